@@ -1,0 +1,200 @@
+"""Throughput of the FDTD time loop on one MI355X.
+
+Workload: the single large plain run at the README shape -- 16 x 16 x 3
+lambda at 0.021 lambda (761 x 761 x 203 cells = 1.18e8, 1714 steps at
+Courant 0.5), 5.4 GHz HH theta 20, a 1 lambda snow layer of Bernoulli(0.3)
+ice voxels over rough soil.  One step = one full Yee time step (H half with
+CPML and E_inc injection, E half with CPML and H_inc injection, phasor DFT on
+the six planes) of that domain.  Materials, CPML state, phasor sums and the
+incident beam are resident in HBM before the timed region; the beam is an
+analytic Gaussian-tapered plane wave (synthetic) of the real beam's shape.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+
+Prints one JSON line (see the keys below).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'FDTD Mcell-updates/s + material-models/hr batched vs plain; p50/p99 latency'
+HBM_PEAK_GBS = 8000.0
+
+
+def cpml_cells(nx, ny, nz, np_):
+    """Cells each half-step's CPML passes touch (cpu/field_updates.py:66-191,
+    243-366 index ranges): d/u (np-1)(ny-1)(nx-1), l/r (nz-1)(ny-1)(np-1),
+    f/b (nz-1)(np-1)(nx-1), two of each."""
+    return 2 * ((np_ - 1) * (ny - 1) * (nx - 1) + (nz - 1) * (ny - 1) * (np_ - 1)
+                + (nz - 1) * (np_ - 1) * (nx - 1))
+
+
+def algorithmic_bytes(nx, ny, nz, np_):
+    """Compulsory HBM bytes per launch of each kernel.
+
+    H half: read E(3) + H(3), write H(3) = 36 B/cell; each CPML face cell
+    reads and writes two psi values = 16 B; E_inc injection reads 2 floats
+    per TFSF-plane cell.  E half: read H(3) + E(3) + Ca + Cb, write E(3) =
+    44 B/cell, same CPML and injection terms.  DFT: per plane sample, four
+    float4 running sums read and written (128 B) plus four field values.
+    """
+    n = nx * ny * nz
+    nc = cpml_cells(nx, ny, nz, np_)
+    plane = nx * ny
+    nxp, nyp, nzp = nx - 2 * np_, ny - 2 * np_, nz - 2 * np_
+    samples = 2 * nyp * nxp + 2 * nzp * nyp + 2 * nzp * nxp
+    return {'h': 36 * n + 16 * nc + 8 * plane,
+            'e': 44 * n + 16 * nc + 8 * plane,
+            'dft': 144 * samples,
+            'cells': n, 'cpml_cells': nc, 'dft_samples': samples}
+
+
+def cpu_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(pc, Ca, Cb, target_s=15.0):
+    """The oracle (C restatement of the reference's CPU loops, OpenMP over
+    z slabs as cpu/fdtd_sim.py does) timed on a bounded sample of the same
+    workload: a few steps of the full domain with injection active."""
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    sys.path.insert(0, os.path.join(ROOT, 'tests'))
+    import oracle
+    import helpers as H
+    from sar_pixel_simulator_amd import workloads
+    oracle.build()
+    g = pc.geom_params
+    nthreads = cpu_threads()
+    max_steps = 24
+    beam = workloads.synthetic_beam_planes(pc, 0, max_steps)
+    geo = H.oracle_geom(pc)
+    geo['nt_inc'] = max_steps
+    sim = oracle.OracleSim(geo, Ca, Cb, H.profiles_of(pc), inc=beam, nthreads=nthreads)
+    t0 = time.perf_counter()
+    sim.run(1)
+    t1 = time.perf_counter() - t0
+    n = int(max(2, min(max_steps - 1, target_s / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    sim.run(n)
+    el = time.perf_counter() - t0
+    cells = g.size_x * g.size_y * g.size_z
+    return {'value': cells * n / el / 1e6, 'unit': 'Mcell-updates/s', 'cores': nthreads,
+            'kind': 'port',
+            'sample': f'{n} time steps (after 1 warm-up) of the full {g.size_x}x{g.size_y}x'
+                      f'{g.size_z} domain with injection active, {el:.1f} s'}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=100)
+    ap.add_argument('--warmup', type=int, default=10)
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--ty', type=int, default=0)
+    ap.add_argument('--zchunk', type=int, default=-1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({'error': 'this build runs on one GPU; multi-GPU runs are not '
+                                   'part of it (see DESIGN.md, sharding)',
+                          'n_gpus': args.gpus}))
+        sys.exit(2)
+
+    from sar_pixel_simulator_amd import _native as native, workloads
+    from sar_pixel_simulator_amd.hip import common, tuning
+    from sar_pixel_simulator_amd.materials import compute_material_arrays
+
+    pc = workloads.large_params()
+    g = pc.geom_params
+    print(f'bench: grid {g.size_x}x{g.size_y}x{g.size_z}, nt {g.size_t}, '
+          f'nt_inc {g.size_t_inc}', file=sys.stderr, flush=True)
+    mat = workloads.large_material(pc, seed=0)
+    Ca, Cb = compute_material_arrays(pc, mat)
+    del mat
+
+    native.init(0)
+    info = native.device_info()
+    sim = native.Sim(common.main_geom(pc), common.coeffs_for(pc))
+    ty, zc = tuning.launch_for(g.size_x, g.size_y, g.size_z)
+    if args.ty > 0:
+        ty = args.ty
+    if args.zchunk >= 0:
+        zc = args.zchunk
+    sim.set_launch(ty, zc)
+    beam = native.Beam(g.size_t_inc, g.size_y, g.size_x)
+    n_fill = min(g.size_t_inc, args.warmup + args.steps + 8)
+    beam.upload(0, *workloads.synthetic_beam_planes(pc, 0, n_fill))
+    sim.attach_beam(beam)
+    sim.set_materials(Ca, Cb)
+    print('bench: device state resident; warm-up', file=sys.stderr, flush=True)
+
+    try:
+        import torch
+        sync = torch.cuda.synchronize
+    except Exception:
+        torch = None
+        sync = sim.sync
+
+    sim.run(args.warmup)
+    sim.sync()
+    sync()
+    t0 = time.perf_counter()
+    sim.run(args.steps)
+    sim.sync()
+    sync()
+    wall = time.perf_counter() - t0
+    ms_step = wall * 1e3 / args.steps
+    cells = g.size_x * g.size_y * g.size_z
+    value = cells * args.steps / wall / 1e6
+    print(f'bench: {args.steps} steps in {wall:.3f} s', file=sys.stderr, flush=True)
+
+    # per-kernel HIP-event times on the library's stream
+    k = sim.time_steps(min(args.steps, 50), per_kernel=True)
+    ab = algorithmic_bytes(g.size_x, g.size_y, g.size_z, g.size_pml)
+    dom = 'e' if k['e'] >= k['h'] else 'h'
+    achieved = ab[dom] / (k[dom] * 1e-3) / 1e9
+    roof = {'bound': 'hbm', 'kernel': 'k_e_half' if dom == 'e' else 'k_h_half',
+            'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+            'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None,
+            'alg_bytes_per_launch': ab[dom], 'avg_launch_ms': round(k[dom], 4)}
+    step_alg = ab['h'] + ab['e'] + ab['dft']
+    per_kernel = {'k_h_half_ms': round(k['h'], 4), 'k_e_half_ms': round(k['e'], 4),
+                  'k_dft_ms': round(k['dft'], 4),
+                  'h_GBs': round(ab['h'] / (k['h'] * 1e-3) / 1e9, 1),
+                  'e_GBs': round(ab['e'] / (k['e'] * 1e-3) / 1e9, 1),
+                  'dft_GBs': round(ab['dft'] / (k['dft'] * 1e-3) / 1e9, 1),
+                  'step_alg_GBs': round(step_alg / (ms_step * 1e-3) / 1e9, 1)}
+    sim.destroy()
+    beam.release()
+
+    cpu = None
+    if not args.no_cpu:
+        print('bench: CPU baseline sample', file=sys.stderr, flush=True)
+        cpu = cpu_baseline(pc, Ca, Cb)
+
+    out = {'metric': METRIC, 'value': round(value, 1), 'unit': 'Mcell-updates/s',
+           'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+           'ms_per_step': round(ms_step, 4), 'higher_is_better': True, 'scaling': 'weak',
+           'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
+           'config': {'workload': 'single large plain run: 761x761x203 cells (16x16x3 '
+                                  'lambda at 0.021 lambda, README shape), snow over rough '
+                                  'soil, 5.4 GHz HH theta 20, 1714-step run',
+                      'cells': cells, 'launch': {'ty': ty, 'zchunk': zc},
+                      'device': info['name'], 'arch': info['arch'],
+                      'cu_count': info['cu_count']},
+           'roofline': roof, 'kernels': per_kernel, 'cpu_baseline': cpu}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,182 @@
+/*
+ * fdtd_hip.h — C-ABI of libfdtd_hip.so, the MI355X (gfx950) FDTD engine.
+ *
+ * The reference has no C-ABI on this path: its backends are Python classes
+ * with the protocol
+ *     __init__(params, mpi=False)                 fdtdsarsim/gpu/main_sim.py:45
+ *     run(Ca, Cb, material_name, material_meta)   fdtdsarsim/gpu/main_sim.py:74
+ * driven by pycuda launches of 23 runtime-compiled kernels
+ * (gpu/main_sim.py:216-355, gpu/field_updates.py:9-907).  These entry points
+ * are what such a backend needs from native code, and what a ctypes binding
+ * (sar_pixel_simulator_amd/_native.py) or any other FFI binds.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative FDTD_E* code on
+ *     failure; fdtd_last_error() then returns a thread-local message.
+ *   - Host arrays are C-order float32 (nz, ny, nx), x fastest, as the
+ *     reference's numpy arrays; the library copies them and never keeps the
+ *     caller's pointer.  Complex outputs are interleaved (re, im) float32
+ *     (numpy complex64) or float64 (complex128).
+ *   - Handles belong to the library until destroyed/released.  One handle is
+ *     used by one host thread at a time; different handles are independent.
+ *   - Work is queued on the library's HIP stream for the handle's device;
+ *     fdtd_sim_run() is asynchronous, fdtd_sim_sync() waits.
+ */
+#ifndef FDTD_HIP_H
+#define FDTD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDTD_ABI_VERSION 1
+#define FDTD_MAX_PML 16
+
+/* error codes */
+#define FDTD_OK 0
+#define FDTD_EINVAL -1    /* bad argument, shape or state */
+#define FDTD_EHIP -2      /* HIP runtime error */
+#define FDTD_ENOMEM -3    /* device allocation failed / over budget */
+#define FDTD_ESTATE -4    /* call not valid in the handle's state */
+
+/* simulation kinds */
+#define FDTD_MODE_MAIN 0      /* H, E, phasors; TFSF injection (cpu/main_sim.py:140-195) */
+#define FDTD_MODE_INCIDENT 1  /* E, hard source, H, record, incident phasors (cpu/incident_sim.py:134-161) */
+
+/* phasor accumulation */
+#define FDTD_DFT_COMPENSATED 0  /* float-float running sum per sample */
+#define FDTD_DFT_REFERENCE 1    /* complex64 running sum rounded every step, as cpu/field_updates.py:385-400 */
+
+/* Geometry and run length of one simulation.  Field names follow GeomParams
+ * (fdtdsarsim/parameters.py:57-86). */
+typedef struct FdtdGeom {
+    int32_t nx, ny, nz;     /* size_x, size_y, size_z */
+    int32_t npml;           /* size_pml (<= FDTD_MAX_PML) */
+    int32_t tfsf_z;         /* MAIN: TFSF plane; INCIDENT: feed-in plane */
+    int32_t read_out;       /* INCIDENT: read-out plane; MAIN: unused */
+    int32_t nt;             /* steps the simulation runs (size_t / size_t_inc) */
+    int32_t nt_inc;         /* MAIN: injected steps (size_t_inc); INCIDENT: source steps (size_t_pulse) */
+    int32_t mode;           /* FDTD_MODE_* */
+    int32_t dft_accum;      /* FDTD_DFT_* */
+} FdtdGeom;
+
+/* Update coefficients.  Db, dtm0, dx, dte0dx are the float32 scalars of
+ * cpu/main_sim.py:119-122; omega, dt the float64 values of :101-102; the
+ * eight CPML profiles are Absorbers' 1-D b/c profiles (absorbers.py:121-153)
+ * in the order be_dlf, ce_dlf, bh_dlf, ch_dlf, be_urb, ce_urb, bh_urb, ch_urb,
+ * each npml long (entries past npml ignored). */
+typedef struct FdtdCoeffs {
+    float Db, dtm0, dx, dte0dx;
+    double omega, dt;
+    float prof[8][FDTD_MAX_PML];
+} FdtdCoeffs;
+
+typedef struct FdtdDeviceInfo {
+    int32_t device;
+    int32_t cu_count;
+    int32_t xcd_count;
+    int32_t warp_size;
+    int64_t total_mem;
+    int64_t free_mem;
+    char arch[64];
+    char name[128];
+} FdtdDeviceInfo;
+
+typedef struct FdtdSim FdtdSim;
+typedef struct FdtdBeam FdtdBeam;
+typedef struct FdtdBatch FdtdBatch;
+
+/* ---- library ---------------------------------------------------------- */
+int fdtd_abi_version(void);
+const char *fdtd_last_error(void);
+/* Select the device for subsequent handle creation on this thread
+ * (replaces experiment.py:33's CUDA_DEVICE and pycuda.autoinit). */
+int fdtd_init(int device);
+int fdtd_device_info(FdtdDeviceInfo *out);
+
+/* ---- one simulation (MainSimGpu's device state, gpu/main_sim.py:512-787) */
+/* Bytes of device memory a simulation of this geometry holds. */
+int fdtd_sim_footprint(const FdtdGeom *g, int64_t *bytes);
+int fdtd_sim_create(const FdtdGeom *g, const FdtdCoeffs *c, FdtdSim **out);
+void fdtd_sim_destroy(FdtdSim *s);
+/* Per-cell Ca, Cb (nz, ny, nx) float32 (gpu/main_sim.py:203-204). */
+int fdtd_sim_set_materials(FdtdSim *s, const float *Ca, const float *Cb);
+/* Same value in every cell (the incident simulation's vacuum,
+ * cpu/incident_sim.py:115-116). */
+int fdtd_sim_set_uniform_material(FdtdSim *s, float ca, float cb);
+/* INCIDENT: hard-source planes, ex and ey each (n_src, ny, nx) float32
+ * (imaginary parts of the excitation, cpu/incident_sim.py:110-111,142-144). */
+int fdtd_sim_set_source(FdtdSim *s, const float *src_ex, const float *src_ey, int n_src);
+/* MAIN: beam injected at tfsf_z; INCIDENT: beam the read-out planes are
+ * recorded into.  The sim keeps a reference until destroyed or re-attached. */
+int fdtd_sim_attach_beam(FdtdSim *s, FdtdBeam *b);
+/* Zero fields, CPML state and phasors; step counter to 0
+ * (gpu/main_sim.py:691-787). */
+int fdtd_sim_reset(FdtdSim *s);
+/* Queue nsteps time steps (asynchronous).  Steps past geom.nt are no-ops. */
+int fdtd_sim_run(FdtdSim *s, int nsteps);
+int fdtd_sim_sync(FdtdSim *s);
+int fdtd_sim_step_count(FdtdSim *s, int *t);
+/* Phasor `index` in the reference's save order (cpu/main_sim.py:220-229:
+ * ex_ph_d ey_ph_d hx_ph_d hy_ph_d, ..._u, ey_ph_l ez_ph_l hy_ph_l hz_ph_l,
+ * ..._r, ex_ph_f ez_ph_f hx_ph_f hz_ph_f, ..._b) as complex64 with the
+ * reference's shape.  INCIDENT: indices 0..3 are ex/ey/hx/hy_inc_phasor. */
+int fdtd_sim_get_phasor(FdtdSim *s, int index, float *out_c64);
+/* Field component 0..5 = ex ey ez hx hy hz, (nz, ny, nx) float32. */
+int fdtd_sim_get_field(FdtdSim *s, int comp, float *out);
+/* Overwrite a field component from host (nz, ny, nx) float32 (tests). */
+int fdtd_sim_set_field(FdtdSim *s, int comp, const float *in);
+/* CPML state psi 0..23 in the reference's shapes (cpu/fdtd_sim.py:165-187):
+ * E d,u: Exz Eyz; l,r: Eyx Ezx; f,b: Exy Ezy; then H likewise. */
+int fdtd_sim_get_psi(FdtdSim *s, int index, float *out);
+/* Tile shape of the fused stencil kernels: ty rows per block (1,2,4,8) and
+ * z-chunk length (0 = whole column). */
+int fdtd_sim_set_launch(FdtdSim *s, int ty, int zchunk);
+/* Run nsteps and report HIP-event times on the library stream: whole run and
+ * per-kernel averages (ms).  Any pointer may be NULL. */
+int fdtd_sim_time_steps(FdtdSim *s, int nsteps, float *ms_total, float *ms_h,
+                        float *ms_e, float *ms_dft);
+
+/* Near-to-far transform of the top-face phasors (near2far.py:28-156), FP64
+ * on the device.  thetas in degrees; out = E_theta, E_phi, H_theta, H_phi,
+ * each ntheta complex128 (interleaved), i.e. 8*ntheta doubles. */
+int fdtd_sim_near2far(FdtdSim *s, const double *thetas_deg, int ntheta,
+                      double phi_deg, double r, double freq, double *out);
+/* Same transform of caller-supplied complex64 top-face phasors, each
+ * (ny_ph, nx_ph): ex_u, ey_u, hx_u, hy_u. */
+int fdtd_near2far_host(const float *ex_u, const float *ey_u, const float *hx_u,
+                       const float *hy_u, int nx_ph, int ny_ph, int nz_ph,
+                       double dx, const double *thetas_deg, int ntheta,
+                       double phi_deg, double r, double freq, double *out);
+
+/* ---- incident beam (the (nt_inc, ny, nx) ex/ey/hx/hy datasets of the
+ * incident HDF5 file, cpu/incident_sim.py:95-98), resident in HBM --------- */
+/* ex, ey, hx, hy each (nt, ny, nx) float32; all NULL = zero-filled beam to
+ * record into. */
+int fdtd_beam_create(int nt, int ny, int nx, const float *ex, const float *ey,
+                     const float *hx, const float *hy, FdtdBeam **out);
+/* Overwrite steps [t0, t0 + n) of every component from host planes, each
+ * (n, ny, nx) float32. */
+int fdtd_beam_upload(FdtdBeam *b, int t0, int n, const float *ex, const float *ey,
+                     const float *hx, const float *hy);
+int fdtd_beam_get(FdtdBeam *b, int comp, float *out);
+void fdtd_beam_release(FdtdBeam *b);
+
+/* ---- batches: many simulations stepped by one launch per kernel ------- */
+/* The batch references the sims (they must outlive it).  Sims may differ
+ * in size and mode; each keeps its own step counter. */
+int fdtd_batch_create(FdtdSim *const *sims, int n, FdtdBatch **out);
+void fdtd_batch_destroy(FdtdBatch *b);
+int fdtd_batch_set_launch(FdtdBatch *b, int ty, int zchunk);
+/* Queue nsteps steps of every member; use_graph != 0 captures the step
+ * sequence into a HIP graph on first use and replays it. */
+int fdtd_batch_run(FdtdBatch *b, int nsteps, int use_graph);
+int fdtd_batch_sync(FdtdBatch *b);
+int fdtd_batch_time_steps(FdtdBatch *b, int nsteps, int use_graph, float *ms_total);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FDTD_HIP_H */

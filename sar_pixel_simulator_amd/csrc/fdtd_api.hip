@@ -1,0 +1,1014 @@
+// C-ABI of libfdtd_hip.so: handles, device memory, launch sequencing and
+// HIP graphs around the kernels in fdtd_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "fdtd_kernels.h"
+
+namespace fdtd {
+__global__ void k_h_half(const Desc *, const int4 *, int);
+__global__ void k_e_half(const Desc *, const int4 *, int);
+__global__ void k_dft(const Desc *, int, unsigned int *);
+__global__ void k_phasor_out(const float4 *, long long, float2 *);
+__global__ void k_n2f_partial(const float2 *, const float2 *, const float2 *, const float2 *, int, int, int,
+                              const double2 *, const double2 *, double *);
+__global__ void k_n2f_final(const double *, int, int, const double4 *, const double2 *, double2, double,
+                            double2 *);
+}  // namespace fdtd
+
+using namespace fdtd;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+static thread_local int g_device = 0;
+
+static int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(FDTD_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                              \
+    } while (0)
+
+#define CHECK(cond, ...)                                      \
+    do {                                                      \
+        if (!(cond)) return fail(FDTD_EINVAL, __VA_ARGS__);   \
+    } while (0)
+
+static long long pitch_of(int nx) { return ((long long)nx + 15) / 16 * 16; }
+
+// ---------------------------------------------------------------------------
+// handles
+// ---------------------------------------------------------------------------
+struct FdtdBeam {
+    int device = 0;
+    int nt = 0, ny = 0, nx = 0;
+    float *d = nullptr;
+    std::atomic<int> refs{1};
+};
+
+static void beam_unref(FdtdBeam *b) {
+    if (b && --b->refs == 0) {
+        (void)hipSetDevice(b->device);
+        (void)hipFree(b->d);
+        delete b;
+    }
+}
+
+// A set of simulations stepped together: one launch per kernel per step.
+struct Group {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Desc *d_descs = nullptr;
+    int4 *d_tiles = nullptr;
+    unsigned int *d_done = nullptr;
+    int ntiles = 0, nmodels = 0, mode = FDTD_MODE_MAIN;
+    long long max_dft_items = 0;
+    Launch launch;
+    std::vector<FdtdSim *> members;
+    std::map<int, hipGraphExec_t> graphs;
+    std::map<int, hipGraph_t> graph_src;
+};
+
+struct FdtdSim {
+    int device = 0;
+    FdtdGeom g{};
+    FdtdCoeffs c{};
+    Desc h{};                    // host copy of the descriptor
+    long long pitch = 0, ncell = 0;
+    float *fields = nullptr;     // 6 * ncell
+    float *mat = nullptr;        // 2 * ncell (Ca, Cb)
+    float *psi = nullptr;        // all 24 arrays
+    long long psi_len[24] = {};
+    float4 *acc = nullptr;
+    double4 *tw64 = nullptr;
+    float4 *tw32 = nullptr;
+    float *src = nullptr;
+    int *step = nullptr;
+    FdtdBeam *beam = nullptr;
+    Group self;                  // batch of one
+    float2 *scratch_c64 = nullptr;
+    long long scratch_len = 0;
+};
+
+struct FdtdBatch {
+    Group grp;
+};
+
+// ---------------------------------------------------------------------------
+// geometry bookkeeping
+// ---------------------------------------------------------------------------
+static int check_geom(const FdtdGeom *g) {
+    CHECK(g, "geom is NULL");
+    CHECK(g->nx >= 3 && g->ny >= 3 && g->nz >= 3, "grid too small: %d x %d x %d", g->nx, g->ny, g->nz);
+    CHECK(g->nz < 65536, "nz %d too large for the tile encoding", g->nz);
+    CHECK(g->npml >= 2 && g->npml <= FDTD_MAX_PML, "npml %d outside [2, %d]", g->npml, FDTD_MAX_PML);
+    CHECK(2 * g->npml + 2 <= g->nx && 2 * g->npml + 2 <= g->ny && 2 * g->npml + 2 <= g->nz,
+          "grid %d x %d x %d too small for npml %d", g->nx, g->ny, g->nz, g->npml);
+    CHECK(g->nt >= 0 && g->nt_inc >= 0, "negative step counts");
+    CHECK(g->mode == FDTD_MODE_MAIN || g->mode == FDTD_MODE_INCIDENT, "unknown mode %d", g->mode);
+    CHECK(g->dft_accum == FDTD_DFT_COMPENSATED || g->dft_accum == FDTD_DFT_REFERENCE,
+          "unknown dft_accum %d", g->dft_accum);
+    if (g->mode == FDTD_MODE_MAIN) {
+        CHECK(g->tfsf_z >= 0 && g->tfsf_z < g->nz, "tfsf_z %d outside [0, %d)", g->tfsf_z, g->nz);
+    } else {
+        CHECK(g->tfsf_z >= 0 && g->tfsf_z < g->nz, "feed-in plane %d outside [0, %d)", g->tfsf_z, g->nz);
+        CHECK(g->read_out >= 1 && g->read_out < g->nz, "read-out plane %d outside [1, %d)", g->read_out, g->nz);
+    }
+    return FDTD_OK;
+}
+
+struct Sizes {
+    long long pitch, ncell, psi_len[24], psi_total, nsamp, acc_total;
+    int nfaces;
+    int kind[kMaxFaces], pos[kMaxFaces], na[kMaxFaces], nb[kMaxFaces];
+    long long face_off[kMaxFaces + 1];
+};
+
+static Sizes sizes_of(const FdtdGeom *g) {
+    Sizes s{};
+    const long long nx = g->nx, ny = g->ny, nz = g->nz, np = g->npml;
+    s.pitch = pitch_of(g->nx);
+    s.ncell = nz * ny * s.pitch;
+    const long long du = np * ny * s.pitch, lr = nz * ny * np, fb = nz * np * s.pitch;
+    for (int k = 0; k < 24; ++k) {
+        const int face = (k % 12) / 2;   // d u l r f b
+        s.psi_len[k] = (face < 2) ? du : (face < 4) ? lr : fb;
+        s.psi_total += s.psi_len[k];
+    }
+    const int nxp = g->nx - 2 * g->npml, nyp = g->ny - 2 * g->npml, nzp = g->nz - 2 * g->npml;
+    if (g->mode == FDTD_MODE_MAIN) {
+        s.nfaces = 6;
+        const int kinds[6] = {kFaceZ, kFaceZ, kFaceX, kFaceX, kFaceY, kFaceY};
+        const int pos[6] = {g->npml + 1, g->nz - g->npml - 1, g->npml + 1, g->nx - g->npml - 1,
+                            g->npml + 1, g->ny - g->npml - 1};
+        const int na[6] = {nyp, nyp, nzp, nzp, nzp, nzp};
+        const int nb[6] = {nxp, nxp, nyp, nyp, nxp, nxp};
+        for (int f = 0; f < 6; ++f) {
+            s.kind[f] = kinds[f]; s.pos[f] = pos[f]; s.na[f] = na[f]; s.nb[f] = nb[f];
+        }
+    } else {
+        s.nfaces = 1;
+        s.kind[0] = kFaceZ; s.pos[0] = g->read_out; s.na[0] = nyp; s.nb[0] = nxp;
+    }
+    s.face_off[0] = 0;
+    for (int f = 0; f < s.nfaces; ++f) s.face_off[f + 1] = s.face_off[f] + (long long)s.na[f] * s.nb[f];
+    s.nsamp = s.face_off[s.nfaces];
+    s.acc_total = 4 * s.nsamp;
+    (void)nx;
+    return s;
+}
+
+static long long footprint_bytes(const FdtdGeom *g) {
+    const Sizes s = sizes_of(g);
+    long long b = 6 * s.ncell * 4 + s.psi_total * 4 + s.acc_total * 16;
+    if (g->mode == FDTD_MODE_MAIN) b += 2 * s.ncell * 4;
+    b += (long long)(g->nt + 1) * (32 + 32) + 4096;
+    return b;
+}
+
+// Twiddles of the running DFT for step t: exp(-1j*omega*tt*dt) with the
+// phase formed as numba forms it, theta = (-omega * tt) * dt
+// (cpu/field_updates.py:387,395).  MAIN: E at tt = t, H at tt = t - 0.5;
+// INCIDENT swaps them (phasors_incident, :478,486).
+static void make_twiddles(const FdtdGeom *g, const FdtdCoeffs *c, std::vector<double4> &w64,
+                          std::vector<float4> &w32) {
+    const int n = g->nt > 0 ? g->nt : 1;
+    w64.resize(n);
+    w32.resize(2 * (size_t)n);
+    auto split = [](double v, float &hi, float &lo) {
+        hi = (float)v;
+        lo = (float)(v - (double)hi);
+    };
+    for (int t = 0; t < n; ++t) {
+        double te = (double)t, th = (double)t - 0.5;
+        if (g->mode == FDTD_MODE_INCIDENT) std::swap(te, th);
+        const double pe = (-c->omega * te) * c->dt;
+        const double ph = (-c->omega * th) * c->dt;
+        const double ce = std::cos(pe), se = std::sin(pe), ch = std::cos(ph), sh = std::sin(ph);
+        w64[t] = make_double4(ce, se, ch, sh);
+        float4 a, b;
+        split(0.5 * ce, a.x, a.y);
+        split(0.5 * se, a.z, a.w);
+        split(0.25 * ch, b.x, b.y);
+        split(0.25 * sh, b.z, b.w);
+        w32[2 * (size_t)t] = a;
+        w32[2 * (size_t)t + 1] = b;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// groups (launch sequencing)
+// ---------------------------------------------------------------------------
+static void group_clear_graphs(Group &G) {
+    for (auto &kv : G.graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto &kv : G.graph_src) (void)hipGraphDestroy(kv.second);
+    G.graphs.clear();
+    G.graph_src.clear();
+}
+
+static int group_build(Group &G) {
+    CHECK(!G.members.empty(), "empty group");
+    HIP_TRY(hipSetDevice(G.device));
+    const int ty = G.launch.ty;
+    CHECK(ty == 1 || ty == 2 || ty == 4, "ty must be 1, 2 or 4 (got %d)", ty);
+    std::vector<int4> tiles;
+    std::vector<Desc> descs;
+    G.mode = G.members[0]->g.mode;
+    G.max_dft_items = 0;
+    for (size_t m = 0; m < G.members.size(); ++m) {
+        FdtdSim *s = G.members[m];
+        CHECK(s->g.mode == G.mode, "a group cannot mix MAIN and INCIDENT simulations");
+        CHECK(s->device == G.device, "group members must share a device");
+        descs.push_back(s->h);
+        const int nz = s->g.nz, zc = G.launch.zchunk > 0 ? G.launch.zchunk : nz;
+        for (int z0 = 0; z0 < nz; z0 += zc) {
+            const int z1 = std::min(nz, z0 + zc);
+            for (int y0 = 0; y0 < s->g.ny; y0 += ty)
+                for (int x0 = 0; x0 < s->g.nx; x0 += kWave)
+                    tiles.push_back(make_int4((int)m, x0, y0, z0 | (z1 << 16)));
+        }
+        G.max_dft_items = std::max(G.max_dft_items, s->h.dft_items);
+    }
+    group_clear_graphs(G);
+    (void)hipFree(G.d_descs);
+    (void)hipFree(G.d_tiles);
+    G.d_descs = nullptr;
+    G.d_tiles = nullptr;
+    HIP_TRY(hipMalloc(&G.d_descs, descs.size() * sizeof(Desc)));
+    HIP_TRY(hipMalloc(&G.d_tiles, tiles.size() * sizeof(int4)));
+    if (!G.d_done) {
+        HIP_TRY(hipMalloc(&G.d_done, sizeof(unsigned int)));
+        HIP_TRY(hipMemsetAsync(G.d_done, 0, sizeof(unsigned int), G.stream));
+    }
+    HIP_TRY(hipMemcpyAsync(G.d_descs, descs.data(), descs.size() * sizeof(Desc), hipMemcpyHostToDevice,
+                           G.stream));
+    HIP_TRY(hipMemcpyAsync(G.d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice,
+                           G.stream));
+    HIP_TRY(hipStreamSynchronize(G.stream));
+    G.ntiles = (int)tiles.size();
+    G.nmodels = (int)descs.size();
+    return FDTD_OK;
+}
+
+static void group_free(Group &G) {
+    (void)hipSetDevice(G.device);
+    group_clear_graphs(G);
+    (void)hipFree(G.d_descs);
+    (void)hipFree(G.d_tiles);
+    (void)hipFree(G.d_done);
+    if (G.stream) (void)hipStreamDestroy(G.stream);
+    G = Group();
+}
+
+enum KernelId { KH = 0, KE = 1, KD = 2 };
+
+static void launch_kernel(Group &G, KernelId k) {
+    const dim3 blk(kWave, G.launch.ty);
+    if (k == KH) {
+        hipLaunchKernelGGL(k_h_half, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
+    } else if (k == KE) {
+        hipLaunchKernelGGL(k_e_half, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
+    } else {
+        const unsigned gx = (unsigned)std::max(1LL, (G.max_dft_items + 255) / 256);
+        hipLaunchKernelGGL(k_dft, dim3(gx, G.nmodels), dim3(256), 0, G.stream, G.d_descs, G.nmodels,
+                           G.d_done);
+    }
+}
+
+static void launch_step(Group &G) {
+    if (G.mode == FDTD_MODE_MAIN) {   // cpu/main_sim.py:140-195: H, E, phasors
+        launch_kernel(G, KH);
+        launch_kernel(G, KE);
+    } else {                          // cpu/incident_sim.py:134-161: E, H, record + phasors
+        launch_kernel(G, KE);
+        launch_kernel(G, KH);
+    }
+    launch_kernel(G, KD);
+}
+
+static int group_run(Group &G, int nsteps, bool use_graph) {
+    CHECK(nsteps >= 0, "negative step count");
+    if (nsteps == 0) return FDTD_OK;
+    HIP_TRY(hipSetDevice(G.device));
+    if (!use_graph) {
+        for (int s = 0; s < nsteps; ++s) launch_step(G);
+        HIP_TRY(hipGetLastError());
+        return FDTD_OK;
+    }
+    // Replay a captured chunk of steps; the chunk length is bounded so graph
+    // instantiation stays cheap, and the remainder runs as its own graph.
+    const int chunk = std::min(nsteps, 64);
+    auto replay = [&](int len) -> int {
+        auto it = G.graphs.find(len);
+        if (it == G.graphs.end()) {
+            hipGraph_t graph;
+            HIP_TRY(hipStreamBeginCapture(G.stream, hipStreamCaptureModeThreadLocal));
+            for (int s = 0; s < len; ++s) launch_step(G);
+            HIP_TRY(hipStreamEndCapture(G.stream, &graph));
+            hipGraphExec_t exec;
+            HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            G.graph_src[len] = graph;
+            it = G.graphs.emplace(len, exec).first;
+        }
+        HIP_TRY(hipGraphLaunch(it->second, G.stream));
+        return FDTD_OK;
+    };
+    int done = 0;
+    while (done + chunk <= nsteps) {
+        int rc = replay(chunk);
+        if (rc) return rc;
+        done += chunk;
+    }
+    if (done < nsteps) {
+        int rc = replay(nsteps - done);
+        if (rc) return rc;
+    }
+    return FDTD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// library
+// ---------------------------------------------------------------------------
+extern "C" int fdtd_abi_version(void) { return FDTD_ABI_VERSION; }
+extern "C" const char *fdtd_last_error(void) { return g_err.c_str(); }
+
+extern "C" int fdtd_init(int device) {
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    CHECK(device >= 0 && device < n, "device %d not present (%d devices)", device, n);
+    HIP_TRY(hipSetDevice(device));
+    g_device = device;
+    return FDTD_OK;
+}
+
+extern "C" int fdtd_device_info(FdtdDeviceInfo *out) {
+    CHECK(out, "out is NULL");
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, g_device));
+    HIP_TRY(hipSetDevice(g_device));
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot));
+    memset(out, 0, sizeof *out);
+    out->device = g_device;
+    out->cu_count = p.multiProcessorCount;
+    out->xcd_count = kXcds;
+    out->warp_size = p.warpSize;
+    out->total_mem = (int64_t)tot;
+    out->free_mem = (int64_t)fr;
+    snprintf(out->arch, sizeof out->arch, "%s", p.gcnArchName);
+    snprintf(out->name, sizeof out->name, "%s", p.name);
+    return FDTD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// simulations
+// ---------------------------------------------------------------------------
+extern "C" int fdtd_sim_footprint(const FdtdGeom *g, int64_t *bytes) {
+    int rc = check_geom(g);
+    if (rc) return rc;
+    CHECK(bytes, "bytes is NULL");
+    *bytes = footprint_bytes(g);
+    return FDTD_OK;
+}
+
+static int sim_alloc(FdtdSim *s) {
+    const FdtdGeom *g = &s->g;
+    const Sizes z = sizes_of(g);
+    s->pitch = z.pitch;
+    s->ncell = z.ncell;
+    HIP_TRY(hipMalloc(&s->fields, 6 * z.ncell * sizeof(float)));
+    if (g->mode == FDTD_MODE_MAIN) HIP_TRY(hipMalloc(&s->mat, 2 * z.ncell * sizeof(float)));
+    HIP_TRY(hipMalloc(&s->psi, z.psi_total * sizeof(float)));
+    HIP_TRY(hipMalloc(&s->acc, z.acc_total * sizeof(float4)));
+    std::vector<double4> w64;
+    std::vector<float4> w32;
+    make_twiddles(g, &s->c, w64, w32);
+    HIP_TRY(hipMalloc(&s->tw64, w64.size() * sizeof(double4)));
+    HIP_TRY(hipMalloc(&s->tw32, w32.size() * sizeof(float4)));
+    HIP_TRY(hipMemcpy(s->tw64, w64.data(), w64.size() * sizeof(double4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->tw32, w32.data(), w32.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&s->step, sizeof(int)));
+    // the field arrays' padding columns are read as x+1 neighbours never,
+    // but keep all device memory defined
+    HIP_TRY(hipMemset(s->fields, 0, 6 * z.ncell * sizeof(float)));
+    if (s->mat) HIP_TRY(hipMemset(s->mat, 0, 2 * z.ncell * sizeof(float)));
+
+    Desc &d = s->h;
+    memset(&d, 0, sizeof d);
+    d.nx = g->nx; d.ny = g->ny; d.nz = g->nz; d.npml = g->npml;
+    d.tfsf_z = g->tfsf_z; d.read_out = g->read_out; d.nt = g->nt; d.nt_inc = g->nt_inc;
+    d.mode = g->mode; d.dft_accum = g->dft_accum; d.uniform = 0; d.n_src = 0;
+    d.sy = z.pitch; d.sz = (long long)g->ny * z.pitch;
+    d.du_sz = (long long)g->ny * z.pitch;
+    d.lr_sz = (long long)g->ny * g->npml;
+    d.fb_sz = (long long)g->npml * z.pitch;
+    d.Db = s->c.Db; d.dtm0 = s->c.dtm0; d.dx = s->c.dx; d.dte0dx = s->c.dte0dx;
+    memcpy(d.prof, s->c.prof, sizeof d.prof);
+    for (int k = 0; k < 6; ++k) d.f[k] = s->fields + k * z.ncell;
+    if (s->mat) {
+        d.ca = s->mat;
+        d.cb = s->mat + z.ncell;
+    }
+    long long off = 0;
+    for (int k = 0; k < 24; ++k) {
+        d.psi[k] = s->psi + off;
+        s->psi_len[k] = z.psi_len[k];
+        off += z.psi_len[k];
+    }
+    d.acc = s->acc;
+    for (int k = 0; k < 4 * z.nfaces; ++k) {
+        const int f = k / 4;
+        d.acc_off[k] = 4 * z.face_off[f] + (long long)(k % 4) * (z.face_off[f + 1] - z.face_off[f]);
+    }
+    // acc layout: face f occupies [4*face_off[f], 4*face_off[f+1]); within it,
+    // component c starts at c * face_size.
+    d.acc_off[24] = z.acc_total;
+    d.nfaces = z.nfaces;
+    for (int f = 0; f < z.nfaces; ++f) {
+        d.face_kind[f] = z.kind[f]; d.face_pos[f] = z.pos[f];
+        d.face_na[f] = z.na[f]; d.face_nb[f] = z.nb[f];
+    }
+    for (int f = 0; f <= z.nfaces; ++f) d.face_off[f] = z.face_off[f];
+    d.dft_items = z.nsamp;
+    d.tw64 = s->tw64;
+    d.tw32 = s->tw32;
+    d.step = s->step;
+    return FDTD_OK;
+}
+
+static void sim_free(FdtdSim *s) {
+    (void)hipSetDevice(s->device);
+    group_free(s->self);
+    (void)hipFree(s->fields);
+    (void)hipFree(s->mat);
+    (void)hipFree(s->psi);
+    (void)hipFree(s->acc);
+    (void)hipFree(s->tw64);
+    (void)hipFree(s->tw32);
+    (void)hipFree(s->src);
+    (void)hipFree(s->step);
+    (void)hipFree(s->scratch_c64);
+    beam_unref(s->beam);
+}
+
+static int sim_refresh(FdtdSim *s) {
+    // descriptor changed on the host: rebuild the batch-of-one
+    return group_build(s->self);
+}
+
+extern "C" int fdtd_sim_create(const FdtdGeom *g, const FdtdCoeffs *c, FdtdSim **out) {
+    int rc = check_geom(g);
+    if (rc) return rc;
+    CHECK(c && out, "NULL argument");
+    *out = nullptr;
+    FdtdSim *s = new FdtdSim();
+    s->device = g_device;
+    s->g = *g;
+    s->c = *c;
+    hipError_t e = hipSetDevice(s->device);
+    if (e != hipSuccess) {
+        delete s;
+        return fail(FDTD_EHIP, "hipSetDevice: %s", hipGetErrorString(e));
+    }
+    rc = sim_alloc(s);
+    if (rc == FDTD_OK) {
+        s->self.device = s->device;
+        e = hipStreamCreateWithFlags(&s->self.stream, hipStreamNonBlocking);
+        if (e != hipSuccess) rc = fail(FDTD_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    if (rc == FDTD_OK) {
+        s->self.members = {s};
+        rc = group_build(s->self);
+    }
+    if (rc == FDTD_OK) rc = fdtd_sim_reset(s);
+    if (rc != FDTD_OK) {
+        std::string keep = g_err;
+        sim_free(s);
+        delete s;
+        g_err = keep;
+        return rc;
+    }
+    *out = s;
+    return FDTD_OK;
+}
+
+extern "C" void fdtd_sim_destroy(FdtdSim *s) {
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->self.stream) (void)hipStreamSynchronize(s->self.stream);
+    sim_free(s);
+    delete s;
+}
+
+static int copy_volume_in(FdtdSim *s, float *dst, const float *src) {
+    HIP_TRY(hipMemcpy2DAsync(dst, s->pitch * sizeof(float), src, (size_t)s->g.nx * sizeof(float),
+                             (size_t)s->g.nx * sizeof(float), (size_t)s->g.ny * s->g.nz,
+                             hipMemcpyHostToDevice, s->self.stream));
+    return FDTD_OK;
+}
+
+static int copy_volume_out(FdtdSim *s, float *dst, const float *src) {
+    HIP_TRY(hipMemcpy2DAsync(dst, (size_t)s->g.nx * sizeof(float), src, s->pitch * sizeof(float),
+                             (size_t)s->g.nx * sizeof(float), (size_t)s->g.ny * s->g.nz,
+                             hipMemcpyDeviceToHost, s->self.stream));
+    HIP_TRY(hipStreamSynchronize(s->self.stream));
+    return FDTD_OK;
+}
+
+extern "C" int fdtd_sim_set_materials(FdtdSim *s, const float *Ca, const float *Cb) {
+    CHECK(s && Ca && Cb, "NULL argument");
+    CHECK(s->g.mode == FDTD_MODE_MAIN, "per-cell materials are for MAIN simulations");
+    HIP_TRY(hipSetDevice(s->device));
+    int rc = copy_volume_in(s, s->mat, Ca);
+    if (!rc) rc = copy_volume_in(s, s->mat + s->ncell, Cb);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s->self.stream));
+    if (s->h.uniform) {
+        s->h.uniform = 0;
+        return sim_refresh(s);
+    }
+    return FDTD_OK;
+}
+
+extern "C" int fdtd_sim_set_uniform_material(FdtdSim *s, float ca, float cb) {
+    CHECK(s, "NULL sim");
+    s->h.uniform = 1;
+    s->h.ca_u = ca;
+    s->h.cb_u = cb;
+    return sim_refresh(s);
+}
+
+extern "C" int fdtd_sim_set_source(FdtdSim *s, const float *src_ex, const float *src_ey, int n_src) {
+    CHECK(s && src_ex && src_ey, "NULL argument");
+    CHECK(s->g.mode == FDTD_MODE_INCIDENT, "hard sources are for INCIDENT simulations");
+    CHECK(n_src >= 0, "negative n_src");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->self.stream));
+    (void)hipFree(s->src);
+    s->src = nullptr;
+    const size_t plane = (size_t)s->g.ny * s->g.nx;
+    const size_t n = (size_t)n_src * plane;
+    if (n) {
+        HIP_TRY(hipMalloc(&s->src, 2 * n * sizeof(float)));
+        HIP_TRY(hipMemcpy(s->src, src_ex, n * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(s->src + n, src_ey, n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    s->h.src = s->src;
+    s->h.src_cs = (long long)n;
+    s->h.n_src = n_src;
+    return sim_refresh(s);
+}
+
+extern "C" int fdtd_sim_attach_beam(FdtdSim *s, FdtdBeam *b) {
+    CHECK(s && b, "NULL argument");
+    CHECK(b->ny == s->g.ny && b->nx == s->g.nx, "beam plane %d x %d does not match grid %d x %d", b->ny,
+          b->nx, s->g.ny, s->g.nx);
+    CHECK(b->device == s->device, "beam and simulation are on different devices");
+    if (s->g.mode == FDTD_MODE_MAIN)
+        CHECK(b->nt >= s->g.nt_inc, "beam has %d steps, simulation injects %d", b->nt, s->g.nt_inc);
+    else
+        CHECK(b->nt >= s->g.nt, "beam has %d steps, incident run records %d", b->nt, s->g.nt);
+    b->refs++;
+    beam_unref(s->beam);
+    s->beam = b;
+    s->h.nt_beam = b->nt;
+    s->h.beam_cs = (long long)b->nt * b->ny * b->nx;
+    if (s->g.mode == FDTD_MODE_MAIN) {
+        s->h.beam = b->d;
+        s->h.rec = nullptr;
+        s->h.dft_items = s->h.face_off[s->h.nfaces];
+    } else {
+        s->h.beam = nullptr;
+        s->h.rec = b->d;
+        s->h.dft_items = s->h.face_off[s->h.nfaces] + 4LL * b->ny * b->nx;
+    }
+    return sim_refresh(s);
+}
+
+extern "C" int fdtd_sim_reset(FdtdSim *s) {
+    CHECK(s, "NULL sim");
+    HIP_TRY(hipSetDevice(s->device));
+    const Sizes z = sizes_of(&s->g);
+    hipStream_t st = s->self.stream;
+    HIP_TRY(hipMemsetAsync(s->fields, 0, 6 * z.ncell * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(s->psi, 0, z.psi_total * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(s->acc, 0, z.acc_total * sizeof(float4), st));
+    HIP_TRY(hipMemsetAsync(s->step, 0, sizeof(int), st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return FDTD_OK;
+}
+
+static int check_ready(FdtdSim *s) {
+    if (s->g.mode == FDTD_MODE_MAIN) {
+        CHECK(s->h.uniform || s->mat, "materials not set");
+        if (s->g.nt_inc > 0) CHECK(s->beam, "no beam attached (simulation injects %d steps)", s->g.nt_inc);
+    } else {
+        CHECK(s->h.uniform, "INCIDENT simulations need a uniform material");
+        CHECK(s->g.nt_inc == 0 || s->src, "hard source not set");
+    }
+    return FDTD_OK;
+}
+
+extern "C" int fdtd_sim_run(FdtdSim *s, int nsteps) {
+    CHECK(s, "NULL sim");
+    int rc = check_ready(s);
+    if (rc) return rc;
+    return group_run(s->self, nsteps, false);
+}
+
+extern "C" int fdtd_sim_sync(FdtdSim *s) {
+    CHECK(s, "NULL sim");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->self.stream));
+    return FDTD_OK;
+}
+
+extern "C" int fdtd_sim_step_count(FdtdSim *s, int *t) {
+    CHECK(s && t, "NULL argument");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipMemcpyAsync(t, s->step, sizeof(int), hipMemcpyDeviceToHost, s->self.stream));
+    HIP_TRY(hipStreamSynchronize(s->self.stream));
+    return FDTD_OK;
+}
+
+static int ensure_scratch(FdtdSim *s, long long n) {
+    if (s->scratch_len >= n) return FDTD_OK;
+    (void)hipFree(s->scratch_c64);
+    s->scratch_c64 = nullptr;
+    s->scratch_len = 0;
+    HIP_TRY(hipMalloc(&s->scratch_c64, n * sizeof(float2)));
+    s->scratch_len = n;
+    return FDTD_OK;
+}
+
+extern "C" int fdtd_sim_get_phasor(FdtdSim *s, int index, float *out_c64) {
+    CHECK(s && out_c64, "NULL argument");
+    CHECK(index >= 0 && index < 4 * s->h.nfaces, "phasor index %d outside [0, %d)", index, 4 * s->h.nfaces);
+    HIP_TRY(hipSetDevice(s->device));
+    const int f = index / 4;
+    const long long n = s->h.face_off[f + 1] - s->h.face_off[f];
+    int rc = ensure_scratch(s, n);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_phasor_out, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->self.stream,
+                       s->acc + s->h.acc_off[index], n, s->scratch_c64);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out_c64, s->scratch_c64, n * sizeof(float2), hipMemcpyDeviceToHost, s->self.stream));
+    HIP_TRY(hipStreamSynchronize(s->self.stream));
+    return FDTD_OK;
+}
+
+extern "C" int fdtd_sim_get_field(FdtdSim *s, int comp, float *out) {
+    CHECK(s && out, "NULL argument");
+    CHECK(comp >= 0 && comp < 6, "field component %d outside [0, 6)", comp);
+    HIP_TRY(hipSetDevice(s->device));
+    return copy_volume_out(s, out, s->fields + comp * s->ncell);
+}
+
+extern "C" int fdtd_sim_set_field(FdtdSim *s, int comp, const float *in) {
+    CHECK(s && in, "NULL argument");
+    CHECK(comp >= 0 && comp < 6, "field component %d outside [0, 6)", comp);
+    HIP_TRY(hipSetDevice(s->device));
+    int rc = copy_volume_in(s, s->fields + comp * s->ncell, in);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s->self.stream));
+    return FDTD_OK;
+}
+
+extern "C" int fdtd_sim_get_psi(FdtdSim *s, int index, float *out) {
+    CHECK(s && out, "NULL argument");
+    CHECK(index >= 0 && index < 24, "psi index %d outside [0, 24)", index);
+    HIP_TRY(hipSetDevice(s->device));
+    const int face = (index % 12) / 2;
+    const FdtdGeom &g = s->g;
+    const float *src = s->h.psi[index];
+    if (face == 2 || face == 3) {   // l/r: compact (nz, ny, npml)
+        HIP_TRY(hipMemcpyAsync(out, src, (size_t)s->psi_len[index] * sizeof(float), hipMemcpyDeviceToHost,
+                               s->self.stream));
+    } else {
+        const size_t rows = (face < 2) ? (size_t)g.npml * g.ny : (size_t)g.nz * g.npml;
+        HIP_TRY(hipMemcpy2DAsync(out, (size_t)g.nx * sizeof(float), src, s->pitch * sizeof(float),
+                                 (size_t)g.nx * sizeof(float), rows, hipMemcpyDeviceToHost, s->self.stream));
+    }
+    HIP_TRY(hipStreamSynchronize(s->self.stream));
+    return FDTD_OK;
+}
+
+extern "C" int fdtd_sim_set_launch(FdtdSim *s, int ty, int zchunk) {
+    CHECK(s, "NULL sim");
+    CHECK(zchunk >= 0, "negative zchunk");
+    s->self.launch.ty = ty;
+    s->self.launch.zchunk = zchunk;
+    return group_build(s->self);
+}
+
+static int time_group(Group &G, int nsteps, bool graph, float *ms_total, float *ms_h, float *ms_e,
+                      float *ms_dft) {
+    CHECK(nsteps > 0, "nsteps must be positive");
+    HIP_TRY(hipSetDevice(G.device));
+    hipEvent_t ev[2];
+    HIP_TRY(hipEventCreate(&ev[0]));
+    HIP_TRY(hipEventCreate(&ev[1]));
+    HIP_TRY(hipEventRecord(ev[0], G.stream));
+    int rc = group_run(G, nsteps, graph);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(ev[1], G.stream));
+    HIP_TRY(hipEventSynchronize(ev[1]));
+    float tot = 0;
+    HIP_TRY(hipEventElapsedTime(&tot, ev[0], ev[1]));
+    if (ms_total) *ms_total = tot;
+    if (ms_h || ms_e || ms_dft) {
+        // per-kernel: bracket each launch of the same step sequence
+        std::vector<hipEvent_t> evs(6 * (size_t)nsteps);
+        for (auto &e : evs) HIP_TRY(hipEventCreate(&e));
+        const KernelId order_main[3] = {KH, KE, KD}, order_inc[3] = {KE, KH, KD};
+        const KernelId *order = G.mode == FDTD_MODE_MAIN ? order_main : order_inc;
+        for (int s = 0; s < nsteps; ++s)
+            for (int k = 0; k < 3; ++k) {
+                HIP_TRY(hipEventRecord(evs[6 * s + 2 * k], G.stream));
+                launch_kernel(G, order[k]);
+                HIP_TRY(hipEventRecord(evs[6 * s + 2 * k + 1], G.stream));
+            }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventSynchronize(evs.back()));
+        double acc[3] = {0, 0, 0};
+        for (int s = 0; s < nsteps; ++s)
+            for (int k = 0; k < 3; ++k) {
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, evs[6 * s + 2 * k], evs[6 * s + 2 * k + 1]));
+                acc[order[k]] += ms;
+            }
+        for (auto &e : evs) (void)hipEventDestroy(e);
+        if (ms_h) *ms_h = (float)(acc[KH] / nsteps);
+        if (ms_e) *ms_e = (float)(acc[KE] / nsteps);
+        if (ms_dft) *ms_dft = (float)(acc[KD] / nsteps);
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    return FDTD_OK;
+}
+
+extern "C" int fdtd_sim_time_steps(FdtdSim *s, int nsteps, float *ms_total, float *ms_h, float *ms_e,
+                                   float *ms_dft) {
+    CHECK(s, "NULL sim");
+    int rc = check_ready(s);
+    if (rc) return rc;
+    return time_group(s->self, nsteps, false, ms_total, ms_h, ms_e, ms_dft);
+}
+
+// ---------------------------------------------------------------------------
+// near-to-far
+// ---------------------------------------------------------------------------
+static int near2far_run(int device, hipStream_t st, const float2 *ex, const float2 *ey, const float2 *hx,
+                        const float2 *hy, int nxp, int nyp, int nzp, double dx, const double *thetas_deg,
+                        int ntheta, double phi_deg, double r, double freq, double *out) {
+    CHECK(nxp >= 3 && nyp >= 3, "phasor plane %d x %d too small", nyp, nxp);
+    CHECK(nxp % 2 == 1 && nyp % 2 == 1, "composite Simpson needs odd sample counts (got %d x %d)", nyp, nxp);
+    CHECK(ntheta > 0 && thetas_deg && out, "bad theta arguments");
+    const double pi = 3.141592653589793;
+    const double eta0 = 376.730313668, c0 = 299792458.0;
+    const double lam = c0 / freq, k = 2.0 * pi / lam;
+    const double phi = phi_deg * (pi / 180.0);
+    // x_loc = arange(-(n-1)//2, (n-1)//2 + 1) * dx  (near2far.py:71-73)
+    const int x0 = -((nxp - 1) / 2), y0 = -((nyp - 1) / 2);   // n odd: (n-1) even
+    const double z_top = (double)((nzp - 1) / 2) * dx;
+    auto simpson_w = [](int n, int i) { return (i == 0 || i == n - 1) ? 1.0 : (i % 2 ? 4.0 : 2.0); };
+    std::vector<double2> phx((size_t)ntheta * nxp), phy((size_t)ntheta * nyp), zph(ntheta);
+    std::vector<double4> trig(ntheta);
+    const double cp = std::cos(phi), sp = std::sin(phi);
+    for (int t = 0; t < ntheta; ++t) {
+        const double th = thetas_deg[t] * (pi / 180.0);
+        const double st = std::sin(th), ct = std::cos(th);
+        const double a = st * cp, b = st * sp;
+        for (int i = 0; i < nxp; ++i) {
+            const double xv = (double)(x0 + i) * dx, w = simpson_w(nxp, i) * dx / 3.0;
+            const double p = k * (xv * a);
+            phx[(size_t)t * nxp + i] = make_double2(w * std::cos(p), w * std::sin(p));
+        }
+        for (int j = 0; j < nyp; ++j) {
+            const double yv = (double)(y0 + j) * dx, w = simpson_w(nyp, j) * dx / 3.0;
+            const double p = k * (yv * b);
+            phy[(size_t)t * nyp + j] = make_double2(w * std::cos(p), w * std::sin(p));
+        }
+        const double pz = k * (z_top * ct);
+        zph[t] = make_double2(std::cos(pz), std::sin(pz));
+        trig[t] = make_double4(ct, st, cp, sp);
+    }
+    // ff_weight = 1j * k * exp(-1j * k * r) / (4 pi r)   (near2far.py:76-77)
+    const double kr = k * r;
+    const double2 wgt = make_double2(k * std::sin(kr) / (4.0 * pi * r), k * std::cos(kr) / (4.0 * pi * r));
+    const int nsplit = std::max(1, std::min(64, nyp / 8));
+    double2 *d_phx, *d_phy, *d_zph, *d_out;
+    double4 *d_trig;
+    double *d_part;
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipMalloc(&d_phx, phx.size() * sizeof(double2)));
+    HIP_TRY(hipMalloc(&d_phy, phy.size() * sizeof(double2)));
+    HIP_TRY(hipMalloc(&d_zph, zph.size() * sizeof(double2)));
+    HIP_TRY(hipMalloc(&d_trig, trig.size() * sizeof(double4)));
+    HIP_TRY(hipMalloc(&d_part, (size_t)ntheta * nsplit * 8 * sizeof(double)));
+    HIP_TRY(hipMalloc(&d_out, (size_t)4 * ntheta * sizeof(double2)));
+    HIP_TRY(hipMemcpyAsync(d_phx, phx.data(), phx.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_phy, phy.data(), phy.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_zph, zph.data(), zph.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_trig, trig.data(), trig.size() * sizeof(double4), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_n2f_partial, dim3(ntheta, nsplit), dim3(256), 0, st, ex, ey, hx, hy, nxp, nyp, nsplit,
+                       d_phx, d_phy, d_part);
+    hipLaunchKernelGGL(k_n2f_final, dim3((ntheta + 63) / 64), dim3(64), 0, st, d_part, ntheta, nsplit, d_trig,
+                       d_zph, wgt, eta0, d_out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, d_out, (size_t)4 * ntheta * sizeof(double2), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    (void)hipFree(d_phx);
+    (void)hipFree(d_phy);
+    (void)hipFree(d_zph);
+    (void)hipFree(d_trig);
+    (void)hipFree(d_part);
+    (void)hipFree(d_out);
+    return FDTD_OK;
+}
+
+extern "C" int fdtd_sim_near2far(FdtdSim *s, const double *thetas_deg, int ntheta, double phi_deg, double r,
+                                 double freq, double *out) {
+    CHECK(s, "NULL sim");
+    CHECK(s->g.mode == FDTD_MODE_MAIN, "near-to-far needs a MAIN simulation");
+    HIP_TRY(hipSetDevice(s->device));
+    const long long n = s->h.face_off[2] - s->h.face_off[1];   // top (u) face
+    int rc = ensure_scratch(s, 4 * n);
+    if (rc) return rc;
+    for (int c = 0; c < 4; ++c)   // ex_ph_u ey_ph_u hx_ph_u hy_ph_u = phasors 4..7
+        hipLaunchKernelGGL(k_phasor_out, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->self.stream,
+                           s->acc + s->h.acc_off[4 + c], n, s->scratch_c64 + c * n);
+    HIP_TRY(hipGetLastError());
+    const int np = s->g.npml;
+    return near2far_run(s->device, s->self.stream, s->scratch_c64, s->scratch_c64 + n, s->scratch_c64 + 2 * n,
+                        s->scratch_c64 + 3 * n, s->g.nx - 2 * np, s->g.ny - 2 * np, s->g.nz - 2 * np,
+                        (double)s->c.dx, thetas_deg, ntheta, phi_deg, r, freq, out);
+}
+
+extern "C" int fdtd_near2far_host(const float *ex_u, const float *ey_u, const float *hx_u, const float *hy_u,
+                                  int nx_ph, int ny_ph, int nz_ph, double dx, const double *thetas_deg,
+                                  int ntheta, double phi_deg, double r, double freq, double *out) {
+    CHECK(ex_u && ey_u && hx_u && hy_u, "NULL phasor");
+    CHECK(nx_ph > 0 && ny_ph > 0, "bad plane size");
+    HIP_TRY(hipSetDevice(g_device));
+    const size_t n = (size_t)nx_ph * ny_ph;
+    float2 *d;
+    HIP_TRY(hipMalloc(&d, 4 * n * sizeof(float2)));
+    hipStream_t st;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    const float *src[4] = {ex_u, ey_u, hx_u, hy_u};
+    for (int c = 0; c < 4; ++c)
+        HIP_TRY(hipMemcpyAsync(d + c * n, src[c], n * sizeof(float2), hipMemcpyHostToDevice, st));
+    int rc = near2far_run(g_device, st, d, d + n, d + 2 * n, d + 3 * n, nx_ph, ny_ph, nz_ph, dx, thetas_deg,
+                          ntheta, phi_deg, r, freq, out);
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    (void)hipFree(d);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// beams
+// ---------------------------------------------------------------------------
+extern "C" int fdtd_beam_create(int nt, int ny, int nx, const float *ex, const float *ey, const float *hx,
+                                const float *hy, FdtdBeam **out) {
+    CHECK(out, "out is NULL");
+    CHECK(nt >= 0 && ny > 0 && nx > 0, "bad beam shape (%d, %d, %d)", nt, ny, nx);
+    const bool have = ex || ey || hx || hy;
+    CHECK(!have || (ex && ey && hx && hy), "give all four beam components or none");
+    HIP_TRY(hipSetDevice(g_device));
+    FdtdBeam *b = new FdtdBeam();
+    b->device = g_device;
+    b->nt = nt; b->ny = ny; b->nx = nx;
+    const size_t n = (size_t)nt * ny * nx;
+    hipError_t e = hipMalloc(&b->d, std::max<size_t>(1, 4 * n) * sizeof(float));
+    if (e != hipSuccess) {
+        delete b;
+        return fail(FDTD_ENOMEM, "beam allocation of %zu bytes failed: %s", 4 * n * sizeof(float),
+                    hipGetErrorString(e));
+    }
+    const float *src[4] = {ex, ey, hx, hy};
+    for (int c = 0; c < 4; ++c) {
+        e = have ? hipMemcpy(b->d + c * n, src[c], n * sizeof(float), hipMemcpyHostToDevice)
+                 : hipMemset(b->d + c * n, 0, n * sizeof(float));
+        if (e != hipSuccess) {
+            (void)hipFree(b->d);
+            delete b;
+            return fail(FDTD_EHIP, "beam upload failed: %s", hipGetErrorString(e));
+        }
+    }
+    *out = b;
+    return FDTD_OK;
+}
+
+extern "C" int fdtd_beam_upload(FdtdBeam *b, int t0, int n, const float *ex, const float *ey, const float *hx,
+                                const float *hy) {
+    CHECK(b && ex && ey && hx && hy, "NULL argument");
+    CHECK(t0 >= 0 && n >= 0 && t0 + n <= b->nt, "steps [%d, %d) outside the beam's %d", t0, t0 + n, b->nt);
+    HIP_TRY(hipSetDevice(b->device));
+    const size_t plane = (size_t)b->ny * b->nx, total = (size_t)b->nt * plane;
+    const float *src[4] = {ex, ey, hx, hy};
+    for (int c = 0; c < 4; ++c)
+        HIP_TRY(hipMemcpy(b->d + c * total + (size_t)t0 * plane, src[c], (size_t)n * plane * sizeof(float),
+                          hipMemcpyHostToDevice));
+    return FDTD_OK;
+}
+
+extern "C" int fdtd_beam_get(FdtdBeam *b, int comp, float *out) {
+    CHECK(b && out, "NULL argument");
+    CHECK(comp >= 0 && comp < 4, "beam component %d outside [0, 4)", comp);
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipDeviceSynchronize());
+    const size_t n = (size_t)b->nt * b->ny * b->nx;
+    HIP_TRY(hipMemcpy(out, b->d + comp * n, n * sizeof(float), hipMemcpyDeviceToHost));
+    return FDTD_OK;
+}
+
+extern "C" void fdtd_beam_release(FdtdBeam *b) { beam_unref(b); }
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+extern "C" int fdtd_batch_create(FdtdSim *const *sims, int n, FdtdBatch **out) {
+    CHECK(sims && n > 0 && out, "bad arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        CHECK(sims[i], "sim %d is NULL", i);
+        int rc = check_ready(sims[i]);
+        if (rc) return rc;
+    }
+    FdtdBatch *b = new FdtdBatch();
+    b->grp.device = sims[0]->device;
+    b->grp.members.assign(sims, sims + n);
+    (void)hipSetDevice(b->grp.device);
+    hipError_t e = hipStreamCreateWithFlags(&b->grp.stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete b;
+        return fail(FDTD_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    // members' own streams may still hold work (uploads, resets)
+    for (int i = 0; i < n; ++i) (void)hipStreamSynchronize(sims[i]->self.stream);
+    int rc = group_build(b->grp);
+    if (rc) {
+        std::string keep = g_err;
+        group_free(b->grp);
+        delete b;
+        g_err = keep;
+        return rc;
+    }
+    *out = b;
+    return FDTD_OK;
+}
+
+extern "C" void fdtd_batch_destroy(FdtdBatch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->grp.device);
+    (void)hipStreamSynchronize(b->grp.stream);
+    group_free(b->grp);
+    delete b;
+}
+
+extern "C" int fdtd_batch_set_launch(FdtdBatch *b, int ty, int zchunk) {
+    CHECK(b, "NULL batch");
+    b->grp.launch.ty = ty;
+    b->grp.launch.zchunk = zchunk;
+    return group_build(b->grp);
+}
+
+extern "C" int fdtd_batch_run(FdtdBatch *b, int nsteps, int use_graph) {
+    CHECK(b, "NULL batch");
+    return group_run(b->grp, nsteps, use_graph != 0);
+}
+
+extern "C" int fdtd_batch_sync(FdtdBatch *b) {
+    CHECK(b, "NULL batch");
+    HIP_TRY(hipSetDevice(b->grp.device));
+    HIP_TRY(hipStreamSynchronize(b->grp.stream));
+    return FDTD_OK;
+}
+
+extern "C" int fdtd_batch_time_steps(FdtdBatch *b, int nsteps, int use_graph, float *ms_total) {
+    CHECK(b, "NULL batch");
+    return time_group(b->grp, nsteps, use_graph != 0, ms_total, nullptr, nullptr, nullptr);
+}

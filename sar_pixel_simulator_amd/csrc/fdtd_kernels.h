@@ -1,0 +1,78 @@
+// Device-side data layout and kernels of the MI355X FDTD engine.
+//
+// Layout in HBM (per simulation):
+//   fields ex ey ez hx hy hz   float32 [nz][ny][pitch], pitch = nx rounded up
+//                               to 16 floats (64-byte rows), x fastest
+//   Ca, Cb                     float32 [nz][ny][pitch] (absent for uniform media)
+//   CPML psi, 24 arrays        d/u [npml][ny][pitch], l/r [nz][ny][npml],
+//                               f/b [nz][npml][pitch]
+//   phasor accumulators        float4 (hi.re, hi.im, lo.re, lo.im) per sample,
+//                               24 planes in the reference's save order
+//   twiddles                   per step: double4 (cos,sin E; cos,sin H) and the
+//                               scaled float hi/lo split of the same values
+//   beam                       float32 [4][nt_beam][ny][nx] (ex ey hx hy)
+//
+// Every kernel reads a per-simulation descriptor (Desc) from device memory,
+// so one launch can step many simulations of different sizes (a batch), and
+// reads its step counter from device memory, so a captured HIP graph of the
+// step sequence replays without parameter updates.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fdtd_hip.h"
+
+namespace fdtd {
+
+constexpr int kWave = 64;          // CDNA wavefront width; tiles are 64 cells wide in x
+constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dealt round-robin
+constexpr int kMaxFaces = 6;
+
+enum FaceKind : int { kFaceZ = 0, kFaceX = 1, kFaceY = 2 };
+
+// psi slots (reference names, cpu/fdtd_sim.py:165-187)
+enum Psi : int {
+    pExz_d, pEyz_d, pExz_u, pEyz_u, pEyx_l, pEzx_l, pEyx_r, pEzx_r,
+    pExy_f, pEzy_f, pExy_b, pEzy_b,
+    pHxz_d, pHyz_d, pHxz_u, pHyz_u, pHyx_l, pHzx_l, pHyx_r, pHzx_r,
+    pHxy_f, pHzy_f, pHxy_b, pHzy_b,
+};
+// profile rows of Desc::prof
+enum Prof : int { BE_DLF, CE_DLF, BH_DLF, CH_DLF, BE_URB, CE_URB, BH_URB, CH_URB };
+
+struct alignas(16) Desc {
+    int nx, ny, nz, npml;
+    int tfsf_z, read_out, nt, nt_inc;
+    int mode, dft_accum, uniform, n_src;
+    int nt_beam, nfaces, pad0, pad1;
+    long long sy, sz;              // field strides (elements)
+    long long du_sz, lr_sz, fb_sz; // psi plane strides
+    float Db, dtm0, dx, dte0dx;
+    float ca_u, cb_u, pad2, pad3;
+    float prof[8][FDTD_MAX_PML];
+    float *f[6];                   // ex ey ez hx hy hz
+    const float *ca, *cb;
+    float *psi[24];
+    const float *beam;             // MAIN: injected beam
+    float *rec;                    // INCIDENT: recorded beam
+    long long beam_cs;             // beam component stride = nt_beam * ny * nx
+    const float *src;              // INCIDENT: hard source [2][n_src][ny][nx]
+    long long src_cs;
+    float4 *acc;                   // phasor accumulators
+    long long acc_off[25];         // start of phasor k (float4 units); [24] = total
+    int face_kind[kMaxFaces], face_pos[kMaxFaces];
+    int face_na[kMaxFaces], face_nb[kMaxFaces];
+    long long face_off[kMaxFaces + 1];
+    long long dft_items;           // samples of all faces (+ record items, INCIDENT)
+    const double4 *tw64;           // [nt]
+    const float4 *tw32;            // [2 * nt]: E then H, scaled by 1/2 and 1/4, hi/lo split
+    int *step;                     // device step counter
+};
+
+// Launch shape shared by the stencil kernels.
+struct Launch {
+    int ty = 4;       // rows per block (block = 64 x ty)
+    int zchunk = 0;   // z cells per tile (0 = whole column)
+};
+
+}  // namespace fdtd

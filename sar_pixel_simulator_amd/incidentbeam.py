@@ -1,0 +1,268 @@
+"""Incident beam: temporal pulse, tapered-wave excitation and the vacuum
+FDTD run that records the beam injected at the TFSF plane.
+
+Follows fdtdsarsim/incidentbeam.py:25-433 (pulse, Braunisch tapered-wave
+spectrum, FFT synthesis, 10.0 peak normalisation, metadata validation) and
+cpu/incident_sim.py:22-224 (the recording run).  The recording run executes
+on the HIP library in its INCIDENT mode, and the recorded beam stays resident
+in HBM for the main simulations that share its key.
+"""
+import json
+import os.path
+import time
+
+import numpy as np
+
+from . import _native as native
+from .constants import C0
+from .hip import common
+from .parameters import ParameterCollection, inc_filename
+from .utils import log
+
+logger = log.getLog(__name__)
+
+#: Attributes checked on load (incidentbeam.py:105-165).
+META_KEYS = ('datatype', 'frequency', 'polarization', 'theta_angle',
+             'phi_angle', 'duration', 'illumination_width', 'cell_size',
+             'timestep', 'size_t_inc', 'size_t_pulse', 'size_x', 'size_y',
+             'size_pml', 'courant')
+
+# beams recorded in this process, keyed by cache file name
+_RESIDENT = {}
+
+
+def taper_k(k, k_x, k_y, theta, phi, radius, pol):
+    """Tapered-wave spectrum (Braunisch), incidentbeam.py:316-433.
+
+    Returns the (x, y) spectral field components for wavenumber ``k`` on the
+    (ny, nx) spatial-frequency grid.  Same dtype flow as the reference: k_z
+    is formed in complex64, the rest in float64/complex128.
+    """
+    k_p = np.hypot(k_x, k_y)
+    k_z = np.conj(np.sqrt((k ** 2 - k_p ** 2).astype(np.complex64)))
+    theta = np.radians(theta)
+    phi = np.radians(phi)
+    k_ix = k * np.sin(theta) * np.cos(phi)
+    k_iy = k * np.sin(theta) * np.sin(phi)
+    k_ip = np.hypot(k_ix, k_iy)
+    k_iz = np.sqrt(k ** 2 - k_ip ** 2)
+    gauss = ((radius ** 2) / (4 * np.pi)) * np.exp(
+        -(radius ** 2) * (((k_x - k_ix) ** 2) + ((k_y - k_iy) ** 2)) / 4)
+
+    pos = k_p > 0.0
+    kp = np.where(pos, k_p, 1.0)
+    hor_x = np.where(pos, k_y / kp, np.sin(phi)).astype(np.complex128)
+    hor_y = np.where(pos, -k_x / kp, -np.cos(phi)).astype(np.complex128)
+    ver_x = np.where(pos, (k_z * k_x) / (k * kp), np.cos(phi)).astype(np.complex128)
+    ver_y = np.where(pos, (k_z * k_y) / (k * kp), np.sin(phi)).astype(np.complex128)
+    ver_z = np.where(pos, k_p / k, 0.0).astype(np.complex128)
+
+    if k_ip > 0:
+        hor_ix, hor_iy = k_iy / k_ip, -k_ix / k_ip
+        ver_ix, ver_iy = (k_iz * k_ix) / (k * k_ip), (k_iz * k_iy) / (k * k_ip)
+        ver_iz = k_ip / k
+    else:
+        hor_ix, hor_iy = np.sin(phi), -np.cos(phi)
+        ver_ix, ver_iy, ver_iz = np.cos(phi), np.sin(phi), 0.0
+
+    if pol == 'HH':
+        Eh, Ev = 1, 0
+    elif pol == 'VV':
+        Eh, Ev = 0, 1
+    elif pol == 'RHCP':
+        Eh, Ev = -1j, 1
+    elif pol == 'LHCP':
+        Eh, Ev = 1j, 1
+    else:
+        raise ValueError('Invalid polarization value')
+
+    eh = Eh * ((hor_ix * hor_x) + (hor_iy * hor_y)) \
+        + Ev * ((ver_ix * hor_x) + (ver_iy * hor_y))
+    ev = Eh * ((hor_ix * ver_x) + (hor_iy * ver_y)) \
+        + Ev * ((ver_ix * ver_x) + (ver_iy * ver_y) + (ver_iz * ver_z))
+    return gauss * ((eh * hor_x) + (ev * ver_x)), \
+        gauss * ((eh * hor_y) + (ev * ver_y))
+
+
+class IncidentBeam:
+    """Incident beam of one parameter set (incidentbeam.py:25-69)."""
+
+    def __init__(self, params: ParameterCollection):
+        self.params = params
+        self.inc_beam_params = params.inc_beam_params
+        self.geom_params = params.geom_params
+        self.processing_params = params.processing_params
+        self.display_params = params.display_params
+        self.beam = None            # native.Beam resident on the device
+        self.phasors = None         # dict of the four incident phasors
+
+    # -- public API of the reference ------------------------------------------
+    def setup(self):
+        """Load the cached beam or generate it (incidentbeam.py:41-69)."""
+        t0 = time.monotonic()
+        key = inc_filename(self.params)
+        if key in _RESIDENT and not self.inc_beam_params.generate_new:
+            self.beam, self.phasors = _RESIDENT[key]
+        elif not self.inc_beam_params.generate_new and self.inc_beam_exists():
+            logger.info('Loading existing incident beam files')
+            self.load_existing()
+        else:
+            logger.info('Generating new incident beam')
+            self.create_pulse()
+            pulse_ex, pulse_ey = self.create_inc_excitation()
+            self.beam, self.phasors = run_incident(self.params, pulse_ex, pulse_ey)
+            _RESIDENT[key] = (self.beam, self.phasors)
+            self.save()
+        logger.info(f'Incident beam setup complete ({time.monotonic() - t0:.3f}s)')
+
+    def load_existing(self):
+        """Read and validate the cached beam, then upload it to the device."""
+        with np.load(inc_filename(self.params), allow_pickle=False) as f:
+            fields = [f[c] for c in ('ex', 'ey', 'hx', 'hy')]
+            self.check_array_sizes(*fields)
+            self.check_metadata(json.loads(str(f['attrs'])))
+            self.phasors = {n: f['phasors/' + n] for n in native.INC_PHASOR_NAMES}
+        g = self.geom_params
+        self.beam = native.Beam(g.size_t_inc, g.size_y, g.size_x, *fields)
+        _RESIDENT[inc_filename(self.params)] = (self.beam, self.phasors)
+
+    def check_array_sizes(self, inc_ex, inc_ey, inc_hx, inc_hy):
+        """ValueError on a wrong shape or dtype (incidentbeam.py:82-103)."""
+        g = self.geom_params
+        want = (g.size_t_inc, g.size_y, g.size_x)
+        for a in (inc_ex, inc_ey, inc_hx, inc_hy):
+            if a.shape != want:
+                raise ValueError(f'A component of the incident beam has the wrong '
+                                 f'shape. Got {a.shape}, expected {want}.')
+            if a.dtype != self.processing_params.datatype:
+                raise ValueError('Incident beam has incorrect datatype. Please delete '
+                                 'the incident beam file and re-run the simulation.')
+
+    def check_metadata(self, meta):
+        """ValueError when a stored attribute differs from the current
+        parameters or the run did not complete (incidentbeam.py:105-165)."""
+        want = beam_metadata(self.params)
+        for key in META_KEYS:
+            if meta.get(key) != want[key]:
+                raise ValueError(f'Incident beam metadata inconsistency ({key})')
+        if meta.get('incident_complete') is not True:
+            raise ValueError('Incident beam simulation was not completed')
+
+    def inc_beam_exists(self):
+        return os.path.isfile(inc_filename(self.params))
+
+    def create_pulse(self):
+        """Gaussian-tapered complex sinusoid and its spectrum
+        (incidentbeam.py:174-204)."""
+        g, b = self.geom_params, self.inc_beam_params
+        dt, nt_inc, nt_pulse = g.timestep, g.size_t_inc, g.size_t_pulse
+        self.time_vals = np.linspace(0.0, dt * (nt_inc - 1), num=nt_inc, endpoint=True)
+        sig = b.duration / b.frequency / (2.0 * np.sqrt(2.0 * np.log(10.0)))
+        om = 2.0 * np.pi * b.frequency
+        self.pulse = 1.0 * np.exp(-(self.time_vals - (dt * nt_pulse / 2)) ** 2
+                                  / (sig ** 2)) * np.exp(1j * om * self.time_vals)
+        self.pulse_fft = np.fft.fft(self.pulse)
+        self.pulse_freqs = np.fft.fftfreq(self.pulse.size, dt)
+
+    def create_inc_excitation(self):
+        """Ex, Ey excitation of the incident simulation, (nt_inc, ny, nx)
+        complex64 (incidentbeam.py:206-313)."""
+        g, b = self.geom_params, self.inc_beam_params
+        dtype_c = {'float32': np.complex64, 'float64': np.complex128}[
+            self.processing_params.datatype]
+        width = b.illumination_width * C0 / b.frequency
+        gauss_radius = width / (2.0 * np.sqrt(2.0 * np.log(10.0)))
+        nt_inc, nx, ny, dx = g.size_t_inc, g.size_x, g.size_y, g.cell_size
+        kx = 2 * np.pi * np.repeat(np.atleast_2d(np.fft.fftfreq(nx, dx)), ny, axis=0)
+        kx = np.fft.fftshift(kx.reshape((ny, nx)), axes=1)
+        ky = 2 * np.pi * np.repeat(np.atleast_2d(np.fft.fftfreq(ny, dx)).transpose(),
+                                   nx, axis=1)
+        ky = np.fft.fftshift(ky.reshape((ny, nx)), axes=0)
+        pulse_ex = np.zeros((nt_inc, ny, nx), dtype_c)
+        pulse_ey = np.zeros((nt_inc, ny, nx), dtype_c)
+        all_k = 2 * np.pi * self.pulse_freqs / C0
+        sh, ish, fft2 = np.fft.fftshift, np.fft.ifftshift, np.fft.fft2
+        for k_idx in range(nt_inc):
+            k_val = all_k[k_idx]
+            if k_val == 0.0:
+                continue
+            sx, sy = taper_k(k_val, kx, ky, b.theta_angle, b.phi_angle,
+                             gauss_radius, b.polarization)
+            w = 1.0 * self.pulse_fft[k_idx]
+            pulse_ex[k_idx] = w * sh(fft2(ish(sx)))
+            pulse_ey[k_idx] = w * sh(fft2(ish(sy)))
+        pulse_ex = np.fft.ifft(pulse_ex, axis=0).astype(dtype_c, copy=False)
+        pulse_ey = np.fft.ifft(pulse_ey, axis=0).astype(dtype_c, copy=False)
+        maxval = float(max(np.abs(pulse_ex).max(), np.abs(pulse_ey).max()))
+        scale = 10.0 / maxval
+        pulse_ex = (pulse_ex.astype(np.complex128) * scale).astype(dtype_c)
+        pulse_ey = (pulse_ey.astype(np.complex128) * scale).astype(dtype_c)
+        return pulse_ex, pulse_ey
+
+    # -- storage ---------------------------------------------------------------
+    def save(self):
+        """Write the beam, its phasors and metadata (the incident HDF5 layout,
+        cpu/incident_sim.py:95-98, :167-171, :195-224) as .npz."""
+        g = self.geom_params
+        arrays = {c: self.beam.component(i) for i, c in enumerate(('ex', 'ey', 'hx', 'hy'))}
+        for n in native.INC_PHASOR_NAMES:
+            arrays['phasors/' + n] = self.phasors[n]
+        meta = beam_metadata(self.params)
+        meta['incident_complete'] = True
+        arrays['attrs'] = np.array(json.dumps(meta))
+        fname = inc_filename(self.params)
+        tmp = fname + '.tmp.npz'
+        np.savez(tmp, **arrays)
+        os.replace(tmp, fname)
+        logger.info(f'Incident beam written to {fname} ({g.size_t_inc} steps)')
+
+
+def beam_metadata(params: ParameterCollection):
+    p, b, g = params.processing_params, params.inc_beam_params, params.geom_params
+    return {'datatype': p.datatype, 'frequency': b.frequency,
+            'polarization': b.polarization, 'theta_angle': b.theta_angle,
+            'phi_angle': b.phi_angle, 'duration': b.duration,
+            'illumination_width': b.illumination_width, 'cell_size': g.cell_size,
+            'timestep': g.timestep, 'size_t_inc': g.size_t_inc,
+            'size_t_pulse': g.size_t_pulse, 'size_x': g.size_x, 'size_y': g.size_y,
+            'size_pml': g.size_pml, 'courant': g.courant}
+
+
+def run_incident(params: ParameterCollection, pulse_ex, pulse_ey, device=None):
+    """Run the vacuum incident simulation on the device and record the beam.
+
+    Returns (native.Beam, incident phasor dict).  The excitation's imaginary
+    parts drive a hard source on [1:-1, 0:-1] of the feed-in plane for the
+    first size_t_pulse steps (cpu/incident_sim.py:109-111, :141-144).
+    """
+    common.require_float32(params)
+    g = params.geom_params
+    native.init(params_device(params) if device is None else device)
+    geom = common.incident_geom(params)
+    sim = native.Sim(geom, common.coeffs_for(params))
+    try:
+        sim.set_uniform_material(1.0, common.incident_cb(params))
+        n_src = geom.nt_inc
+        src_ex = np.ascontiguousarray(np.imag(pulse_ex[:n_src]), dtype=np.float32)
+        src_ey = np.ascontiguousarray(np.imag(pulse_ey[:n_src]), dtype=np.float32)
+        sim.set_source(src_ex, src_ey)
+        beam = native.Beam(g.size_t_inc, g.size_y, g.size_x)
+        sim.attach_beam(beam)
+        sim.run(g.size_t_inc)
+        sim.sync()
+        phasors = sim.phasors()
+    finally:
+        sim.destroy()
+    return beam, phasors
+
+
+def params_device(params: ParameterCollection):
+    d = params.processing_params.gpu_num
+    return d[0] if isinstance(d, (list, tuple)) else int(d)
+
+
+def clear_resident():
+    """Drop this process's device-resident beams."""
+    for beam, _ in _RESIDENT.values():
+        beam.release()
+    _RESIDENT.clear()

@@ -1,0 +1,9 @@
+from .materialmodel import MaterialModel
+from .materials import (compute_material_arrays, all_vacuum, flat_surface,
+                        rough_dielectric, fill_below_surface)
+from .roughsurface import gaussian_correlated_surface
+from . import synthetic
+
+__all__ = ['MaterialModel', 'compute_material_arrays', 'all_vacuum',
+           'flat_surface', 'rough_dielectric', 'fill_below_surface',
+           'gaussian_correlated_surface', 'synthetic']

@@ -1,0 +1,226 @@
+"""HIP library vs the oracle on the same seeded inputs (needs an MI355X).
+
+Tolerances:
+  fields, CPML psi, recorded beam, phasors in 'reference' accumulation:
+      bit-identical (same operation order, -ffp-contract=off on both sides)
+  phasors in 'compensated' accumulation: relative L2 <= 1e-5 against the
+      oracle's complex64 running sum (the float-float sum is the more
+      accurate one; see test_compensated_dft_is_closer_to_exact_sum)
+  far field (FP64, device vs scipy Simpson): relative <= 1e-9
+"""
+import numpy as np
+import pytest
+
+import helpers as H
+import oracle
+from sar_pixel_simulator_amd import _native as native
+from sar_pixel_simulator_amd.hip import common
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def make_sim(pc, Ca=None, Cb=None, beam_planes=None, launch=None):
+    native.init(0)
+    sim = native.Sim(common.main_geom(pc), common.coeffs_for(pc))
+    if launch:
+        sim.set_launch(*launch)
+    g = pc.geom_params
+    if beam_planes is not None:
+        b = native.Beam(g.size_t_inc, g.size_y, g.size_x, *beam_planes)
+        sim.attach_beam(b)
+    if Ca is not None:
+        sim.set_materials(Ca, Cb)
+    return sim
+
+
+def assert_same(a, b, what):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, what
+    if not np.array_equal(a, b):
+        bad = np.argwhere(a != b)
+        i = tuple(bad[0])
+        raise AssertionError(f'{what}: {len(bad)} of {a.size} differ; first {i}: '
+                             f'{a[i]!r} vs {b[i]!r}')
+
+
+@pytest.fixture(scope='module')
+def soil60():
+    pc, Ca, Cb = H.soil_case(steps=60, dft_accumulation='reference')
+    beam = H.synthetic_beam(pc)
+    ref = H.oracle_main(pc, Ca, Cb, beam)
+    return pc, Ca, Cb, beam, ref
+
+
+def test_fields_psi_phasors_bitwise(soil60):
+    pc, Ca, Cb, beam, ref = soil60
+    say('case: soil 33x31x35, 60 steps, reference DFT')
+    sim = make_sim(pc, Ca, Cb, beam)
+    sim.run(60)
+    sim.sync()
+    assert sim.step_count() == 60
+    for n in native.FIELD_NAMES:
+        assert_same(sim.field(n), ref.fields[n], n)
+    for i, n in enumerate(native.PSI_NAMES):
+        assert_same(sim.psi(i), ref.psi[n], n)
+    ph = sim.phasors()
+    oph = ref.phasors()
+    for n in native.PHASOR_NAMES:
+        assert_same(ph[n], oph[n], n)
+    assert np.abs(oph['ex_ph_u']).max() > 0
+    sim.destroy()
+
+
+@pytest.mark.parametrize('launch', [(1, 0), (2, 0), (4, 0), (4, 7), (1, 5)])
+def test_launch_shapes_bitwise(soil60, launch):
+    pc, Ca, Cb, beam, ref = soil60
+    say('case: launch', launch)
+    sim = make_sim(pc, Ca, Cb, beam, launch=launch)
+    sim.run(60)
+    sim.sync()
+    for n in native.FIELD_NAMES:
+        assert_same(sim.field(n), ref.fields[n], f'{n} {launch}')
+    sim.destroy()
+
+
+def test_compensated_phasors_close(soil60):
+    pc, Ca, Cb, beam, ref = soil60
+    say('case: compensated DFT')
+    pc.processing_params.dft_accumulation = 'compensated'
+    try:
+        sim = make_sim(pc, Ca, Cb, beam)
+    finally:
+        pc.processing_params.dft_accumulation = 'reference'
+    sim.run(60)
+    sim.sync()
+    ph, oph = sim.phasors(), ref.phasors()
+    for n in native.PHASOR_NAMES:
+        assert H.rel_l2(ph[n], oph[n]) <= 1e-5, n
+    sim.destroy()
+
+
+def test_steps_past_end_are_noops(soil60):
+    pc, Ca, Cb, beam, ref = soil60
+    sim = make_sim(pc, Ca, Cb, beam)
+    sim.run(75)
+    sim.sync()
+    assert sim.step_count() == 60
+    assert_same(sim.field('ey'), ref.fields['ey'], 'ey after extra steps')
+    sim.destroy()
+
+
+def test_uneven_grid_bitwise():
+    pc, Ca, Cb = H.soil_case(nx=37, ny=29, nz=41, nz_mat=17, steps=40, seed=11)
+    say('case: uneven 37x29x41, 40 steps')
+    beam = H.synthetic_beam(pc)
+    ref = H.oracle_main(pc, Ca, Cb, beam)
+    sim = make_sim(pc, Ca, Cb, beam, launch=(2, 9))
+    sim.run(40)
+    sim.sync()
+    for n in native.FIELD_NAMES:
+        assert_same(sim.field(n), ref.fields[n], n)
+    sim.destroy()
+
+
+def test_incident_run_bitwise():
+    pc = H.soil_case(nx=31, ny=33, nz=35, nz_mat=14, dft_accumulation='reference')[0]
+    say('case: incident run 31x33, nt', pc.geom_params.size_t_inc)
+    pex, pey = H.excitation(pc)
+    rec, oph = H.oracle_incident(pc, pex, pey)
+    from sar_pixel_simulator_amd.incidentbeam import run_incident
+    beam, ph = run_incident(pc, pex, pey, device=0)
+    for c in range(4):
+        assert_same(beam.component(c), rec[c], f'beam component {c}')
+    for n in native.INC_PHASOR_NAMES:
+        assert_same(ph[n], oph[n], n)
+    assert np.abs(rec[1]).max() > 0.1
+    beam.release()
+
+
+def test_batch_equals_solo():
+    cases = [H.soil_case(nx=33, ny=31, nz=35, nz_mat=14, steps=30, seed=s) for s in (1, 2)]
+    cases.append(H.soil_case(nx=45, ny=39, nz=37, nz_mat=15, steps=30, seed=5))
+    say('case: batch of 3 uneven sims, 30 steps')
+    solo, batch_sims = [], []
+    for pc, Ca, Cb in cases:
+        beam = H.synthetic_beam(pc)
+        s = make_sim(pc, Ca, Cb, beam)
+        s.run(30)
+        s.sync()
+        solo.append({n: s.field(n) for n in native.FIELD_NAMES})
+        s.destroy()
+        batch_sims.append(make_sim(pc, Ca, Cb, beam))
+    for use_graph in (False, True):
+        for s in batch_sims:
+            s.reset()
+        b = native.Batch(batch_sims)
+        b.run(17, use_graph=use_graph)
+        b.run(13, use_graph=use_graph)
+        b.sync()
+        for k, s in enumerate(batch_sims):
+            assert s.step_count() == 30
+            for n in native.FIELD_NAMES:
+                assert_same(s.field(n), solo[k][n], f'member {k} {n} graph={use_graph}')
+        b.destroy()
+    for s in batch_sims:
+        s.destroy()
+
+
+def test_near2far_matches_scipy(soil60):
+    pc, Ca, Cb, beam, ref = soil60
+    g = pc.geom_params
+    oph = ref.phasors()
+    nxp, nyp, nzp = g.size_x - 20, g.size_y - 20, g.size_z - 20
+    th = np.linspace(-80.0, 80.0, 161)
+    want = oracle.far_field(oph, nxp, nyp, nzp, g.cell_size, 2000.0, th, [0.0],
+                            pc.inc_beam_params.frequency)
+    native.init(0)
+    got = native.near2far_host(oph['ex_ph_u'], oph['ey_ph_u'], oph['hx_ph_u'], oph['hy_ph_u'],
+                               nxp, nyp, nzp, g.cell_size, th, 0.0, 2000.0,
+                               pc.inc_beam_params.frequency)
+    for k in ('E_theta', 'E_phi', 'H_theta', 'H_phi'):
+        scale = np.abs(want[k]).max()
+        assert np.abs(got[k] - want[k]).max() <= 1e-9 * scale, k
+    # from the device accumulators directly
+    sim = make_sim(pc, Ca, Cb, beam)
+    sim.run(60)
+    sim.sync()
+    got2 = sim.near2far(th, 0.0, 2000.0, pc.inc_beam_params.frequency)
+    for k in ('E_theta', 'E_phi'):
+        scale = np.abs(want[k]).max()
+        assert np.abs(got2[k] - want[k]).max() <= 1e-9 * scale, k
+    sim.destroy()
+
+
+def test_linearity_at_large_size():
+    """Size-independent property at the single-large-run shape: doubling the
+    beam doubles every field exactly (all operations commute with x2)."""
+    from sar_pixel_simulator_amd import workloads
+    pc = workloads.large_params()
+    g = pc.geom_params
+    say('case: linearity', g.size_x, g.size_y, g.size_z)
+    steps = 4
+    Ca = np.ones((g.size_z, g.size_y, g.size_x), np.float32)
+    Cb = np.full_like(Ca, np.float32(common.update_scalars(pc)['dte0dx']))
+    rng = np.random.default_rng(0)
+    Ca[:g.size_z_material] = rng.uniform(0.9, 1.0, (g.size_z_material, g.size_y, g.size_x))
+    planes = workloads.synthetic_beam_planes(pc, 0, steps)
+    native.init(0)
+    out = []
+    for scale in (1.0, 2.0):
+        sim = native.Sim(common.main_geom(pc), common.coeffs_for(pc))
+        b = native.Beam(g.size_t_inc, g.size_y, g.size_x)
+        b.upload(0, *(scale * planes))
+        sim.attach_beam(b)
+        sim.set_materials(Ca, Cb)
+        sim.run(steps)
+        sim.sync()
+        out.append(sim.field('ey'))
+        sim.destroy()
+        b.release()
+    assert np.isfinite(out[0]).all()
+    assert np.abs(out[0]).max() > 0
+    assert_same(out[1], 2.0 * out[0], 'ey(2 beam) vs 2 ey(beam)')
