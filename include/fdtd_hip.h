@@ -70,6 +70,7 @@ typedef struct FdtdGeom {
 typedef struct FdtdCoeffs {
     float Db, dtm0, dx, dte0dx;
     double omega, dt;
+    double cell_size;       /* float64 dx [m] for the near-to-far transform (processresults.py:197) */
     float prof[8][FDTD_MAX_PML];
 } FdtdCoeffs;
 

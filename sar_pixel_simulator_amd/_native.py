@@ -76,6 +76,7 @@ class FdtdCoeffs(ctypes.Structure):
     _fields_ = [('Db', ctypes.c_float), ('dtm0', ctypes.c_float),
                 ('dx', ctypes.c_float), ('dte0dx', ctypes.c_float),
                 ('omega', ctypes.c_double), ('dt', ctypes.c_double),
+                ('cell_size', ctypes.c_double),
                 ('prof', (ctypes.c_float * FDTD_MAX_PML) * 8)]
 
 
@@ -203,13 +204,14 @@ def make_geom(nx, ny, nz, npml, tfsf_z, nt, nt_inc, mode=MODE_MAIN,
     return g
 
 
-def make_coeffs(Db, dtm0, dx, dte0dx, omega, dt, profiles):
+def make_coeffs(Db, dtm0, dx, dte0dx, omega, dt, profiles, cell_size=None):
     """``profiles``: the eight 1-D CPML profiles (be_dlf, ce_dlf, bh_dlf,
     ch_dlf, be_urb, ce_urb, bh_urb, ch_urb), each npml float32 values."""
     c = FdtdCoeffs()
     c.Db, c.dtm0, c.dx, c.dte0dx = (float(np.float32(v))
                                     for v in (Db, dtm0, dx, dte0dx))
     c.omega, c.dt = float(omega), float(dt)
+    c.cell_size = float(dx if cell_size is None else cell_size)
     for k, prof in enumerate(profiles):
         prof = np.asarray(prof, dtype=np.float32)
         if prof.size > FDTD_MAX_PML:

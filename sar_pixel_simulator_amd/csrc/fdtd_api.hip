@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -55,7 +56,16 @@ static int fail(int code, const char *fmt, ...) {
         if (!(cond)) return fail(FDTD_EINVAL, __VA_ARGS__);   \
     } while (0)
 
-static long long pitch_of(int nx) { return ((long long)nx + 15) / 16 * 16; }
+// Row pitch of the device arrays: nx rounded up to FDTD_PITCH_ALIGN floats
+// (environment, default 16 = 64-byte rows; 1 = unpadded).
+static long long pitch_of(int nx) {
+    static const long long align = [] {
+        const char *v = getenv("FDTD_PITCH_ALIGN");
+        long long a = v ? atoll(v) : 16;
+        return (a >= 1 && a <= 256) ? a : 16;
+    }();
+    return ((long long)nx + align - 1) / align * align;
+}
 
 // ---------------------------------------------------------------------------
 // handles
@@ -521,19 +531,50 @@ extern "C" void fdtd_sim_destroy(FdtdSim *s) {
     delete s;
 }
 
+// Host (nz, ny, nx) <-> device (nz, ny, pitch).  Pageable host memory goes
+// through one contiguous device staging buffer with a single bulk copy; the
+// pitch change is a device-to-device 2-D copy.  (A pitched copy straight from
+// pageable memory is staged row by row by the runtime: ~1e5 small copies for
+// a large grid.)
 static int copy_volume_in(FdtdSim *s, float *dst, const float *src) {
-    HIP_TRY(hipMemcpy2DAsync(dst, s->pitch * sizeof(float), src, (size_t)s->g.nx * sizeof(float),
-                             (size_t)s->g.nx * sizeof(float), (size_t)s->g.ny * s->g.nz,
-                             hipMemcpyHostToDevice, s->self.stream));
+    const size_t nx = (size_t)s->g.nx, rows = (size_t)s->g.ny * s->g.nz;
+    if ((long long)nx == s->pitch) {
+        HIP_TRY(hipMemcpyAsync(dst, src, nx * rows * sizeof(float), hipMemcpyHostToDevice, s->self.stream));
+        return FDTD_OK;
+    }
+    float *tmp = nullptr;
+    HIP_TRY(hipMalloc(&tmp, nx * rows * sizeof(float)));
+    hipError_t e = hipMemcpyAsync(tmp, src, nx * rows * sizeof(float), hipMemcpyHostToDevice, s->self.stream);
+    if (e == hipSuccess)
+        e = hipMemcpy2DAsync(dst, s->pitch * sizeof(float), tmp, nx * sizeof(float), nx * sizeof(float), rows,
+                             hipMemcpyDeviceToDevice, s->self.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->self.stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(FDTD_EHIP, "volume upload failed: %s", hipGetErrorString(e));
+    return FDTD_OK;
+}
+
+static int copy_rows_out(FdtdSim *s, float *dst, const float *src, size_t rows) {
+    const size_t nx = (size_t)s->g.nx;
+    if ((long long)nx == s->pitch) {
+        HIP_TRY(hipMemcpyAsync(dst, src, nx * rows * sizeof(float), hipMemcpyDeviceToHost, s->self.stream));
+        HIP_TRY(hipStreamSynchronize(s->self.stream));
+        return FDTD_OK;
+    }
+    float *tmp = nullptr;
+    HIP_TRY(hipMalloc(&tmp, nx * rows * sizeof(float)));
+    hipError_t e = hipMemcpy2DAsync(tmp, nx * sizeof(float), src, s->pitch * sizeof(float), nx * sizeof(float),
+                                    rows, hipMemcpyDeviceToDevice, s->self.stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dst, tmp, nx * rows * sizeof(float), hipMemcpyDeviceToHost, s->self.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->self.stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(FDTD_EHIP, "volume download failed: %s", hipGetErrorString(e));
     return FDTD_OK;
 }
 
 static int copy_volume_out(FdtdSim *s, float *dst, const float *src) {
-    HIP_TRY(hipMemcpy2DAsync(dst, (size_t)s->g.nx * sizeof(float), src, s->pitch * sizeof(float),
-                             (size_t)s->g.nx * sizeof(float), (size_t)s->g.ny * s->g.nz,
-                             hipMemcpyDeviceToHost, s->self.stream));
-    HIP_TRY(hipStreamSynchronize(s->self.stream));
-    return FDTD_OK;
+    return copy_rows_out(s, dst, src, (size_t)s->g.ny * s->g.nz);
 }
 
 extern "C" int fdtd_sim_set_materials(FdtdSim *s, const float *Ca, const float *Cb) {
@@ -705,13 +746,11 @@ extern "C" int fdtd_sim_get_psi(FdtdSim *s, int index, float *out) {
     if (face == 2 || face == 3) {   // l/r: compact (nz, ny, npml)
         HIP_TRY(hipMemcpyAsync(out, src, (size_t)s->psi_len[index] * sizeof(float), hipMemcpyDeviceToHost,
                                s->self.stream));
-    } else {
-        const size_t rows = (face < 2) ? (size_t)g.npml * g.ny : (size_t)g.nz * g.npml;
-        HIP_TRY(hipMemcpy2DAsync(out, (size_t)g.nx * sizeof(float), src, s->pitch * sizeof(float),
-                                 (size_t)g.nx * sizeof(float), rows, hipMemcpyDeviceToHost, s->self.stream));
+        HIP_TRY(hipStreamSynchronize(s->self.stream));
+        return FDTD_OK;
     }
-    HIP_TRY(hipStreamSynchronize(s->self.stream));
-    return FDTD_OK;
+    const size_t rows = (face < 2) ? (size_t)g.npml * g.ny : (size_t)g.nz * g.npml;
+    return copy_rows_out(s, out, src, rows);
 }
 
 extern "C" int fdtd_sim_set_launch(FdtdSim *s, int ty, int zchunk) {
@@ -863,7 +902,7 @@ extern "C" int fdtd_sim_near2far(FdtdSim *s, const double *thetas_deg, int nthet
     const int np = s->g.npml;
     return near2far_run(s->device, s->self.stream, s->scratch_c64, s->scratch_c64 + n, s->scratch_c64 + 2 * n,
                         s->scratch_c64 + 3 * n, s->g.nx - 2 * np, s->g.ny - 2 * np, s->g.nz - 2 * np,
-                        (double)s->c.dx, thetas_deg, ntheta, phi_deg, r, freq, out);
+                        s->c.cell_size, thetas_deg, ntheta, phi_deg, r, freq, out);
 }
 
 extern "C" int fdtd_near2far_host(const float *ex_u, const float *ey_u, const float *hx_u, const float *hy_u,

@@ -42,7 +42,8 @@ def coeffs_for(params: ParameterCollection, absorbers: Absorbers = None):
     ab = absorbers or Absorbers(params)
     s = update_scalars(params)
     return native.make_coeffs(s['Db'], s['dtm0'], s['dx'], s['dte0dx'],
-                              s['omega'], s['dt'], ab.profiles())
+                              s['omega'], s['dt'], ab.profiles(),
+                              cell_size=params.geom_params.cell_size)
 
 
 def main_geom(params: ParameterCollection):

@@ -1,0 +1,240 @@
+"""Single-GPU serving loop with continuous batching.
+
+Replaces the reference's one-process-per-GPU MPI farm
+(experiment_mpi.py:86-255: rank 0 sends Ca/Cb to an idle worker, the worker
+runs one model and reports SUCCESS).  Here requests queue on the host;
+at every chunk boundary the loop
+
+1. retires members that have run all their steps: far field and sigma0
+   from the device-resident phasor sums, then frees their memory;
+2. admits queued requests in arrival order while a slot is free and the
+   request's HBM footprint (fields, coefficients, CPML state, phasor sums,
+   plus its incident beam unless already resident) fits the budget;
+3. rebuilds the batch when membership changed and runs the next chunk of
+   time steps for every member in one launch per kernel per step.
+
+Requests that arrive while others run join at the next boundary, so their
+steps overlap on the device.  Each member's arithmetic does not depend on the
+other members, so a result is bit-identical alone and in the mix.
+
+The device side is behind a small backend interface (``HipBackend``) so the
+scheduling logic can be exercised without a GPU.
+"""
+import collections
+import dataclasses
+import time
+from typing import Any, Callable, Optional
+
+import numpy as np
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    params: Any                      # ParameterCollection with geometry set
+    material: Any = None             # MaterialModel, or None when Ca/Cb given
+    Ca: Optional[np.ndarray] = None
+    Cb: Optional[np.ndarray] = None
+    name: str = ''
+    arrival: float = 0.0             # seconds on the server clock
+
+
+@dataclasses.dataclass
+class Result:
+    rid: int
+    name: str
+    arrival: float
+    admitted: float
+    finished: float
+    steps: int
+    output: Any = None
+    error: Optional[str] = None
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+class AdmissionError(RuntimeError):
+    """The request can never fit the HBM budget."""
+
+
+class HipBackend:
+    """Device operations the loop needs, on libfdtd_hip."""
+
+    def __init__(self, device=0, launch=None):
+        from .. import _native as native
+        from .batch import BeamCache
+        native.init(device)
+        self.native = native
+        self.beams = BeamCache()
+        self.launch = launch
+
+    def footprint(self, req):
+        from .batch import DeviceModel
+        return DeviceModel.footprint(req.params) + self.beams.bytes_for(req.params)
+
+    def steps(self, req):
+        return req.params.geom_params.size_t
+
+    def admit(self, req):
+        from .batch import DeviceModel
+        from ..materials import compute_material_arrays
+        Ca, Cb = (req.Ca, req.Cb) if req.Ca is not None else \
+            compute_material_arrays(req.params, req.material)
+        return DeviceModel(req.params, Ca, Cb, self.beams, req.name)
+
+    def make_batch(self, members):
+        from .batch import DeviceBatch
+        return DeviceBatch(members, self.launch)
+
+    def run(self, batch, nsteps):
+        batch.run(nsteps, use_graph=True)
+
+    def sync(self, batch):
+        batch.sync()
+
+    def drop_batch(self, batch):
+        batch.destroy()
+
+    def collect(self, member):
+        return member.sigma0()
+
+    def release(self, member):
+        member.release()
+
+
+class Server:
+    """Continuous-batching request loop on one device."""
+
+    def __init__(self, backend, hbm_budget, max_slots=64, chunk_steps=32,
+                 clock: Callable[[], float] = None):
+        self.backend = backend
+        self.budget = int(hbm_budget)
+        self.max_slots = int(max_slots)
+        self.chunk = int(chunk_steps)
+        self.clock = clock or time.monotonic
+        self.queue = collections.deque()
+        self.active = []                 # (request, member, footprint, admitted, steps_done, nt)
+        self.used = 0
+        self.batch = None
+        self.results = []
+        self.chunks = 0
+        self.peak_used = 0
+        self.peak_slots = 0
+
+    # -- queue ----------------------------------------------------------------
+    def submit(self, req: Request):
+        fp = self.backend.footprint(req)
+        if fp > self.budget:
+            raise AdmissionError(f'request {req.rid} needs {fp} bytes, budget {self.budget}')
+        self.queue.append(req)
+
+    def idle(self):
+        return not self.queue and not self.active
+
+    # -- one boundary + chunk --------------------------------------------------
+    def _retire(self):
+        keep = []
+        changed = False
+        for entry in self.active:
+            req, member, fp, t_admit, done, nt = entry
+            if done >= nt:
+                out, err = None, None
+                try:
+                    out = self.backend.collect(member)
+                except Exception as e:          # a failing request does not stop the loop
+                    err = f'{type(e).__name__}: {e}'
+                self.backend.release(member)
+                self.used -= fp
+                self.results.append(Result(req.rid, req.name, req.arrival, t_admit,
+                                           self.clock(), nt, out, err))
+                changed = True
+            else:
+                keep.append(entry)
+        self.active = keep
+        return changed
+
+    def _admit(self):
+        changed = False
+        while self.queue and len(self.active) < self.max_slots:
+            req = self.queue[0]
+            fp = self.backend.footprint(req)
+            if self.used + fp > self.budget:
+                break                           # FIFO: wait for memory to free up
+            self.queue.popleft()
+            try:
+                member = self.backend.admit(req)
+            except Exception as e:
+                now = self.clock()
+                self.results.append(Result(req.rid, req.name, req.arrival, now, now, 0, None,
+                                           f'{type(e).__name__}: {e}'))
+                continue
+            self.used += fp
+            self.active.append([req, member, fp, self.clock(), 0, self.backend.steps(req)])
+            changed = True
+        self.peak_used = max(self.peak_used, self.used)
+        self.peak_slots = max(self.peak_slots, len(self.active))
+        return changed
+
+    def step(self):
+        """Retire, admit, run one chunk.  Returns the number of active members."""
+        changed = self._retire()
+        changed = self._admit() or changed
+        if changed and self.batch is not None:
+            self.backend.drop_batch(self.batch)
+            self.batch = None
+        if not self.active:
+            return 0
+        if self.batch is None:
+            self.batch = self.backend.make_batch([e[1] for e in self.active])
+        n = min(self.chunk, max(e[5] - e[4] for e in self.active))
+        self.backend.run(self.batch, n)
+        self.backend.sync(self.batch)
+        for e in self.active:
+            e[4] = min(e[5], e[4] + n)
+        self.chunks += 1
+        return len(self.active)
+
+    def drain(self):
+        while not self.idle():
+            self.step()
+        self.close()
+
+    def close(self):
+        if self.batch is not None:
+            self.backend.drop_batch(self.batch)
+            self.batch = None
+
+    def serve(self, requests, time_scale=1.0):
+        """Replay ``requests`` (sorted by arrival, seconds from start) against
+        the server clock: each is submitted once its arrival time has passed;
+        the loop runs until all are done.  Returns the results."""
+        t0 = self.clock()
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        while pending or not self.idle():
+            now = self.clock() - t0
+            while pending and pending[0].arrival * time_scale <= now:
+                r = pending.popleft()
+                r.arrival = t0 + r.arrival * time_scale
+                self.submit(r)
+            if self.idle() and pending:
+                # nothing to do until the next arrival
+                wait = pending[0].arrival * time_scale - (self.clock() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.05))
+                continue
+            self.step()
+        self.close()
+        return self.results
+
+
+def latency_stats(results):
+    lat = np.array([r.latency for r in results if r.error is None])
+    if lat.size == 0:
+        return {}
+    span = max(r.finished for r in results) - min(r.arrival for r in results)
+    return {'count': int(lat.size), 'p50_s': float(np.percentile(lat, 50)),
+            'p95_s': float(np.percentile(lat, 95)), 'p99_s': float(np.percentile(lat, 99)),
+            'mean_s': float(lat.mean()), 'span_s': float(span),
+            'models_per_hour': float(lat.size / span * 3600.0) if span > 0 else None}

@@ -55,3 +55,34 @@ def autotune(sim, nx, ny, nz, arch='gfx950', steps=10, candidates=CANDIDATES, sa
             json.dump(data, f, indent=1, sort_keys=True)
         os.replace(tmp, CACHE)
     return best, results
+
+
+def _tune_workload(name):
+    """Build a ready simulation of a named workload with uniform-value inputs
+    (timing does not depend on the values) and autotune it."""
+    import numpy as np
+    from .. import _native as native, workloads
+    from . import common
+    pc = {'large': workloads.large_params, 'plain': workloads.plain_params}[name]()
+    g = pc.geom_params
+    native.init(0)
+    sim = native.Sim(common.main_geom(pc), common.coeffs_for(pc))
+    beam = native.Beam(g.size_t_inc, g.size_y, g.size_x)
+    beam.upload(0, *workloads.synthetic_beam_planes(pc, 0, 16))
+    sim.attach_beam(beam)
+    shape = (g.size_z, g.size_y, g.size_x)
+    sim.set_materials(np.ones(shape, np.float32),
+                      np.full(shape, common.update_scalars(pc)['dte0dx'], np.float32))
+    best, res = autotune(sim, g.size_x, g.size_y, g.size_z)
+    sim.destroy()
+    beam.release()
+    return best, res
+
+
+if __name__ == '__main__':
+    import sys
+    for name in (sys.argv[1:] or ['large']):
+        best, res = _tune_workload(name)
+        for shape, ms in sorted(res.items(), key=lambda kv: kv[1]):
+            print(f'{name} ty={shape[0]} zchunk={shape[1]}: {ms:.4f} ms/step', flush=True)
+        print(f'{name}: best {best}', flush=True)

@@ -1,0 +1,410 @@
+"""Pins for the oracle (the CPU restatement the HIP library is checked
+against).  The reference has no tests or golden vectors, so each check here
+fixes the oracle with mathematics or physics a plausible mistake would
+break:
+
+* constant fields + zero update coefficients: every phasor equals the field
+  interpolated to the common Yee node times a geometric series -> pins the
+  plane positions, sample indexing, 2-/4-point averaging, the twiddle sign
+  and the H half-step offset (closed form)
+* brute-force Python loops of the reference kernels on a tiny grid -> pins
+  the C transcription of every pass (index ranges, signs, operation order)
+* zero stays zero; doubling the input doubles every value bit for bit
+* TFSF in vacuum: the scattered-field region stays dark -> pins the
+  injection signs and the incident/main coupling
+* CPML reflection vs a domain twice as large -> pins the absorber
+* flat dielectric vs PEC plate: specular far-field power ratio equals the
+  Fresnel |Gamma|^2 -> pins injection, DFT, near-to-far and sigma0 end to end
+* uniform current sheet: far field at broadside in closed form
+"""
+import numpy as np
+import pytest
+
+import helpers as H
+import oracle
+from sar_pixel_simulator_amd import workloads
+from sar_pixel_simulator_amd.hip import common
+from sar_pixel_simulator_amd.materials import compute_material_arrays, flat_surface, all_vacuum
+
+
+def _tiny_geom(nx=9, ny=8, nz=10, npml=3, **kw):
+    g = dict(nx=nx, ny=ny, nz=nz, npml=npml, tfsf_z=5, nt_inc=0, Db=np.float32(0.37),
+             dtm0=np.float32(1.3e-3), dx=np.float32(0.01), dte0dx=np.float32(2.1),
+             omega=2 * np.pi * 5.4e9, dt=4.6e-12)
+    g.update(kw)
+    return g
+
+
+def test_zero_stays_zero():
+    pc, Ca, Cb = H.soil_case(steps=30)
+    g = pc.geom_params
+    beam = np.zeros((4, g.size_t_inc, g.size_y, g.size_x), np.float32)
+    sim = H.oracle_main(pc, Ca, Cb, beam)
+    for a in sim.fields.values():
+        assert not a.any()
+    assert not any(np.abs(v).max() for v in sim.phasors().values())
+
+
+def test_doubling_is_exact():
+    pc, Ca, Cb = H.soil_case(steps=40)
+    beam = H.synthetic_beam(pc)
+    a = H.oracle_main(pc, Ca, Cb, beam)
+    b = H.oracle_main(pc, Ca, Cb, 2 * beam)
+    for n in oracle.FIELD_NAMES:
+        assert np.array_equal(b.fields[n], 2 * a.fields[n])
+    assert np.abs(a.fields['ey']).max() > 0
+
+
+# ---------------------------------------------------------------------------
+# phasor DFT closed form
+# ---------------------------------------------------------------------------
+# Yee staggering of each component, in half cells (x, y, z) (Taflove fig 3.1)
+STAGGER = {'ex': (1, 0, 0), 'ey': (0, 1, 0), 'ez': (0, 0, 1),
+           'hx': (0, 1, 1), 'hy': (1, 0, 1), 'hz': (1, 1, 0)}
+COEF = {'ex': (3, 1, 2, 5), 'ey': (-7, 2, 1, -1), 'ez': (11, -1, 3, 2),
+        'hx': (5, 3, -2, 1), 'hy': (-2, 1, 1, 4), 'hz': (9, -3, 2, -1)}
+
+
+def _linear_field(name, nz, ny, nx):
+    a, b, c, d = COEF[name]
+    sx, sy, sz = STAGGER[name]
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing='ij')
+    return (a + b * (2 * x + sx) + c * (2 * y + sy) + d * (2 * z + sz)).astype(np.float32)
+
+
+def _node_value(name, X, Y, Z):
+    a, b, c, d = COEF[name]
+    return a + b * (2 * X) + c * (2 * Y) + d * (2 * Z)
+
+
+def test_dft_closed_form():
+    nx, ny, nz, npml, nt = 27, 25, 29, 10, 37
+    geo = _tiny_geom(nx=nx, ny=ny, nz=nz, npml=npml, Db=np.float32(0), dtm0=np.float32(0),
+                     dte0dx=np.float32(0))
+    ones = np.ones((nz, ny, nx), np.float32)
+    prof = np.full(8 * npml, 0.5, np.float32)
+    sim = oracle.OracleSim(geo, ones, 0 * ones, prof)
+    for n in oracle.FIELD_NAMES:
+        sim.fields[n][...] = _linear_field(n, nz, ny, nx)
+    sim.run(nt)
+    for n in oracle.FIELD_NAMES:                  # coefficients zero: fields unchanged
+        assert np.array_equal(sim.fields[n], _linear_field(n, nz, ny, nx))
+    w = geo['omega'] * geo['dt']
+    geo_sum = (1 - np.exp(-1j * w * nt)) / (1 - np.exp(-1j * w))   # sum_t e^{-j w t}
+    series = {'e': geo_sum, 'h': np.exp(0.5j * w) * geo_sum}       # H sampled at t - 1/2
+    pos = {'d': npml + 1, 'u': nz - npml - 1, 'l': npml + 1, 'r': nx - npml - 1,
+           'f': npml + 1, 'b': ny - npml - 1}
+    ph = sim.phasors()
+    for name, arr in ph.items():
+        comp, face = name[:2], name[-1]
+        A, B = arr.shape
+        want = np.zeros((A, B), np.complex128)
+        for a in range(1, A):
+            for b in range(1, B):
+                if face in 'du':
+                    X, Y, Z = npml + b, npml + a, pos[face]
+                elif face in 'lr':
+                    X, Y, Z = pos[face], npml + b, npml + a
+                else:
+                    X, Y, Z = npml + b, pos[face], npml + a
+                want[a, b] = _node_value(comp, X, Y, Z) * series[comp[0]]
+        assert np.all(arr[0, :] == 0) and np.all(arr[:, 0] == 0), name
+        err = np.abs(arr - want).max() / np.abs(want).max()
+        assert err < 1e-5, (name, err)
+
+
+# ---------------------------------------------------------------------------
+# brute-force loops on a tiny grid
+# ---------------------------------------------------------------------------
+f32 = np.float32
+
+
+def _brute_h(F, P, prof, g):
+    nz, ny, nx, n = g['nz'], g['ny'], g['nx'], g['npml']
+    db, dtm0, dx = f32(g['Db']), f32(g['dtm0']), f32(g['dx'])
+    ex, ey, ez, hx, hy, hz = (F[k] for k in oracle.FIELD_NAMES)
+    bh_d, ch_d, bh_u, ch_u = prof[2], prof[3], prof[6], prof[7]
+    for z in range(nz - 1):
+        for y in range(ny - 1):
+            for x in range(1, nx - 1):
+                hx[z, y, x] = f32(hx[z, y, x] - f32(db * f32(f32(ez[z, y + 1, x] - ez[z, y, x]) - f32(ey[z + 1, y, x] - ey[z, y, x]))))
+    for z in range(nz - 1):
+        for y in range(1, ny - 1):
+            for x in range(nx - 1):
+                hy[z, y, x] = f32(hy[z, y, x] - f32(db * f32(f32(ex[z + 1, y, x] - ex[z, y, x]) - f32(ez[z, y, x + 1] - ez[z, y, x]))))
+    for z in range(1, nz - 1):
+        for y in range(ny - 1):
+            for x in range(nx - 1):
+                hz[z, y, x] = f32(hz[z, y, x] - f32(db * f32(f32(ey[z, y, x + 1] - ey[z, y, x]) - f32(ex[z, y + 1, x] - ex[z, y, x]))))
+
+    def psi(p, b, c, d):
+        return f32(f32(b * p) + f32(f32(c * d) / dx))
+
+    for z in range(n - 1):            # down
+        for y in range(ny - 1):
+            for x in range(nx - 1):
+                P['psiHyz_d'][z, y, x] = psi(P['psiHyz_d'][z, y, x], bh_d[z], ch_d[z], f32(ex[z + 1, y, x] - ex[z, y, x]))
+                P['psiHxz_d'][z, y, x] = psi(P['psiHxz_d'][z, y, x], bh_d[z], ch_d[z], f32(ey[z + 1, y, x] - ey[z, y, x]))
+                hy[z, y, x] = f32(hy[z, y, x] - f32(dtm0 * P['psiHyz_d'][z, y, x]))
+                hx[z, y, x] = f32(hx[z, y, x] + f32(dtm0 * P['psiHxz_d'][z, y, x]))
+    for z in range(n - 1):            # up
+        zf = nz - n + z
+        for y in range(ny - 1):
+            for x in range(nx - 1):
+                P['psiHyz_u'][z, y, x] = psi(P['psiHyz_u'][z, y, x], bh_u[z], ch_u[z], f32(ex[zf + 1, y, x] - ex[zf, y, x]))
+                P['psiHxz_u'][z, y, x] = psi(P['psiHxz_u'][z, y, x], bh_u[z], ch_u[z], f32(ey[zf + 1, y, x] - ey[zf, y, x]))
+                hy[zf, y, x] = f32(hy[zf, y, x] - f32(dtm0 * P['psiHyz_u'][z, y, x]))
+                hx[zf, y, x] = f32(hx[zf, y, x] + f32(dtm0 * P['psiHxz_u'][z, y, x]))
+    for z in range(nz - 1):           # left / right
+        for y in range(ny - 1):
+            for x in range(n - 1):
+                P['psiHzx_l'][z, y, x] = psi(P['psiHzx_l'][z, y, x], bh_d[x], ch_d[x], f32(ey[z, y, x + 1] - ey[z, y, x]))
+                P['psiHyx_l'][z, y, x] = psi(P['psiHyx_l'][z, y, x], bh_d[x], ch_d[x], f32(ez[z, y, x + 1] - ez[z, y, x]))
+                hz[z, y, x] = f32(hz[z, y, x] - f32(dtm0 * P['psiHzx_l'][z, y, x]))
+                hy[z, y, x] = f32(hy[z, y, x] + f32(dtm0 * P['psiHyx_l'][z, y, x]))
+                xf = nx - n + x
+                P['psiHzx_r'][z, y, x] = psi(P['psiHzx_r'][z, y, x], bh_u[x], ch_u[x], f32(ey[z, y, xf + 1] - ey[z, y, xf]))
+                P['psiHyx_r'][z, y, x] = psi(P['psiHyx_r'][z, y, x], bh_u[x], ch_u[x], f32(ez[z, y, xf + 1] - ez[z, y, xf]))
+                hz[z, y, xf] = f32(hz[z, y, xf] - f32(dtm0 * P['psiHzx_r'][z, y, x]))
+                hy[z, y, xf] = f32(hy[z, y, xf] + f32(dtm0 * P['psiHyx_r'][z, y, x]))
+    for z in range(nz - 1):           # front / back
+        for y in range(n - 1):
+            for x in range(nx - 1):
+                P['psiHxy_f'][z, y, x] = psi(P['psiHxy_f'][z, y, x], bh_d[y], ch_d[y], f32(ez[z, y + 1, x] - ez[z, y, x]))
+                P['psiHzy_f'][z, y, x] = psi(P['psiHzy_f'][z, y, x], bh_d[y], ch_d[y], f32(ex[z, y + 1, x] - ex[z, y, x]))
+                hx[z, y, x] = f32(hx[z, y, x] - f32(dtm0 * P['psiHxy_f'][z, y, x]))
+                hz[z, y, x] = f32(hz[z, y, x] + f32(dtm0 * P['psiHzy_f'][z, y, x]))
+                yf = ny - n + y
+                P['psiHxy_b'][z, y, x] = psi(P['psiHxy_b'][z, y, x], bh_u[y], ch_u[y], f32(ez[z, yf + 1, x] - ez[z, yf, x]))
+                P['psiHzy_b'][z, y, x] = psi(P['psiHzy_b'][z, y, x], bh_u[y], ch_u[y], f32(ex[z, yf + 1, x] - ex[z, yf, x]))
+                hx[z, yf, x] = f32(hx[z, yf, x] - f32(dtm0 * P['psiHxy_b'][z, y, x]))
+                hz[z, yf, x] = f32(hz[z, yf, x] + f32(dtm0 * P['psiHzy_b'][z, y, x]))
+
+
+def _brute_e(F, P, prof, g, Ca, Cb):
+    nz, ny, nx, n = g['nz'], g['ny'], g['nx'], g['npml']
+    dx = f32(g['dx'])
+    ex, ey, ez, hx, hy, hz = (F[k] for k in oracle.FIELD_NAMES)
+    be_d, ce_d, be_u, ce_u = prof[0], prof[1], prof[4], prof[5]
+    for z in range(1, nz - 1):
+        for y in range(1, ny - 1):
+            for x in range(nx - 1):
+                ex[z, y, x] = f32(f32(ex[z, y, x] * Ca[z, y, x]) + f32(Cb[z, y, x] * f32(f32(hz[z, y, x] - hz[z, y - 1, x]) - f32(hy[z, y, x] - hy[z - 1, y, x]))))
+    for z in range(1, nz - 1):
+        for y in range(ny - 1):
+            for x in range(1, nx - 1):
+                ey[z, y, x] = f32(f32(ey[z, y, x] * Ca[z, y, x]) + f32(Cb[z, y, x] * f32(f32(hx[z, y, x] - hx[z - 1, y, x]) - f32(hz[z, y, x] - hz[z, y, x - 1]))))
+    for z in range(nz - 1):
+        for y in range(1, ny - 1):
+            for x in range(1, nx - 1):
+                ez[z, y, x] = f32(f32(ez[z, y, x] * Ca[z, y, x]) + f32(Cb[z, y, x] * f32(f32(hy[z, y, x] - hy[z, y, x - 1]) - f32(hx[z, y, x] - hx[z, y - 1, x]))))
+
+    def psi(p, b, c, d):
+        return f32(f32(b * p) + f32(f32(c * d) / dx))
+
+    for z in range(1, n):             # down
+        for y in range(ny - 1):
+            for x in range(nx - 1):
+                P['psiEyz_d'][z, y, x] = psi(P['psiEyz_d'][z, y, x], be_d[z], ce_d[z], f32(hx[z, y, x] - hx[z - 1, y, x]))
+                P['psiExz_d'][z, y, x] = psi(P['psiExz_d'][z, y, x], be_d[z], ce_d[z], f32(hy[z, y, x] - hy[z - 1, y, x]))
+                ey[z, y, x] = f32(ey[z, y, x] + f32(f32(Cb[z, y, x] * dx) * P['psiEyz_d'][z, y, x]))
+                ex[z, y, x] = f32(ex[z, y, x] - f32(f32(Cb[z, y, x] * dx) * P['psiExz_d'][z, y, x]))
+    for z in range(n - 1):            # up
+        zf = nz - n + z
+        for y in range(ny - 1):
+            for x in range(nx - 1):
+                P['psiEyz_u'][z, y, x] = psi(P['psiEyz_u'][z, y, x], be_u[z], ce_u[z], f32(hx[zf, y, x] - hx[zf - 1, y, x]))
+                P['psiExz_u'][z, y, x] = psi(P['psiExz_u'][z, y, x], be_u[z], ce_u[z], f32(hy[zf, y, x] - hy[zf - 1, y, x]))
+                ey[zf, y, x] = f32(ey[zf, y, x] + f32(f32(Cb[zf, y, x] * dx) * P['psiEyz_u'][z, y, x]))
+                ex[zf, y, x] = f32(ex[zf, y, x] - f32(f32(Cb[zf, y, x] * dx) * P['psiExz_u'][z, y, x]))
+    for z in range(nz - 1):           # left / right
+        for y in range(ny - 1):
+            for x in range(1, n):
+                P['psiEzx_l'][z, y, x] = psi(P['psiEzx_l'][z, y, x], be_d[x], ce_d[x], f32(hy[z, y, x] - hy[z, y, x - 1]))
+                P['psiEyx_l'][z, y, x] = psi(P['psiEyx_l'][z, y, x], be_d[x], ce_d[x], f32(hz[z, y, x] - hz[z, y, x - 1]))
+                ez[z, y, x] = f32(ez[z, y, x] + f32(f32(Cb[z, y, x] * dx) * P['psiEzx_l'][z, y, x]))
+                ey[z, y, x] = f32(ey[z, y, x] - f32(f32(Cb[z, y, x] * dx) * P['psiEyx_l'][z, y, x]))
+            for x in range(n - 1):
+                xf = nx - n + x
+                P['psiEzx_r'][z, y, x] = psi(P['psiEzx_r'][z, y, x], be_u[x], ce_u[x], f32(hy[z, y, xf] - hy[z, y, xf - 1]))
+                P['psiEyx_r'][z, y, x] = psi(P['psiEyx_r'][z, y, x], be_u[x], ce_u[x], f32(hz[z, y, xf] - hz[z, y, xf - 1]))
+                ez[z, y, xf] = f32(ez[z, y, xf] + f32(f32(Cb[z, y, xf] * dx) * P['psiEzx_r'][z, y, x]))
+                ey[z, y, xf] = f32(ey[z, y, xf] - f32(f32(Cb[z, y, xf] * dx) * P['psiEyx_r'][z, y, x]))
+    for z in range(nz - 1):           # front / back
+        for y in range(1, n):
+            for x in range(nx - 1):
+                P['psiExy_f'][z, y, x] = psi(P['psiExy_f'][z, y, x], be_d[y], ce_d[y], f32(hz[z, y, x] - hz[z, y - 1, x]))
+                P['psiEzy_f'][z, y, x] = psi(P['psiEzy_f'][z, y, x], be_d[y], ce_d[y], f32(hx[z, y, x] - hx[z, y - 1, x]))
+                ex[z, y, x] = f32(ex[z, y, x] + f32(f32(Cb[z, y, x] * dx) * P['psiExy_f'][z, y, x]))
+                ez[z, y, x] = f32(ez[z, y, x] - f32(f32(Cb[z, y, x] * dx) * P['psiEzy_f'][z, y, x]))
+        for y in range(n - 1):
+            yf = ny - n + y
+            for x in range(nx - 1):
+                P['psiExy_b'][z, y, x] = psi(P['psiExy_b'][z, y, x], be_u[y], ce_u[y], f32(hz[z, yf, x] - hz[z, yf - 1, x]))
+                P['psiEzy_b'][z, y, x] = psi(P['psiEzy_b'][z, y, x], be_u[y], ce_u[y], f32(hx[z, yf, x] - hx[z, yf - 1, x]))
+                ex[z, yf, x] = f32(ex[z, yf, x] + f32(f32(Cb[z, yf, x] * dx) * P['psiExy_b'][z, y, x]))
+                ez[z, yf, x] = f32(ez[z, yf, x] - f32(f32(Cb[z, yf, x] * dx) * P['psiEzy_b'][z, y, x]))
+
+
+def test_passes_match_brute_force_loops():
+    g = _tiny_geom()
+    nz, ny, nx, n = g['nz'], g['ny'], g['nx'], g['npml']
+    rng = np.random.default_rng(42)
+    Ca = rng.uniform(0.5, 1.0, (nz, ny, nx)).astype(np.float32)
+    Cb = rng.uniform(0.5, 3.0, (nz, ny, nx)).astype(np.float32)
+    prof = rng.uniform(-0.5, 1.0, (8, n)).astype(np.float32)
+    sim = oracle.OracleSim(g, Ca, Cb, prof.ravel(), nthreads=3)
+    for k in oracle.FIELD_NAMES:
+        sim.fields[k][...] = rng.standard_normal(sim.fields[k].shape)
+    for k in oracle.PSI_NAMES:
+        sim.psi[k][...] = rng.standard_normal(sim.psi[k].shape)
+    F = {k: v.copy() for k, v in sim.fields.items()}
+    P = {k: v.copy() for k, v in sim.psi.items()}
+    sim.h_half()
+    _brute_h(F, P, prof, g)
+    for k in oracle.FIELD_NAMES:
+        assert np.array_equal(sim.fields[k], F[k]), k
+    for k in oracle.PSI_NAMES[12:]:
+        assert np.array_equal(sim.psi[k], P[k]), k
+    sim.e_half()
+    _brute_e(F, P, prof, g, Ca, Cb)
+    for k in oracle.FIELD_NAMES:
+        assert np.array_equal(sim.fields[k], F[k]), k
+    for k in oracle.PSI_NAMES[:12]:
+        assert np.array_equal(sim.psi[k], P[k]), k
+
+
+# ---------------------------------------------------------------------------
+# physics
+# ---------------------------------------------------------------------------
+def _beam_case(nx, nz, nz_mat, theta, illum, inc_dur, sim_dur, pol='HH'):
+    pc = workloads.base_params(theta=theta, polarization=pol, illumination_width=illum,
+                               incident_duration=inc_dur, simulation_duration=sim_dur,
+                               dft_accumulation='reference')
+    workloads.explicit_geometry(pc, nx, nx, nz, nz_mat, 0.05)
+    pex, pey = H.excitation(pc)
+    rec, inc_ph = H.oracle_incident(pc, pex, pey)
+    return pc, rec, inc_ph
+
+
+@pytest.fixture(scope='module')
+def normal_beam():
+    return _beam_case(61, 71, 20, 0.0, 1.5, 4.0, 11.0)
+
+
+def test_tfsf_vacuum_scattered_region_dark(normal_beam):
+    pc, rec, _ = normal_beam
+    g = pc.geom_params
+    Ca, Cb = compute_material_arrays(pc, next(all_vacuum(pc)))
+    sim = oracle.OracleSim(H.oracle_geom(pc), Ca, Cb, H.profiles_of(pc), inc=rec)
+    n = g.size_pml
+    tot = leak = 0.0
+    for _ in range(g.size_t // 10):
+        sim.run(10)
+        ey = sim.fields['ey']
+        tot = max(tot, np.abs(ey[n + 2:g.tfsf_z - 1, n:-n, n:-n]).max())
+        leak = max(leak, np.abs(ey[g.tfsf_z + 2:-n - 1, n:-n, n:-n]).max())
+    assert tot > 1.0
+    assert leak / tot < 2e-3, leak / tot
+
+
+def _probe_run(N, steps, offset):
+    c = N // 2
+    pc = workloads.small_params(nx=N, ny=N, nz=N, nz_mat=1)
+    g = pc.geom_params
+    g.size_t_inc = 0
+    Ca, Cb = compute_material_arrays(pc, next(all_vacuum(pc)))
+    geo = H.oracle_geom(pc)
+    sim = oracle.OracleSim(geo, Ca, Cb, H.profiles_of(pc))
+    z, y, x = np.meshgrid(*(np.arange(N),) * 3, indexing='ij')
+    sim.fields['ey'][...] = np.exp(-((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2) / 4.0)
+    trace = []
+    for _ in range(steps):
+        sim.run(1)
+        trace.append(sim.fields['ey'][c, c, c + offset])
+    return np.array(trace)
+
+
+def test_cpml_reflection_small():
+    small = _probe_run(41, 90, 7)
+    big = _probe_run(101, 90, 7)
+    assert np.abs(big).max() > 1e-3
+    refl = np.abs(small - big).max() / np.abs(big).max()
+    assert refl < 5e-3, refl
+
+
+def _fresnel_ratio(cell, nx, nz, nz_mat, eps_r):
+    """|E_phi(theta=0)|^2 of a flat dielectric over that of a PEC plate
+    (sigma = 1e10), divided by the plane-wave Fresnel |Gamma|^2."""
+    pc, rec, inc_ph = _beam_case(nx, nz, nz_mat, 0.0, 1.5, 4.0, 11.0)
+    if cell != 0.05:
+        pc = workloads.base_params(theta=0.0, illumination_width=1.5, incident_duration=4.0,
+                                   simulation_duration=11.0, dft_accumulation='reference')
+        workloads.explicit_geometry(pc, nx, nx, nz, nz_mat, cell)
+        pex, pey = H.excitation(pc)
+        rec, inc_ph = H.oracle_incident(pc, pex, pey)
+    g = pc.geom_params
+    n = g.size_pml
+    nxp, nzp = nx - 2 * n, nz - 2 * n
+    th = np.linspace(-80.0, 80.0, 161)
+    out = []
+    for e, s in ((eps_r, 0.0), (1.0, 1.0e10)):
+        Ca, Cb = compute_material_arrays(pc, next(flat_surface(pc, e, s)))
+        sim = oracle.OracleSim(H.oracle_geom(pc), Ca, Cb, H.profiles_of(pc), inc=rec)
+        sim.run(g.size_t)
+        ff = oracle.far_field(sim.phasors(), nxp, nxp, nzp, g.cell_size, 2000.0, th, [0.0],
+                              pc.inc_beam_params.frequency)
+        p_inc = oracle.incident_power(inc_ph, g.cell_size)
+        out.append((ff, oracle.sigma0(ff, p_inc, nxp, nxp, g.cell_size)[0]))
+    (ff_d, _), (ff_p, s_p) = out
+    i0 = 80                                            # theta = 0
+    nn = np.sqrt(eps_r)
+    ratio = abs(ff_d['E_phi'][i0]) ** 2 / abs(ff_p['E_phi'][i0]) ** 2
+    return ratio / ((1 - nn) / (1 + nn)) ** 2, ff_p, s_p
+
+
+def test_fresnel_flat_plates_second_order():
+    """The specular power ratio converges to Fresnel at second order in the
+    cell size (Yee's scheme): the error at 0.035 lambda is (0.035/0.05)^2 of
+    the error at 0.05 lambda."""
+    r1, ff_p, s_p = _fresnel_ratio(0.05, 61, 71, 20, 4.0)
+    r2, _, _ = _fresnel_ratio(0.035, 81, 97, 26, 4.0)
+    e1, e2 = r1 - 1, r2 - 1
+    assert 0 < e1 < 0.15 and 0 < e2 < 0.08, (r1, r2)
+    assert 0.3 < e2 / e1 < 0.7, e2 / e1             # (0.035 / 0.05)^2 = 0.49
+    i0 = 80
+    # HH at normal incidence: cross-polar (E_theta) far weaker than co-polar
+    assert abs(ff_p['E_theta'][i0]) < 1e-2 * abs(ff_p['E_phi'][i0])
+    # the PEC plate's specular lobe dominates the far field
+    assert s_p[i0] == s_p.max() and s_p[i0] > 100 * s_p[10]
+
+
+def test_incident_power_closed_form():
+    n, dx, e0 = 21, 0.01, 3.0
+    eta = 376.730313668
+    inc = {'ex_inc_phasor': np.full((n, n), e0, np.complex64),
+           'ey_inc_phasor': np.zeros((n, n), np.complex64),
+           'hx_inc_phasor': np.zeros((n, n), np.complex64),
+           'hy_inc_phasor': np.full((n, n), -e0 / eta, np.complex64)}
+    want = 0.5 * e0 * (e0 / eta) * ((n - 1) * dx) ** 2     # downward Poynting flux
+    from sar_pixel_simulator_amd import processresults
+    assert oracle.incident_power(inc, dx) == pytest.approx(want, rel=1e-6)
+    assert processresults.incident_power(inc, dx) == pytest.approx(want, rel=1e-6)
+
+
+def test_far_field_uniform_sheet_broadside():
+    n, dx, nz, f = 31, 0.002, 41, 5.4e9
+    one = np.ones((n, n), np.complex64)
+    zero = np.zeros((n, n), np.complex64)
+    ph = {'ex_ph_u': zero, 'ey_ph_u': one, 'hx_ph_u': zero, 'hy_ph_u': zero}   # M_x = 1
+    r = 2000.0
+    ff = oracle.far_field(ph, n, n, nz, dx, r, np.array([0.0]), [0.0], f)
+    k = 2 * np.pi * f / 299792458
+    area = ((n - 1) * dx) ** 2
+    w = 1j * k * np.exp(-1j * k * r) / (4 * np.pi * r)
+    ztop = ((nz - 1) // 2) * dx
+    want_ephi = w * area * np.exp(1j * k * ztop)       # L_theta = area e^{jkz}, E_phi = w L_theta
+    assert abs(ff['E_phi'] - want_ephi) < 1e-12 * abs(want_ephi)
+    assert abs(ff['E_theta']) < 1e-12 * abs(want_ephi)
+    assert abs(ff['H_theta'] - (-w * area * np.exp(1j * k * ztop) / 376.730313668)) \
+        < 1e-12 * abs(want_ephi)

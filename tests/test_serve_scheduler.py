@@ -1,0 +1,155 @@
+"""Scheduling logic of the serving loop against a fake device backend: HBM
+admission, slots, FIFO order, continuous joining at chunk boundaries, error
+isolation and latency statistics.  (Device results of the same loop are
+checked on the GPU in test_gpu_batch_serve.py.)"""
+import types
+
+import pytest
+
+from sar_pixel_simulator_amd.hip.serve import (AdmissionError, Request, Server,
+                                               latency_stats)
+
+
+class FakeMember:
+    def __init__(self, req, nt):
+        self.req, self.nt, self.steps = req, nt, 0
+
+
+class FakeBackend:
+    def __init__(self, sizes, steps, fail_collect=(), fail_admit=()):
+        self.sizes, self.nsteps = sizes, steps
+        self.fail_collect, self.fail_admit = set(fail_collect), set(fail_admit)
+        self.batches = []
+        self.released = []
+
+    def footprint(self, req):
+        return self.sizes[req.rid]
+
+    def steps(self, req):
+        return self.nsteps[req.rid]
+
+    def admit(self, req):
+        if req.rid in self.fail_admit:
+            raise MemoryError('device allocation failed')
+        return FakeMember(req, self.nsteps[req.rid])
+
+    def make_batch(self, members):
+        b = types.SimpleNamespace(members=list(members), runs=[])
+        self.batches.append(b)
+        return b
+
+    def run(self, batch, n):
+        batch.runs.append(n)
+        for m in batch.members:
+            m.steps = min(m.nt, m.steps + n)
+
+    def sync(self, batch):
+        pass
+
+    def drop_batch(self, batch):
+        pass
+
+    def collect(self, m):
+        if m.req.rid in self.fail_collect:
+            raise RuntimeError('far field failed')
+        return {'rid': m.req.rid, 'steps_run': m.steps}
+
+    def release(self, m):
+        self.released.append(m.req.rid)
+
+
+class Clock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        self.t += 1.0
+        return self.t
+
+
+def _reqs(n):
+    return [Request(rid=i, params=None, name=f'r{i}') for i in range(n)]
+
+
+def test_all_complete_with_exact_steps():
+    n = 7
+    be = FakeBackend({i: 10 for i in range(n)}, {i: 30 + 7 * i for i in range(n)})
+    s = Server(be, hbm_budget=1000, max_slots=64, chunk_steps=16, clock=Clock())
+    for r in _reqs(n):
+        s.submit(r)
+    s.drain()
+    assert sorted(r.rid for r in s.results) == list(range(n))
+    for r in s.results:
+        assert r.error is None and r.output['steps_run'] == 30 + 7 * r.rid
+    assert sorted(be.released) == list(range(n))
+    assert s.used == 0
+
+
+def test_budget_and_slots_bind():
+    n = 9
+    be = FakeBackend({i: 40 for i in range(n)}, {i: 20 for i in range(n)})
+    s = Server(be, hbm_budget=100, max_slots=64, chunk_steps=8, clock=Clock())
+    for r in _reqs(n):
+        s.submit(r)
+    s.drain()
+    assert s.peak_used <= 100 and s.peak_slots == 2
+    assert all(len(b.members) <= 2 for b in be.batches)
+    be2 = FakeBackend({i: 1 for i in range(n)}, {i: 20 for i in range(n)})
+    s2 = Server(be2, hbm_budget=100, max_slots=3, chunk_steps=8, clock=Clock())
+    for r in _reqs(n):
+        s2.submit(r)
+    s2.drain()
+    assert s2.peak_slots == 3 and len(s2.results) == n
+
+
+def test_fifo_and_oversize_rejected():
+    be = FakeBackend({0: 60, 1: 60, 2: 10, 3: 500}, {0: 8, 1: 8, 2: 8, 3: 8})
+    s = Server(be, hbm_budget=100, chunk_steps=8, clock=Clock())
+    with pytest.raises(AdmissionError):
+        s.submit(Request(rid=3, params=None))
+    for r in _reqs(3):
+        s.submit(r)
+    s.drain()
+    admitted = sorted(s.results, key=lambda r: r.admitted)
+    # request 2 fits next to 0 but waits behind 1 (arrival order is kept)
+    assert [r.rid for r in admitted] == [0, 1, 2]
+
+
+def test_late_arrival_joins_running_batch():
+    be = FakeBackend({0: 10, 1: 10}, {0: 64, 1: 16})
+    s = Server(be, hbm_budget=100, chunk_steps=8, clock=Clock())
+    s.submit(Request(rid=0, params=None))
+    s.step()
+    s.step()
+    s.submit(Request(rid=1, params=None))   # arrives while 0 is running
+    s.step()
+    assert [m.req.rid for m in be.batches[-1].members] == [0, 1]
+    s.drain()
+    res = {r.rid: r for r in s.results}
+    assert res[1].finished < res[0].finished      # the short late one overtakes
+    assert res[0].output['steps_run'] == 64 and res[1].output['steps_run'] == 16
+
+
+def test_failures_are_isolated():
+    be = FakeBackend({i: 1 for i in range(4)}, {i: 8 for i in range(4)},
+                     fail_collect={1}, fail_admit={2})
+    s = Server(be, hbm_budget=100, chunk_steps=8, clock=Clock())
+    for r in _reqs(4):
+        s.submit(r)
+    s.drain()
+    res = {r.rid: r for r in s.results}
+    assert res[1].error.startswith('RuntimeError') and res[2].error.startswith('MemoryError')
+    assert res[0].error is None and res[3].error is None
+    assert 1 in be.released and 2 not in be.released
+
+
+def test_latency_stats():
+    be = FakeBackend({i: 1 for i in range(5)}, {i: 8 * (i + 1) for i in range(5)})
+    s = Server(be, hbm_budget=100, chunk_steps=8, clock=Clock())
+    for r in _reqs(5):
+        s.submit(r)
+    s.drain()
+    st = latency_stats(s.results)
+    assert st['count'] == 5
+    assert st['p50_s'] <= st['p95_s'] <= st['p99_s']
+    assert st['models_per_hour'] > 0
