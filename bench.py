@@ -36,22 +36,35 @@ def cpml_cells(nx, ny, nz, np_):
                 + (nz - 1) * (np_ - 1) * (nx - 1))
 
 
-def algorithmic_bytes(nx, ny, nz, np_):
+def uniform_top(Ca, Cb):
+    """First z of the run of top planes whose cells all hold one (Ca, Cb)
+    pair (the vacuum above the material); the E kernel reads no Ca/Cb there."""
+    nz = Ca.shape[0]
+    a0, b0 = Ca[-1].flat[0], Cb[-1].flat[0]
+    z = nz
+    while z > 0 and np.all(Ca[z - 1] == a0) and np.all(Cb[z - 1] == b0):
+        z -= 1
+    return z
+
+
+def algorithmic_bytes(nx, ny, nz, np_, uni_z=None):
     """Compulsory HBM bytes per launch of each kernel.
 
     H half: read E(3) + H(3), write H(3) = 36 B/cell; each CPML face cell
     reads and writes two psi values = 16 B; E_inc injection reads 2 floats
     per TFSF-plane cell.  E half: read H(3) + E(3) + Ca + Cb, write E(3) =
-    44 B/cell, same CPML and injection terms.  DFT: per plane sample, four
+    44 B/cell (36 B on the uniform top planes z >= uni_z, where Ca/Cb are
+    scalars), same CPML and injection terms.  DFT: per plane sample, four
     float4 running sums read and written (128 B) plus four field values.
     """
     n = nx * ny * nz
     nc = cpml_cells(nx, ny, nz, np_)
     plane = nx * ny
+    n_uni = (nz - (nz if uni_z is None else uni_z)) * plane
     nxp, nyp, nzp = nx - 2 * np_, ny - 2 * np_, nz - 2 * np_
     samples = 2 * nyp * nxp + 2 * nzp * nyp + 2 * nzp * nxp
     return {'h': 36 * n + 16 * nc + 8 * plane,
-            'e': 44 * n + 16 * nc + 8 * plane,
+            'e': 44 * n - 8 * n_uni + 16 * nc + 8 * plane,
             'dft': 144 * samples,
             'cells': n, 'cpml_cells': nc, 'dft_samples': samples}
 
@@ -101,6 +114,8 @@ def main():
     ap.add_argument('--steps', type=int, default=100)
     ap.add_argument('--warmup', type=int, default=10)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--no-extras', action='store_true',
+                    help='skip the batched-vs-plain and serving legs')
     ap.add_argument('--ty', type=int, default=0)
     ap.add_argument('--zchunk', type=int, default=-1)
     args = ap.parse_args()
@@ -154,7 +169,8 @@ def main():
 
     # per-kernel HIP-event times on the library's stream
     k = sim.time_steps(min(args.steps, 50), per_kernel=True)
-    ab = algorithmic_bytes(g.size_x, g.size_y, g.size_z, g.size_pml)
+    uni_z = uniform_top(Ca, Cb)
+    ab = algorithmic_bytes(g.size_x, g.size_y, g.size_z, g.size_pml, uni_z)
     dom = 'e' if k['e'] >= k['h'] else 'h'
     achieved = ab[dom] / (k[dom] * 1e-3) / 1e9
     roof = {'bound': 'hbm', 'kernel': 'k_e_half' if dom == 'e' else 'k_h_half',
@@ -176,6 +192,17 @@ def main():
         print('bench: CPU baseline sample', file=sys.stderr, flush=True)
         cpu = cpu_baseline(pc, Ca, Cb)
 
+    extras = {}
+    if not args.no_extras:
+        from sar_pixel_simulator_amd.hip import measure
+
+        def log(msg):
+            print(msg, file=sys.stderr, flush=True)
+        print('bench: batched vs plain (uneven soil models)', file=sys.stderr, flush=True)
+        extras['batched_vs_plain'] = measure.batched_vs_plain(count=8, log=log)
+        print('bench: serving (Poisson canopy requests)', file=sys.stderr, flush=True)
+        extras['serving'] = measure.serving(n=16, log=log)
+
     out = {'metric': METRIC, 'value': round(value, 1), 'unit': 'Mcell-updates/s',
            'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
            'ms_per_step': round(ms_step, 4), 'higher_is_better': True, 'scaling': 'weak',
@@ -184,9 +211,11 @@ def main():
                                   'lambda at 0.021 lambda, README shape), snow over rough '
                                   'soil, 5.4 GHz HH theta 20, 1714-step run',
                       'cells': cells, 'launch': {'ty': ty, 'zchunk': zc},
+                      'uniform_planes_from_z': int(uni_z),
                       'device': info['name'], 'arch': info['arch'],
                       'cu_count': info['cu_count']},
            'roofline': roof, 'kernels': per_kernel, 'cpu_baseline': cpu}
+    out.update(extras)
     print(json.dumps(out), flush=True)
 
 

@@ -119,6 +119,7 @@ struct FdtdSim {
     Group self;                  // batch of one
     float2 *scratch_c64 = nullptr;
     long long scratch_len = 0;
+    bool mat_set = false;        // materials uploaded or uniform value given
 };
 
 struct FdtdBatch {
@@ -428,7 +429,7 @@ static int sim_alloc(FdtdSim *s) {
     memset(&d, 0, sizeof d);
     d.nx = g->nx; d.ny = g->ny; d.nz = g->nz; d.npml = g->npml;
     d.tfsf_z = g->tfsf_z; d.read_out = g->read_out; d.nt = g->nt; d.nt_inc = g->nt_inc;
-    d.mode = g->mode; d.dft_accum = g->dft_accum; d.uniform = 0; d.n_src = 0;
+    d.mode = g->mode; d.dft_accum = g->dft_accum; d.uni_z = g->nz; d.n_src = 0;
     d.sy = z.pitch; d.sz = (long long)g->ny * z.pitch;
     d.du_sz = (long long)g->ny * z.pitch;
     d.lr_sz = (long long)g->ny * g->npml;
@@ -585,18 +586,31 @@ extern "C" int fdtd_sim_set_materials(FdtdSim *s, const float *Ca, const float *
     if (!rc) rc = copy_volume_in(s, s->mat + s->ncell, Cb);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(s->self.stream));
-    if (s->h.uniform) {
-        s->h.uniform = 0;
-        return sim_refresh(s);
+    // Top planes where every cell holds the same (Ca, Cb) -- the vacuum that
+    // compute_material_arrays appends -- are read as scalars by k_e_half.
+    const size_t plane = (size_t)s->g.ny * s->g.nx;
+    const float ca0 = Ca[(size_t)(s->g.nz - 1) * plane], cb0 = Cb[(size_t)(s->g.nz - 1) * plane];
+    int zu = s->g.nz;
+    for (int z = s->g.nz - 1; z >= 0; --z) {
+        const float *a = Ca + (size_t)z * plane, *b = Cb + (size_t)z * plane;
+        bool same = true;
+        for (size_t i = 0; i < plane && same; ++i) same = (a[i] == ca0) & (b[i] == cb0);
+        if (!same) break;
+        zu = z;
     }
-    return FDTD_OK;
+    s->mat_set = true;
+    s->h.uni_z = zu;
+    s->h.ca_u = ca0;
+    s->h.cb_u = cb0;
+    return sim_refresh(s);
 }
 
 extern "C" int fdtd_sim_set_uniform_material(FdtdSim *s, float ca, float cb) {
     CHECK(s, "NULL sim");
-    s->h.uniform = 1;
+    s->h.uni_z = 0;
     s->h.ca_u = ca;
     s->h.cb_u = cb;
+    s->mat_set = true;
     return sim_refresh(s);
 }
 
@@ -662,10 +676,10 @@ extern "C" int fdtd_sim_reset(FdtdSim *s) {
 
 static int check_ready(FdtdSim *s) {
     if (s->g.mode == FDTD_MODE_MAIN) {
-        CHECK(s->h.uniform || s->mat, "materials not set");
+        CHECK(s->mat_set, "materials not set");
         if (s->g.nt_inc > 0) CHECK(s->beam, "no beam attached (simulation injects %d steps)", s->g.nt_inc);
     } else {
-        CHECK(s->h.uniform, "INCIDENT simulations need a uniform material");
+        CHECK(s->mat_set && s->h.uni_z == 0, "INCIDENT simulations need a uniform material");
         CHECK(s->g.nt_inc == 0 || s->src, "hard source not set");
     }
     return FDTD_OK;

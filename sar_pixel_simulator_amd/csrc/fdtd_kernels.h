@@ -43,7 +43,7 @@ enum Prof : int { BE_DLF, CE_DLF, BH_DLF, CH_DLF, BE_URB, CE_URB, BH_URB, CH_URB
 struct alignas(16) Desc {
     int nx, ny, nz, npml;
     int tfsf_z, read_out, nt, nt_inc;
-    int mode, dft_accum, uniform, n_src;
+    int mode, dft_accum, uni_z, n_src;   // planes z >= uni_z hold (ca_u, cb_u) everywhere
     int nt_beam, nfaces, pad0, pad1;
     long long sy, sz;              // field strides (elements)
     long long du_sz, lr_sz, fb_sz; // psi plane strides

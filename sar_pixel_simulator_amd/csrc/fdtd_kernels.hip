@@ -194,7 +194,7 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     const float *__restrict__ Hz = d->f[5];
     const float *__restrict__ Ca = d->ca;
     const float *__restrict__ Cb = d->cb;
-    const bool uniform = d->uniform != 0;
+    const int uni_z = d->uni_z;
     const float ca_u = d->ca_u, cb_u = d->cb_u, dx = d->dx;
 
     const bool xlo = x >= 1, ylo = y >= 1;
@@ -226,8 +226,9 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
             const float hz_xm = xlo ? Hz[o - 1] : 0.f;
             const float hx_ym = ylo ? Hx[o - sy] : 0.f;
             const float hy_xm = xlo ? Hy[o - 1] : 0.f;
-            const float ca = uniform ? ca_u : Ca[o];
-            const float cb = uniform ? cb_u : Cb[o];
+            const bool uni = z >= uni_z;                  // uniform (e.g. vacuum) plane
+            const float ca = uni ? ca_u : Ca[o];
+            const float cb = uni ? cb_u : Cb[o];
             const bool zlo = z >= 1;
             if (zlo && ex_xy) ex = (ex * ca) + (cb * ((hz_c - hz_ym) - (hy_c - hy_m)));
             if (zlo && ey_xy) ey = (ey * ca) + (cb * ((hx_c - hx_m) - (hz_c - hz_xm)));

@@ -1,0 +1,164 @@
+"""Throughput and latency measurements of the batched and served paths.
+
+* ``batched_vs_plain``: material-models/hour of the same set of models run
+  one at a time (the reference's experiment loop) and as one device batch.
+  Both count, per model, the Ca/Cb upload, all time steps and the far field
+  plus sigma0 from the device phasor sums; the incident beams are generated
+  beforehand (cached per beam key) and reported separately.
+* ``serving``: a Poisson stream of requests through the continuous-batching
+  server with an HBM budget; p50/p95/p99 latency and models/hour.
+"""
+import time
+
+import numpy as np
+
+from .. import _native as native
+from .. import workloads
+from ..materials import compute_material_arrays, synthetic
+from . import batch as B
+from .serve import HipBackend, Request, Server, latency_stats
+
+
+def _models_for(cases, beams):
+    return [B.DeviceModel(pc, Ca, Cb, beams, name) for pc, Ca, Cb, name in cases]
+
+
+def batched_workload(count, seed=1000):
+    """Soil realizations of uneven lateral size {3,4,6,8} lambda, depth
+    U[1,2] lambda, cell 0.03 lambda, HH and VV (``count`` seeds x 2 pols)."""
+    cases = []
+    for i, pc in workloads.batched_params_list(count=count, seed=seed):
+        m = workloads.batched_material(pc, i)
+        Ca, Cb = compute_material_arrays(pc, m)
+        cases.append((pc, Ca, Cb, f'soil_{i}_{pc.inc_beam_params.polarization}'))
+    return cases
+
+
+def batched_vs_plain(count=8, chunk=64, device=0, log=print):
+    native.init(device)
+    t0 = time.perf_counter()
+    cases = batched_workload(count)
+    beams = B.BeamCache()
+    for pc, _, _, _ in cases:
+        beams.get(pc)
+    t_beams = time.perf_counter() - t0
+    cells = [pc.geom_params.size_x * pc.geom_params.size_y * pc.geom_params.size_z
+             for pc, _, _, _ in cases]
+    steps = [pc.geom_params.size_t for pc, _, _, _ in cases]
+    log(f'batched_vs_plain: {len(cases)} models, {len(beams)} beams in {t_beams:.1f} s')
+
+    # plain: one model at a time
+    models = []
+    t0 = time.perf_counter()
+    plain_back = []
+    for pc, Ca, Cb, name in cases:
+        m = B.DeviceModel(pc, Ca, Cb, beams, name)
+        B.run_plain([m])
+        plain_back.append(m.sigma0()['sigma_h'])
+        m.release()
+    t_plain = time.perf_counter() - t0
+    log(f'batched_vs_plain: plain {t_plain:.2f} s')
+
+    # batched: all resident, one launch per kernel per step
+    t0 = time.perf_counter()
+    models = _models_for(cases, beams)
+    B.run_batched(models, chunk=chunk)
+    batch_back = [m.sigma0()['sigma_h'] for m in models]
+    t_batch = time.perf_counter() - t0
+    for m in models:
+        m.release()
+    beams.release()
+    same = all(np.array_equal(a, b) for a, b in zip(plain_back, batch_back))
+    log(f'batched_vs_plain: batched {t_batch:.2f} s, identical results: {same}')
+    n = len(cases)
+    cu = float(np.dot(cells, steps))
+    return {'models': n, 'beams': len(set(map(B.beam_key, [c[0] for c in cases]))),
+            'beam_setup_s': round(t_beams, 2),
+            'plain_s': round(t_plain, 3), 'batched_s': round(t_batch, 3),
+            'plain_models_per_hour': round(n / t_plain * 3600, 1),
+            'batched_models_per_hour': round(n / t_batch * 3600, 1),
+            'speedup': round(t_plain / t_batch, 2),
+            'batched_Mcell_updates_per_s': round(cu / t_batch / 1e6, 1),
+            'plain_Mcell_updates_per_s': round(cu / t_plain / 1e6, 1),
+            'identical_results': bool(same),
+            'grid_cells_min_max': [int(min(cells)), int(max(cells))],
+            'steps': int(max(steps))}
+
+
+def serving_requests(n, rate_hz, seed=7, lateral=(4.0, 6.0), height=(2.0, 3.0),
+                     thetas=(20.0, 30.0, 40.0), pols=('HH', 'VV'), cell=0.025):
+    """Poisson arrivals of vegetation-canopy-over-soil requests.  Lateral
+    sizes and heights are drawn from a few discrete values so that beams are
+    shared between requests (a beam is keyed by grid size, angle and
+    polarisation)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for k in range(n):
+        t += rng.exponential(1.0 / rate_hz)
+        lat = float(rng.choice(lateral))
+        h = float(rng.choice(height))
+        pc = workloads.base_params(polarization=str(rng.choice(pols)),
+                                   theta=float(rng.choice(thetas)), incident_duration=10.0,
+                                   illumination_width=2.0, simulation_duration=12.0,
+                                   material_fraction=0.6)
+        workloads.set_geom_params(pc, lat, lat, h, 12.0, cell, 'wavelength')
+        g = pc.geom_params
+        soil = max(2, g.size_z_material // 3)
+        m = synthetic.canopy_over_soil(g.size_x, g.size_y, g.size_z_material, g.cell_size,
+                                       workloads.lam_of(pc), soil, seed=1000 + k,
+                                       density=float(rng.uniform(2.0, 6.0)))
+        reqs.append(Request(rid=k, params=pc, material=m, name=f'req{k}', arrival=t))
+    return reqs
+
+
+def serving(n=24, rate_hz=None, budget_gb=None, max_slots=16, chunk=48, device=0, log=print):
+    """Serve ``n`` Poisson requests; the rate defaults to 1.5x the measured
+    one-at-a-time service rate so admission binds."""
+    native.init(device)
+    reqs = serving_requests(n, rate_hz=1.0)
+    be = HipBackend(device)
+    for r in reqs:                         # beams first: reported separately
+        be.beams.get(r.params)
+    fps = [be.footprint(r) for r in reqs]
+    # one-at-a-time service time of the first few requests sets the rate
+    if rate_hz is None:
+        t0 = time.perf_counter()
+        k = min(3, n)
+        for r in reqs[:k]:
+            m = be.admit(r)
+            B.run_plain([m])
+            m.sigma0()
+            m.release()
+        svc = (time.perf_counter() - t0) / k
+        rate_hz = 1.5 / svc
+    gaps = np.diff([0.0] + [r.arrival for r in reqs])
+    t = 0.0
+    for r, gap in zip(reqs, gaps):       # rescale the Poisson stream to the rate
+        t += gap / rate_hz
+        r.arrival = t
+    budget = int((budget_gb or 0) * (1 << 30)) or int(4 * np.median(fps))
+    srv = Server(be, hbm_budget=budget, max_slots=max_slots, chunk_steps=chunk)
+    t0 = time.perf_counter()
+    res = srv.serve(reqs)
+    wall = time.perf_counter() - t0
+    be.beams.release()
+    st = latency_stats(res)
+    st.update({'requests': n, 'errors': sum(r.error is not None for r in res),
+               'arrival_rate_hz': round(rate_hz, 3), 'hbm_budget_bytes': budget,
+               'peak_hbm_bytes': srv.peak_used, 'peak_members': srv.peak_slots,
+               'chunks': srv.chunks, 'wall_s': round(wall, 2)})
+    log(f'serving: {st}')
+    return st
+
+
+if __name__ == '__main__':
+    import json
+    import sys
+    what = sys.argv[1:] or ['batched', 'serving']
+    out = {}
+    if 'batched' in what:
+        out['batched'] = batched_vs_plain()
+    if 'serving' in what:
+        out['serving'] = serving()
+    print(json.dumps(out))

@@ -102,18 +102,19 @@ def canopy_over_soil(nx, ny, nz_mat, dx, lam, soil_cells, seed, density,
                       rng.uniform(0, ny * dx), rng.uniform(0, nx * dx)])
         e_c = rng.uniform(10.0, 25.0)
         s_c = rng.uniform(0.1, 0.5)
-        half = 0.5 * ln + r
-        lo = np.maximum(((c - half) / dx).astype(int), 0)
-        hi = np.minimum(((c + half) / dx).astype(int) + 2,
+        # axis-aligned extent of the cylinder: L/2 |u_i| + r sqrt(1 - u_i^2)
+        half = 0.5 * ln * np.abs(u) + r * np.sqrt(np.maximum(0.0, 1.0 - u * u))
+        lo = np.maximum(np.floor((c - half) / dx).astype(int), 0)
+        hi = np.minimum(np.ceil((c + half) / dx).astype(int) + 1,
                         [nz_mat, ny, nx])
         if np.any(hi <= lo):
             continue
-        Z, Y, X = np.meshgrid(zc[lo[0]:hi[0]], yc[lo[1]:hi[1]],
-                              xc[lo[2]:hi[2]], indexing='ij')
-        d = np.stack([Z - c[0], Y - c[1], X - c[2]], axis=-1)
-        t = d @ u
-        perp = np.linalg.norm(d - t[..., None] * u, axis=-1)
-        inside = (np.abs(t) <= 0.5 * ln) & (perp <= r)
+        dz = (zc[lo[0]:hi[0]] - c[0])[:, None, None]
+        dy = (yc[lo[1]:hi[1]] - c[1])[None, :, None]
+        dxx = (xc[lo[2]:hi[2]] - c[2])[None, None, :]
+        t = dz * u[0] + dy * u[1] + dxx * u[2]
+        perp2 = dz * dz + dy * dy + dxx * dxx - t * t
+        inside = (np.abs(t) <= 0.5 * ln) & (perp2 <= r * r)
         sub_e = eps[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
         sub_s = sig[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
         sub_e[inside] = e_c
