@@ -116,6 +116,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-extras', action='store_true',
                     help='skip the batched-vs-plain and serving legs')
+    ap.add_argument('--split', action='store_true',
+                    help='split H / E kernels instead of the fused stepper')
     ap.add_argument('--ty', type=int, default=0)
     ap.add_argument('--zchunk', type=int, default=-1)
     args = ap.parse_args()
@@ -139,8 +141,10 @@ def main():
 
     native.init(0)
     info = native.device_info()
+    fused = not args.split
     sim = native.Sim(common.main_geom(pc), common.coeffs_for(pc))
-    ty, zc = tuning.launch_for(g.size_x, g.size_y, g.size_z)
+    sim.set_stepper(fused)
+    ty, zc = tuning.launch_for(g.size_x, g.size_y, g.size_z, fused=fused)
     if args.ty > 0:
         ty = args.ty
     if args.zchunk >= 0:

@@ -132,11 +132,17 @@ int fdtd_sim_set_field(FdtdSim *s, int comp, const float *in);
 /* CPML state psi 0..23 in the reference's shapes (cpu/fdtd_sim.py:165-187):
  * E d,u: Exz Eyz; l,r: Eyx Ezx; f,b: Exy Ezy; then H likewise. */
 int fdtd_sim_get_psi(FdtdSim *s, int index, float *out);
-/* Tile shape of the fused stencil kernels: ty rows per block (1,2,4,8) and
- * z-chunk length (0 = whole column). */
+/* MAIN only, before the first step: 1 = fused stepper (one kernel computes
+ * H then E per step; fields double-buffered), 0 = split stepper (an H
+ * kernel, then an E kernel, in place).  Both give identical results. */
+int fdtd_sim_set_stepper(FdtdSim *s, int fused);
+/* Tile shape: split stepper ty rows per 64-wide block (1, 2, 4); fused
+ * stepper ty rows per lane (4, 8); zchunk z cells per tile (0 = whole
+ * column). */
 int fdtd_sim_set_launch(FdtdSim *s, int ty, int zchunk);
 /* Run nsteps and report HIP-event times on the library stream: whole run and
- * per-kernel averages (ms).  Any pointer may be NULL. */
+ * per-kernel averages (ms); with the fused stepper ms_h is the fused kernel
+ * and ms_e is 0.  Any pointer may be NULL. */
 int fdtd_sim_time_steps(FdtdSim *s, int nsteps, float *ms_total, float *ms_h,
                         float *ms_e, float *ms_dft);
 

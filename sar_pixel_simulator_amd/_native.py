@@ -46,7 +46,8 @@ EXPORTS = [
     'fdtd_sim_set_source', 'fdtd_sim_attach_beam', 'fdtd_sim_reset',
     'fdtd_sim_run', 'fdtd_sim_sync', 'fdtd_sim_step_count',
     'fdtd_sim_get_phasor', 'fdtd_sim_get_field', 'fdtd_sim_set_field',
-    'fdtd_sim_get_psi', 'fdtd_sim_set_launch', 'fdtd_sim_time_steps',
+    'fdtd_sim_get_psi', 'fdtd_sim_set_stepper', 'fdtd_sim_set_launch',
+    'fdtd_sim_time_steps',
     'fdtd_sim_near2far', 'fdtd_near2far_host',
     'fdtd_beam_create', 'fdtd_beam_upload', 'fdtd_beam_get', 'fdtd_beam_release',
     'fdtd_batch_create', 'fdtd_batch_destroy', 'fdtd_batch_set_launch',
@@ -117,6 +118,7 @@ _SIGS = {
     'fdtd_sim_get_field': ([_P, _I, _FP], _I),
     'fdtd_sim_set_field': ([_P, _I, _FP], _I),
     'fdtd_sim_get_psi': ([_P, _I, _FP], _I),
+    'fdtd_sim_set_stepper': ([_P, _I], _I),
     'fdtd_sim_set_launch': ([_P, _I, _I], _I),
     'fdtd_sim_time_steps': ([_P, _I, _FP, _FP, _FP, _FP], _I),
     'fdtd_sim_near2far': ([_P, _DP, _I, _D, _D, _D, _DP], _I),
@@ -358,6 +360,10 @@ class Sim:
 
     def set_launch(self, ty, zchunk=0):
         _check(lib().fdtd_sim_set_launch(self._h, int(ty), int(zchunk)))
+
+    def set_stepper(self, fused):
+        """True: fused H+E kernel per step; False: split H and E kernels."""
+        _check(lib().fdtd_sim_set_stepper(self._h, 1 if fused else 0))
 
     def time_steps(self, nsteps, per_kernel=True):
         """Run ``nsteps`` and return HIP-event times (ms) measured on the

@@ -17,7 +17,10 @@
 namespace fdtd {
 __global__ void k_h_half(const Desc *, const int4 *, int);
 __global__ void k_e_half(const Desc *, const int4 *, int);
-__global__ void k_dft(const Desc *, int, unsigned int *);
+__global__ void k_dft(const Desc *);
+__global__ void k_advance(const Desc *, int);
+template <int R>
+__global__ void k_step_fused(const Desc *, const int4 *, int);
 __global__ void k_phasor_out(const float4 *, long long, float2 *);
 __global__ void k_n2f_partial(const float2 *, const float2 *, const float2 *, const float2 *, int, int, int,
                               const double2 *, const double2 *, double *);
@@ -91,8 +94,7 @@ struct Group {
     hipStream_t stream = nullptr;
     Desc *d_descs = nullptr;
     int4 *d_tiles = nullptr;
-    unsigned int *d_done = nullptr;
-    int ntiles = 0, nmodels = 0, mode = FDTD_MODE_MAIN;
+    int ntiles = 0, nmodels = 0, mode = FDTD_MODE_MAIN, fused = 0;
     long long max_dft_items = 0;
     Launch launch;
     std::vector<FdtdSim *> members;
@@ -106,7 +108,7 @@ struct FdtdSim {
     FdtdCoeffs c{};
     Desc h{};                    // host copy of the descriptor
     long long pitch = 0, ncell = 0;
-    float *fields = nullptr;     // 6 * ncell
+    float *fields = nullptr;     // 6 * ncell per buffer set (MAIN: 2 sets)
     float *mat = nullptr;        // 2 * ncell (Ca, Cb)
     float *psi = nullptr;        // all 24 arrays
     long long psi_len[24] = {};
@@ -190,9 +192,20 @@ static Sizes sizes_of(const FdtdGeom *g) {
     return s;
 }
 
+// MAIN simulations hold two buffer sets of fields and H-CPML state (the
+// fused stepper ping-pongs between them); INCIDENT ones one set.
+static int nsets_of(const FdtdGeom *g) { return g->mode == FDTD_MODE_MAIN ? 2 : 1; }
+
+static long long psi_h_len(const Sizes &s) {
+    long long n = 0;
+    for (int k = 12; k < 24; ++k) n += s.psi_len[k];
+    return n;
+}
+
 static long long footprint_bytes(const FdtdGeom *g) {
     const Sizes s = sizes_of(g);
-    long long b = 6 * s.ncell * 4 + s.psi_total * 4 + s.acc_total * 16;
+    const int ns = nsets_of(g);
+    long long b = 6 * s.ncell * 4 * ns + (s.psi_total + (ns - 1) * psi_h_len(s)) * 4 + s.acc_total * 16;
     if (g->mode == FDTD_MODE_MAIN) b += 2 * s.ncell * 4;
     b += (long long)(g->nt + 1) * (32 + 32) + 4096;
     return b;
@@ -242,21 +255,28 @@ static int group_build(Group &G) {
     CHECK(!G.members.empty(), "empty group");
     HIP_TRY(hipSetDevice(G.device));
     const int ty = G.launch.ty;
-    CHECK(ty == 1 || ty == 2 || ty == 4, "ty must be 1, 2 or 4 (got %d)", ty);
     std::vector<int4> tiles;
     std::vector<Desc> descs;
     G.mode = G.members[0]->g.mode;
+    G.fused = G.members[0]->h.fused;
+    if (G.fused)
+        CHECK(ty == 4 || ty == 8, "fused stepper: ty (rows per lane) must be 4 or 8 (got %d)", ty);
+    else
+        CHECK(ty == 1 || ty == 2 || ty == 4, "split stepper: ty must be 1, 2 or 4 (got %d)", ty);
+    // fused tiles own 63 columns (lane 0 is the halo), split tiles 64
+    const int tx = G.fused ? kWave - 1 : kWave;
     G.max_dft_items = 0;
     for (size_t m = 0; m < G.members.size(); ++m) {
         FdtdSim *s = G.members[m];
         CHECK(s->g.mode == G.mode, "a group cannot mix MAIN and INCIDENT simulations");
+        CHECK(s->h.fused == G.fused, "a group cannot mix fused and split steppers");
         CHECK(s->device == G.device, "group members must share a device");
         descs.push_back(s->h);
         const int nz = s->g.nz, zc = G.launch.zchunk > 0 ? G.launch.zchunk : nz;
         for (int z0 = 0; z0 < nz; z0 += zc) {
             const int z1 = std::min(nz, z0 + zc);
             for (int y0 = 0; y0 < s->g.ny; y0 += ty)
-                for (int x0 = 0; x0 < s->g.nx; x0 += kWave)
+                for (int x0 = 0; x0 < s->g.nx; x0 += tx)
                     tiles.push_back(make_int4((int)m, x0, y0, z0 | (z1 << 16)));
         }
         G.max_dft_items = std::max(G.max_dft_items, s->h.dft_items);
@@ -268,10 +288,6 @@ static int group_build(Group &G) {
     G.d_tiles = nullptr;
     HIP_TRY(hipMalloc(&G.d_descs, descs.size() * sizeof(Desc)));
     HIP_TRY(hipMalloc(&G.d_tiles, tiles.size() * sizeof(int4)));
-    if (!G.d_done) {
-        HIP_TRY(hipMalloc(&G.d_done, sizeof(unsigned int)));
-        HIP_TRY(hipMemsetAsync(G.d_done, 0, sizeof(unsigned int), G.stream));
-    }
     HIP_TRY(hipMemcpyAsync(G.d_descs, descs.data(), descs.size() * sizeof(Desc), hipMemcpyHostToDevice,
                            G.stream));
     HIP_TRY(hipMemcpyAsync(G.d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice,
@@ -287,28 +303,35 @@ static void group_free(Group &G) {
     group_clear_graphs(G);
     (void)hipFree(G.d_descs);
     (void)hipFree(G.d_tiles);
-    (void)hipFree(G.d_done);
     if (G.stream) (void)hipStreamDestroy(G.stream);
     G = Group();
 }
 
-enum KernelId { KH = 0, KE = 1, KD = 2 };
+enum KernelId { KH = 0, KE = 1, KD = 2, KF = 3 };
 
 static void launch_kernel(Group &G, KernelId k) {
     const dim3 blk(kWave, G.launch.ty);
-    if (k == KH) {
+    if (k == KF) {
+        const dim3 grid((G.ntiles + 3) / 4), wblk(kWave, 4);
+        if (G.launch.ty == 8)
+            hipLaunchKernelGGL(k_step_fused<8>, grid, wblk, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
+        else
+            hipLaunchKernelGGL(k_step_fused<4>, grid, wblk, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
+    } else if (k == KH) {
         hipLaunchKernelGGL(k_h_half, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
     } else if (k == KE) {
         hipLaunchKernelGGL(k_e_half, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
     } else {
         const unsigned gx = (unsigned)std::max(1LL, (G.max_dft_items + 255) / 256);
-        hipLaunchKernelGGL(k_dft, dim3(gx, G.nmodels), dim3(256), 0, G.stream, G.d_descs, G.nmodels,
-                           G.d_done);
+        hipLaunchKernelGGL(k_dft, dim3(gx, G.nmodels), dim3(256), 0, G.stream, G.d_descs);
+        hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, G.stream, G.d_descs, G.nmodels);
     }
 }
 
 static void launch_step(Group &G) {
-    if (G.mode == FDTD_MODE_MAIN) {   // cpu/main_sim.py:140-195: H, E, phasors
+    if (G.mode == FDTD_MODE_MAIN && G.fused) {   // H then E in one kernel, phasors
+        launch_kernel(G, KF);
+    } else if (G.mode == FDTD_MODE_MAIN) {       // cpu/main_sim.py:140-195: H, E, phasors
         launch_kernel(G, KH);
         launch_kernel(G, KE);
     } else {                          // cpu/incident_sim.py:134-161: E, H, record + phasors
@@ -408,9 +431,11 @@ static int sim_alloc(FdtdSim *s) {
     const Sizes z = sizes_of(g);
     s->pitch = z.pitch;
     s->ncell = z.ncell;
-    HIP_TRY(hipMalloc(&s->fields, 6 * z.ncell * sizeof(float)));
+    const int ns = nsets_of(g);
+    const long long psi_alloc = z.psi_total + (ns - 1) * psi_h_len(z);
+    HIP_TRY(hipMalloc(&s->fields, 6 * z.ncell * ns * sizeof(float)));
     if (g->mode == FDTD_MODE_MAIN) HIP_TRY(hipMalloc(&s->mat, 2 * z.ncell * sizeof(float)));
-    HIP_TRY(hipMalloc(&s->psi, z.psi_total * sizeof(float)));
+    HIP_TRY(hipMalloc(&s->psi, psi_alloc * sizeof(float)));
     HIP_TRY(hipMalloc(&s->acc, z.acc_total * sizeof(float4)));
     std::vector<double4> w64;
     std::vector<float4> w32;
@@ -422,7 +447,7 @@ static int sim_alloc(FdtdSim *s) {
     HIP_TRY(hipMalloc(&s->step, sizeof(int)));
     // the field arrays' padding columns are read as x+1 neighbours never,
     // but keep all device memory defined
-    HIP_TRY(hipMemset(s->fields, 0, 6 * z.ncell * sizeof(float)));
+    HIP_TRY(hipMemset(s->fields, 0, 6 * z.ncell * ns * sizeof(float)));
     if (s->mat) HIP_TRY(hipMemset(s->mat, 0, 2 * z.ncell * sizeof(float)));
 
     Desc &d = s->h;
@@ -436,16 +461,28 @@ static int sim_alloc(FdtdSim *s) {
     d.fb_sz = (long long)g->npml * z.pitch;
     d.Db = s->c.Db; d.dtm0 = s->c.dtm0; d.dx = s->c.dx; d.dte0dx = s->c.dte0dx;
     memcpy(d.prof, s->c.prof, sizeof d.prof);
-    for (int k = 0; k < 6; ++k) d.f[k] = s->fields + k * z.ncell;
+    for (int k = 0; k < 6; ++k) {
+        d.f[0][k] = s->fields + k * z.ncell;
+        d.f[1][k] = s->fields + ((ns - 1) * 6 + k) * z.ncell;
+    }
+    d.fused = 0;
     if (s->mat) {
         d.ca = s->mat;
         d.cb = s->mat + z.ncell;
     }
     long long off = 0;
     for (int k = 0; k < 24; ++k) {
-        d.psi[k] = s->psi + off;
+        d.psi[0][k] = s->psi + off;
         s->psi_len[k] = z.psi_len[k];
         off += z.psi_len[k];
+    }
+    for (int k = 0; k < 24; ++k) {
+        if (k < 12 || ns == 1) {
+            d.psi[1][k] = d.psi[0][k];
+        } else {
+            d.psi[1][k] = s->psi + off;
+            off += z.psi_len[k];
+        }
     }
     d.acc = s->acc;
     for (int k = 0; k < 4 * z.nfaces; ++k) {
@@ -666,8 +703,9 @@ extern "C" int fdtd_sim_reset(FdtdSim *s) {
     HIP_TRY(hipSetDevice(s->device));
     const Sizes z = sizes_of(&s->g);
     hipStream_t st = s->self.stream;
-    HIP_TRY(hipMemsetAsync(s->fields, 0, 6 * z.ncell * sizeof(float), st));
-    HIP_TRY(hipMemsetAsync(s->psi, 0, z.psi_total * sizeof(float), st));
+    const int ns = nsets_of(&s->g);
+    HIP_TRY(hipMemsetAsync(s->fields, 0, 6 * z.ncell * ns * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(s->psi, 0, (z.psi_total + (ns - 1) * psi_h_len(z)) * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(s->acc, 0, z.acc_total * sizeof(float4), st));
     HIP_TRY(hipMemsetAsync(s->step, 0, sizeof(int), st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -733,18 +771,33 @@ extern "C" int fdtd_sim_get_phasor(FdtdSim *s, int index, float *out_c64) {
     return FDTD_OK;
 }
 
+// Buffer set holding the current fields (reads the device step counter).
+static int host_current_set(FdtdSim *s, int *set) {
+    int t = 0;
+    int rc = fdtd_sim_step_count(s, &t);
+    if (rc) return rc;
+    *set = current_set(s->h, t);
+    return FDTD_OK;
+}
+
 extern "C" int fdtd_sim_get_field(FdtdSim *s, int comp, float *out) {
     CHECK(s && out, "NULL argument");
     CHECK(comp >= 0 && comp < 6, "field component %d outside [0, 6)", comp);
     HIP_TRY(hipSetDevice(s->device));
-    return copy_volume_out(s, out, s->fields + comp * s->ncell);
+    int set = 0;
+    int rc = host_current_set(s, &set);
+    if (rc) return rc;
+    return copy_volume_out(s, out, s->h.f[set][comp]);
 }
 
 extern "C" int fdtd_sim_set_field(FdtdSim *s, int comp, const float *in) {
     CHECK(s && in, "NULL argument");
     CHECK(comp >= 0 && comp < 6, "field component %d outside [0, 6)", comp);
     HIP_TRY(hipSetDevice(s->device));
-    int rc = copy_volume_in(s, s->fields + comp * s->ncell, in);
+    int set = 0;
+    int rc = host_current_set(s, &set);
+    if (rc) return rc;
+    rc = copy_volume_in(s, s->h.f[set][comp], in);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(s->self.stream));
     return FDTD_OK;
@@ -756,7 +809,10 @@ extern "C" int fdtd_sim_get_psi(FdtdSim *s, int index, float *out) {
     HIP_TRY(hipSetDevice(s->device));
     const int face = (index % 12) / 2;
     const FdtdGeom &g = s->g;
-    const float *src = s->h.psi[index];
+    int set = 0;
+    int rc = host_current_set(s, &set);
+    if (rc) return rc;
+    const float *src = s->h.psi[set][index];
     if (face == 2 || face == 3) {   // l/r: compact (nz, ny, npml)
         HIP_TRY(hipMemcpyAsync(out, src, (size_t)s->psi_len[index] * sizeof(float), hipMemcpyDeviceToHost,
                                s->self.stream));
@@ -765,6 +821,21 @@ extern "C" int fdtd_sim_get_psi(FdtdSim *s, int index, float *out) {
     }
     const size_t rows = (face < 2) ? (size_t)g.npml * g.ny : (size_t)g.nz * g.npml;
     return copy_rows_out(s, out, src, rows);
+}
+
+extern "C" int fdtd_sim_set_stepper(FdtdSim *s, int fused) {
+    CHECK(s, "NULL sim");
+    CHECK(fused == 0 || fused == 1, "fused must be 0 or 1");
+    CHECK(!fused || s->g.mode == FDTD_MODE_MAIN, "the fused stepper runs MAIN simulations");
+    int t = 0;
+    int rc = fdtd_sim_step_count(s, &t);
+    if (rc) return rc;
+    CHECK(t == 0, "the stepper can only change before the first step (reset first)");
+    if (s->h.fused == fused) return FDTD_OK;
+    s->h.fused = fused;
+    s->self.launch = Launch();
+    if (fused) s->self.launch.ty = 8;
+    return sim_refresh(s);
 }
 
 extern "C" int fdtd_sim_set_launch(FdtdSim *s, int ty, int zchunk) {
@@ -792,27 +863,29 @@ static int time_group(Group &G, int nsteps, bool graph, float *ms_total, float *
     if (ms_total) *ms_total = tot;
     if (ms_h || ms_e || ms_dft) {
         // per-kernel: bracket each launch of the same step sequence
-        std::vector<hipEvent_t> evs(6 * (size_t)nsteps);
+        const KernelId order_main[3] = {KH, KE, KD}, order_inc[3] = {KE, KH, KD}, order_fused[2] = {KF, KD};
+        const KernelId *order = G.mode != FDTD_MODE_MAIN ? order_inc : G.fused ? order_fused : order_main;
+        const int nk = (G.mode == FDTD_MODE_MAIN && G.fused) ? 2 : 3;
+        std::vector<hipEvent_t> evs(2 * nk * (size_t)nsteps);
         for (auto &e : evs) HIP_TRY(hipEventCreate(&e));
-        const KernelId order_main[3] = {KH, KE, KD}, order_inc[3] = {KE, KH, KD};
-        const KernelId *order = G.mode == FDTD_MODE_MAIN ? order_main : order_inc;
         for (int s = 0; s < nsteps; ++s)
-            for (int k = 0; k < 3; ++k) {
-                HIP_TRY(hipEventRecord(evs[6 * s + 2 * k], G.stream));
+            for (int k = 0; k < nk; ++k) {
+                HIP_TRY(hipEventRecord(evs[2 * nk * s + 2 * k], G.stream));
                 launch_kernel(G, order[k]);
-                HIP_TRY(hipEventRecord(evs[6 * s + 2 * k + 1], G.stream));
+                HIP_TRY(hipEventRecord(evs[2 * nk * s + 2 * k + 1], G.stream));
             }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventSynchronize(evs.back()));
-        double acc[3] = {0, 0, 0};
+        double acc[4] = {0, 0, 0, 0};
         for (int s = 0; s < nsteps; ++s)
-            for (int k = 0; k < 3; ++k) {
+            for (int k = 0; k < nk; ++k) {
                 float ms = 0;
-                HIP_TRY(hipEventElapsedTime(&ms, evs[6 * s + 2 * k], evs[6 * s + 2 * k + 1]));
+                HIP_TRY(hipEventElapsedTime(&ms, evs[2 * nk * s + 2 * k], evs[2 * nk * s + 2 * k + 1]));
                 acc[order[k]] += ms;
             }
         for (auto &e : evs) (void)hipEventDestroy(e);
-        if (ms_h) *ms_h = (float)(acc[KH] / nsteps);
+        // fused stepper: ms_h reports the fused H+E kernel, ms_e is 0
+        if (ms_h) *ms_h = (float)((acc[KH] + acc[KF]) / nsteps);
         if (ms_e) *ms_e = (float)(acc[KE] / nsteps);
         if (ms_dft) *ms_dft = (float)(acc[KD] / nsteps);
     }

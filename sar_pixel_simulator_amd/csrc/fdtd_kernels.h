@@ -44,15 +44,15 @@ struct alignas(16) Desc {
     int nx, ny, nz, npml;
     int tfsf_z, read_out, nt, nt_inc;
     int mode, dft_accum, uni_z, n_src;   // planes z >= uni_z hold (ca_u, cb_u) everywhere
-    int nt_beam, nfaces, pad0, pad1;
+    int nt_beam, nfaces, fused, pad1;    // fused: the fused stepper advances this sim
     long long sy, sz;              // field strides (elements)
     long long du_sz, lr_sz, fb_sz; // psi plane strides
     float Db, dtm0, dx, dte0dx;
     float ca_u, cb_u, pad2, pad3;
     float prof[8][FDTD_MAX_PML];
-    float *f[6];                   // ex ey ez hx hy hz
+    float *f[2][6];                // ex ey ez hx hy hz; set t&1 current (fused), else set 0
     const float *ca, *cb;
-    float *psi[24];
+    float *psi[2][24];             // E psi: set 0 only (set 1 aliases it); H psi: both sets
     const float *beam;             // MAIN: injected beam
     float *rec;                    // INCIDENT: recorded beam
     long long beam_cs;             // beam component stride = nt_beam * ny * nx
@@ -69,10 +69,16 @@ struct alignas(16) Desc {
     int *step;                     // device step counter
 };
 
-// Launch shape shared by the stencil kernels.
+// Launch shape of the stencil kernels.  Split stepper: blocks of 64 x ty
+// threads, one cell column per thread.  Fused stepper: one wavefront per
+// 63 x ty tile, ty in {4, 8} rows per lane.  zchunk: z cells per tile
+// (0 = whole column).
 struct Launch {
-    int ty = 4;       // rows per block (block = 64 x ty)
-    int zchunk = 0;   // z cells per tile (0 = whole column)
+    int ty = 4;
+    int zchunk = 0;
 };
+
+// Buffer set holding the fields at step t.
+__host__ __device__ inline int current_set(const Desc &d, int t) { return d.fused ? (t & 1) : 0; }
 
 }  // namespace fdtd

@@ -26,15 +26,219 @@ __device__ __forceinline__ int load_step(const int *p) {
 }
 
 // ---------------------------------------------------------------------------
-// H half-step: Yee H update (cpu/field_updates.py:198-240), the six H-CPML
-// corrections in the reference order d,u,l,r,f,b (:243-366,
-// cpu/fdtd_sim.py:104-133) and, in MAIN mode, the E_inc injection at tfsf_z
-// (cpu/main_sim.py:147-150) -- fused per cell.  Every correction of a cell
-// reads E only, so applying them in sequence per cell gives the values of
-// the reference's separate passes exactly.
-//
-// One thread owns an (x, y) column of a tile and marches z upward; ex(z+1)
-// and ey(z+1) loaded at plane z are reused from registers at plane z+1.
+// Per-cell updates shared by the split and the fused steppers.
+// ---------------------------------------------------------------------------
+
+// E values around an H node (x, y, z) at the old time level.
+struct EAround {
+    float ex_c, ex_n, ex_yp;   // ex(z), ex(z+1), ex(y+1)
+    float ey_c, ey_n, ey_xp;   // ey(z), ey(z+1), ey(x+1)
+    float ez_c, ez_yp, ez_xp;  // ez(z), ez(y+1), ez(x+1)
+};
+
+// H half-step of one cell: Yee update (cpu/field_updates.py:198-240), the
+// six H-CPML corrections in the reference order d,u,l,r,f,b (:243-366,
+// cpu/fdtd_sim.py:104-133), then the E_inc injection at tfsf_z in MAIN mode
+// (cpu/main_sim.py:147-150).  Every correction reads E only, so applying
+// them in sequence per cell gives the values of the reference's separate
+// passes exactly.  psi is read from psi_rd and, when psi_wr is not null,
+// the updated value is written there (psi_wr == psi_rd updates in place).
+__device__ __forceinline__ void h_cell(const Desc *__restrict__ d, int x, int y, int z, int t,
+                                       const EAround &e, float &hx, float &hy, float &hz,
+                                       float *const *psi_rd, float *const *psi_wr) {
+    const int nx = d->nx, ny = d->ny, nz = d->nz, np = d->npml;
+    const long long sy = d->sy;
+    const float Db = d->Db, dtm0 = d->dtm0, dx = d->dx;
+    const bool xin = x < nx - 1, yin = y < ny - 1;
+    if (z < nz - 1) {
+        if (x >= 1 && xin && yin) hx -= Db * ((e.ez_yp - e.ez_c) - (e.ey_n - e.ey_c));
+        if (y >= 1 && yin && xin) hy -= Db * ((e.ex_n - e.ex_c) - (e.ez_xp - e.ez_c));
+        if (z >= 1 && yin && xin) hz -= Db * ((e.ey_xp - e.ey_c) - (e.ex_yp - e.ex_c));
+        if (yin && xin) {
+            if (z < np - 1) {                                       // down
+                const float bh = d->prof[BH_DLF][z], ch = d->prof[CH_DLF][z];
+                const long long p = (long long)z * d->du_sz + (long long)y * sy + x;
+                const float a = (bh * psi_rd[pHyz_d][p]) + ((ch * (e.ex_n - e.ex_c)) / dx);
+                const float b = (bh * psi_rd[pHxz_d][p]) + ((ch * (e.ey_n - e.ey_c)) / dx);
+                if (psi_wr) { psi_wr[pHyz_d][p] = a; psi_wr[pHxz_d][p] = b; }
+                hy -= dtm0 * a;
+                hx += dtm0 * b;
+            }
+            if (z >= nz - np) {                                     // up
+                const int zl = z - (nz - np);
+                const float bh = d->prof[BH_URB][zl], ch = d->prof[CH_URB][zl];
+                const long long p = (long long)zl * d->du_sz + (long long)y * sy + x;
+                const float a = (bh * psi_rd[pHyz_u][p]) + ((ch * (e.ex_n - e.ex_c)) / dx);
+                const float b = (bh * psi_rd[pHxz_u][p]) + ((ch * (e.ey_n - e.ey_c)) / dx);
+                if (psi_wr) { psi_wr[pHyz_u][p] = a; psi_wr[pHxz_u][p] = b; }
+                hy -= dtm0 * a;
+                hx += dtm0 * b;
+            }
+        }
+        if (yin) {
+            if (x < np - 1) {                                       // left
+                const float bh = d->prof[BH_DLF][x], ch = d->prof[CH_DLF][x];
+                const long long p = (long long)z * d->lr_sz + (long long)y * np + x;
+                const float a = (bh * psi_rd[pHzx_l][p]) + ((ch * (e.ey_xp - e.ey_c)) / dx);
+                const float b = (bh * psi_rd[pHyx_l][p]) + ((ch * (e.ez_xp - e.ez_c)) / dx);
+                if (psi_wr) { psi_wr[pHzx_l][p] = a; psi_wr[pHyx_l][p] = b; }
+                hz -= dtm0 * a;
+                hy += dtm0 * b;
+            }
+            if (x >= nx - np && xin) {                              // right
+                const int xl = x - (nx - np);
+                const float bh = d->prof[BH_URB][xl], ch = d->prof[CH_URB][xl];
+                const long long p = (long long)z * d->lr_sz + (long long)y * np + xl;
+                const float a = (bh * psi_rd[pHzx_r][p]) + ((ch * (e.ey_xp - e.ey_c)) / dx);
+                const float b = (bh * psi_rd[pHyx_r][p]) + ((ch * (e.ez_xp - e.ez_c)) / dx);
+                if (psi_wr) { psi_wr[pHzx_r][p] = a; psi_wr[pHyx_r][p] = b; }
+                hz -= dtm0 * a;
+                hy += dtm0 * b;
+            }
+        }
+        if (xin) {
+            if (y < np - 1) {                                       // front
+                const float bh = d->prof[BH_DLF][y], ch = d->prof[CH_DLF][y];
+                const long long p = (long long)z * d->fb_sz + (long long)y * sy + x;
+                const float a = (bh * psi_rd[pHxy_f][p]) + ((ch * (e.ez_yp - e.ez_c)) / dx);
+                const float b = (bh * psi_rd[pHzy_f][p]) + ((ch * (e.ex_yp - e.ex_c)) / dx);
+                if (psi_wr) { psi_wr[pHxy_f][p] = a; psi_wr[pHzy_f][p] = b; }
+                hx -= dtm0 * a;
+                hz += dtm0 * b;
+            }
+            if (y >= ny - np && yin) {                              // back
+                const int yl = y - (ny - np);
+                const float bh = d->prof[BH_URB][yl], ch = d->prof[CH_URB][yl];
+                const long long p = (long long)z * d->fb_sz + (long long)yl * sy + x;
+                const float a = (bh * psi_rd[pHxy_b][p]) + ((ch * (e.ez_yp - e.ez_c)) / dx);
+                const float b = (bh * psi_rd[pHzy_b][p]) + ((ch * (e.ex_yp - e.ex_c)) / dx);
+                if (psi_wr) { psi_wr[pHxy_b][p] = a; psi_wr[pHzy_b][p] = b; }
+                hx -= dtm0 * a;
+                hz += dtm0 * b;
+            }
+        }
+    }
+    if (d->mode == FDTD_MODE_MAIN && t < d->nt_inc && z == d->tfsf_z) {
+        const long long q = ((long long)t * ny + y) * nx + x;
+        hx += Db * d->beam[1 * d->beam_cs + q];   // ey_inc
+        hy -= Db * d->beam[0 * d->beam_cs + q];   // ex_inc
+    }
+}
+
+// H values around an E node (x, y, z) at the new time level.
+struct HAround {
+    float hx_c, hy_c, hz_c;     // at the node
+    float hx_zm, hy_zm;         // z-1
+    float hz_ym, hx_ym;         // y-1
+    float hz_xm, hy_xm;         // x-1
+};
+
+// E half-step of one cell: Yee update with per-cell Ca, Cb
+// (cpu/field_updates.py:18-63), the six E-CPML corrections d,u,l,r,f,b
+// (:66-191, psi updated in place), then the H_inc injection at tfsf_z (MAIN,
+// cpu/main_sim.py:158-161) or the hard source at the feed-in plane
+// (INCIDENT, cpu/incident_sim.py:142-144).  o is the cell's field offset.
+__device__ __forceinline__ void e_cell(const Desc *__restrict__ d, int x, int y, int z, int t, long long o,
+                                       const HAround &h, float &ex, float &ey, float &ez) {
+    const int nx = d->nx, ny = d->ny, nz = d->nz, np = d->npml;
+    const long long sy = d->sy;
+    const float dx = d->dx;
+    const bool xlo = x >= 1, ylo = y >= 1, zlo = z >= 1;
+    const bool xin = x < nx - 1, yin = y < ny - 1;
+    float *const *psi = d->psi[0];
+    if (z < nz - 1) {
+        const bool uni = z >= d->uni_z;                        // uniform (e.g. vacuum) plane
+        const float ca = uni ? d->ca_u : d->ca[o];
+        const float cb = uni ? d->cb_u : d->cb[o];
+        if (zlo && ylo && yin && xin) ex = (ex * ca) + (cb * ((h.hz_c - h.hz_ym) - (h.hy_c - h.hy_zm)));
+        if (zlo && yin && xlo && xin) ey = (ey * ca) + (cb * ((h.hx_c - h.hx_zm) - (h.hz_c - h.hz_xm)));
+        if (ylo && yin && xlo && xin) ez = (ez * ca) + (cb * ((h.hy_c - h.hy_xm) - (h.hx_c - h.hx_ym)));
+        const float cbdx = cb * dx;
+        if (yin && xin) {
+            if (zlo && z < np) {                                    // down
+                const float be = d->prof[BE_DLF][z], ce = d->prof[CE_DLF][z];
+                const long long p = (long long)z * d->du_sz + (long long)y * sy + x;
+                const float a = (be * psi[pEyz_d][p]) + ((ce * (h.hx_c - h.hx_zm)) / dx);
+                const float b = (be * psi[pExz_d][p]) + ((ce * (h.hy_c - h.hy_zm)) / dx);
+                psi[pEyz_d][p] = a; psi[pExz_d][p] = b;
+                ey += cbdx * a;
+                ex -= cbdx * b;
+            }
+            if (z >= nz - np) {                                     // up
+                const int zl = z - (nz - np);
+                const float be = d->prof[BE_URB][zl], ce = d->prof[CE_URB][zl];
+                const long long p = (long long)zl * d->du_sz + (long long)y * sy + x;
+                const float a = (be * psi[pEyz_u][p]) + ((ce * (h.hx_c - h.hx_zm)) / dx);
+                const float b = (be * psi[pExz_u][p]) + ((ce * (h.hy_c - h.hy_zm)) / dx);
+                psi[pEyz_u][p] = a; psi[pExz_u][p] = b;
+                ey += cbdx * a;
+                ex -= cbdx * b;
+            }
+        }
+        if (yin) {
+            if (xlo && x < np) {                                    // left
+                const float be = d->prof[BE_DLF][x], ce = d->prof[CE_DLF][x];
+                const long long p = (long long)z * d->lr_sz + (long long)y * np + x;
+                const float a = (be * psi[pEzx_l][p]) + ((ce * (h.hy_c - h.hy_xm)) / dx);
+                const float b = (be * psi[pEyx_l][p]) + ((ce * (h.hz_c - h.hz_xm)) / dx);
+                psi[pEzx_l][p] = a; psi[pEyx_l][p] = b;
+                ez += cbdx * a;
+                ey -= cbdx * b;
+            }
+            if (x >= nx - np && xin) {                              // right
+                const int xl = x - (nx - np);
+                const float be = d->prof[BE_URB][xl], ce = d->prof[CE_URB][xl];
+                const long long p = (long long)z * d->lr_sz + (long long)y * np + xl;
+                const float a = (be * psi[pEzx_r][p]) + ((ce * (h.hy_c - h.hy_xm)) / dx);
+                const float b = (be * psi[pEyx_r][p]) + ((ce * (h.hz_c - h.hz_xm)) / dx);
+                psi[pEzx_r][p] = a; psi[pEyx_r][p] = b;
+                ez += cbdx * a;
+                ey -= cbdx * b;
+            }
+        }
+        if (xin) {
+            if (ylo && y < np) {                                    // front
+                const float be = d->prof[BE_DLF][y], ce = d->prof[CE_DLF][y];
+                const long long p = (long long)z * d->fb_sz + (long long)y * sy + x;
+                const float a = (be * psi[pExy_f][p]) + ((ce * (h.hz_c - h.hz_ym)) / dx);
+                const float b = (be * psi[pEzy_f][p]) + ((ce * (h.hx_c - h.hx_ym)) / dx);
+                psi[pExy_f][p] = a; psi[pEzy_f][p] = b;
+                ex += cbdx * a;
+                ez -= cbdx * b;
+            }
+            if (y >= ny - np && yin) {                              // back
+                const int yl = y - (ny - np);
+                const float be = d->prof[BE_URB][yl], ce = d->prof[CE_URB][yl];
+                const long long p = (long long)z * d->fb_sz + (long long)yl * sy + x;
+                const float a = (be * psi[pExy_b][p]) + ((ce * (h.hz_c - h.hz_ym)) / dx);
+                const float b = (be * psi[pEzy_b][p]) + ((ce * (h.hx_c - h.hx_ym)) / dx);
+                psi[pExy_b][p] = a; psi[pEzy_b][p] = b;
+                ex += cbdx * a;
+                ez -= cbdx * b;
+            }
+        }
+    }
+    if (z == d->tfsf_z) {
+        if (d->mode == FDTD_MODE_MAIN) {
+            if (t < d->nt_inc) {
+                const long long q = ((long long)t * ny + y) * nx + x;
+                ey += d->dte0dx * d->beam[2 * d->beam_cs + q];   // hx_inc
+                ex -= d->dte0dx * d->beam[3 * d->beam_cs + q];   // hy_inc
+            }
+        } else if (t < d->n_src && ylo && yin && xin) {
+            const long long q = ((long long)t * ny + y) * nx + x;
+            ex = d->src[q];
+            ey = d->src[d->src_cs + q];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Split stepper: H half-step kernel, then E half-step kernel, fields updated
+// in place (set 0).  Used for INCIDENT runs (E before H) and available for
+// MAIN runs.  One thread owns an (x, y) column of a 64 x ty tile and marches
+// z upward, reusing the plane above (H half: ex, ey) or below (E half: hx,
+// hy) from registers.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles) {
@@ -42,286 +246,229 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     const Desc *__restrict__ d = descs + tl.x;
     const int t = load_step(d->step);
     if (t >= d->nt) return;
-    const int nx = d->nx, ny = d->ny, nz = d->nz, np = d->npml;
+    const int nx = d->nx, ny = d->ny, nz = d->nz;
     const int x = tl.y + threadIdx.x, y = tl.z + threadIdx.y;
     if (x >= nx || y >= ny) return;
     const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
-
     const long long sy = d->sy, sz = d->sz;
-    const float *__restrict__ Ex = d->f[0];
-    const float *__restrict__ Ey = d->f[1];
-    const float *__restrict__ Ez = d->f[2];
-    float *__restrict__ Hx = d->f[3];
-    float *__restrict__ Hy = d->f[4];
-    float *__restrict__ Hz = d->f[5];
-    const float Db = d->Db, dtm0 = d->dtm0, dx = d->dx;
-
+    const float *__restrict__ Ex = d->f[0][0];
+    const float *__restrict__ Ey = d->f[0][1];
+    const float *__restrict__ Ez = d->f[0][2];
+    float *__restrict__ Hx = d->f[0][3];
+    float *__restrict__ Hy = d->f[0][4];
+    float *__restrict__ Hz = d->f[0][5];
+    float *const *psi = d->psi[0];
     const bool xin = x < nx - 1, yin = y < ny - 1;
-    const bool hx_xy = x >= 1 && xin && yin;
-    const bool hy_xy = y >= 1 && yin && xin;
-    const bool hz_xy = yin && xin;
-    const bool f_l = x < np - 1;
-    const bool f_r = x >= nx - np && xin;
-    const bool f_f = y < np - 1;
-    const bool f_b = y >= ny - np && yin;
-    const bool inject = d->mode == FDTD_MODE_MAIN && t < d->nt_inc;
-    const int tz = d->tfsf_z;
 
     long long o = (long long)z0 * sz + (long long)y * sy + x;
-    float ex_c = Ex[o], ey_c = Ey[o];
+    EAround e;
+    e.ex_c = Ex[o];
+    e.ey_c = Ey[o];
     for (int z = z0; z < z1; ++z, o += sz) {
-        const bool zin = z < nz - 1;
         float hx = Hx[o], hy = Hy[o], hz = Hz[o];
-        float ex_n = 0.f, ey_n = 0.f;
-        if (zin) {
-            ex_n = Ex[o + sz];
-            ey_n = Ey[o + sz];
-            const float ez_c = Ez[o];
-            const float ez_yp = yin ? Ez[o + sy] : 0.f;
-            const float ez_xp = xin ? Ez[o + 1] : 0.f;
-            const float ey_xp = xin ? Ey[o + 1] : 0.f;
-            const float ex_yp = yin ? Ex[o + sy] : 0.f;
-            if (hx_xy) hx -= Db * ((ez_yp - ez_c) - (ey_n - ey_c));
-            if (hy_xy) hy -= Db * ((ex_n - ex_c) - (ez_xp - ez_c));
-            if (z >= 1 && hz_xy) hz -= Db * ((ey_xp - ey_c) - (ex_yp - ex_c));
-            if (hz_xy) {
-                if (z < np - 1) {                                   // down
-                    const float bh = d->prof[BH_DLF][z], ch = d->prof[CH_DLF][z];
-                    const long long p = (long long)z * d->du_sz + (long long)y * sy + x;
-                    float *py = d->psi[pHyz_d], *px = d->psi[pHxz_d];
-                    const float a = (bh * py[p]) + ((ch * (ex_n - ex_c)) / dx);
-                    const float b = (bh * px[p]) + ((ch * (ey_n - ey_c)) / dx);
-                    py[p] = a; px[p] = b;
-                    hy -= dtm0 * a;
-                    hx += dtm0 * b;
-                }
-                if (z >= nz - np) {                                 // up
-                    const int zl = z - (nz - np);
-                    const float bh = d->prof[BH_URB][zl], ch = d->prof[CH_URB][zl];
-                    const long long p = (long long)zl * d->du_sz + (long long)y * sy + x;
-                    float *py = d->psi[pHyz_u], *px = d->psi[pHxz_u];
-                    const float a = (bh * py[p]) + ((ch * (ex_n - ex_c)) / dx);
-                    const float b = (bh * px[p]) + ((ch * (ey_n - ey_c)) / dx);
-                    py[p] = a; px[p] = b;
-                    hy -= dtm0 * a;
-                    hx += dtm0 * b;
-                }
-            }
-            if (yin) {
-                if (f_l) {                                          // left
-                    const float bh = d->prof[BH_DLF][x], ch = d->prof[CH_DLF][x];
-                    const long long p = (long long)z * d->lr_sz + (long long)y * np + x;
-                    float *pz = d->psi[pHzx_l], *py = d->psi[pHyx_l];
-                    const float a = (bh * pz[p]) + ((ch * (ey_xp - ey_c)) / dx);
-                    const float b = (bh * py[p]) + ((ch * (ez_xp - ez_c)) / dx);
-                    pz[p] = a; py[p] = b;
-                    hz -= dtm0 * a;
-                    hy += dtm0 * b;
-                }
-                if (f_r) {                                          // right
-                    const int xl = x - (nx - np);
-                    const float bh = d->prof[BH_URB][xl], ch = d->prof[CH_URB][xl];
-                    const long long p = (long long)z * d->lr_sz + (long long)y * np + xl;
-                    float *pz = d->psi[pHzx_r], *py = d->psi[pHyx_r];
-                    const float a = (bh * pz[p]) + ((ch * (ey_xp - ey_c)) / dx);
-                    const float b = (bh * py[p]) + ((ch * (ez_xp - ez_c)) / dx);
-                    pz[p] = a; py[p] = b;
-                    hz -= dtm0 * a;
-                    hy += dtm0 * b;
-                }
-            }
-            if (xin) {
-                if (f_f) {                                          // front
-                    const float bh = d->prof[BH_DLF][y], ch = d->prof[CH_DLF][y];
-                    const long long p = (long long)z * d->fb_sz + (long long)y * sy + x;
-                    float *pxy = d->psi[pHxy_f], *pzy = d->psi[pHzy_f];
-                    const float a = (bh * pxy[p]) + ((ch * (ez_yp - ez_c)) / dx);
-                    const float b = (bh * pzy[p]) + ((ch * (ex_yp - ex_c)) / dx);
-                    pxy[p] = a; pzy[p] = b;
-                    hx -= dtm0 * a;
-                    hz += dtm0 * b;
-                }
-                if (f_b) {                                          // back
-                    const int yl = y - (ny - np);
-                    const float bh = d->prof[BH_URB][yl], ch = d->prof[CH_URB][yl];
-                    const long long p = (long long)z * d->fb_sz + (long long)yl * sy + x;
-                    float *pxy = d->psi[pHxy_b], *pzy = d->psi[pHzy_b];
-                    const float a = (bh * pxy[p]) + ((ch * (ez_yp - ez_c)) / dx);
-                    const float b = (bh * pzy[p]) + ((ch * (ex_yp - ex_c)) / dx);
-                    pxy[p] = a; pzy[p] = b;
-                    hx -= dtm0 * a;
-                    hz += dtm0 * b;
-                }
-            }
+        e.ex_n = e.ey_n = 0.f;
+        if (z < nz - 1) {
+            e.ex_n = Ex[o + sz];
+            e.ey_n = Ey[o + sz];
+            e.ez_c = Ez[o];
+            e.ez_yp = yin ? Ez[o + sy] : 0.f;
+            e.ez_xp = xin ? Ez[o + 1] : 0.f;
+            e.ey_xp = xin ? Ey[o + 1] : 0.f;
+            e.ex_yp = yin ? Ex[o + sy] : 0.f;
         }
-        if (inject && z == tz) {
-            const long long q = ((long long)t * ny + y) * nx + x;
-            hx += Db * d->beam[1 * d->beam_cs + q];   // ey_inc
-            hy -= Db * d->beam[0 * d->beam_cs + q];   // ex_inc
-        }
+        h_cell(d, x, y, z, t, e, hx, hy, hz, psi, psi);
         Hx[o] = hx;
         Hy[o] = hy;
         Hz[o] = hz;
-        ex_c = ex_n;
-        ey_c = ey_n;
+        e.ex_c = e.ex_n;
+        e.ey_c = e.ey_n;
     }
 }
 
-// ---------------------------------------------------------------------------
-// E half-step: Yee E update with per-cell Ca, Cb (cpu/field_updates.py:18-63),
-// the six E-CPML corrections d,u,l,r,f,b (:66-191) and either the H_inc
-// injection at tfsf_z (MAIN, cpu/main_sim.py:158-161) or the hard source at
-// the feed-in plane (INCIDENT, cpu/incident_sim.py:142-144).  hx(z-1),
-// hy(z-1) carry in registers up the column.
-// ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles) {
     const int4 tl = tiles[xcd_tile(blockIdx.x, ntiles)];
     const Desc *__restrict__ d = descs + tl.x;
     const int t = load_step(d->step);
     if (t >= d->nt) return;
-    const int nx = d->nx, ny = d->ny, nz = d->nz, np = d->npml;
+    const int nx = d->nx, ny = d->ny, nz = d->nz;
     const int x = tl.y + threadIdx.x, y = tl.z + threadIdx.y;
     if (x >= nx || y >= ny) return;
     const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
-
     const long long sy = d->sy, sz = d->sz;
-    float *__restrict__ Ex = d->f[0];
-    float *__restrict__ Ey = d->f[1];
-    float *__restrict__ Ez = d->f[2];
-    const float *__restrict__ Hx = d->f[3];
-    const float *__restrict__ Hy = d->f[4];
-    const float *__restrict__ Hz = d->f[5];
-    const float *__restrict__ Ca = d->ca;
-    const float *__restrict__ Cb = d->cb;
-    const int uni_z = d->uni_z;
-    const float ca_u = d->ca_u, cb_u = d->cb_u, dx = d->dx;
-
+    float *__restrict__ Ex = d->f[0][0];
+    float *__restrict__ Ey = d->f[0][1];
+    float *__restrict__ Ez = d->f[0][2];
+    const float *__restrict__ Hx = d->f[0][3];
+    const float *__restrict__ Hy = d->f[0][4];
+    const float *__restrict__ Hz = d->f[0][5];
     const bool xlo = x >= 1, ylo = y >= 1;
-    const bool xin = x < nx - 1, yin = y < ny - 1;
-    const bool ex_xy = ylo && yin && xin;
-    const bool ey_xy = yin && xlo && xin;
-    const bool ez_xy = ylo && yin && xlo && xin;
-    const bool f_l = xlo && x < np;
-    const bool f_r = x >= nx - np && xin;
-    const bool f_f = ylo && y < np;
-    const bool f_b = y >= ny - np && yin;
-    const bool main_mode = d->mode == FDTD_MODE_MAIN;
-    const bool inject = main_mode && t < d->nt_inc;
-    const bool hard = !main_mode && t < d->n_src && ylo && yin && xin;
-    const int tz = d->tfsf_z;
 
     long long o = (long long)z0 * sz + (long long)y * sy + x;
-    float hx_m = 0.f, hy_m = 0.f;
+    HAround h;
+    h.hx_zm = h.hy_zm = 0.f;
     if (z0 >= 1) {
-        hx_m = Hx[o - sz];
-        hy_m = Hy[o - sz];
+        h.hx_zm = Hx[o - sz];
+        h.hy_zm = Hy[o - sz];
     }
     for (int z = z0; z < z1; ++z, o += sz) {
-        const float hx_c = Hx[o], hy_c = Hy[o];
+        h.hx_c = Hx[o];
+        h.hy_c = Hy[o];
         float ex = Ex[o], ey = Ey[o], ez = Ez[o];
         if (z < nz - 1) {
-            const float hz_c = Hz[o];
-            const float hz_ym = ylo ? Hz[o - sy] : 0.f;
-            const float hz_xm = xlo ? Hz[o - 1] : 0.f;
-            const float hx_ym = ylo ? Hx[o - sy] : 0.f;
-            const float hy_xm = xlo ? Hy[o - 1] : 0.f;
-            const bool uni = z >= uni_z;                  // uniform (e.g. vacuum) plane
-            const float ca = uni ? ca_u : Ca[o];
-            const float cb = uni ? cb_u : Cb[o];
-            const bool zlo = z >= 1;
-            if (zlo && ex_xy) ex = (ex * ca) + (cb * ((hz_c - hz_ym) - (hy_c - hy_m)));
-            if (zlo && ey_xy) ey = (ey * ca) + (cb * ((hx_c - hx_m) - (hz_c - hz_xm)));
-            if (ez_xy) ez = (ez * ca) + (cb * ((hy_c - hy_xm) - (hx_c - hx_ym)));
-            const float cbdx = cb * dx;
-            if (yin && xin) {
-                if (zlo && z < np) {                                // down
-                    const float be = d->prof[BE_DLF][z], ce = d->prof[CE_DLF][z];
-                    const long long p = (long long)z * d->du_sz + (long long)y * sy + x;
-                    float *pey = d->psi[pEyz_d], *pex = d->psi[pExz_d];
-                    const float a = (be * pey[p]) + ((ce * (hx_c - hx_m)) / dx);
-                    const float b = (be * pex[p]) + ((ce * (hy_c - hy_m)) / dx);
-                    pey[p] = a; pex[p] = b;
-                    ey += cbdx * a;
-                    ex -= cbdx * b;
-                }
-                if (z >= nz - np) {                                 // up
-                    const int zl = z - (nz - np);
-                    const float be = d->prof[BE_URB][zl], ce = d->prof[CE_URB][zl];
-                    const long long p = (long long)zl * d->du_sz + (long long)y * sy + x;
-                    float *pey = d->psi[pEyz_u], *pex = d->psi[pExz_u];
-                    const float a = (be * pey[p]) + ((ce * (hx_c - hx_m)) / dx);
-                    const float b = (be * pex[p]) + ((ce * (hy_c - hy_m)) / dx);
-                    pey[p] = a; pex[p] = b;
-                    ey += cbdx * a;
-                    ex -= cbdx * b;
-                }
-            }
-            if (yin) {
-                if (f_l) {                                          // left
-                    const float be = d->prof[BE_DLF][x], ce = d->prof[CE_DLF][x];
-                    const long long p = (long long)z * d->lr_sz + (long long)y * np + x;
-                    float *pez = d->psi[pEzx_l], *pey = d->psi[pEyx_l];
-                    const float a = (be * pez[p]) + ((ce * (hy_c - hy_xm)) / dx);
-                    const float b = (be * pey[p]) + ((ce * (hz_c - hz_xm)) / dx);
-                    pez[p] = a; pey[p] = b;
-                    ez += cbdx * a;
-                    ey -= cbdx * b;
-                }
-                if (f_r) {                                          // right
-                    const int xl = x - (nx - np);
-                    const float be = d->prof[BE_URB][xl], ce = d->prof[CE_URB][xl];
-                    const long long p = (long long)z * d->lr_sz + (long long)y * np + xl;
-                    float *pez = d->psi[pEzx_r], *pey = d->psi[pEyx_r];
-                    const float a = (be * pez[p]) + ((ce * (hy_c - hy_xm)) / dx);
-                    const float b = (be * pey[p]) + ((ce * (hz_c - hz_xm)) / dx);
-                    pez[p] = a; pey[p] = b;
-                    ez += cbdx * a;
-                    ey -= cbdx * b;
-                }
-            }
-            if (xin) {
-                if (f_f) {                                          // front
-                    const float be = d->prof[BE_DLF][y], ce = d->prof[CE_DLF][y];
-                    const long long p = (long long)z * d->fb_sz + (long long)y * sy + x;
-                    float *pexy = d->psi[pExy_f], *pezy = d->psi[pEzy_f];
-                    const float a = (be * pexy[p]) + ((ce * (hz_c - hz_ym)) / dx);
-                    const float b = (be * pezy[p]) + ((ce * (hx_c - hx_ym)) / dx);
-                    pexy[p] = a; pezy[p] = b;
-                    ex += cbdx * a;
-                    ez -= cbdx * b;
-                }
-                if (f_b) {                                          // back
-                    const int yl = y - (ny - np);
-                    const float be = d->prof[BE_URB][yl], ce = d->prof[CE_URB][yl];
-                    const long long p = (long long)z * d->fb_sz + (long long)yl * sy + x;
-                    float *pexy = d->psi[pExy_b], *pezy = d->psi[pEzy_b];
-                    const float a = (be * pexy[p]) + ((ce * (hz_c - hz_ym)) / dx);
-                    const float b = (be * pezy[p]) + ((ce * (hx_c - hx_ym)) / dx);
-                    pexy[p] = a; pezy[p] = b;
-                    ex += cbdx * a;
-                    ez -= cbdx * b;
-                }
-            }
+            h.hz_c = Hz[o];
+            h.hz_ym = ylo ? Hz[o - sy] : 0.f;
+            h.hz_xm = xlo ? Hz[o - 1] : 0.f;
+            h.hx_ym = ylo ? Hx[o - sy] : 0.f;
+            h.hy_xm = xlo ? Hy[o - 1] : 0.f;
         }
-        if (z == tz) {
-            if (inject) {
-                const long long q = ((long long)t * ny + y) * nx + x;
-                ey += d->dte0dx * d->beam[2 * d->beam_cs + q];   // hx_inc
-                ex -= d->dte0dx * d->beam[3 * d->beam_cs + q];   // hy_inc
-            } else if (hard) {
-                const long long q = ((long long)t * ny + y) * nx + x;
-                ex = d->src[q];
-                ey = d->src[d->src_cs + q];
-            }
-        }
+        e_cell(d, x, y, z, t, o, h, ex, ey, ez);
         Ex[o] = ex;
         Ey[o] = ey;
         Ez[o] = ez;
-        hx_m = hx_c;
-        hy_m = hy_c;
+        h.hx_zm = h.hx_c;
+        h.hy_zm = h.hy_c;
     }
 }
+
+// ---------------------------------------------------------------------------
+// Fused stepper (MAIN): one kernel per time step computes H(t+1/2) and then
+// E(t+1) plane by plane, so each field value is read once and written once
+// per step (56 B/cell instead of the split stepper's 80).  Fields and H-CPML
+// state are double-buffered: step t reads set t&1 and writes set 1-(t&1).
+//
+// A wavefront owns a tile of 63 x R cells: lane 0 and row 0 are a one-cell
+// halo whose H values (needed by E at x-1, y-1) are recomputed from the old
+// level instead of being exchanged; the owner tile computes the same values
+// and is the only one that writes them.  Each lane carries R+1 rows in
+// registers and marches z upward: E(z+1) read at plane z is E(z) at plane
+// z+1, H(z) computed at plane z is H(z-1) at plane z+1.  x+1 / x-1
+// neighbours come from the adjacent lane (wavefront shuffles), y+1 / y-1
+// neighbours from the adjacent row in registers.  A tile starting at z0 > 0
+// recomputes H at z0-1 (not stored) for its first E plane.
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ void __launch_bounds__(256)
+k_step_fused(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles) {
+    const int nblk = (ntiles + 3) / 4;
+    const int tile = xcd_tile(blockIdx.x, nblk) * 4 + (int)threadIdx.y;
+    if (tile >= ntiles) return;                        // uniform per wavefront
+    const int4 tl = tiles[tile];
+    const Desc *__restrict__ d = descs + tl.x;
+    const int t = load_step(d->step);
+    if (t >= d->nt) return;
+    const int nx = d->nx, ny = d->ny, nz = d->nz;
+    const int lx = threadIdx.x;
+    const int x = tl.y - 1 + lx;                       // lane 0: halo column x0-1
+    const int y0 = tl.z;                               // rows y0-1 (halo) .. y0+R-1
+    const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
+    const int p = t & 1;
+    const long long sy = d->sy, sz = d->sz;
+    const float *__restrict__ Exo = d->f[p][0];
+    const float *__restrict__ Eyo = d->f[p][1];
+    const float *__restrict__ Ezo = d->f[p][2];
+    const float *__restrict__ Hxo = d->f[p][3];
+    const float *__restrict__ Hyo = d->f[p][4];
+    const float *__restrict__ Hzo = d->f[p][5];
+    float *__restrict__ Exn = d->f[1 - p][0];
+    float *__restrict__ Eyn = d->f[1 - p][1];
+    float *__restrict__ Ezn = d->f[1 - p][2];
+    float *__restrict__ Hxn = d->f[1 - p][3];
+    float *__restrict__ Hyn = d->f[1 - p][4];
+    float *__restrict__ Hzn = d->f[1 - p][5];
+    float *const *psi_rd = d->psi[p];
+    float *const *psi_wr = d->psi[1 - p];
+    const bool vx = x >= 0 && x < nx;
+    const bool ox = lx >= 1 && vx;
+    const bool last_lane = lx == kWave - 1;
+    const int zs = z0 > 0 ? z0 - 1 : 0;
+
+    float exc[R + 1], eyc[R + 1], hxm[R + 1], hym[R + 1];
+#pragma unroll
+    for (int r = 0; r <= R; ++r) {
+        const int y = y0 - 1 + r;
+        const bool ok = vx && y >= 0 && y < ny;
+        const long long o = (long long)zs * sz + (long long)y * sy + x;
+        exc[r] = ok ? Exo[o] : 0.f;
+        eyc[r] = ok ? Eyo[o] : 0.f;
+        hxm[r] = 0.f;
+        hym[r] = 0.f;
+    }
+    for (int z = zs; z < z1; ++z) {
+        const bool zin = z < nz - 1;
+        const bool store = z >= z0;
+        float ezr[R + 2];
+#pragma unroll
+        for (int r = 0; r <= R + 1; ++r) {
+            const int y = y0 - 1 + r;
+            const bool ok = vx && y >= 0 && y < ny;
+            ezr[r] = ok ? Ezo[(long long)z * sz + (long long)y * sy + x] : 0.f;
+        }
+        float ex_extra;
+        {
+            const int y = y0 + R;
+            ex_extra = (vx && y < ny) ? Exo[(long long)z * sz + (long long)y * sy + x] : 0.f;
+        }
+        float hz_prev = 0.f, hx_prev = 0.f;
+#pragma unroll
+        for (int r = 0; r <= R; ++r) {
+            const int y = y0 - 1 + r;
+            const bool ok = vx && y >= 0 && y < ny;
+            const long long o = (long long)z * sz + (long long)y * sy + x;
+            EAround e;
+            e.ex_c = exc[r];
+            e.ey_c = eyc[r];
+            e.ez_c = ezr[r];
+            e.ex_n = (ok && zin) ? Exo[o + sz] : 0.f;
+            e.ey_n = (ok && zin) ? Eyo[o + sz] : 0.f;
+            e.ex_yp = (r < R) ? exc[r + 1] : ex_extra;
+            e.ez_yp = ezr[r + 1];
+            e.ey_xp = __shfl_down(e.ey_c, 1, kWave);
+            e.ez_xp = __shfl_down(e.ez_c, 1, kWave);
+            if (last_lane) {                               // x+1 belongs to the next tile
+                const bool okx = ok && x + 1 < nx;
+                e.ey_xp = okx ? Eyo[o + 1] : 0.f;
+                e.ez_xp = okx ? Ezo[o + 1] : 0.f;
+            }
+            float hx = 0.f, hy = 0.f, hz = 0.f;
+            const bool own = ox && r >= 1 && y < ny;
+            if (ok) {
+                hx = Hxo[o];
+                hy = Hyo[o];
+                hz = Hzo[o];
+                h_cell(d, x, y, z, t, e, hx, hy, hz, psi_rd, (own && store) ? psi_wr : nullptr);
+            }
+            const float hz_xm = __shfl_up(hz, 1, kWave);
+            const float hy_xm = __shfl_up(hy, 1, kWave);
+            if (r >= 1) {
+                if (own && store) {
+                    HAround h;
+                    h.hx_c = hx; h.hy_c = hy; h.hz_c = hz;
+                    h.hx_zm = hxm[r]; h.hy_zm = hym[r];
+                    h.hz_ym = hz_prev; h.hx_ym = hx_prev;
+                    h.hz_xm = hz_xm; h.hy_xm = hy_xm;
+                    float ex = e.ex_c, ey = e.ey_c, ez = e.ez_c;
+                    e_cell(d, x, y, z, t, o, h, ex, ey, ez);
+                    Exn[o] = ex;
+                    Eyn[o] = ey;
+                    Ezn[o] = ez;
+                    Hxn[o] = hx;
+                    Hyn[o] = hy;
+                    Hzn[o] = hz;
+                }
+                hxm[r] = hx;
+                hym[r] = hy;
+            }
+            hz_prev = hz;
+            hx_prev = hx;
+            exc[r] = e.ex_n;
+            eyc[r] = e.ey_n;
+        }
+    }
+}
+
+template __global__ void k_step_fused<4>(const Desc *, const int4 *, int);
+template __global__ void k_step_fused<8>(const Desc *, const int4 *, int);
 
 // ---------------------------------------------------------------------------
 // Running single-frequency DFT on the phasor planes
@@ -336,8 +483,7 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
 // averaging factor) and added with TwoSum, so the accumulated value carries
 // ~44 significant bits in float32 arithmetic only.
 //
-// The last block to finish advances every member's step counter, after all
-// blocks have read it; the next launch in the stream sees the new value.
+// k_advance then moves every member's step counter to the next step.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void acc_ref(float4 &a, float s, double half, double c, double sn) {
     const double avg = half * (double)s;
@@ -364,100 +510,97 @@ __device__ __forceinline__ void acc_comp(float4 &a, float s, float4 w) {
 }
 
 __global__ void __launch_bounds__(256)
-k_dft(const Desc *__restrict__ descs, int nmodels, unsigned int *done) {
+k_dft(const Desc *__restrict__ descs) {
     const Desc *__restrict__ d = descs + blockIdx.y;
     const int t = load_step(d->step);
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < d->nt && i < d->dft_items) {
-        const long long nsamp = d->face_off[d->nfaces];
-        const long long sy = d->sy, sz = d->sz;
-        const float *__restrict__ Ex = d->f[0];
-        const float *__restrict__ Ey = d->f[1];
-        const float *__restrict__ Ez = d->f[2];
-        const float *__restrict__ Hx = d->f[3];
-        const float *__restrict__ Hy = d->f[4];
-        const float *__restrict__ Hz = d->f[5];
-        if (i < nsamp) {
-            int f = 0;
-            while (f + 1 < d->nfaces && i >= d->face_off[f + 1]) ++f;
-            const long long j = i - d->face_off[f];
-            const int nb = d->face_nb[f];
-            const int a = (int)(j / nb), b = (int)(j % nb);
-            if (a >= 1 && b >= 1) {
-                const int np = d->npml, pos = d->face_pos[f];
-                float s[4];
-                switch (d->face_kind[f]) {
-                case kFaceZ: {   // (a, b) = (y, x); ex ey hx hy
-                    const long long o = (long long)pos * sz + (long long)(np + a) * sy + (np + b);
-                    s[0] = Ex[o] + Ex[o - 1];
-                    s[1] = Ey[o] + Ey[o - sy];
-                    s[2] = Hx[o] + Hx[o - sy] + Hx[o - sz] + Hx[o - sz - sy];
-                    s[3] = Hy[o] + Hy[o - 1] + Hy[o - sz] + Hy[o - sz - 1];
-                    break;
-                }
-                case kFaceX: {   // (a, b) = (z, y); ey ez hy hz
-                    const long long o = (long long)(np + a) * sz + (long long)(np + b) * sy + pos;
-                    s[0] = Ey[o] + Ey[o - sy];
-                    s[1] = Ez[o] + Ez[o - sz];
-                    s[2] = Hy[o] + Hy[o - 1] + Hy[o - sz] + Hy[o - sz - 1];
-                    s[3] = Hz[o] + Hz[o - 1] + Hz[o - sy] + Hz[o - sy - 1];
-                    break;
-                }
-                default: {       // kFaceY, (a, b) = (z, x); ex ez hx hz
-                    const long long o = (long long)(np + a) * sz + (long long)pos * sy + (np + b);
-                    s[0] = Ex[o] + Ex[o - 1];
-                    s[1] = Ez[o] + Ez[o - sz];
-                    s[2] = Hx[o] + Hx[o - sy] + Hx[o - sz] + Hx[o - sz - sy];
-                    s[3] = Hz[o] + Hz[o - sy] + Hz[o - 1] + Hz[o - sy - 1];
-                    break;
-                }
-                }
-                float4 *acc = d->acc;
-                if (d->dft_accum == FDTD_DFT_REFERENCE) {
-                    const double4 w = d->tw64[t];
-                    for (int c = 0; c < 4; ++c) {
-                        float4 &v = acc[d->acc_off[4 * f + c] + j];
-                        float4 r = v;
-                        if (c < 2) acc_ref(r, s[c], 0.5, w.x, w.y);
-                        else acc_ref(r, s[c], 0.25, w.z, w.w);
-                        v = r;
-                    }
-                } else {
-                    const float4 we = d->tw32[2 * t], wh = d->tw32[2 * t + 1];
-                    for (int c = 0; c < 4; ++c) {
-                        float4 &v = acc[d->acc_off[4 * f + c] + j];
-                        float4 r = v;
-                        acc_comp(r, s[c], c < 2 ? we : wh);
-                        v = r;
-                    }
-                }
-            }
-        } else if (d->rec) {
-            // INCIDENT: record the read-out plane of ex, ey, hx, hy at step t
-            // (cpu/incident_sim.py:153-156).
-            const long long k = i - nsamp;
-            const long long plane = (long long)d->ny * d->nx;
-            const int c = (int)(k / plane);
-            const long long r = k % plane;
-            const int yy = (int)(r / d->nx), xx = (int)(r % d->nx);
-            const int src_comp[4] = {0, 1, 3, 4};
-            const float v = d->f[src_comp[c]][(long long)d->read_out * sz + (long long)yy * sy + xx];
-            d->rec[c * d->beam_cs + (long long)t * plane + r] = v;
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= d->nt || i >= d->dft_items) return;
+    const int nsamp = (int)d->face_off[d->nfaces];
+    const long long sy = d->sy, sz = d->sz;
+    // the step's kernels have written set current_set(t + 1)
+    const int cs = current_set(*d, t + 1);
+    const float *__restrict__ Ex = d->f[cs][0];
+    const float *__restrict__ Ey = d->f[cs][1];
+    const float *__restrict__ Ez = d->f[cs][2];
+    const float *__restrict__ Hx = d->f[cs][3];
+    const float *__restrict__ Hy = d->f[cs][4];
+    const float *__restrict__ Hz = d->f[cs][5];
+    if (i < nsamp) {
+        int f = 0;
+        while (f + 1 < d->nfaces && i >= (int)d->face_off[f + 1]) ++f;
+        const int j = i - (int)d->face_off[f];
+        const int nb = d->face_nb[f];
+        const int a = j / nb, b = j - a * nb;
+        if (a < 1 || b < 1) return;
+        const int np = d->npml, pos = d->face_pos[f];
+        float s[4];
+        switch (d->face_kind[f]) {
+        case kFaceZ: {   // (a, b) = (y, x); ex ey hx hy
+            const long long o = (long long)pos * sz + (long long)(np + a) * sy + (np + b);
+            s[0] = Ex[o] + Ex[o - 1];
+            s[1] = Ey[o] + Ey[o - sy];
+            s[2] = Hx[o] + Hx[o - sy] + Hx[o - sz] + Hx[o - sz - sy];
+            s[3] = Hy[o] + Hy[o - 1] + Hy[o - sz] + Hy[o - sz - 1];
+            break;
         }
+        case kFaceX: {   // (a, b) = (z, y); ey ez hy hz
+            const long long o = (long long)(np + a) * sz + (long long)(np + b) * sy + pos;
+            s[0] = Ey[o] + Ey[o - sy];
+            s[1] = Ez[o] + Ez[o - sz];
+            s[2] = Hy[o] + Hy[o - 1] + Hy[o - sz] + Hy[o - sz - 1];
+            s[3] = Hz[o] + Hz[o - 1] + Hz[o - sy] + Hz[o - sy - 1];
+            break;
+        }
+        default: {       // kFaceY, (a, b) = (z, x); ex ez hx hz
+            const long long o = (long long)(np + a) * sz + (long long)pos * sy + (np + b);
+            s[0] = Ex[o] + Ex[o - 1];
+            s[1] = Ez[o] + Ez[o - sz];
+            s[2] = Hx[o] + Hx[o - sy] + Hx[o - sz] + Hx[o - sz - sy];
+            s[3] = Hz[o] + Hz[o - sy] + Hz[o - 1] + Hz[o - sy - 1];
+            break;
+        }
+        }
+        float4 *acc = d->acc;
+        if (d->dft_accum == FDTD_DFT_REFERENCE) {
+            const double4 w = d->tw64[t];
+            for (int c = 0; c < 4; ++c) {
+                float4 &v = acc[d->acc_off[4 * f + c] + j];
+                float4 r = v;
+                if (c < 2) acc_ref(r, s[c], 0.5, w.x, w.y);
+                else acc_ref(r, s[c], 0.25, w.z, w.w);
+                v = r;
+            }
+        } else {
+            const float4 we = d->tw32[2 * t], wh = d->tw32[2 * t + 1];
+            for (int c = 0; c < 4; ++c) {
+                float4 &v = acc[d->acc_off[4 * f + c] + j];
+                float4 r = v;
+                acc_comp(r, s[c], c < 2 ? we : wh);
+                v = r;
+            }
+        }
+    } else if (d->rec) {
+        // INCIDENT: record the read-out plane of ex, ey, hx, hy at step t
+        // (cpu/incident_sim.py:153-156).
+        const int k = i - nsamp;
+        const int plane = d->ny * d->nx;
+        const int c = k / plane;
+        const int r = k - c * plane;
+        const int yy = r / d->nx, xx = r - yy * d->nx;
+        const int src_comp[4] = {0, 1, 3, 4};
+        const float v = d->f[cs][src_comp[c]][(long long)d->read_out * sz + (long long)yy * sy + xx];
+        d->rec[c * d->beam_cs + (long long)t * plane + r] = v;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned int total = gridDim.x * gridDim.y;
-        const unsigned int prev = atomicAdd(done, 1u);
-        if (prev == total - 1) {
-            for (int m = 0; m < nmodels; ++m) {
-                int *sp = descs[m].step;
-                if (load_step(sp) < descs[m].nt) atomicAdd(sp, 1);
-            }
-            atomicExch(done, 0u);
-            __threadfence();
-        }
+}
+
+// Advance every member's step counter after the step's kernels (a separate
+// launch, so no kernel of the step reads a counter another has changed).
+__global__ void __launch_bounds__(64)
+k_advance(const Desc *__restrict__ descs, int nmodels) {
+    for (int m = threadIdx.x; m < nmodels; m += blockDim.x) {
+        int *sp = descs[m].step;
+        const int t = load_step(sp);
+        if (t < descs[m].nt) __hip_atomic_store(sp, t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 

@@ -76,7 +76,7 @@ class DeviceModel:
         g = params.geom_params
         self.nt = g.size_t
         self.geom = common.main_geom(params)
-        self.sim = native.Sim(self.geom, common.coeffs_for(params))
+        self.sim = common.new_main_sim(params)
         self.beam, self.inc_phasors, self.p_inc = (None, None, None)
         if g.size_t_inc > 0:
             self.beam, self.inc_phasors, self.p_inc = beams.get(params)
@@ -126,6 +126,8 @@ class DeviceBatch:
     def __init__(self, models, launch=None):
         self.models = list(models)
         self.batch = native.Batch([m.sim for m in self.models])
+        if launch is None and common.FUSED_STEPPER:
+            launch = (8, 0)
         if launch:
             self.batch.set_launch(*launch)
 

@@ -9,6 +9,22 @@ from ..parameters import ParameterCollection
 DFT_MODES = {'compensated': native.DFT_COMPENSATED,
              'reference': native.DFT_REFERENCE}
 
+#: Stepper of MAIN simulations: the fused H+E kernel (one pass over the
+#: fields per step).  The split H / E kernels give identical results.
+FUSED_STEPPER = True
+
+
+def new_main_sim(params, absorbers=None, fused=None):
+    """A native.Sim for the main simulation of ``params`` with the default
+    stepper and the tuned launch shape for its grid."""
+    from . import tuning
+    g = params.geom_params
+    sim = native.Sim(main_geom(params), coeffs_for(params, absorbers))
+    fused = FUSED_STEPPER if fused is None else fused
+    sim.set_stepper(fused)
+    sim.set_launch(*tuning.launch_for(g.size_x, g.size_y, g.size_z, fused=fused))
+    return sim
+
 
 def require_float32(params: ParameterCollection):
     dt = params.processing_params.datatype

@@ -71,7 +71,7 @@ class MainSimHip:
         self.ny_ph = g.size_y - 2 * g.size_pml
         self.nz_ph = g.size_z - 2 * g.size_pml
         self.geom = common.main_geom(params)
-        self.sim = native.Sim(self.geom, common.coeffs_for(params, self.absorbers))
+        self.sim = common.new_main_sim(params, self.absorbers)
         self._beam = None
 
     def _attach_beam(self):

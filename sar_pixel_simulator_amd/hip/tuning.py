@@ -1,20 +1,25 @@
-"""Tile-shape selection for the fused stencil kernels.
+"""Tile-shape selection for the stencil kernels.
 
-A launch shape is (ty, zchunk): blocks of 64 x ty threads, each owning a
-64 x ty column tile, marching zchunk cells in z (0 = whole column).  The
-table below holds measured choices per shape class; ``autotune`` measures
-candidates on the device and records the best in a JSON cache keyed by
-device arch and grid dims.
+A launch shape is (ty, zchunk).  Split stepper: blocks of 64 x ty threads,
+one cell column per thread, ty in {1, 2, 4}.  Fused stepper: one wavefront
+per 63 x ty tile with ty rows per lane, ty in {4, 8}.  zchunk: z cells per
+tile (0 = whole column).  ``autotune`` measures candidates on the device and
+records the best per (stepper, arch, grid) in a JSON cache; ``launch_for``
+reads it, falling back to a default.
 """
 import json
 import os
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'tuning_cache.json')
-CANDIDATES = [(1, 0), (2, 0), (4, 0), (1, 64), (2, 64), (4, 64), (4, 32), (2, 32)]
+CANDIDATES = {
+    'fused': [(8, 0), (4, 0), (8, 64), (8, 48), (8, 32), (8, 24), (4, 64), (4, 32)],
+    'split': [(1, 0), (2, 0), (4, 0), (1, 64), (2, 64), (4, 64), (4, 32), (2, 32)],
+}
+DEFAULT = {'fused': (8, 0), 'split': (4, 0)}
 
 
-def _key(nx, ny, nz, arch='gfx950'):
-    return f'{arch}:{nx}x{ny}x{nz}'
+def _key(nx, ny, nz, fused, arch='gfx950'):
+    return f'{"fused" if fused else "split"}:{arch}:{nx}x{ny}x{nz}'
 
 
 def _load():
@@ -25,19 +30,20 @@ def _load():
         return {}
 
 
-def launch_for(nx, ny, nz, arch='gfx950'):
-    """Cached choice for this grid, else the default (ty=4, whole column)."""
-    hit = _load().get(_key(nx, ny, nz, arch))
+def launch_for(nx, ny, nz, fused=True, arch='gfx950'):
+    """Cached choice for this grid, else the stepper's default."""
+    hit = _load().get(_key(nx, ny, nz, fused, arch))
     if hit:
         return int(hit['ty']), int(hit['zchunk'])
-    return 4, 0
+    return DEFAULT['fused' if fused else 'split']
 
 
-def autotune(sim, nx, ny, nz, arch='gfx950', steps=10, candidates=CANDIDATES, save=True):
-    """Time each candidate shape on ``sim`` (a ready native.Sim) and keep the
-    fastest.  Returns ((ty, zchunk), {shape: ms_per_step})."""
+def autotune(sim, nx, ny, nz, fused=True, arch='gfx950', steps=10, candidates=None, save=True):
+    """Time each candidate shape on ``sim`` (a ready native.Sim whose stepper
+    is already chosen) and keep the fastest.  Returns ((ty, zchunk),
+    {shape: ms_per_step})."""
     results = {}
-    for ty, zc in candidates:
+    for ty, zc in candidates or CANDIDATES['fused' if fused else 'split']:
         sim.set_launch(ty, zc)
         sim.reset()
         sim.run(2)
@@ -48,8 +54,8 @@ def autotune(sim, nx, ny, nz, arch='gfx950', steps=10, candidates=CANDIDATES, sa
     sim.set_launch(*best)
     if save:
         data = _load()
-        data[_key(nx, ny, nz, arch)] = {'ty': best[0], 'zchunk': best[1],
-                                        'ms_per_step': results[best]}
+        data[_key(nx, ny, nz, fused, arch)] = {'ty': best[0], 'zchunk': best[1],
+                                               'ms_per_step': results[best]}
         tmp = CACHE + '.tmp'
         with open(tmp, 'w') as f:
             json.dump(data, f, indent=1, sort_keys=True)
@@ -57,7 +63,7 @@ def autotune(sim, nx, ny, nz, arch='gfx950', steps=10, candidates=CANDIDATES, sa
     return best, results
 
 
-def _tune_workload(name):
+def _tune_workload(name, fused):
     """Build a ready simulation of a named workload with uniform-value inputs
     (timing does not depend on the values) and autotune it."""
     import numpy as np
@@ -67,13 +73,14 @@ def _tune_workload(name):
     g = pc.geom_params
     native.init(0)
     sim = native.Sim(common.main_geom(pc), common.coeffs_for(pc))
+    sim.set_stepper(fused)
     beam = native.Beam(g.size_t_inc, g.size_y, g.size_x)
     beam.upload(0, *workloads.synthetic_beam_planes(pc, 0, 16))
     sim.attach_beam(beam)
     shape = (g.size_z, g.size_y, g.size_x)
     sim.set_materials(np.ones(shape, np.float32),
                       np.full(shape, common.update_scalars(pc)['dte0dx'], np.float32))
-    best, res = autotune(sim, g.size_x, g.size_y, g.size_z)
+    best, res = autotune(sim, g.size_x, g.size_y, g.size_z, fused=fused)
     sim.destroy()
     beam.release()
     return best, res
@@ -81,8 +88,11 @@ def _tune_workload(name):
 
 if __name__ == '__main__':
     import sys
-    for name in (sys.argv[1:] or ['large']):
-        best, res = _tune_workload(name)
-        for shape, ms in sorted(res.items(), key=lambda kv: kv[1]):
-            print(f'{name} ty={shape[0]} zchunk={shape[1]}: {ms:.4f} ms/step', flush=True)
-        print(f'{name}: best {best}', flush=True)
+    args = sys.argv[1:] or ['large']
+    for name in args:
+        for fused in (True, False):
+            best, res = _tune_workload(name, fused)
+            tag = 'fused' if fused else 'split'
+            for shape, ms in sorted(res.items(), key=lambda kv: kv[1]):
+                print(f'{name} {tag} ty={shape[0]} zchunk={shape[1]}: {ms:.4f} ms/step', flush=True)
+            print(f'{name} {tag}: best {best}', flush=True)

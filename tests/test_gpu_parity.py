@@ -4,8 +4,8 @@ Tolerances:
   fields, CPML psi, recorded beam, phasors in 'reference' accumulation:
       bit-identical (same operation order, -ffp-contract=off on both sides)
   phasors in 'compensated' accumulation: relative L2 <= 1e-5 against the
-      oracle's complex64 running sum (the float-float sum is the more
-      accurate one; see test_compensated_dft_is_closer_to_exact_sum)
+      oracle's complex64 running sum; the float-float sum is the more
+      accurate one (test_compensated_dft_is_closer_to_exact_sum)
   far field (FP64, device vs scipy Simpson): relative <= 1e-9
 """
 import numpy as np
@@ -23,9 +23,10 @@ def say(*a):
     print(*a, flush=True)
 
 
-def make_sim(pc, Ca=None, Cb=None, beam_planes=None, launch=None):
+def make_sim(pc, Ca=None, Cb=None, beam_planes=None, launch=None, fused=True):
     native.init(0)
     sim = native.Sim(common.main_geom(pc), common.coeffs_for(pc))
+    sim.set_stepper(fused)
     if launch:
         sim.set_launch(*launch)
     g = pc.geom_params
@@ -55,10 +56,11 @@ def soil60():
     return pc, Ca, Cb, beam, ref
 
 
-def test_fields_psi_phasors_bitwise(soil60):
+@pytest.mark.parametrize('fused', [False, True], ids=['split', 'fused'])
+def test_fields_psi_phasors_bitwise(soil60, fused):
     pc, Ca, Cb, beam, ref = soil60
-    say('case: soil 33x31x35, 60 steps, reference DFT')
-    sim = make_sim(pc, Ca, Cb, beam)
+    say('case: soil 33x31x35, 60 steps, reference DFT, fused', fused)
+    sim = make_sim(pc, Ca, Cb, beam, fused=fused)
     sim.run(60)
     sim.sync()
     assert sim.step_count() == 60
@@ -74,11 +76,14 @@ def test_fields_psi_phasors_bitwise(soil60):
     sim.destroy()
 
 
-@pytest.mark.parametrize('launch', [(1, 0), (2, 0), (4, 0), (4, 7), (1, 5)])
-def test_launch_shapes_bitwise(soil60, launch):
+@pytest.mark.parametrize('fused,launch', [(False, (1, 0)), (False, (2, 0)), (False, (4, 0)),
+                                          (False, (4, 7)), (False, (1, 5)), (True, (4, 0)),
+                                          (True, (8, 0)), (True, (8, 7)), (True, (4, 5)),
+                                          (True, (8, 1))])
+def test_launch_shapes_bitwise(soil60, fused, launch):
     pc, Ca, Cb, beam, ref = soil60
-    say('case: launch', launch)
-    sim = make_sim(pc, Ca, Cb, beam, launch=launch)
+    say('case: fused', fused, 'launch', launch)
+    sim = make_sim(pc, Ca, Cb, beam, launch=launch, fused=fused)
     sim.run(60)
     sim.sync()
     for n in native.FIELD_NAMES:
@@ -117,7 +122,7 @@ def test_uneven_grid_bitwise():
     say('case: uneven 37x29x41, 40 steps')
     beam = H.synthetic_beam(pc)
     ref = H.oracle_main(pc, Ca, Cb, beam)
-    sim = make_sim(pc, Ca, Cb, beam, launch=(2, 9))
+    sim = make_sim(pc, Ca, Cb, beam, launch=(8, 9))
     sim.run(40)
     sim.sync()
     for n in native.FIELD_NAMES:
@@ -138,6 +143,27 @@ def test_incident_run_bitwise():
         assert_same(ph[n], oph[n], n)
     assert np.abs(rec[1]).max() > 0.1
     beam.release()
+
+
+def test_split_and_fused_agree_long_run():
+    """300 steps on the plain workload's grid: both steppers bit-identical."""
+    from sar_pixel_simulator_amd import workloads
+    from sar_pixel_simulator_amd.materials import compute_material_arrays
+    pc = workloads.plain_params()
+    Ca, Cb = compute_material_arrays(pc, workloads.plain_material(pc, 2))
+    beam = H.synthetic_beam(pc)
+    say('case: 81^3, 300 steps, split vs fused')
+    out = []
+    for fused in (False, True):
+        sim = make_sim(pc, Ca, Cb, beam, fused=fused)
+        sim.run(300)
+        sim.sync()
+        out.append(({n: sim.field(n) for n in native.FIELD_NAMES}, sim.phasors()))
+        sim.destroy()
+    for n in native.FIELD_NAMES:
+        assert_same(out[1][0][n], out[0][0][n], n)
+    for n in native.PHASOR_NAMES:
+        assert_same(out[1][1][n], out[0][1][n], n)
 
 
 def test_batch_equals_solo():
@@ -224,3 +250,43 @@ def test_linearity_at_large_size():
     assert np.isfinite(out[0]).all()
     assert np.abs(out[0]).max() > 0
     assert_same(out[1], 2.0 * out[0], 'ey(2 beam) vs 2 ey(beam)')
+
+
+def _exact_dft_z_face(pc, Ca, Cb, beam, steps, face_z):
+    """FP64 running DFT of the top-face ex samples, accumulated from the
+    oracle's float32 field history step by step (the exact sum the two
+    accumulation modes approximate)."""
+    g = pc.geom_params
+    n = g.size_pml
+    s = H.scalars(pc)
+    sim = oracle.OracleSim(H.oracle_geom(pc), Ca, Cb, H.profiles_of(pc), inc=beam)
+    nyp, nxp = g.size_y - 2 * n, g.size_x - 2 * n
+    acc = np.zeros((nyp, nxp), np.complex128)
+    for t in range(steps):
+        sim.run(1)
+        ex = sim.fields['ex'][face_z, n:n + nyp, :]
+        a = (ex[:, n:n + nxp] + ex[:, n - 1:n - 1 + nxp]).astype(np.float64)   # float32 sums
+        th = (-s['omega'] * t) * s['dt']
+        acc += 0.5 * a * np.exp(1j * th)
+    acc[0, :] = 0
+    acc[:, 0] = 0
+    return acc
+
+
+def test_compensated_dft_is_closer_to_exact_sum():
+    pc, Ca, Cb = H.soil_case(steps=120, seed=9)
+    beam = H.synthetic_beam(pc)
+    g = pc.geom_params
+    say('case: DFT accumulation modes vs exact sum, 120 steps')
+    exact = _exact_dft_z_face(pc, Ca, Cb, beam, 120, g.size_z - g.size_pml - 1)
+    err = {}
+    for mode in ('reference', 'compensated'):
+        pc.processing_params.dft_accumulation = mode
+        sim = make_sim(pc, Ca, Cb, beam)
+        sim.run(120)
+        sim.sync()
+        err[mode] = H.rel_l2(sim.phasor(4), exact)      # ex_ph_u
+        sim.destroy()
+    say('relative L2 error vs exact sum', err)
+    assert err['compensated'] <= err['reference']
+    assert err['compensated'] < 1e-6 and err['reference'] < 1e-5
