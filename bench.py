@@ -89,7 +89,7 @@ def cpu_baseline(pc, Ca, Cb, target_s=15.0):
     oracle.build()
     g = pc.geom_params
     nthreads = cpu_threads()
-    max_steps = 24
+    max_steps = 160
     beam = workloads.synthetic_beam_planes(pc, 0, max_steps)
     geo = H.oracle_geom(pc)
     geo['nt_inc'] = max_steps
@@ -116,8 +116,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-extras', action='store_true',
                     help='skip the batched-vs-plain and serving legs')
-    ap.add_argument('--split', action='store_true',
-                    help='split H / E kernels instead of the fused stepper')
+    ap.add_argument('--stepper', choices=['split', 'fused'], default=None,
+                    help='time-step kernels (default: the library default)')
     ap.add_argument('--ty', type=int, default=0)
     ap.add_argument('--zchunk', type=int, default=-1)
     args = ap.parse_args()
@@ -141,7 +141,7 @@ def main():
 
     native.init(0)
     info = native.device_info()
-    fused = not args.split
+    fused = common.FUSED_STEPPER if args.stepper is None else args.stepper == 'fused'
     sim = native.Sim(common.main_geom(pc), common.coeffs_for(pc))
     sim.set_stepper(fused)
     ty, zc = tuning.launch_for(g.size_x, g.size_y, g.size_z, fused=fused)
@@ -175,19 +175,31 @@ def main():
     k = sim.time_steps(min(args.steps, 50), per_kernel=True)
     uni_z = uniform_top(Ca, Cb)
     ab = algorithmic_bytes(g.size_x, g.size_y, g.size_z, g.size_pml, uni_z)
-    dom = 'e' if k['e'] >= k['h'] else 'h'
-    achieved = ab[dom] / (k[dom] * 1e-3) / 1e9
-    roof = {'bound': 'hbm', 'kernel': 'k_e_half' if dom == 'e' else 'k_h_half',
-            'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-            'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None,
-            'alg_bytes_per_launch': ab[dom], 'avg_launch_ms': round(k[dom], 4)}
-    step_alg = ab['h'] + ab['e'] + ab['dft']
-    per_kernel = {'k_h_half_ms': round(k['h'], 4), 'k_e_half_ms': round(k['e'], 4),
-                  'k_dft_ms': round(k['dft'], 4),
-                  'h_GBs': round(ab['h'] / (k['h'] * 1e-3) / 1e9, 1),
-                  'e_GBs': round(ab['e'] / (k['e'] * 1e-3) / 1e9, 1),
-                  'dft_GBs': round(ab['dft'] / (k['dft'] * 1e-3) / 1e9, 1),
-                  'step_alg_GBs': round(step_alg / (ms_step * 1e-3) / 1e9, 1)}
+
+    def gbs(nbytes, ms):
+        return round(nbytes / (ms * 1e-3) / 1e9, 1) if ms > 0 else None
+
+    if fused:
+        # one kernel does both halves: E and H each read once and written
+        # once (the H-half reads of E and E-half reads of H are gone)
+        fb = ab['h'] + ab['e'] - 24 * ab['cells']
+        kern = {'name': 'k_step_fused', 'bytes': fb, 'ms': k['h']}
+        per_kernel = {'k_step_fused_ms': round(k['h'], 4), 'k_dft_ms': round(k['dft'], 4),
+                      'fused_GBs': gbs(fb, k['h']), 'dft_GBs': gbs(ab['dft'], k['dft'])}
+        step_alg = fb + ab['dft']
+    else:
+        dom = 'e' if k['e'] >= k['h'] else 'h'
+        kern = {'name': 'k_e_half' if dom == 'e' else 'k_h_half', 'bytes': ab[dom], 'ms': k[dom]}
+        per_kernel = {'k_h_half_ms': round(k['h'], 4), 'k_e_half_ms': round(k['e'], 4),
+                      'k_dft_ms': round(k['dft'], 4), 'h_GBs': gbs(ab['h'], k['h']),
+                      'e_GBs': gbs(ab['e'], k['e']), 'dft_GBs': gbs(ab['dft'], k['dft'])}
+        step_alg = ab['h'] + ab['e'] + ab['dft']
+    per_kernel['step_alg_GBs'] = gbs(step_alg, ms_step)
+    achieved = kern['bytes'] / (kern['ms'] * 1e-3) / 1e9
+    roof = {'bound': 'hbm', 'kernel': kern['name'], 'achieved': round(achieved, 1),
+            'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
+            'traffic': None, 'alg_bytes_per_launch': kern['bytes'],
+            'avg_launch_ms': round(kern['ms'], 4)}
     sim.destroy()
     beam.release()
 
@@ -214,7 +226,8 @@ def main():
            'config': {'workload': 'single large plain run: 761x761x203 cells (16x16x3 '
                                   'lambda at 0.021 lambda, README shape), snow over rough '
                                   'soil, 5.4 GHz HH theta 20, 1714-step run',
-                      'cells': cells, 'launch': {'ty': ty, 'zchunk': zc},
+                      'cells': cells, 'stepper': 'fused' if fused else 'split',
+                      'launch': {'ty': ty, 'zchunk': zc},
                       'uniform_planes_from_z': int(uni_z),
                       'device': info['name'], 'arch': info['arch'],
                       'cu_count': info['cu_count']},

@@ -9,9 +9,10 @@ from ..parameters import ParameterCollection
 DFT_MODES = {'compensated': native.DFT_COMPENSATED,
              'reference': native.DFT_REFERENCE}
 
-#: Stepper of MAIN simulations: the fused H+E kernel (one pass over the
-#: fields per step).  The split H / E kernels give identical results.
-FUSED_STEPPER = True
+#: Stepper of MAIN simulations: False = split H and E kernels (in place),
+#: True = fused H+E kernel (double-buffered).  Both give identical results;
+#: the split stepper is the faster one today.
+FUSED_STEPPER = False
 
 
 def new_main_sim(params, absorbers=None, fused=None):
