@@ -137,8 +137,8 @@ int fdtd_sim_get_psi(FdtdSim *s, int index, float *out);
  * kernel, then an E kernel, in place).  Both give identical results. */
 int fdtd_sim_set_stepper(FdtdSim *s, int fused);
 /* Tile shape: split stepper ty rows per 64-wide block (1, 2, 4); fused
- * stepper ty rows per lane (4, 8); zchunk z cells per tile (0 = whole
- * column). */
+ * stepper ty rows per 64-wide block (8, 16; row 0 and lane 0 are a halo);
+ * zchunk z cells per tile (0 = whole column). */
 int fdtd_sim_set_launch(FdtdSim *s, int ty, int zchunk);
 /* Run nsteps and report HIP-event times on the library stream: whole run and
  * per-kernel averages (ms); with the fused stepper ms_h is the fused kernel

@@ -260,11 +260,13 @@ static int group_build(Group &G) {
     G.mode = G.members[0]->g.mode;
     G.fused = G.members[0]->h.fused;
     if (G.fused)
-        CHECK(ty == 4 || ty == 8, "fused stepper: ty (rows per lane) must be 4 or 8 (got %d)", ty);
+        CHECK(ty == 8 || ty == 16, "fused stepper: ty (rows per block) must be 8 or 16 (got %d)", ty);
     else
         CHECK(ty == 1 || ty == 2 || ty == 4, "split stepper: ty must be 1, 2 or 4 (got %d)", ty);
-    // fused tiles own 63 columns (lane 0 is the halo), split tiles 64
+    // fused tiles own 63 x (ty - 1) cells (lane 0 and row 0 are halo),
+    // split tiles 64 x ty
     const int tx = G.fused ? kWave - 1 : kWave;
+    const int tys = G.fused ? ty - 1 : ty;
     G.max_dft_items = 0;
     for (size_t m = 0; m < G.members.size(); ++m) {
         FdtdSim *s = G.members[m];
@@ -275,7 +277,7 @@ static int group_build(Group &G) {
         const int nz = s->g.nz, zc = G.launch.zchunk > 0 ? G.launch.zchunk : nz;
         for (int z0 = 0; z0 < nz; z0 += zc) {
             const int z1 = std::min(nz, z0 + zc);
-            for (int y0 = 0; y0 < s->g.ny; y0 += ty)
+            for (int y0 = 0; y0 < s->g.ny; y0 += tys)
                 for (int x0 = 0; x0 < s->g.nx; x0 += tx)
                     tiles.push_back(make_int4((int)m, x0, y0, z0 | (z1 << 16)));
         }
@@ -312,11 +314,12 @@ enum KernelId { KH = 0, KE = 1, KD = 2, KF = 3 };
 static void launch_kernel(Group &G, KernelId k) {
     const dim3 blk(kWave, G.launch.ty);
     if (k == KF) {
-        const dim3 grid((G.ntiles + 3) / 4), wblk(kWave, 4);
-        if (G.launch.ty == 8)
-            hipLaunchKernelGGL(k_step_fused<8>, grid, wblk, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
+        if (G.launch.ty == 16)
+            hipLaunchKernelGGL(k_step_fused<16>, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles,
+                               G.ntiles);
         else
-            hipLaunchKernelGGL(k_step_fused<4>, grid, wblk, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
+            hipLaunchKernelGGL(k_step_fused<8>, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles,
+                               G.ntiles);
     } else if (k == KH) {
         hipLaunchKernelGGL(k_h_half, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
     } else if (k == KE) {

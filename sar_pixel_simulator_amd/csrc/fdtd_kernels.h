@@ -69,10 +69,10 @@ struct alignas(16) Desc {
     int *step;                     // device step counter
 };
 
-// Launch shape of the stencil kernels.  Split stepper: blocks of 64 x ty
-// threads, one cell column per thread.  Fused stepper: one wavefront per
-// 63 x ty tile, ty in {4, 8} rows per lane.  zchunk: z cells per tile
-// (0 = whole column).
+// Launch shape of the stencil kernels: blocks of 64 x ty threads, one cell
+// column per thread.  Split stepper: ty in {1, 2, 4}, tiles 64 x ty.  Fused
+// stepper: ty in {8, 16}, tiles own 63 x (ty - 1) cells (lane 0 and row 0
+// are halo).  zchunk: z cells per tile (0 = whole column).
 struct Launch {
     int ty = 4;
     int zchunk = 0;

@@ -337,30 +337,28 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
 // per step (56 B/cell instead of the split stepper's 80).  Fields and H-CPML
 // state are double-buffered: step t reads set t&1 and writes set 1-(t&1).
 //
-// A wavefront owns a tile of 63 x R cells: lane 0 and row 0 are a one-cell
-// halo whose H values (needed by E at x-1, y-1) are recomputed from the old
-// level instead of being exchanged; the owner tile computes the same values
-// and is the only one that writes them.  Each lane carries R+1 rows in
-// registers and marches z upward: E(z+1) read at plane z is E(z) at plane
-// z+1, H(z) computed at plane z is H(z-1) at plane z+1.  x+1 / x-1
-// neighbours come from the adjacent lane (wavefront shuffles), y+1 / y-1
-// neighbours from the adjacent row in registers.  A tile starting at z0 > 0
-// recomputes H at z0-1 (not stored) for its first E plane.
+// A block of 64 x TY threads owns a 63 x (TY-1) tile: lane 0 and row 0 are
+// a one-cell halo whose H values (needed by E at x-1, y-1) are recomputed
+// from the old level rather than exchanged between blocks; the owner tile
+// computes the same values and is the only one that writes them.  Each
+// thread holds one (x, y) column and marches z upward: E(z+1) read at plane
+// z is E(z) at plane z+1, H(z) computed at plane z is H(z-1) at plane z+1.
+// H at x-1 comes from the neighbouring lane (wavefront shuffle); H at y-1
+// from the row below through LDS, one barrier per plane.  A tile starting
+// at z0 > 0 recomputes H at z0-1 (not stored) for its first E plane.
 // ---------------------------------------------------------------------------
-template <int R>
-__global__ void __launch_bounds__(256)
+template <int TY>
+__global__ void __launch_bounds__(kWave * TY)
 k_step_fused(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles) {
-    const int nblk = (ntiles + 3) / 4;
-    const int tile = xcd_tile(blockIdx.x, nblk) * 4 + (int)threadIdx.y;
-    if (tile >= ntiles) return;                        // uniform per wavefront
-    const int4 tl = tiles[tile];
+    __shared__ float lds_hz[2][TY][kWave];
+    __shared__ float lds_hx[2][TY][kWave];
+    const int4 tl = tiles[xcd_tile(blockIdx.x, ntiles)];
     const Desc *__restrict__ d = descs + tl.x;
     const int t = load_step(d->step);
-    if (t >= d->nt) return;
+    if (t >= d->nt) return;                            // uniform per block
     const int nx = d->nx, ny = d->ny, nz = d->nz;
-    const int lx = threadIdx.x;
-    const int x = tl.y - 1 + lx;                       // lane 0: halo column x0-1
-    const int y0 = tl.z;                               // rows y0-1 (halo) .. y0+R-1
+    const int lx = threadIdx.x, ly = threadIdx.y;
+    const int x = tl.y - 1 + lx, y = tl.z - 1 + ly;    // lane 0 / row 0: halo
     const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
     const int p = t & 1;
     const long long sy = d->sy, sz = d->sz;
@@ -376,99 +374,75 @@ k_step_fused(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int
     float *__restrict__ Hxn = d->f[1 - p][3];
     float *__restrict__ Hyn = d->f[1 - p][4];
     float *__restrict__ Hzn = d->f[1 - p][5];
-    float *const *psi_rd = d->psi[p];
-    float *const *psi_wr = d->psi[1 - p];
-    const bool vx = x >= 0 && x < nx;
-    const bool ox = lx >= 1 && vx;
+    const bool ok = x >= 0 && x < nx && y >= 0 && y < ny;
+    const bool own = ok && lx >= 1 && ly >= 1;
+    const bool xin = x < nx - 1, yin = y < ny - 1;
     const bool last_lane = lx == kWave - 1;
     const int zs = z0 > 0 ? z0 - 1 : 0;
 
-    float exc[R + 1], eyc[R + 1], hxm[R + 1], hym[R + 1];
-#pragma unroll
-    for (int r = 0; r <= R; ++r) {
-        const int y = y0 - 1 + r;
-        const bool ok = vx && y >= 0 && y < ny;
-        const long long o = (long long)zs * sz + (long long)y * sy + x;
-        exc[r] = ok ? Exo[o] : 0.f;
-        eyc[r] = ok ? Eyo[o] : 0.f;
-        hxm[r] = 0.f;
-        hym[r] = 0.f;
-    }
-    for (int z = zs; z < z1; ++z) {
+    long long o = (long long)zs * sz + (long long)y * sy + x;
+    EAround e;
+    e.ex_c = ok ? Exo[o] : 0.f;
+    e.ey_c = ok ? Eyo[o] : 0.f;
+    float hx_zm = 0.f, hy_zm = 0.f;
+    for (int z = zs; z < z1; ++z, o += sz) {
+        const int buf = z & 1;
         const bool zin = z < nz - 1;
-        const bool store = z >= z0;
-        float ezr[R + 2];
-#pragma unroll
-        for (int r = 0; r <= R + 1; ++r) {
-            const int y = y0 - 1 + r;
-            const bool ok = vx && y >= 0 && y < ny;
-            ezr[r] = ok ? Ezo[(long long)z * sz + (long long)y * sy + x] : 0.f;
-        }
-        float ex_extra;
-        {
-            const int y = y0 + R;
-            ex_extra = (vx && y < ny) ? Exo[(long long)z * sz + (long long)y * sy + x] : 0.f;
-        }
-        float hz_prev = 0.f, hx_prev = 0.f;
-#pragma unroll
-        for (int r = 0; r <= R; ++r) {
-            const int y = y0 - 1 + r;
-            const bool ok = vx && y >= 0 && y < ny;
-            const long long o = (long long)z * sz + (long long)y * sy + x;
-            EAround e;
-            e.ex_c = exc[r];
-            e.ey_c = eyc[r];
-            e.ez_c = ezr[r];
-            e.ex_n = (ok && zin) ? Exo[o + sz] : 0.f;
-            e.ey_n = (ok && zin) ? Eyo[o + sz] : 0.f;
-            e.ex_yp = (r < R) ? exc[r + 1] : ex_extra;
-            e.ez_yp = ezr[r + 1];
-            e.ey_xp = __shfl_down(e.ey_c, 1, kWave);
-            e.ez_xp = __shfl_down(e.ez_c, 1, kWave);
-            if (last_lane) {                               // x+1 belongs to the next tile
-                const bool okx = ok && x + 1 < nx;
-                e.ey_xp = okx ? Eyo[o + 1] : 0.f;
-                e.ez_xp = okx ? Ezo[o + 1] : 0.f;
-            }
-            float hx = 0.f, hy = 0.f, hz = 0.f;
-            const bool own = ox && r >= 1 && y < ny;
-            if (ok) {
-                hx = Hxo[o];
-                hy = Hyo[o];
-                hz = Hzo[o];
-                h_cell(d, x, y, z, t, e, hx, hy, hz, psi_rd, (own && store) ? psi_wr : nullptr);
-            }
-            const float hz_xm = __shfl_up(hz, 1, kWave);
-            const float hy_xm = __shfl_up(hy, 1, kWave);
-            if (r >= 1) {
-                if (own && store) {
-                    HAround h;
-                    h.hx_c = hx; h.hy_c = hy; h.hz_c = hz;
-                    h.hx_zm = hxm[r]; h.hy_zm = hym[r];
-                    h.hz_ym = hz_prev; h.hx_ym = hx_prev;
-                    h.hz_xm = hz_xm; h.hy_xm = hy_xm;
-                    float ex = e.ex_c, ey = e.ey_c, ez = e.ez_c;
-                    e_cell(d, x, y, z, t, o, h, ex, ey, ez);
-                    Exn[o] = ex;
-                    Eyn[o] = ey;
-                    Ezn[o] = ez;
-                    Hxn[o] = hx;
-                    Hyn[o] = hy;
-                    Hzn[o] = hz;
+        float hx = 0.f, hy = 0.f, hz = 0.f;
+        e.ex_n = e.ey_n = e.ez_c = e.ez_yp = e.ex_yp = 0.f;
+        if (ok) {
+            hx = Hxo[o];
+            hy = Hyo[o];
+            hz = Hzo[o];
+            e.ez_c = Ezo[o];
+            if (zin) {
+                e.ex_n = Exo[o + sz];
+                e.ey_n = Eyo[o + sz];
+                if (yin) {
+                    e.ez_yp = Ezo[o + sy];
+                    e.ex_yp = Exo[o + sy];
                 }
-                hxm[r] = hx;
-                hym[r] = hy;
             }
-            hz_prev = hz;
-            hx_prev = hx;
-            exc[r] = e.ex_n;
-            eyc[r] = e.ey_n;
         }
+        e.ey_xp = __shfl_down(e.ey_c, 1, kWave);
+        e.ez_xp = __shfl_down(e.ez_c, 1, kWave);
+        if (last_lane) {                               // x+1 belongs to the next tile
+            const bool okx = ok && xin;
+            e.ey_xp = okx ? Eyo[o + 1] : 0.f;
+            e.ez_xp = okx ? Ezo[o + 1] : 0.f;
+        }
+        const bool store = z >= z0;
+        if (ok) h_cell(d, x, y, z, t, e, hx, hy, hz, d->psi[p], (own && store) ? d->psi[1 - p] : nullptr);
+        lds_hz[buf][ly][lx] = hz;
+        lds_hx[buf][ly][lx] = hx;
+        const float hz_xm = __shfl_up(hz, 1, kWave);
+        const float hy_xm = __shfl_up(hy, 1, kWave);
+        __syncthreads();
+        if (own && store) {
+            HAround h;
+            h.hx_c = hx; h.hy_c = hy; h.hz_c = hz;
+            h.hx_zm = hx_zm; h.hy_zm = hy_zm;
+            h.hz_ym = lds_hz[buf][ly - 1][lx];
+            h.hx_ym = lds_hx[buf][ly - 1][lx];
+            h.hz_xm = hz_xm; h.hy_xm = hy_xm;
+            float ex = e.ex_c, ey = e.ey_c, ez = e.ez_c;
+            e_cell(d, x, y, z, t, o, h, ex, ey, ez);
+            Exn[o] = ex;
+            Eyn[o] = ey;
+            Ezn[o] = ez;
+            Hxn[o] = hx;
+            Hyn[o] = hy;
+            Hzn[o] = hz;
+        }
+        hx_zm = hx;
+        hy_zm = hy;
+        e.ex_c = e.ex_n;
+        e.ey_c = e.ey_n;
     }
 }
 
-template __global__ void k_step_fused<4>(const Desc *, const int4 *, int);
 template __global__ void k_step_fused<8>(const Desc *, const int4 *, int);
+template __global__ void k_step_fused<16>(const Desc *, const int4 *, int);
 
 // ---------------------------------------------------------------------------
 // Running single-frequency DFT on the phasor planes

@@ -1,8 +1,8 @@
 """Tile-shape selection for the stencil kernels.
 
-A launch shape is (ty, zchunk).  Split stepper: blocks of 64 x ty threads,
-one cell column per thread, ty in {1, 2, 4}.  Fused stepper: one wavefront
-per 63 x ty tile with ty rows per lane, ty in {4, 8}.  zchunk: z cells per
+A launch shape is (ty, zchunk): blocks of 64 x ty threads, one cell column
+per thread.  Split stepper: ty in {1, 2, 4}.  Fused stepper: ty in {8, 16}
+(row 0 and lane 0 of a block are a recomputed halo).  zchunk: z cells per
 tile (0 = whole column).  ``autotune`` measures candidates on the device and
 records the best per (stepper, arch, grid) in a JSON cache; ``launch_for``
 reads it, falling back to a default.
@@ -12,7 +12,7 @@ import os
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'tuning_cache.json')
 CANDIDATES = {
-    'fused': [(8, 0), (4, 0), (8, 64), (8, 48), (8, 32), (8, 24), (4, 64), (4, 32)],
+    'fused': [(8, 0), (16, 0), (8, 64), (8, 32), (16, 64), (16, 32), (8, 24), (16, 24)],
     'split': [(1, 0), (2, 0), (4, 0), (1, 64), (2, 64), (4, 64), (4, 32), (2, 32)],
 }
 DEFAULT = {'fused': (8, 0), 'split': (4, 0)}

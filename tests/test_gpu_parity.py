@@ -77,8 +77,8 @@ def test_fields_psi_phasors_bitwise(soil60, fused):
 
 
 @pytest.mark.parametrize('fused,launch', [(False, (1, 0)), (False, (2, 0)), (False, (4, 0)),
-                                          (False, (4, 7)), (False, (1, 5)), (True, (4, 0)),
-                                          (True, (8, 0)), (True, (8, 7)), (True, (4, 5)),
+                                          (False, (4, 7)), (False, (1, 5)), (True, (8, 0)),
+                                          (True, (16, 0)), (True, (8, 7)), (True, (16, 5)),
                                           (True, (8, 1))])
 def test_launch_shapes_bitwise(soil60, fused, launch):
     pc, Ca, Cb, beam, ref = soil60
@@ -122,7 +122,7 @@ def test_uneven_grid_bitwise():
     say('case: uneven 37x29x41, 40 steps')
     beam = H.synthetic_beam(pc)
     ref = H.oracle_main(pc, Ca, Cb, beam)
-    sim = make_sim(pc, Ca, Cb, beam, launch=(8, 9))
+    sim = make_sim(pc, Ca, Cb, beam, launch=(16, 9))
     sim.run(40)
     sim.sync()
     for n in native.FIELD_NAMES:
