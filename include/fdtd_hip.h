@@ -171,6 +171,27 @@ int fdtd_beam_upload(FdtdBeam *b, int t0, int n, const float *ex, const float *e
 int fdtd_beam_get(FdtdBeam *b, int comp, float *out);
 void fdtd_beam_release(FdtdBeam *b);
 
+/* Tapered-wave beam description for fdtd_beam_synthesize. */
+typedef struct FdtdBeamSpec {
+    int32_t polarization;     /* 0 HH, 1 VV, 2 RHCP, 3 LHCP */
+    float cb_vacuum;          /* Cb of the incident simulation's vacuum */
+    double theta_deg, phi_deg;
+    double gauss_radius;      /* [m]: illumination width / (2 sqrt(2 ln 10)) */
+    const double *pulse_fft;  /* nt complex128 (re, im): FFT of the temporal pulse */
+    const double *all_k;      /* nt wavenumbers 2 pi f_k / c of the FFT bins */
+} FdtdBeamSpec;
+
+/* Whole incident-beam pipeline on the device (IncidentBeam.setup's
+ * generate branch, incidentbeam.py:51-60): excitation synthesis
+ * (create_inc_excitation / taper_k, incidentbeam.py:206-433: spectra per
+ * temporal bin, FP64 2-D FFTs, complex64 inverse FFT over time, peak
+ * normalisation to 10), then the INCIDENT-mode simulation `g` driven by the
+ * excitation's imaginary parts, recording geom.nt steps into a new beam.
+ * c->cell_size is the spatial sample spacing.  inc_phasors_c64 receives the
+ * four incident phasors, each (ny - 2 npml, nx - 2 npml) complex64. */
+int fdtd_beam_synthesize(const FdtdGeom *g, const FdtdCoeffs *c, const FdtdBeamSpec *spec,
+                         FdtdBeam **out, float *inc_phasors_c64);
+
 /* ---- batches: many simulations stepped by one launch per kernel ------- */
 /* The batch references the sims (they must outlive it).  Sims may differ
  * in size and mode; each keeps its own step counter. */

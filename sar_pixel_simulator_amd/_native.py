@@ -50,6 +50,7 @@ EXPORTS = [
     'fdtd_sim_time_steps',
     'fdtd_sim_near2far', 'fdtd_near2far_host',
     'fdtd_beam_create', 'fdtd_beam_upload', 'fdtd_beam_get', 'fdtd_beam_release',
+    'fdtd_beam_synthesize',
     'fdtd_batch_create', 'fdtd_batch_destroy', 'fdtd_batch_set_launch',
     'fdtd_batch_run', 'fdtd_batch_sync', 'fdtd_batch_time_steps',
 ]
@@ -79,6 +80,17 @@ class FdtdCoeffs(ctypes.Structure):
                 ('omega', ctypes.c_double), ('dt', ctypes.c_double),
                 ('cell_size', ctypes.c_double),
                 ('prof', (ctypes.c_float * FDTD_MAX_PML) * 8)]
+
+
+class FdtdBeamSpec(ctypes.Structure):
+    _fields_ = [('polarization', ctypes.c_int32), ('cb_vacuum', ctypes.c_float),
+                ('theta_deg', ctypes.c_double), ('phi_deg', ctypes.c_double),
+                ('gauss_radius', ctypes.c_double),
+                ('pulse_fft', ctypes.POINTER(ctypes.c_double)),
+                ('all_k', ctypes.POINTER(ctypes.c_double))]
+
+
+POLARIZATIONS = {'HH': 0, 'VV': 1, 'RHCP': 2, 'LHCP': 3}
 
 
 class FdtdDeviceInfo(ctypes.Structure):
@@ -128,6 +140,8 @@ _SIGS = {
     'fdtd_beam_upload': ([_P, _I, _I, _FP, _FP, _FP, _FP], _I),
     'fdtd_beam_get': ([_P, _I, _FP], _I),
     'fdtd_beam_release': ([_P], None),
+    'fdtd_beam_synthesize': ([ctypes.POINTER(FdtdGeom), ctypes.POINTER(FdtdCoeffs),
+                              ctypes.POINTER(FdtdBeamSpec), ctypes.POINTER(_P), _FP], _I),
     'fdtd_batch_create': ([ctypes.POINTER(_P), _I, ctypes.POINTER(_P)], _I),
     'fdtd_batch_destroy': ([_P], None),
     'fdtd_batch_set_launch': ([_P, _I, _I], _I),
@@ -270,6 +284,32 @@ class Beam:
             self.release()
         except Exception:
             pass
+
+
+def beam_synthesize(geom, coeffs, polarization, theta_deg, phi_deg, gauss_radius,
+                    pulse_fft, all_k, cb_vacuum):
+    """Device beam pipeline (fdtd_beam_synthesize): returns (Beam, incident
+    phasors dict).  ``geom`` is the INCIDENT-mode geometry."""
+    pf = np.ascontiguousarray(np.asarray(pulse_fft, np.complex128))
+    ak = np.ascontiguousarray(all_k, dtype=np.float64)
+    if pf.size != geom.nt or ak.size != geom.nt:
+        raise ValueError('pulse spectrum and wavenumbers need geom.nt entries')
+    spec = FdtdBeamSpec()
+    spec.polarization = POLARIZATIONS[polarization]
+    spec.cb_vacuum = float(np.float32(cb_vacuum))
+    spec.theta_deg, spec.phi_deg = float(theta_deg), float(phi_deg)
+    spec.gauss_radius = float(gauss_radius)
+    spec.pulse_fft = pf.view(np.float64).ctypes.data_as(_DP)
+    spec.all_k = _dptr(ak)
+    nyp, nxp = geom.ny - 2 * geom.npml, geom.nx - 2 * geom.npml
+    ph = np.empty((4, nyp, nxp), np.complex64)
+    h = _P()
+    _check(lib().fdtd_beam_synthesize(ctypes.byref(geom), ctypes.byref(coeffs), ctypes.byref(spec),
+                                      ctypes.byref(h), ph.view(np.float32).ctypes.data_as(_FP)))
+    beam = Beam.__new__(Beam)
+    beam.nt, beam.ny, beam.nx = geom.nt, geom.ny, geom.nx
+    beam._h = h
+    return beam, {n: ph[i] for i, n in enumerate(INC_PHASOR_NAMES)}
 
 
 class Sim:

@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB_NAME = 'libfdtd_hip.so'
 LIB_PATH = os.path.join(HERE, LIB_NAME)
-SOURCES = ['fdtd_api.hip', 'fdtd_kernels.hip']
+SOURCES = ['fdtd_api.hip', 'fdtd_kernels.hip', 'fdtd_beam.hip']
+LIBS = ['-L/opt/rocm/lib', '-lhipfft', '-Wl,-rpath,/opt/rocm/lib']
 ARCH = os.environ.get('FDTD_OFFLOAD_ARCH', 'gfx950')
 
 HIPCC_FLAGS = [
@@ -37,7 +38,7 @@ def build(verbose=False):
     """Compile the HIP library; raises CalledProcessError on failure."""
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB_PATH + '.tmp'
-    cmd = [hipcc()] + HIPCC_FLAGS + ['-o', tmp] + srcs
+    cmd = [hipcc()] + HIPCC_FLAGS + ['-o', tmp] + srcs + LIBS
     if verbose:
         print(' '.join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -1076,6 +1076,54 @@ extern "C" int fdtd_beam_get(FdtdBeam *b, int comp, float *out) {
 
 extern "C" void fdtd_beam_release(FdtdBeam *b) { beam_unref(b); }
 
+int beam_synthesize_source(const double *pulse_fft, const double *all_k, int nt_inc, int n_src, int nx, int ny,
+                           double dx, double theta_deg, double phi_deg, double radius, int pol, hipStream_t st,
+                           float **src_out, char *err, size_t errlen);
+
+extern "C" int fdtd_beam_synthesize(const FdtdGeom *g, const FdtdCoeffs *c, const FdtdBeamSpec *spec,
+                                    FdtdBeam **out, float *inc_phasors_c64) {
+    CHECK(g && c && spec && out && inc_phasors_c64, "NULL argument");
+    CHECK(g->mode == FDTD_MODE_INCIDENT, "beam synthesis runs an INCIDENT simulation");
+    CHECK(spec->pulse_fft && spec->all_k, "pulse spectrum not given");
+    CHECK(g->nt > 0 && g->nt_inc >= 0 && g->nt_inc <= g->nt, "bad step counts (nt %d, n_src %d)", g->nt,
+          g->nt_inc);
+    *out = nullptr;
+    FdtdSim *s = nullptr;
+    int rc = fdtd_sim_create(g, c, &s);
+    if (rc) return rc;
+    FdtdBeam *b = nullptr;
+    auto cleanup = [&](int code) {
+        std::string keep = g_err;
+        fdtd_sim_destroy(s);
+        if (b) beam_unref(b);
+        g_err = keep;
+        return code;
+    };
+    rc = fdtd_sim_set_uniform_material(s, 1.0f, spec->cb_vacuum);
+    if (rc) return cleanup(rc);
+    float *d_src = nullptr;
+    char err[256] = {0};
+    if (beam_synthesize_source(spec->pulse_fft, spec->all_k, g->nt, g->nt_inc, g->nx, g->ny, c->cell_size,
+                               spec->theta_deg, spec->phi_deg, spec->gauss_radius, spec->polarization,
+                               s->self.stream, &d_src, err, sizeof err) != 0)
+        return cleanup(fail(FDTD_EHIP, "beam synthesis: %s", err));
+    s->src = d_src;                                 // the sim owns it now
+    s->h.src = d_src;
+    s->h.src_cs = (long long)g->nt_inc * g->ny * g->nx;
+    s->h.n_src = g->nt_inc;
+    rc = sim_refresh(s);
+    if (!rc) rc = fdtd_beam_create(g->nt, g->ny, g->nx, nullptr, nullptr, nullptr, nullptr, &b);
+    if (!rc) rc = fdtd_sim_attach_beam(s, b);
+    if (!rc) rc = fdtd_sim_run(s, g->nt);
+    if (!rc) rc = fdtd_sim_sync(s);
+    const size_t n = (size_t)(g->ny - 2 * g->npml) * (g->nx - 2 * g->npml);
+    for (int i = 0; i < 4 && !rc; ++i) rc = fdtd_sim_get_phasor(s, i, inc_phasors_c64 + 2 * n * i);
+    if (rc) return cleanup(rc);
+    fdtd_sim_destroy(s);
+    *out = b;
+    return FDTD_OK;
+}
+
 // ---------------------------------------------------------------------------
 // batches
 // ---------------------------------------------------------------------------

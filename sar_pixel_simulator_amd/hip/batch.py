@@ -11,7 +11,7 @@ state and phasor sums, so results are identical to running it alone.
 import numpy as np
 
 from .. import _native as native
-from ..incidentbeam import IncidentBeam, run_incident
+from ..incidentbeam import synthesize_on_device
 from ..materials import compute_material_arrays
 from ..processresults import THETAS, RADIUS, incident_power, scattering_coefficients, \
     farfield_power, backscatter
@@ -21,8 +21,9 @@ from . import common
 class BeamCache:
     """Incident beams resident in HBM, keyed by what the beam depends on (the
     reference's incident-file key, parameters.py:308-324, plus the values its
-    metadata check compares, incidentbeam.py:105-165).  A beam is generated on first use: host
-    excitation synthesis, then the incident FDTD on the device."""
+    metadata check compares, incidentbeam.py:105-165).  A beam is generated on
+    first use, entirely on the device (excitation synthesis and incident
+    FDTD)."""
 
     def __init__(self):
         self._beams = {}
@@ -37,10 +38,7 @@ class BeamCache:
         k = beam_key(params)
         hit = self._beams.get(k)
         if hit is None:
-            ib = IncidentBeam(params)
-            ib.create_pulse()
-            pex, pey = ib.create_inc_excitation()
-            beam, ph = run_incident(params, pex, pey)
+            beam, ph = synthesize_on_device(params)
             hit = (beam, ph, incident_power(ph, params.geom_params.cell_size))
             self._beams[k] = hit
         return hit

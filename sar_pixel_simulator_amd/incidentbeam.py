@@ -107,10 +107,8 @@ class IncidentBeam:
             logger.info('Loading existing incident beam files')
             self.load_existing()
         else:
-            logger.info('Generating new incident beam')
-            self.create_pulse()
-            pulse_ex, pulse_ey = self.create_inc_excitation()
-            self.beam, self.phasors = run_incident(self.params, pulse_ex, pulse_ey)
+            logger.info('Generating new incident beam on the device')
+            self.beam, self.phasors = synthesize_on_device(self.params)
             _RESIDENT[key] = (self.beam, self.phasors)
             self.save()
         logger.info(f'Incident beam setup complete ({time.monotonic() - t0:.3f}s)')
@@ -254,6 +252,33 @@ def run_incident(params: ParameterCollection, pulse_ex, pulse_ey, device=None):
     finally:
         sim.destroy()
     return beam, phasors
+
+
+def beam_spectrum(params: ParameterCollection):
+    """Host-side inputs of the device beam pipeline: the temporal pulse's
+    FFT and bin wavenumbers (incidentbeam.py:174-204, :255-257) and the
+    Gaussian footprint radius (:231-233)."""
+    ib = IncidentBeam(params)
+    ib.create_pulse()
+    b = params.inc_beam_params
+    width = b.illumination_width * C0 / b.frequency
+    gauss_radius = width / (2.0 * np.sqrt(2.0 * np.log(10.0)))
+    all_k = 2 * np.pi * ib.pulse_freqs / C0
+    return ib.pulse_fft, all_k, gauss_radius
+
+
+def synthesize_on_device(params: ParameterCollection, device=None):
+    """Excitation synthesis and incident simulation both on the device
+    (libfdtd_hip fdtd_beam_synthesize).  Returns (native.Beam, incident
+    phasor dict); the same beam as ``create_inc_excitation`` followed by
+    ``run_incident`` up to FFT rounding."""
+    common.require_float32(params)
+    native.init(params_device(params) if device is None else device)
+    b = params.inc_beam_params
+    pulse_fft, all_k, radius = beam_spectrum(params)
+    return native.beam_synthesize(common.incident_geom(params), common.coeffs_for(params),
+                                  b.polarization, b.theta_angle, b.phi_angle, radius,
+                                  pulse_fft, all_k, common.incident_cb(params))
 
 
 def params_device(params: ParameterCollection):
