@@ -14,9 +14,14 @@ analytic Gaussian-tapered plane wave (synthetic) of the real beam's shape.
 Prints one JSON line (see the keys below).
 """
 import argparse
+import glob
 import json
 import os
+import shutil
+import sqlite3
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -26,6 +31,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = 'FDTD Mcell-updates/s + material-models/hr batched vs plain; p50/p99 latency'
 HBM_PEAK_GBS = 8000.0
+PROBE_BYTES = 1 << 30          # calibration copy: 1 GiB read + 1 GiB written
+PMC_COUNTERS = ('FETCH_SIZE', 'WRITE_SIZE')
 
 
 def cpml_cells(nx, ny, nz, np_):
@@ -108,6 +115,71 @@ def cpu_baseline(pc, Ca, Cb, target_s=15.0):
                       f'{g.size_z} domain with injection active, {el:.1f} s'}
 
 
+def rocprofv3():
+    for cand in (shutil.which('rocprofv3'), '/opt/rocm/bin/rocprofv3'):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def pmc_pass(counter, child_args, out_dir):
+    """One ``rocprofv3 --pmc <counter>`` run of a short child bench (a few
+    steps, then the calibration copy).  Returns the path of its database."""
+    d = os.path.join(out_dir, counter)
+    cmd = ['timeout', '-s', 'KILL', '120', rocprofv3(), '--pmc', counter, '-d', d, '-o', 'pmc',
+           '--', sys.executable, os.path.join(ROOT, 'bench.py'), '--pmc-child'] + child_args
+    r = subprocess.run(cmd, cwd=ROOT, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    dbs = glob.glob(os.path.join(d, '**', '*results.db'), recursive=True)
+    if r.returncode != 0 or not dbs:
+        tail = '\n'.join(r.stdout.splitlines()[-5:])
+        raise RuntimeError(f'rocprofv3 --pmc {counter} exited {r.returncode}: {tail}')
+    return dbs[0]
+
+
+def counter_kib(db, counter, kernel):
+    """Average value (KiB per dispatch) of ``counter`` over the dispatches
+    whose kernel name contains ``kernel``."""
+    c = sqlite3.connect(db)
+    try:
+        rows = [v for name, v in c.execute(
+            'select kernel_name, value from counters_collection where counter_name = ?',
+            (counter,)) if kernel in name]
+    finally:
+        c.close()
+    if not rows:
+        raise RuntimeError(f'no {kernel} dispatch with {counter} in {db}')
+    return sum(rows) / len(rows)
+
+
+def measure_traffic(kernel, child_args, keep_dir=None):
+    """HBM bytes per launch of ``kernel`` from FETCH_SIZE and WRITE_SIZE,
+    each scaled by what the same counter reads for the calibration copy of
+    PROBE_BYTES (k_copy_probe<1>: PROBE_BYTES fetched, PROBE_BYTES written)."""
+    if rocprofv3() is None:
+        return None, {'note': 'rocprofv3 not found'}
+    out_dir = keep_dir or tempfile.mkdtemp(prefix='fdtd_pmc_')
+    detail = {}
+    try:
+        total = 0.0
+        for ctr in PMC_COUNTERS:
+            db = pmc_pass(ctr, child_args, out_dir)
+            k_kib = counter_kib(db, ctr, 'fdtd::' + kernel)
+            c_kib = counter_kib(db, ctr, 'fdtd::k_copy_probe<1>')
+            scale = PROBE_BYTES / (c_kib * 1024.0)
+            nbytes = k_kib * 1024.0 * scale
+            total += nbytes
+            key = ctr.split('_')[0].lower()
+            detail[key + '_bytes'] = int(nbytes)
+            detail[key + '_raw_kib'] = round(k_kib, 1)
+            detail[key + '_scale'] = round(scale, 4)
+        return int(total), detail
+    except (RuntimeError, sqlite3.Error) as e:
+        return None, {'note': str(e)[:300]}
+    finally:
+        if keep_dir is None:
+            shutil.rmtree(out_dir, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -120,7 +192,14 @@ def main():
                     help='time-step kernels (default: the library default)')
     ap.add_argument('--ty', type=int, default=0)
     ap.add_argument('--zchunk', type=int, default=-1)
+    ap.add_argument('--no-traffic', action='store_true',
+                    help='skip the two rocprofv3 --pmc passes that measure HBM traffic')
+    ap.add_argument('--pmc-dir', default=None,
+                    help='keep the PMC databases in this directory')
+    ap.add_argument('--pmc-child', action='store_true', help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        args.no_cpu = args.no_extras = args.no_traffic = True
     if args.gpus != 1:
         print(json.dumps({'error': 'this build runs on one GPU; multi-GPU runs are not '
                                    'part of it (see DESIGN.md, sharding)',
@@ -162,6 +241,14 @@ def main():
     # the steps run on (nothing else runs on the device).
     sim.run(args.warmup)
     sim.sync()
+    if args.pmc_child:
+        # profiled child: a few steps, then the calibration copy
+        sim.run(args.steps)
+        sim.sync()
+        native.probe_copy(PROBE_BYTES, vec=1, reps=1)
+        sim.destroy()
+        beam.release()
+        return
     t0 = time.perf_counter()
     sim.run(args.steps)
     sim.sync()
@@ -196,12 +283,27 @@ def main():
         step_alg = ab['h'] + ab['e'] + ab['dft']
     per_kernel['step_alg_GBs'] = gbs(step_alg, ms_step)
     achieved = kern['bytes'] / (kern['ms'] * 1e-3) / 1e9
-    roof = {'bound': 'hbm', 'kernel': kern['name'], 'achieved': round(achieved, 1),
-            'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
-            'traffic': None, 'alg_bytes_per_launch': kern['bytes'],
-            'avg_launch_ms': round(kern['ms'], 4)}
     sim.destroy()
     beam.release()
+    # practical ceiling: a plain streaming copy with the same 4-byte accesses
+    # (and with 16-byte accesses)
+    copy1_ms, copy1 = native.probe_copy(PROBE_BYTES, vec=1, reps=10)
+    copy4_ms, copy4 = native.probe_copy(PROBE_BYTES, vec=4, reps=10)
+    traffic, tdetail = None, {}
+    if not args.no_traffic:
+        print('bench: PMC traffic passes (FETCH_SIZE, WRITE_SIZE)', file=sys.stderr, flush=True)
+        child = ['--steps', '4', '--warmup', '1', '--stepper', 'fused' if fused else 'split',
+                 '--ty', str(ty), '--zchunk', str(zc)]
+        traffic, tdetail = measure_traffic(kern['name'], child, args.pmc_dir)
+    roof = {'bound': 'hbm', 'kernel': kern['name'], 'achieved': round(achieved, 1),
+            'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
+            'traffic': traffic, 'alg_bytes_per_launch': kern['bytes'],
+            'avg_launch_ms': round(kern['ms'], 4),
+            'traffic_detail': tdetail,
+            'copy_probe_GBs': {'4B_per_lane': round(copy1, 1), '16B_per_lane': round(copy4, 1)},
+            'frac_of_copy_probe': round(achieved / copy1, 4)}
+    if traffic:
+        roof['traffic_GBs'] = round(traffic / (kern['ms'] * 1e-3) / 1e9, 1)
 
     cpu = None
     if not args.no_cpu:

@@ -204,6 +204,15 @@ int fdtd_batch_run(FdtdBatch *b, int nsteps, int use_graph);
 int fdtd_batch_sync(FdtdBatch *b);
 int fdtd_batch_time_steps(FdtdBatch *b, int nsteps, int use_graph, float *ms_total);
 
+/* ---- measurement support (no counterpart in the reference) ------------ */
+/* Copy nbytes between two fresh device buffers reps times (after one
+ * untimed copy) with a plain streaming kernel, vec floats per thread (1 or
+ * 4), and report the average HIP-event time of one copy.  The vec = 1 form
+ * makes the stencil kernels' coalesced 4-byte accesses: its bandwidth is the
+ * practical ceiling they are measured against, and its known byte count
+ * calibrates the PMC byte counters. */
+int fdtd_probe_copy(int64_t nbytes, int vec, int reps, float *ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif

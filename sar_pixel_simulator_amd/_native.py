@@ -53,6 +53,7 @@ EXPORTS = [
     'fdtd_beam_synthesize',
     'fdtd_batch_create', 'fdtd_batch_destroy', 'fdtd_batch_set_launch',
     'fdtd_batch_run', 'fdtd_batch_sync', 'fdtd_batch_time_steps',
+    'fdtd_probe_copy',
 ]
 
 
@@ -148,6 +149,7 @@ _SIGS = {
     'fdtd_batch_run': ([_P, _I, _I], _I),
     'fdtd_batch_sync': ([_P], _I),
     'fdtd_batch_time_steps': ([_P, _I, _I, _FP], _I),
+    'fdtd_probe_copy': ([ctypes.c_int64, _I, _I, _FP], _I),
 }
 
 
@@ -209,6 +211,14 @@ def device_info():
             'xcd_count': info.xcd_count, 'warp_size': info.warp_size,
             'total_mem': info.total_mem, 'free_mem': info.free_mem,
             'arch': info.arch.decode(), 'name': info.name.decode()}
+
+
+def probe_copy(nbytes, vec=1, reps=5):
+    """Average HIP-event time (ms) of one plain device copy of nbytes with
+    vec floats per thread; returns (ms, GB/s counting read + write)."""
+    ms = ctypes.c_float()
+    _check(lib().fdtd_probe_copy(int(nbytes), int(vec), int(reps), ctypes.byref(ms)))
+    return ms.value, 2 * nbytes / (ms.value * 1e-3) / 1e9
 
 
 def make_geom(nx, ny, nz, npml, tfsf_z, nt, nt_inc, mode=MODE_MAIN,

@@ -26,6 +26,8 @@ __global__ void k_n2f_partial(const float2 *, const float2 *, const float2 *, co
                               const double2 *, const double2 *, double *);
 __global__ void k_n2f_final(const double *, int, int, const double4 *, const double2 *, double2, double,
                             double2 *);
+template <int V>
+__global__ void k_copy_probe(const float *, float *, long long);
 }  // namespace fdtd
 
 using namespace fdtd;
@@ -416,6 +418,52 @@ extern "C" int fdtd_device_info(FdtdDeviceInfo *out) {
     snprintf(out->arch, sizeof out->arch, "%s", p.gcnArchName);
     snprintf(out->name, sizeof out->name, "%s", p.name);
     return FDTD_OK;
+}
+
+// Plain streaming copy between two fresh device buffers of nbytes each,
+// timed with HIP events on a stream of its own (default priority).
+extern "C" int fdtd_probe_copy(int64_t nbytes, int vec, int reps, float *ms_per_copy) {
+    CHECK(nbytes >= 1024 && nbytes % 16 == 0, "nbytes must be a multiple of 16, at least 1024");
+    CHECK(vec == 1 || vec == 4, "vec must be 1 or 4");
+    CHECK(reps >= 1 && ms_per_copy, "bad reps / output");
+    HIP_TRY(hipSetDevice(g_device));
+    const long long n = nbytes / 4;
+    const long long threads = vec == 4 ? n / 4 : n;
+    const long long blocks = (threads + 255) / 256;
+    CHECK(blocks <= 0x7fffffffLL, "copy too large for one launch");
+    float *src = nullptr, *dst = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int rc = FDTD_OK;
+    auto hip_ok = [&](hipError_t e, const char *what) {
+        if (e != hipSuccess && rc == FDTD_OK) rc = fail(FDTD_EHIP, "%s: %s", what, hipGetErrorString(e));
+        return rc == FDTD_OK;
+    };
+    if (hip_ok(hipMalloc(&src, nbytes), "hipMalloc") && hip_ok(hipMalloc(&dst, nbytes), "hipMalloc") &&
+        hip_ok(hipStreamCreate(&st), "hipStreamCreate") && hip_ok(hipEventCreate(&ev[0]), "hipEventCreate") &&
+        hip_ok(hipEventCreate(&ev[1]), "hipEventCreate") &&
+        hip_ok(hipMemsetAsync(src, 0x3f, nbytes, st), "hipMemsetAsync")) {
+        for (int r = 0; r <= reps && rc == FDTD_OK; ++r) {   // r = 0: warm-up
+            if (r == 1) hip_ok(hipEventRecord(ev[0], st), "hipEventRecord");
+            if (vec == 4)
+                hipLaunchKernelGGL(k_copy_probe<4>, dim3((unsigned)blocks), dim3(256), 0, st, src, dst, n);
+            else
+                hipLaunchKernelGGL(k_copy_probe<1>, dim3((unsigned)blocks), dim3(256), 0, st, src, dst, n);
+            hip_ok(hipGetLastError(), "k_copy_probe");
+        }
+        float ms = 0;
+        if (hip_ok(hipEventRecord(ev[1], st), "hipEventRecord") &&
+            hip_ok(hipEventSynchronize(ev[1]), "hipEventSynchronize") &&
+            hip_ok(hipEventElapsedTime(&ms, ev[0], ev[1]), "hipEventElapsedTime"))
+            *ms_per_copy = ms / reps;
+    }
+    if (st) hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize");
+    for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (st) (void)hipStreamDestroy(st);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    return rc;
 }
 
 // ---------------------------------------------------------------------------

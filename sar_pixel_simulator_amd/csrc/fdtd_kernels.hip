@@ -589,6 +589,25 @@ k_phasor_out(const float4 *__restrict__ acc, long long n, float2 *__restrict__ o
 }
 
 // ---------------------------------------------------------------------------
+// Streaming copy probe: one element of width V floats per thread, the same
+// coalesced access the stencil kernels make with V = 1.  Measures the
+// practical copy bandwidth and calibrates the PMC byte counters.
+// ---------------------------------------------------------------------------
+template <int V>
+__global__ void __launch_bounds__(256)
+k_copy_probe(const float *__restrict__ src, float *__restrict__ dst, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (V == 4) {
+        if (i < n / 4) reinterpret_cast<float4 *>(dst)[i] = reinterpret_cast<const float4 *>(src)[i];
+    } else if (i < n) {
+        dst[i] = src[i];
+    }
+}
+
+template __global__ void k_copy_probe<1>(const float *, float *, long long);
+template __global__ void k_copy_probe<4>(const float *, float *, long long);
+
+// ---------------------------------------------------------------------------
 // Near-to-far transform over the top face (near2far.py:28-156), FP64.
 // The integrand is linear in J = (-hy_u, hx_u) and M = (ey_u, -ex_u), so
 // each theta needs the four Simpson-weighted sums of J_x, J_y, M_x, M_y
