@@ -192,6 +192,8 @@ def main():
                     help='time-step kernels (default: the library default)')
     ap.add_argument('--ty', type=int, default=0)
     ap.add_argument('--zchunk', type=int, default=-1)
+    ap.add_argument('--xvec', type=int, default=-1,
+                    help='x cells per lane of the split kernels (0 = automatic)')
     ap.add_argument('--no-traffic', action='store_true',
                     help='skip the two rocprofv3 --pmc passes that measure HBM traffic')
     ap.add_argument('--pmc-dir', default=None,
@@ -223,12 +225,14 @@ def main():
     fused = common.FUSED_STEPPER if args.stepper is None else args.stepper == 'fused'
     sim = native.Sim(common.main_geom(pc), common.coeffs_for(pc))
     sim.set_stepper(fused)
-    ty, zc = tuning.launch_for(g.size_x, g.size_y, g.size_z, fused=fused)
+    ty, zc, xv = tuning.launch_for(g.size_x, g.size_y, g.size_z, fused=fused)
     if args.ty > 0:
         ty = args.ty
     if args.zchunk >= 0:
         zc = args.zchunk
-    sim.set_launch(ty, zc)
+    if args.xvec >= 0:
+        xv = args.xvec
+    sim.set_launch(ty, zc, xv)
     beam = native.Beam(g.size_t_inc, g.size_y, g.size_x)
     n_fill = min(g.size_t_inc, args.warmup + args.steps + 8)
     beam.upload(0, *workloads.synthetic_beam_planes(pc, 0, n_fill))
@@ -293,7 +297,7 @@ def main():
     if not args.no_traffic:
         print('bench: PMC traffic passes (FETCH_SIZE, WRITE_SIZE)', file=sys.stderr, flush=True)
         child = ['--steps', '4', '--warmup', '1', '--stepper', 'fused' if fused else 'split',
-                 '--ty', str(ty), '--zchunk', str(zc)]
+                 '--ty', str(ty), '--zchunk', str(zc), '--xvec', str(xv)]
         traffic, tdetail = measure_traffic(kern['name'], child, args.pmc_dir)
     roof = {'bound': 'hbm', 'kernel': kern['name'], 'achieved': round(achieved, 1),
             'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
@@ -329,7 +333,7 @@ def main():
                                   'lambda at 0.021 lambda, README shape), snow over rough '
                                   'soil, 5.4 GHz HH theta 20, 1714-step run',
                       'cells': cells, 'stepper': 'fused' if fused else 'split',
-                      'launch': {'ty': ty, 'zchunk': zc},
+                      'launch': {'ty': ty, 'zchunk': zc, 'xvec': xv},
                       'uniform_planes_from_z': int(uni_z),
                       'device': info['name'], 'arch': info['arch'],
                       'cu_count': info['cu_count']},

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define FDTD_ABI_VERSION 1
+#define FDTD_ABI_VERSION 2
 #define FDTD_MAX_PML 16
 
 /* error codes */
@@ -136,10 +136,12 @@ int fdtd_sim_get_psi(FdtdSim *s, int index, float *out);
  * H then E per step; fields double-buffered), 0 = split stepper (an H
  * kernel, then an E kernel, in place).  Both give identical results. */
 int fdtd_sim_set_stepper(FdtdSim *s, int fused);
-/* Tile shape: split stepper ty rows per 64-wide block (1, 2, 4); fused
- * stepper ty rows per 64-wide block (8, 16; row 0 and lane 0 are a halo);
- * zchunk z cells per tile (0 = whole column). */
-int fdtd_sim_set_launch(FdtdSim *s, int ty, int zchunk);
+/* Tile shape.  Split stepper: blocks of 64 x ty lanes (ty 1, 2, 4), each
+ * lane owning xvec consecutive x cells (1, 2, 4; 0 = widest that leaves few
+ * lanes idle).  Fused stepper: 64 x ty lanes (ty 8, 16; row 0 and lane 0 are
+ * a halo), xvec 0 or 1.  zchunk: z cells per tile (0 = whole column).  An
+ * invalid shape is rejected and the previous one kept. */
+int fdtd_sim_set_launch(FdtdSim *s, int ty, int zchunk, int xvec);
 /* Run nsteps and report HIP-event times on the library stream: whole run and
  * per-kernel averages (ms); with the fused stepper ms_h is the fused kernel
  * and ms_e is 0.  Any pointer may be NULL. */
@@ -197,7 +199,7 @@ int fdtd_beam_synthesize(const FdtdGeom *g, const FdtdCoeffs *c, const FdtdBeamS
  * in size and mode; each keeps its own step counter. */
 int fdtd_batch_create(FdtdSim *const *sims, int n, FdtdBatch **out);
 void fdtd_batch_destroy(FdtdBatch *b);
-int fdtd_batch_set_launch(FdtdBatch *b, int ty, int zchunk);
+int fdtd_batch_set_launch(FdtdBatch *b, int ty, int zchunk, int xvec);
 /* Queue nsteps steps of every member; use_graph != 0 captures the step
  * sequence into a HIP graph on first use and replays it. */
 int fdtd_batch_run(FdtdBatch *b, int nsteps, int use_graph);

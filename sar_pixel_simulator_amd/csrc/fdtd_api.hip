@@ -15,7 +15,9 @@
 #include "fdtd_kernels.h"
 
 namespace fdtd {
+template <int V>
 __global__ void k_h_half(const Desc *, const int4 *, int);
+template <int V>
 __global__ void k_e_half(const Desc *, const int4 *, int);
 __global__ void k_dft(const Desc *);
 __global__ void k_advance(const Desc *, int);
@@ -62,7 +64,8 @@ static int fail(int code, const char *fmt, ...) {
     } while (0)
 
 // Row pitch of the device arrays: nx rounded up to FDTD_PITCH_ALIGN floats
-// (environment, default 16 = 64-byte rows; 1 = unpadded).
+// (environment, default 16 = 64-byte rows; 1 = unpadded, which limits the
+// split kernels to one cell per lane).
 static long long pitch_of(int nx) {
     static const long long align = [] {
         const char *v = getenv("FDTD_PITCH_ALIGN");
@@ -90,6 +93,16 @@ static void beam_unref(FdtdBeam *b) {
     }
 }
 
+// Launch shape a simulation or batch starts with, per stepper.
+static Launch default_launch(int fused) {
+    Launch l;
+    if (fused) {
+        l.ty = 8;
+        l.xvec = 1;
+    }
+    return l;
+}
+
 // A set of simulations stepped together: one launch per kernel per step.
 struct Group {
     int device = 0;
@@ -97,6 +110,7 @@ struct Group {
     Desc *d_descs = nullptr;
     int4 *d_tiles = nullptr;
     int ntiles = 0, nmodels = 0, mode = FDTD_MODE_MAIN, fused = 0;
+    int xvec = 1;                  // x cells per lane of the split kernels in use
     long long max_dft_items = 0;
     Launch launch;
     std::vector<FdtdSim *> members;
@@ -253,6 +267,25 @@ static void group_clear_graphs(Group &G) {
     G.graph_src.clear();
 }
 
+// Widest lane (4, 2 or 1 x cells) whose tiles leave at most 30 % of the
+// lanes idle over the members' rows (x padding to whole tiles), among the
+// widths every member's row pitch holds.
+static int auto_xvec(const std::vector<FdtdSim *> &members) {
+    for (int v : {4, 2}) {
+        bool fits = true;
+        double used = 0, alloc = 0;
+        for (const FdtdSim *s : members) {
+            fits = fits && s->h.sy % v == 0;
+            const long long tw = (long long)kWave * v;
+            const double rows = (double)s->g.ny * s->g.nz;
+            used += rows * s->g.nx;
+            alloc += rows * (double)((s->g.nx + tw - 1) / tw * tw);
+        }
+        if (fits && alloc <= 1.3 * used) return v;
+    }
+    return 1;
+}
+
 static int group_build(Group &G) {
     CHECK(!G.members.empty(), "empty group");
     HIP_TRY(hipSetDevice(G.device));
@@ -261,13 +294,21 @@ static int group_build(Group &G) {
     std::vector<Desc> descs;
     G.mode = G.members[0]->g.mode;
     G.fused = G.members[0]->h.fused;
-    if (G.fused)
+    const int xv = G.launch.xvec;
+    if (G.fused) {
         CHECK(ty == 8 || ty == 16, "fused stepper: ty (rows per block) must be 8 or 16 (got %d)", ty);
-    else
+        CHECK(xv == 0 || xv == 1, "fused stepper: one x cell per lane (xvec 0 or 1, got %d)", xv);
+        G.xvec = 1;
+    } else {
         CHECK(ty == 1 || ty == 2 || ty == 4, "split stepper: ty must be 1, 2 or 4 (got %d)", ty);
+        CHECK(xv == 0 || xv == 1 || xv == 2 || xv == 4, "xvec must be 0, 1, 2 or 4 (got %d)", xv);
+        G.xvec = xv ? xv : auto_xvec(G.members);
+        for (FdtdSim *s : G.members)
+            CHECK(s->h.sy % G.xvec == 0, "row pitch %lld does not hold whole %d-cell lanes", s->h.sy, G.xvec);
+    }
     // fused tiles own 63 x (ty - 1) cells (lane 0 and row 0 are halo),
-    // split tiles 64 x ty
-    const int tx = G.fused ? kWave - 1 : kWave;
+    // split tiles (64 xvec) x ty
+    const int tx = G.fused ? kWave - 1 : kWave * G.xvec;
     const int tys = G.fused ? ty - 1 : ty;
     G.max_dft_items = 0;
     for (size_t m = 0; m < G.members.size(); ++m) {
@@ -323,9 +364,19 @@ static void launch_kernel(Group &G, KernelId k) {
             hipLaunchKernelGGL(k_step_fused<8>, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles,
                                G.ntiles);
     } else if (k == KH) {
-        hipLaunchKernelGGL(k_h_half, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
+        if (G.xvec == 4)
+            hipLaunchKernelGGL(k_h_half<4>, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
+        else if (G.xvec == 2)
+            hipLaunchKernelGGL(k_h_half<2>, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
+        else
+            hipLaunchKernelGGL(k_h_half<1>, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
     } else if (k == KE) {
-        hipLaunchKernelGGL(k_e_half, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
+        if (G.xvec == 4)
+            hipLaunchKernelGGL(k_e_half<4>, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
+        else if (G.xvec == 2)
+            hipLaunchKernelGGL(k_e_half<2>, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
+        else
+            hipLaunchKernelGGL(k_e_half<1>, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
     } else {
         const unsigned gx = (unsigned)std::max(1LL, (G.max_dft_items + 255) / 256);
         hipLaunchKernelGGL(k_dft, dim3(gx, G.nmodels), dim3(256), 0, G.stream, G.d_descs);
@@ -884,17 +935,25 @@ extern "C" int fdtd_sim_set_stepper(FdtdSim *s, int fused) {
     CHECK(t == 0, "the stepper can only change before the first step (reset first)");
     if (s->h.fused == fused) return FDTD_OK;
     s->h.fused = fused;
-    s->self.launch = Launch();
-    if (fused) s->self.launch.ty = 8;
+    s->self.launch = default_launch(fused);
     return sim_refresh(s);
 }
 
-extern "C" int fdtd_sim_set_launch(FdtdSim *s, int ty, int zchunk) {
+extern "C" int fdtd_sim_set_launch(FdtdSim *s, int ty, int zchunk, int xvec) {
     CHECK(s, "NULL sim");
-    CHECK(zchunk >= 0, "negative zchunk");
+    CHECK(zchunk >= 0 && zchunk < 0x8000, "zchunk out of range");
+    const Launch keep = s->self.launch;
     s->self.launch.ty = ty;
     s->self.launch.zchunk = zchunk;
-    return group_build(s->self);
+    s->self.launch.xvec = xvec;
+    int rc = group_build(s->self);
+    if (rc) {
+        std::string err = g_err;
+        s->self.launch = keep;
+        (void)group_build(s->self);
+        g_err = err;
+    }
+    return rc;
 }
 
 static int time_group(Group &G, int nsteps, bool graph, float *ms_total, float *ms_h, float *ms_e,
@@ -1186,6 +1245,7 @@ extern "C" int fdtd_batch_create(FdtdSim *const *sims, int n, FdtdBatch **out) {
     FdtdBatch *b = new FdtdBatch();
     b->grp.device = sims[0]->device;
     b->grp.members.assign(sims, sims + n);
+    b->grp.launch = default_launch(sims[0]->h.fused);
     (void)hipSetDevice(b->grp.device);
     hipError_t e = hipStreamCreateWithFlags(&b->grp.stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -1214,11 +1274,21 @@ extern "C" void fdtd_batch_destroy(FdtdBatch *b) {
     delete b;
 }
 
-extern "C" int fdtd_batch_set_launch(FdtdBatch *b, int ty, int zchunk) {
+extern "C" int fdtd_batch_set_launch(FdtdBatch *b, int ty, int zchunk, int xvec) {
     CHECK(b, "NULL batch");
+    CHECK(zchunk >= 0 && zchunk < 0x8000, "zchunk out of range");
+    const Launch keep = b->grp.launch;
     b->grp.launch.ty = ty;
     b->grp.launch.zchunk = zchunk;
-    return group_build(b->grp);
+    b->grp.launch.xvec = xvec;
+    int rc = group_build(b->grp);
+    if (rc) {
+        std::string err = g_err;
+        b->grp.launch = keep;
+        (void)group_build(b->grp);
+        g_err = err;
+    }
+    return rc;
 }
 
 extern "C" int fdtd_batch_run(FdtdBatch *b, int nsteps, int use_graph) {

@@ -69,13 +69,16 @@ struct alignas(16) Desc {
     int *step;                     // device step counter
 };
 
-// Launch shape of the stencil kernels: blocks of 64 x ty threads, one cell
-// column per thread.  Split stepper: ty in {1, 2, 4}, tiles 64 x ty.  Fused
-// stepper: ty in {8, 16}, tiles own 63 x (ty - 1) cells (lane 0 and row 0
-// are halo).  zchunk: z cells per tile (0 = whole column).
+// Launch shape of the stencil kernels: blocks of 64 x ty threads marching z.
+// Split stepper: ty in {1, 2, 4}; each lane owns xvec consecutive x cells
+// (1, 2 or 4; 0 = chosen from the grid widths), tiles (64 xvec) x ty.  Fused
+// stepper: ty in {8, 16}, one cell column per lane, tiles own 63 x (ty - 1)
+// cells (lane 0 and row 0 are halo).  zchunk: z cells per tile (0 = whole
+// column).
 struct Launch {
-    int ty = 4;
-    int zchunk = 0;
+    int ty = 2;
+    int zchunk = 16;
+    int xvec = 0;
 };
 
 // Buffer set holding the fields at step t.

@@ -133,13 +133,14 @@ struct HAround {
     float hz_xm, hy_xm;         // x-1
 };
 
-// E half-step of one cell: Yee update with per-cell Ca, Cb
+// E half-step of one cell: Yee update with the cell's Ca, Cb
 // (cpu/field_updates.py:18-63), the six E-CPML corrections d,u,l,r,f,b
 // (:66-191, psi updated in place), then the H_inc injection at tfsf_z (MAIN,
 // cpu/main_sim.py:158-161) or the hard source at the feed-in plane
-// (INCIDENT, cpu/incident_sim.py:142-144).  o is the cell's field offset.
-__device__ __forceinline__ void e_cell(const Desc *__restrict__ d, int x, int y, int z, int t, long long o,
-                                       const HAround &h, float &ex, float &ey, float &ez) {
+// (INCIDENT, cpu/incident_sim.py:142-144).  ca, cb are read only below the
+// top plane (z < nz - 1).
+__device__ __forceinline__ void e_cell(const Desc *__restrict__ d, int x, int y, int z, int t, float ca,
+                                       float cb, const HAround &h, float &ex, float &ey, float &ez) {
     const int nx = d->nx, ny = d->ny, nz = d->nz, np = d->npml;
     const long long sy = d->sy;
     const float dx = d->dx;
@@ -147,9 +148,6 @@ __device__ __forceinline__ void e_cell(const Desc *__restrict__ d, int x, int y,
     const bool xin = x < nx - 1, yin = y < ny - 1;
     float *const *psi = d->psi[0];
     if (z < nz - 1) {
-        const bool uni = z >= d->uni_z;                        // uniform (e.g. vacuum) plane
-        const float ca = uni ? d->ca_u : d->ca[o];
-        const float cb = uni ? d->cb_u : d->cb[o];
         if (zlo && ylo && yin && xin) ex = (ex * ca) + (cb * ((h.hz_c - h.hz_ym) - (h.hy_c - h.hy_zm)));
         if (zlo && yin && xlo && xin) ey = (ey * ca) + (cb * ((h.hx_c - h.hx_zm) - (h.hz_c - h.hz_xm)));
         if (ylo && yin && xlo && xin) ez = (ez * ca) + (cb * ((h.hy_c - h.hy_xm) - (h.hx_c - h.hx_ym)));
@@ -236,19 +234,60 @@ __device__ __forceinline__ void e_cell(const Desc *__restrict__ d, int x, int y,
 // ---------------------------------------------------------------------------
 // Split stepper: H half-step kernel, then E half-step kernel, fields updated
 // in place (set 0).  Used for INCIDENT runs (E before H) and available for
-// MAIN runs.  One thread owns an (x, y) column of a 64 x ty tile and marches
-// z upward, reusing the plane above (H half: ex, ey) or below (E half: hx,
-// hy) from registers.
+// MAIN runs.  A block of 64 x ty lanes covers a (64 V) x ty tile: each lane
+// owns V consecutive x cells of one row, loads and stores them as one
+// V-wide access (16 bytes for V = 4), and marches z upward, reusing the
+// plane above (H half: ex, ey) or below (E half: hx, hy) from registers.
+// The x +- 1 neighbours come from the lane's own cells or, at its ends, from
+// the adjacent lane by a wavefront shuffle (from memory at the tile edge).
+// Rows are padded to a multiple of 16 floats, so a lane's V cells are
+// either all inside the row or all in its padding.
 // ---------------------------------------------------------------------------
+template <int V>
+__device__ __forceinline__ void vld(float (&r)[V], const float *p) {
+    if constexpr (V == 4) {
+        const float4 q = *reinterpret_cast<const float4 *>(p);
+        r[0] = q.x; r[1] = q.y; r[2] = q.z; r[3] = q.w;
+    } else if constexpr (V == 2) {
+        const float2 q = *reinterpret_cast<const float2 *>(p);
+        r[0] = q.x; r[1] = q.y;
+    } else {
+        r[0] = *p;
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void vst(float *p, const float (&r)[V]) {
+    if constexpr (V == 4) {
+        *reinterpret_cast<float4 *>(p) = make_float4(r[0], r[1], r[2], r[3]);
+    } else if constexpr (V == 2) {
+        *reinterpret_cast<float2 *>(p) = make_float2(r[0], r[1]);
+    } else {
+        *p = r[0];
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void vset(float (&r)[V], float v) {
+#pragma unroll
+    for (int k = 0; k < V; ++k) r[k] = v;
+}
+
+template <int V>
 __global__ void __launch_bounds__(256)
 k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles) {
     const int4 tl = tiles[xcd_tile(blockIdx.x, ntiles)];
     const Desc *__restrict__ d = descs + tl.x;
     const int t = load_step(d->step);
-    if (t >= d->nt) return;
+    if (t >= d->nt) return;                            // uniform per block
     const int nx = d->nx, ny = d->ny, nz = d->nz;
-    const int x = tl.y + threadIdx.x, y = tl.z + threadIdx.y;
-    if (x >= nx || y >= ny) return;
+    const int y = tl.z + threadIdx.y;
+    if (y >= ny) return;                               // uniform per wavefront (one row)
+    const int xb = tl.y + V * threadIdx.x;             // the lane's first cell
+    const bool act = xb < nx;
+    // the x + 1 neighbour of the lane's last cell is needed when that cell
+    // is below nx - 1; for lane 63 it lies in the next tile
+    const bool next_mem = threadIdx.x == kWave - 1 && xb + V < nx;
     const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
     const long long sy = d->sy, sz = d->sz;
     const float *__restrict__ Ex = d->f[0][0];
@@ -258,42 +297,83 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     float *__restrict__ Hy = d->f[0][4];
     float *__restrict__ Hz = d->f[0][5];
     float *const *psi = d->psi[0];
-    const bool xin = x < nx - 1, yin = y < ny - 1;
+    const bool yin = y < ny - 1;
 
-    long long o = (long long)z0 * sz + (long long)y * sy + x;
-    EAround e;
-    e.ex_c = Ex[o];
-    e.ey_c = Ey[o];
+    long long o = (long long)z0 * sz + (long long)y * sy + xb;
+    float exc[V], eyc[V];
+    vset(exc, 0.f);
+    vset(eyc, 0.f);
+    if (act) {
+        vld(exc, Ex + o);
+        vld(eyc, Ey + o);
+    }
     for (int z = z0; z < z1; ++z, o += sz) {
-        float hx = Hx[o], hy = Hy[o], hz = Hz[o];
-        e.ex_n = e.ey_n = 0.f;
-        if (z < nz - 1) {
-            e.ex_n = Ex[o + sz];
-            e.ey_n = Ey[o + sz];
-            e.ez_c = Ez[o];
-            e.ez_yp = yin ? Ez[o + sy] : 0.f;
-            e.ez_xp = xin ? Ez[o + 1] : 0.f;
-            e.ey_xp = xin ? Ey[o + 1] : 0.f;
-            e.ex_yp = yin ? Ex[o + sy] : 0.f;
+        const bool zin = z < nz - 1;
+        float hx[V], hy[V], hz[V], exn[V], eyn[V], ezc[V], ezyp[V], exyp[V];
+        vset(exn, 0.f);
+        vset(eyn, 0.f);
+        vset(ezc, 0.f);
+        vset(ezyp, 0.f);
+        vset(exyp, 0.f);
+        if (act) {
+            vld(hx, Hx + o);
+            vld(hy, Hy + o);
+            vld(hz, Hz + o);
+            if (zin) {
+                vld(exn, Ex + o + sz);
+                vld(eyn, Ey + o + sz);
+                vld(ezc, Ez + o);
+                if (yin) {
+                    vld(ezyp, Ez + o + sy);
+                    vld(exyp, Ex + o + sy);
+                }
+            }
         }
-        h_cell(d, x, y, z, t, e, hx, hy, hz, psi, psi);
-        Hx[o] = hx;
-        Hy[o] = hy;
-        Hz[o] = hz;
-        e.ex_c = e.ex_n;
-        e.ey_c = e.ey_n;
+        float ey_next = __shfl_down(eyc[0], 1, kWave);
+        float ez_next = __shfl_down(ezc[0], 1, kWave);
+        if (next_mem && zin) {
+            ey_next = Ey[o + V];
+            ez_next = Ez[o + V];
+        }
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            if (xb + k < nx) {
+                EAround e;
+                e.ex_c = exc[k]; e.ex_n = exn[k]; e.ex_yp = exyp[k];
+                e.ey_c = eyc[k]; e.ey_n = eyn[k];
+                e.ey_xp = k + 1 < V ? eyc[k + 1 < V ? k + 1 : 0] : ey_next;
+                e.ez_c = ezc[k]; e.ez_yp = ezyp[k];
+                e.ez_xp = k + 1 < V ? ezc[k + 1 < V ? k + 1 : 0] : ez_next;
+                h_cell(d, xb + k, y, z, t, e, hx[k], hy[k], hz[k], psi, psi);
+            }
+        }
+        if (act) {
+            vst(Hx + o, hx);
+            vst(Hy + o, hy);
+            vst(Hz + o, hz);
+        }
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            exc[k] = exn[k];
+            eyc[k] = eyn[k];
+        }
     }
 }
 
+template <int V>
 __global__ void __launch_bounds__(256)
 k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles) {
     const int4 tl = tiles[xcd_tile(blockIdx.x, ntiles)];
     const Desc *__restrict__ d = descs + tl.x;
     const int t = load_step(d->step);
-    if (t >= d->nt) return;
+    if (t >= d->nt) return;                            // uniform per block
     const int nx = d->nx, ny = d->ny, nz = d->nz;
-    const int x = tl.y + threadIdx.x, y = tl.z + threadIdx.y;
-    if (x >= nx || y >= ny) return;
+    const int y = tl.z + threadIdx.y;
+    if (y >= ny) return;                               // uniform per wavefront (one row)
+    const int xb = tl.y + V * threadIdx.x;
+    const bool act = xb < nx;
+    const bool first_lane = threadIdx.x == 0;
+    const bool prev_mem = first_lane && xb >= 1 && act;   // x - 1 lies in the previous tile
     const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
     const long long sy = d->sy, sz = d->sz;
     float *__restrict__ Ex = d->f[0][0];
@@ -302,34 +382,88 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     const float *__restrict__ Hx = d->f[0][3];
     const float *__restrict__ Hy = d->f[0][4];
     const float *__restrict__ Hz = d->f[0][5];
-    const bool xlo = x >= 1, ylo = y >= 1;
+    const bool ylo = y >= 1;
+    const int uni_z = d->uni_z;
+    const float ca_u = d->ca_u, cb_u = d->cb_u;
 
-    long long o = (long long)z0 * sz + (long long)y * sy + x;
-    HAround h;
-    h.hx_zm = h.hy_zm = 0.f;
-    if (z0 >= 1) {
-        h.hx_zm = Hx[o - sz];
-        h.hy_zm = Hy[o - sz];
+    long long o = (long long)z0 * sz + (long long)y * sy + xb;
+    float hxzm[V], hyzm[V];
+    vset(hxzm, 0.f);
+    vset(hyzm, 0.f);
+    if (act && z0 >= 1) {
+        vld(hxzm, Hx + o - sz);
+        vld(hyzm, Hy + o - sz);
     }
     for (int z = z0; z < z1; ++z, o += sz) {
-        h.hx_c = Hx[o];
-        h.hy_c = Hy[o];
-        float ex = Ex[o], ey = Ey[o], ez = Ez[o];
-        if (z < nz - 1) {
-            h.hz_c = Hz[o];
-            h.hz_ym = ylo ? Hz[o - sy] : 0.f;
-            h.hz_xm = xlo ? Hz[o - 1] : 0.f;
-            h.hx_ym = ylo ? Hx[o - sy] : 0.f;
-            h.hy_xm = xlo ? Hy[o - 1] : 0.f;
+        const bool zin = z < nz - 1;
+        const bool mat = zin && z < uni_z;             // plane holds per-cell Ca/Cb
+        float hxc[V], hyc[V], hzc[V], hzym[V], hxym[V], ex[V], ey[V], ez[V], ca[V], cb[V];
+        vset(hxc, 0.f);
+        vset(hyc, 0.f);
+        vset(hzc, 0.f);
+        vset(hzym, 0.f);
+        vset(hxym, 0.f);
+        vset(ca, ca_u);
+        vset(cb, cb_u);
+        if (act) {
+            vld(hxc, Hx + o);
+            vld(hyc, Hy + o);
+            vld(ex, Ex + o);
+            vld(ey, Ey + o);
+            vld(ez, Ez + o);
+            if (zin) {
+                vld(hzc, Hz + o);
+                if (ylo) {
+                    vld(hzym, Hz + o - sy);
+                    vld(hxym, Hx + o - sy);
+                }
+                if (mat) {
+                    vld(ca, d->ca + o);
+                    vld(cb, d->cb + o);
+                }
+            }
         }
-        e_cell(d, x, y, z, t, o, h, ex, ey, ez);
-        Ex[o] = ex;
-        Ey[o] = ey;
-        Ez[o] = ez;
-        h.hx_zm = h.hx_c;
-        h.hy_zm = h.hy_c;
+        float hz_prev = __shfl_up(hzc[V - 1], 1, kWave);
+        float hy_prev = __shfl_up(hyc[V - 1], 1, kWave);
+        if (first_lane) {
+            hz_prev = 0.f;
+            hy_prev = 0.f;
+            if (prev_mem && zin) {
+                hz_prev = Hz[o - 1];
+                hy_prev = Hy[o - 1];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            if (xb + k < nx) {
+                HAround h;
+                h.hx_c = hxc[k]; h.hy_c = hyc[k]; h.hz_c = hzc[k];
+                h.hx_zm = hxzm[k]; h.hy_zm = hyzm[k];
+                h.hz_ym = hzym[k]; h.hx_ym = hxym[k];
+                h.hz_xm = k > 0 ? hzc[k > 0 ? k - 1 : 0] : hz_prev;
+                h.hy_xm = k > 0 ? hyc[k > 0 ? k - 1 : 0] : hy_prev;
+                e_cell(d, xb + k, y, z, t, ca[k], cb[k], h, ex[k], ey[k], ez[k]);
+            }
+        }
+        if (act) {
+            vst(Ex + o, ex);
+            vst(Ey + o, ey);
+            vst(Ez + o, ez);
+        }
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            hxzm[k] = hxc[k];
+            hyzm[k] = hyc[k];
+        }
     }
 }
+
+template __global__ void k_h_half<1>(const Desc *, const int4 *, int);
+template __global__ void k_h_half<2>(const Desc *, const int4 *, int);
+template __global__ void k_h_half<4>(const Desc *, const int4 *, int);
+template __global__ void k_e_half<1>(const Desc *, const int4 *, int);
+template __global__ void k_e_half<2>(const Desc *, const int4 *, int);
+template __global__ void k_e_half<4>(const Desc *, const int4 *, int);
 
 // ---------------------------------------------------------------------------
 // Fused stepper (MAIN): one kernel per time step computes H(t+1/2) and then
@@ -426,7 +560,12 @@ k_step_fused(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int
             h.hx_ym = lds_hx[buf][ly - 1][lx];
             h.hz_xm = hz_xm; h.hy_xm = hy_xm;
             float ex = e.ex_c, ey = e.ey_c, ez = e.ez_c;
-            e_cell(d, x, y, z, t, o, h, ex, ey, ez);
+            float ca = d->ca_u, cb = d->cb_u;
+            if (zin && z < d->uni_z) {
+                ca = d->ca[o];
+                cb = d->cb[o];
+            }
+            e_cell(d, x, y, z, t, ca, cb, h, ex, ey, ez);
             Exn[o] = ex;
             Eyn[o] = ey;
             Ezn[o] = ez;

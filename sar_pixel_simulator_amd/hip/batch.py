@@ -124,8 +124,6 @@ class DeviceBatch:
     def __init__(self, models, launch=None):
         self.models = list(models)
         self.batch = native.Batch([m.sim for m in self.models])
-        if launch is None and common.FUSED_STEPPER:
-            launch = (8, 0)
         if launch:
             self.batch.set_launch(*launch)
 

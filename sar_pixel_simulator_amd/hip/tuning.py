@@ -1,23 +1,24 @@
 """Tile-shape selection for the stencil kernels.
 
-A launch shape is (ty, zchunk): blocks of 64 x ty threads, one cell column
-per thread.  Split stepper: ty in {1, 2, 4}.  Fused stepper: ty in {8, 16}
-(row 0 and lane 0 of a block are a recomputed halo).  zchunk: z cells per
-tile (0 = whole column).  ``autotune`` measures candidates on the device and
-records the best per (stepper, arch, grid) in a JSON cache; ``launch_for``
-reads it, falling back to a default.
+A launch shape is (ty, zchunk, xvec): blocks of 64 x ty lanes marching z.
+Split stepper: ty in {1, 2, 4}, each lane owning xvec consecutive x cells
+(1, 2 or 4; 0 = chosen by the library from the grid width).  Fused stepper:
+ty in {8, 16}, one column per lane (row 0 and lane 0 of a block are a
+recomputed halo).  zchunk: z cells per tile (0 = whole column).
+``autotune`` measures candidates on the device and records the best per
+(stepper, arch, grid) in a JSON cache; ``launch_for`` reads it, falling back
+to a default.
 """
 import json
 import os
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'tuning_cache.json')
 CANDIDATES = {
-    'fused': [(8, 0), (16, 0), (8, 64), (8, 32), (16, 64), (16, 32), (8, 24), (16, 24),
-              (8, 16), (16, 16)],
-    'split': [(1, 0), (2, 0), (4, 0), (1, 64), (2, 64), (4, 64), (4, 32), (2, 32), (1, 32),
-              (2, 16), (4, 16)],
+    'fused': [(8, 16, 1), (16, 16, 1), (8, 24, 1), (16, 24, 1), (8, 32, 1), (8, 0, 1)],
+    'split': [(ty, zc, xv) for xv in (4, 2) for ty in (1, 2, 4) for zc in (8, 16, 32)]
+             + [(2, 16, 1)],
 }
-DEFAULT = {'fused': (8, 0), 'split': (4, 0)}
+DEFAULT = {'fused': (8, 16, 1), 'split': (2, 16, 0)}
 
 
 def _key(nx, ny, nz, fused, arch='gfx950'):
@@ -36,7 +37,7 @@ def launch_for(nx, ny, nz, fused=True, arch='gfx950'):
     """Cached choice for this grid, else the stepper's default."""
     hit = _load().get(_key(nx, ny, nz, fused, arch))
     if hit:
-        return int(hit['ty']), int(hit['zchunk'])
+        return int(hit['ty']), int(hit['zchunk']), int(hit.get('xvec', 0))
     return DEFAULT['fused' if fused else 'split']
 
 
@@ -45,19 +46,19 @@ def autotune(sim, nx, ny, nz, fused=True, arch='gfx950', steps=10, candidates=No
     is already chosen) and keep the fastest.  Returns ((ty, zchunk),
     {shape: ms_per_step})."""
     results = {}
-    for ty, zc in candidates or CANDIDATES['fused' if fused else 'split']:
-        sim.set_launch(ty, zc)
+    for shape in candidates or CANDIDATES['fused' if fused else 'split']:
+        sim.set_launch(*shape)
         sim.reset()
         sim.run(2)
         sim.sync()
         ms = sim.time_steps(steps, per_kernel=False)['total'] / steps
-        results[(ty, zc)] = ms
+        results[tuple(shape)] = ms
     best = min(results, key=results.get)
     sim.set_launch(*best)
     if save:
         data = _load()
         data[_key(nx, ny, nz, fused, arch)] = {'ty': best[0], 'zchunk': best[1],
-                                               'ms_per_step': results[best]}
+                                               'xvec': best[2], 'ms_per_step': results[best]}
         tmp = CACHE + '.tmp'
         with open(tmp, 'w') as f:
             json.dump(data, f, indent=1, sort_keys=True)
@@ -101,5 +102,6 @@ if __name__ == '__main__':
             best, res = _tune_workload(name, fused)
             tag = 'fused' if fused else 'split'
             for shape, ms in sorted(res.items(), key=lambda kv: kv[1]):
-                print(f'{name} {tag} ty={shape[0]} zchunk={shape[1]}: {ms:.4f} ms/step', flush=True)
+                print(f'{name} {tag} ty={shape[0]} zchunk={shape[1]} xvec={shape[2]}: '
+                      f'{ms:.4f} ms/step', flush=True)
             print(f'{name} {tag}: best {best}', flush=True)

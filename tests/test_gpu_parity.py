@@ -76,10 +76,15 @@ def test_fields_psi_phasors_bitwise(soil60, fused):
     sim.destroy()
 
 
-@pytest.mark.parametrize('fused,launch', [(False, (1, 0)), (False, (2, 0)), (False, (4, 0)),
-                                          (False, (4, 7)), (False, (1, 5)), (True, (8, 0)),
-                                          (True, (16, 0)), (True, (8, 7)), (True, (16, 5)),
-                                          (True, (8, 1))])
+# (ty, zchunk, xvec): split stepper with 1, 2 and 4 x cells per lane
+# (33 cells: one partly idle tile), fused stepper with its halo tiles
+@pytest.mark.parametrize('fused,launch', [(False, (1, 0, 1)), (False, (2, 0, 2)),
+                                          (False, (4, 0, 4)), (False, (4, 7, 4)),
+                                          (False, (1, 5, 2)), (False, (2, 16, 4)),
+                                          (False, (2, 3, 1)), (False, (1, 1, 4)),
+                                          (True, (8, 0, 1)), (True, (16, 0, 1)),
+                                          (True, (8, 7, 0)), (True, (16, 5, 1)),
+                                          (True, (8, 1, 0))])
 def test_launch_shapes_bitwise(soil60, fused, launch):
     pc, Ca, Cb, beam, ref = soil60
     say('case: fused', fused, 'launch', launch)
@@ -122,11 +127,43 @@ def test_uneven_grid_bitwise():
     say('case: uneven 37x29x41, 40 steps')
     beam = H.synthetic_beam(pc)
     ref = H.oracle_main(pc, Ca, Cb, beam)
-    sim = make_sim(pc, Ca, Cb, beam, launch=(16, 9))
+    sim = make_sim(pc, Ca, Cb, beam, launch=(16, 9, 1))
     sim.run(40)
     sim.sync()
     for n in native.FIELD_NAMES:
         assert_same(sim.field(n), ref.fields[n], n)
+    sim.destroy()
+
+
+@pytest.mark.parametrize('xvec', [4, 2, 1])
+def test_wide_grid_lane_widths_bitwise(xvec):
+    # 301 cells in x: two 256-cell tiles at 4 cells per lane (the second
+    # with 12 active lanes), three at 2, five at 1; x +- 1 neighbours cross
+    # lanes and tiles
+    pc, Ca, Cb = H.soil_case(nx=301, ny=25, nz=27, nz_mat=10, steps=30, seed=7)
+    say('case: wide 301x25x27, 30 steps, xvec', xvec)
+    beam = H.synthetic_beam(pc)
+    ref = H.oracle_main(pc, Ca, Cb, beam)
+    sim = make_sim(pc, Ca, Cb, beam, launch=(2, 8, xvec), fused=False)
+    sim.run(30)
+    sim.sync()
+    for n in native.FIELD_NAMES:
+        assert_same(sim.field(n), ref.fields[n], f'{n} xvec {xvec}')
+    for i, n in enumerate(native.PSI_NAMES):
+        assert_same(sim.psi(i), ref.psi[n], f'{n} xvec {xvec}')
+    sim.destroy()
+
+
+def test_invalid_launch_is_rejected_and_previous_kept(soil60):
+    pc, Ca, Cb, beam, ref = soil60
+    sim = make_sim(pc, Ca, Cb, beam, launch=(2, 0, 4), fused=False)
+    with pytest.raises(native.FdtdError):
+        sim.set_launch(8, 0, 4)          # ty 8 is a fused-stepper shape
+    with pytest.raises(native.FdtdError):
+        sim.set_launch(2, 0, 3)
+    sim.run(60)
+    sim.sync()
+    assert_same(sim.field('hz'), ref.fields['hz'], 'hz after rejected shapes')
     sim.destroy()
 
 
@@ -166,23 +203,27 @@ def test_split_and_fused_agree_long_run():
         assert_same(out[1][1][n], out[0][1][n], n)
 
 
-def test_batch_equals_solo():
+@pytest.mark.parametrize('fused,launch', [(True, None), (False, None), (False, (2, 8, 4))],
+                         ids=['fused', 'split-auto', 'split-4-per-lane'])
+def test_batch_equals_solo(fused, launch):
     cases = [H.soil_case(nx=33, ny=31, nz=35, nz_mat=14, steps=30, seed=s) for s in (1, 2)]
     cases.append(H.soil_case(nx=45, ny=39, nz=37, nz_mat=15, steps=30, seed=5))
-    say('case: batch of 3 uneven sims, 30 steps')
+    say('case: batch of 3 uneven sims, 30 steps, fused', fused, 'launch', launch)
     solo, batch_sims = [], []
     for pc, Ca, Cb in cases:
         beam = H.synthetic_beam(pc)
-        s = make_sim(pc, Ca, Cb, beam)
+        s = make_sim(pc, Ca, Cb, beam, fused=fused)
         s.run(30)
         s.sync()
         solo.append({n: s.field(n) for n in native.FIELD_NAMES})
         s.destroy()
-        batch_sims.append(make_sim(pc, Ca, Cb, beam))
+        batch_sims.append(make_sim(pc, Ca, Cb, beam, fused=fused))
     for use_graph in (False, True):
         for s in batch_sims:
             s.reset()
         b = native.Batch(batch_sims)
+        if launch:
+            b.set_launch(*launch)
         b.run(17, use_graph=use_graph)
         b.run(13, use_graph=use_graph)
         b.sync()
