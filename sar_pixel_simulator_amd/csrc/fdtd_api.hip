@@ -21,8 +21,8 @@ template <int V>
 __global__ void k_e_half(const Desc *, const int4 *, int);
 __global__ void k_dft(const Desc *);
 __global__ void k_advance(const Desc *, int);
-template <int R>
-__global__ void k_step_fused(const Desc *, const int4 *, int);
+template <int V, int R>
+__global__ void k_step_strip(const Desc *, const int4 *, int);
 __global__ void k_phasor_out(const float4 *, long long, float2 *);
 __global__ void k_n2f_partial(const float2 *, const float2 *, const float2 *, const float2 *, int, int, int,
                               const double2 *, const double2 *, double *);
@@ -96,10 +96,7 @@ static void beam_unref(FdtdBeam *b) {
 // Launch shape a simulation or batch starts with, per stepper.
 static Launch default_launch(int fused) {
     Launch l;
-    if (fused) {
-        l.ty = 8;
-        l.xvec = 1;
-    }
+    if (fused) l.ty = 4;
     return l;
 }
 
@@ -295,21 +292,19 @@ static int group_build(Group &G) {
     G.mode = G.members[0]->g.mode;
     G.fused = G.members[0]->h.fused;
     const int xv = G.launch.xvec;
-    if (G.fused) {
-        CHECK(ty == 8 || ty == 16, "fused stepper: ty (rows per block) must be 8 or 16 (got %d)", ty);
-        CHECK(xv == 0 || xv == 1, "fused stepper: one x cell per lane (xvec 0 or 1, got %d)", xv);
-        G.xvec = 1;
-    } else {
+    if (G.fused)
+        CHECK(ty == 2 || ty == 4 || ty == 8, "fused stepper: ty (rows per wavefront) must be 2, 4 or 8 (got %d)",
+              ty);
+    else
         CHECK(ty == 1 || ty == 2 || ty == 4, "split stepper: ty must be 1, 2 or 4 (got %d)", ty);
-        CHECK(xv == 0 || xv == 1 || xv == 2 || xv == 4, "xvec must be 0, 1, 2 or 4 (got %d)", xv);
-        G.xvec = xv ? xv : auto_xvec(G.members);
-        for (FdtdSim *s : G.members)
-            CHECK(s->h.sy % G.xvec == 0, "row pitch %lld does not hold whole %d-cell lanes", s->h.sy, G.xvec);
-    }
-    // fused tiles own 63 x (ty - 1) cells (lane 0 and row 0 are halo),
-    // split tiles (64 xvec) x ty
-    const int tx = G.fused ? kWave - 1 : kWave * G.xvec;
-    const int tys = G.fused ? ty - 1 : ty;
+    CHECK(xv == 0 || xv == 1 || xv == 2 || xv == 4, "xvec must be 0, 1, 2 or 4 (got %d)", xv);
+    G.xvec = xv ? xv : auto_xvec(G.members);
+    for (FdtdSim *s : G.members)
+        CHECK(s->h.sy % G.xvec == 0, "row pitch %lld does not hold whole %d-cell lanes", s->h.sy, G.xvec);
+    // tiles of (64 xvec) x ty cells: split, one row per wavefront of a
+    // 64 x ty block; fused, one wavefront per tile, four tiles per block
+    const int tx = kWave * G.xvec;
+    const int tys = ty;
     G.max_dft_items = 0;
     for (size_t m = 0; m < G.members.size(); ++m) {
         FdtdSim *s = G.members[m];
@@ -357,12 +352,13 @@ enum KernelId { KH = 0, KE = 1, KD = 2, KF = 3 };
 static void launch_kernel(Group &G, KernelId k) {
     const dim3 blk(kWave, G.launch.ty);
     if (k == KF) {
-        if (G.launch.ty == 16)
-            hipLaunchKernelGGL(k_step_fused<16>, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles,
-                               G.ntiles);
-        else
-            hipLaunchKernelGGL(k_step_fused<8>, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles,
-                               G.ntiles);
+        const dim3 grid((G.ntiles + 3) / 4), b4(kWave, 4);
+        auto *fn = &k_step_strip<1, 2>;
+        const int r = G.launch.ty, v = G.xvec;
+        if (v == 4) fn = r == 8 ? &k_step_strip<4, 8> : r == 4 ? &k_step_strip<4, 4> : &k_step_strip<4, 2>;
+        else if (v == 2) fn = r == 8 ? &k_step_strip<2, 8> : r == 4 ? &k_step_strip<2, 4> : &k_step_strip<2, 2>;
+        else fn = r == 8 ? &k_step_strip<1, 8> : r == 4 ? &k_step_strip<1, 4> : &k_step_strip<1, 2>;
+        hipLaunchKernelGGL(fn, grid, b4, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
     } else if (k == KH) {
         if (G.xvec == 4)
             hipLaunchKernelGGL(k_h_half<4>, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);

@@ -69,12 +69,12 @@ struct alignas(16) Desc {
     int *step;                     // device step counter
 };
 
-// Launch shape of the stencil kernels: blocks of 64 x ty threads marching z.
-// Split stepper: ty in {1, 2, 4}; each lane owns xvec consecutive x cells
-// (1, 2 or 4; 0 = chosen from the grid widths), tiles (64 xvec) x ty.  Fused
-// stepper: ty in {8, 16}, one cell column per lane, tiles own 63 x (ty - 1)
-// cells (lane 0 and row 0 are halo).  zchunk: z cells per tile (0 = whole
-// column).
+// Launch shape of the stencil kernels.  Each lane owns xvec consecutive x
+// cells (1, 2 or 4; 0 = chosen from the grid widths) and marches z over
+// zchunk cells (0 = whole column); a tile is (64 xvec) x ty cells.  Split
+// stepper: ty in {1, 2, 4}, one row per wavefront of a 64 x ty block.
+// Fused stepper: ty in {2, 4, 8} rows walked by one wavefront, four
+// wavefronts (tiles) per block.
 struct Launch {
     int ty = 2;
     int zchunk = 16;

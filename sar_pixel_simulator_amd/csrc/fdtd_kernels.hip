@@ -29,6 +29,49 @@ __device__ __forceinline__ int load_step(const int *p) {
 // Per-cell updates shared by the split and the fused steppers.
 // ---------------------------------------------------------------------------
 
+// Field and CPML arrays are addressed as global memory: loads and stores
+// then compile to global_load/global_store, which the waitcnt logic tracks
+// apart from the scalar loads of the descriptor.
+typedef __attribute__((address_space(1))) float gfloat;
+
+__device__ __forceinline__ gfloat *gp(float *p) { return (gfloat *)p; }
+__device__ __forceinline__ const gfloat *gp(const float *p) { return (const gfloat *)p; }
+
+// Scalars of one simulation at one time step, read from the descriptor once
+// per kernel so that the cell updates take them from registers.
+struct StepK {
+    int nx, ny, nz, np;
+    int tfsf_z;
+    bool inj;                      // MAIN and t < nt_inc: incident-field injection
+    bool hard;                     // INCIDENT and t < n_src: hard source
+    float Db, dtm0, dx, dte0dx;
+    long long sy, du_sz, lr_sz, fb_sz;
+    const gfloat *b0, *b1, *b2, *b3;   // plane t of the beam (ex ey hx hy) or source (ex ey)
+};
+
+__device__ __forceinline__ StepK step_k(const Desc *__restrict__ d, int t) {
+    StepK k;
+    k.nx = d->nx; k.ny = d->ny; k.nz = d->nz; k.np = d->npml;
+    k.tfsf_z = d->tfsf_z;
+    k.inj = d->mode == FDTD_MODE_MAIN && t < d->nt_inc;
+    k.hard = d->mode != FDTD_MODE_MAIN && t < d->n_src;
+    k.Db = d->Db; k.dtm0 = d->dtm0; k.dx = d->dx; k.dte0dx = d->dte0dx;
+    k.sy = d->sy; k.du_sz = d->du_sz; k.lr_sz = d->lr_sz; k.fb_sz = d->fb_sz;
+    const long long plane = (long long)d->ny * d->nx * t;
+    k.b0 = k.b1 = k.b2 = k.b3 = nullptr;
+    if (k.inj) {
+        const gfloat *b = gp(d->beam) + plane;
+        k.b0 = b;
+        k.b1 = b + d->beam_cs;
+        k.b2 = b + 2 * d->beam_cs;
+        k.b3 = b + 3 * d->beam_cs;
+    } else if (k.hard) {
+        k.b0 = gp(d->src) + plane;
+        k.b1 = gp(d->src) + d->src_cs + plane;
+    }
+    return k;
+}
+
 // E values around an H node (x, y, z) at the old time level.
 struct EAround {
     float ex_c, ex_n, ex_yp;   // ex(z), ex(z+1), ex(y+1)
@@ -43,86 +86,113 @@ struct EAround {
 // them in sequence per cell gives the values of the reference's separate
 // passes exactly.  psi is read from psi_rd and, when psi_wr is not null,
 // the updated value is written there (psi_wr == psi_rd updates in place).
-__device__ __forceinline__ void h_cell(const Desc *__restrict__ d, int x, int y, int z, int t,
+__device__ __noinline__ void h_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
+                                   EAround e, float *__restrict__ h, float *const *psi_rd,
+                                   float *const *psi_wr);
+
+__device__ __forceinline__ void h_cell(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
                                        const EAround &e, float &hx, float &hy, float &hz,
                                        float *const *psi_rd, float *const *psi_wr) {
-    const int nx = d->nx, ny = d->ny, nz = d->nz, np = d->npml;
-    const long long sy = d->sy;
-    const float Db = d->Db, dtm0 = d->dtm0, dx = d->dx;
+    const int nx = k.nx, ny = k.ny, nz = k.nz, np = k.np;
+    const float Db = k.Db;
     const bool xin = x < nx - 1, yin = y < ny - 1;
     if (z < nz - 1) {
         if (x >= 1 && xin && yin) hx -= Db * ((e.ez_yp - e.ez_c) - (e.ey_n - e.ey_c));
         if (y >= 1 && yin && xin) hy -= Db * ((e.ex_n - e.ex_c) - (e.ez_xp - e.ez_c));
         if (z >= 1 && yin && xin) hz -= Db * ((e.ey_xp - e.ey_c) - (e.ex_yp - e.ex_c));
-        if (yin && xin) {
-            if (z < np - 1) {                                       // down
-                const float bh = d->prof[BH_DLF][z], ch = d->prof[CH_DLF][z];
-                const long long p = (long long)z * d->du_sz + (long long)y * sy + x;
-                const float a = (bh * psi_rd[pHyz_d][p]) + ((ch * (e.ex_n - e.ex_c)) / dx);
-                const float b = (bh * psi_rd[pHxz_d][p]) + ((ch * (e.ey_n - e.ey_c)) / dx);
-                if (psi_wr) { psi_wr[pHyz_d][p] = a; psi_wr[pHxz_d][p] = b; }
-                hy -= dtm0 * a;
-                hx += dtm0 * b;
-            }
-            if (z >= nz - np) {                                     // up
-                const int zl = z - (nz - np);
-                const float bh = d->prof[BH_URB][zl], ch = d->prof[CH_URB][zl];
-                const long long p = (long long)zl * d->du_sz + (long long)y * sy + x;
-                const float a = (bh * psi_rd[pHyz_u][p]) + ((ch * (e.ex_n - e.ex_c)) / dx);
-                const float b = (bh * psi_rd[pHxz_u][p]) + ((ch * (e.ey_n - e.ey_c)) / dx);
-                if (psi_wr) { psi_wr[pHyz_u][p] = a; psi_wr[pHxz_u][p] = b; }
-                hy -= dtm0 * a;
-                hx += dtm0 * b;
-            }
-        }
-        if (yin) {
-            if (x < np - 1) {                                       // left
-                const float bh = d->prof[BH_DLF][x], ch = d->prof[CH_DLF][x];
-                const long long p = (long long)z * d->lr_sz + (long long)y * np + x;
-                const float a = (bh * psi_rd[pHzx_l][p]) + ((ch * (e.ey_xp - e.ey_c)) / dx);
-                const float b = (bh * psi_rd[pHyx_l][p]) + ((ch * (e.ez_xp - e.ez_c)) / dx);
-                if (psi_wr) { psi_wr[pHzx_l][p] = a; psi_wr[pHyx_l][p] = b; }
-                hz -= dtm0 * a;
-                hy += dtm0 * b;
-            }
-            if (x >= nx - np && xin) {                              // right
-                const int xl = x - (nx - np);
-                const float bh = d->prof[BH_URB][xl], ch = d->prof[CH_URB][xl];
-                const long long p = (long long)z * d->lr_sz + (long long)y * np + xl;
-                const float a = (bh * psi_rd[pHzx_r][p]) + ((ch * (e.ey_xp - e.ey_c)) / dx);
-                const float b = (bh * psi_rd[pHyx_r][p]) + ((ch * (e.ez_xp - e.ez_c)) / dx);
-                if (psi_wr) { psi_wr[pHzx_r][p] = a; psi_wr[pHyx_r][p] = b; }
-                hz -= dtm0 * a;
-                hy += dtm0 * b;
-            }
-        }
-        if (xin) {
-            if (y < np - 1) {                                       // front
-                const float bh = d->prof[BH_DLF][y], ch = d->prof[CH_DLF][y];
-                const long long p = (long long)z * d->fb_sz + (long long)y * sy + x;
-                const float a = (bh * psi_rd[pHxy_f][p]) + ((ch * (e.ez_yp - e.ez_c)) / dx);
-                const float b = (bh * psi_rd[pHzy_f][p]) + ((ch * (e.ex_yp - e.ex_c)) / dx);
-                if (psi_wr) { psi_wr[pHxy_f][p] = a; psi_wr[pHzy_f][p] = b; }
-                hx -= dtm0 * a;
-                hz += dtm0 * b;
-            }
-            if (y >= ny - np && yin) {                              // back
-                const int yl = y - (ny - np);
-                const float bh = d->prof[BH_URB][yl], ch = d->prof[CH_URB][yl];
-                const long long p = (long long)z * d->fb_sz + (long long)yl * sy + x;
-                const float a = (bh * psi_rd[pHxy_b][p]) + ((ch * (e.ez_yp - e.ez_c)) / dx);
-                const float b = (bh * psi_rd[pHzy_b][p]) + ((ch * (e.ex_yp - e.ex_c)) / dx);
-                if (psi_wr) { psi_wr[pHxy_b][p] = a; psi_wr[pHzy_b][p] = b; }
-                hx -= dtm0 * a;
-                hz += dtm0 * b;
-            }
+        if (z < np - 1 || z >= nz - np || x < np - 1 || x >= nx - np || y < np - 1 || y >= ny - np) {
+            float h[3] = {hx, hy, hz};
+            h_cpml(d, k, x, y, z, e, h, psi_rd, psi_wr);
+            hx = h[0];
+            hy = h[1];
+            hz = h[2];
         }
     }
-    if (d->mode == FDTD_MODE_MAIN && t < d->nt_inc && z == d->tfsf_z) {
-        const long long q = ((long long)t * ny + y) * nx + x;
-        hx += Db * d->beam[1 * d->beam_cs + q];   // ey_inc
-        hy -= Db * d->beam[0 * d->beam_cs + q];   // ex_inc
+    if (k.inj && z == k.tfsf_z) {
+        const long long q = (long long)y * nx + x;
+        hx += Db * k.b1[q];   // ey_inc
+        hy -= Db * k.b0[q];   // ex_inc
     }
+}
+
+// The six H-CPML corrections of one cell below the top plane, in the
+// reference order d,u,l,r,f,b; h = (hx, hy, hz) after the Yee update.
+// Called only for cells inside an absorber, and not inlined, so that the
+// absorber addressing does not occupy registers in the stencil loops.
+__device__ __noinline__ void h_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
+                                   EAround e, float *__restrict__ h, float *const *psi_rd,
+                                   float *const *psi_wr) {
+    const int nx = k.nx, ny = k.ny, nz = k.nz, np = k.np;
+    const long long sy = k.sy;
+    const float dtm0 = k.dtm0, dx = k.dx;
+    const bool xin = x < nx - 1, yin = y < ny - 1;
+    float hx = h[0], hy = h[1], hz = h[2];
+    if (yin && xin) {
+        if (z < np - 1) {                                       // down
+            const float bh = d->prof[BH_DLF][z], ch = d->prof[CH_DLF][z];
+            const long long p = (long long)z * k.du_sz + (long long)y * sy + x;
+            const float a = (bh * gp(psi_rd[pHyz_d])[p]) + ((ch * (e.ex_n - e.ex_c)) / dx);
+            const float b = (bh * gp(psi_rd[pHxz_d])[p]) + ((ch * (e.ey_n - e.ey_c)) / dx);
+            if (psi_wr) { gp(psi_wr[pHyz_d])[p] = a; gp(psi_wr[pHxz_d])[p] = b; }
+            hy -= dtm0 * a;
+            hx += dtm0 * b;
+        }
+        if (z >= nz - np) {                                     // up
+            const int zl = z - (nz - np);
+            const float bh = d->prof[BH_URB][zl], ch = d->prof[CH_URB][zl];
+            const long long p = (long long)zl * k.du_sz + (long long)y * sy + x;
+            const float a = (bh * gp(psi_rd[pHyz_u])[p]) + ((ch * (e.ex_n - e.ex_c)) / dx);
+            const float b = (bh * gp(psi_rd[pHxz_u])[p]) + ((ch * (e.ey_n - e.ey_c)) / dx);
+            if (psi_wr) { gp(psi_wr[pHyz_u])[p] = a; gp(psi_wr[pHxz_u])[p] = b; }
+            hy -= dtm0 * a;
+            hx += dtm0 * b;
+        }
+    }
+    if (yin) {
+        if (x < np - 1) {                                       // left
+            const float bh = d->prof[BH_DLF][x], ch = d->prof[CH_DLF][x];
+            const long long p = (long long)z * k.lr_sz + (long long)y * np + x;
+            const float a = (bh * gp(psi_rd[pHzx_l])[p]) + ((ch * (e.ey_xp - e.ey_c)) / dx);
+            const float b = (bh * gp(psi_rd[pHyx_l])[p]) + ((ch * (e.ez_xp - e.ez_c)) / dx);
+            if (psi_wr) { gp(psi_wr[pHzx_l])[p] = a; gp(psi_wr[pHyx_l])[p] = b; }
+            hz -= dtm0 * a;
+            hy += dtm0 * b;
+        }
+        if (x >= nx - np && xin) {                              // right
+            const int xl = x - (nx - np);
+            const float bh = d->prof[BH_URB][xl], ch = d->prof[CH_URB][xl];
+            const long long p = (long long)z * k.lr_sz + (long long)y * np + xl;
+            const float a = (bh * gp(psi_rd[pHzx_r])[p]) + ((ch * (e.ey_xp - e.ey_c)) / dx);
+            const float b = (bh * gp(psi_rd[pHyx_r])[p]) + ((ch * (e.ez_xp - e.ez_c)) / dx);
+            if (psi_wr) { gp(psi_wr[pHzx_r])[p] = a; gp(psi_wr[pHyx_r])[p] = b; }
+            hz -= dtm0 * a;
+            hy += dtm0 * b;
+        }
+    }
+    if (xin) {
+        if (y < np - 1) {                                       // front
+            const float bh = d->prof[BH_DLF][y], ch = d->prof[CH_DLF][y];
+            const long long p = (long long)z * k.fb_sz + (long long)y * sy + x;
+            const float a = (bh * gp(psi_rd[pHxy_f])[p]) + ((ch * (e.ez_yp - e.ez_c)) / dx);
+            const float b = (bh * gp(psi_rd[pHzy_f])[p]) + ((ch * (e.ex_yp - e.ex_c)) / dx);
+            if (psi_wr) { gp(psi_wr[pHxy_f])[p] = a; gp(psi_wr[pHzy_f])[p] = b; }
+            hx -= dtm0 * a;
+            hz += dtm0 * b;
+        }
+        if (y >= ny - np && yin) {                              // back
+            const int yl = y - (ny - np);
+            const float bh = d->prof[BH_URB][yl], ch = d->prof[CH_URB][yl];
+            const long long p = (long long)z * k.fb_sz + (long long)yl * sy + x;
+            const float a = (bh * gp(psi_rd[pHxy_b])[p]) + ((ch * (e.ez_yp - e.ez_c)) / dx);
+            const float b = (bh * gp(psi_rd[pHzy_b])[p]) + ((ch * (e.ex_yp - e.ex_c)) / dx);
+            if (psi_wr) { gp(psi_wr[pHxy_b])[p] = a; gp(psi_wr[pHzy_b])[p] = b; }
+            hx -= dtm0 * a;
+            hz += dtm0 * b;
+        }
+    }
+    h[0] = hx;
+    h[1] = hy;
+    h[2] = hz;
 }
 
 // H values around an E node (x, y, z) at the new time level.
@@ -139,117 +209,138 @@ struct HAround {
 // cpu/main_sim.py:158-161) or the hard source at the feed-in plane
 // (INCIDENT, cpu/incident_sim.py:142-144).  ca, cb are read only below the
 // top plane (z < nz - 1).
-__device__ __forceinline__ void e_cell(const Desc *__restrict__ d, int x, int y, int z, int t, float ca,
-                                       float cb, const HAround &h, float &ex, float &ey, float &ez) {
-    const int nx = d->nx, ny = d->ny, nz = d->nz, np = d->npml;
-    const long long sy = d->sy;
-    const float dx = d->dx;
+__device__ __noinline__ void e_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
+                                   float cb, HAround h, float *__restrict__ ef);
+
+__device__ __forceinline__ void e_cell(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
+                                       float ca, float cb, const HAround &h, float &ex, float &ey,
+                                       float &ez) {
+    const int nx = k.nx, ny = k.ny, nz = k.nz, np = k.np;
     const bool xlo = x >= 1, ylo = y >= 1, zlo = z >= 1;
     const bool xin = x < nx - 1, yin = y < ny - 1;
-    float *const *psi = d->psi[0];
     if (z < nz - 1) {
         if (zlo && ylo && yin && xin) ex = (ex * ca) + (cb * ((h.hz_c - h.hz_ym) - (h.hy_c - h.hy_zm)));
         if (zlo && yin && xlo && xin) ey = (ey * ca) + (cb * ((h.hx_c - h.hx_zm) - (h.hz_c - h.hz_xm)));
         if (ylo && yin && xlo && xin) ez = (ez * ca) + (cb * ((h.hy_c - h.hy_xm) - (h.hx_c - h.hx_ym)));
-        const float cbdx = cb * dx;
-        if (yin && xin) {
-            if (zlo && z < np) {                                    // down
-                const float be = d->prof[BE_DLF][z], ce = d->prof[CE_DLF][z];
-                const long long p = (long long)z * d->du_sz + (long long)y * sy + x;
-                const float a = (be * psi[pEyz_d][p]) + ((ce * (h.hx_c - h.hx_zm)) / dx);
-                const float b = (be * psi[pExz_d][p]) + ((ce * (h.hy_c - h.hy_zm)) / dx);
-                psi[pEyz_d][p] = a; psi[pExz_d][p] = b;
-                ey += cbdx * a;
-                ex -= cbdx * b;
-            }
-            if (z >= nz - np) {                                     // up
-                const int zl = z - (nz - np);
-                const float be = d->prof[BE_URB][zl], ce = d->prof[CE_URB][zl];
-                const long long p = (long long)zl * d->du_sz + (long long)y * sy + x;
-                const float a = (be * psi[pEyz_u][p]) + ((ce * (h.hx_c - h.hx_zm)) / dx);
-                const float b = (be * psi[pExz_u][p]) + ((ce * (h.hy_c - h.hy_zm)) / dx);
-                psi[pEyz_u][p] = a; psi[pExz_u][p] = b;
-                ey += cbdx * a;
-                ex -= cbdx * b;
-            }
-        }
-        if (yin) {
-            if (xlo && x < np) {                                    // left
-                const float be = d->prof[BE_DLF][x], ce = d->prof[CE_DLF][x];
-                const long long p = (long long)z * d->lr_sz + (long long)y * np + x;
-                const float a = (be * psi[pEzx_l][p]) + ((ce * (h.hy_c - h.hy_xm)) / dx);
-                const float b = (be * psi[pEyx_l][p]) + ((ce * (h.hz_c - h.hz_xm)) / dx);
-                psi[pEzx_l][p] = a; psi[pEyx_l][p] = b;
-                ez += cbdx * a;
-                ey -= cbdx * b;
-            }
-            if (x >= nx - np && xin) {                              // right
-                const int xl = x - (nx - np);
-                const float be = d->prof[BE_URB][xl], ce = d->prof[CE_URB][xl];
-                const long long p = (long long)z * d->lr_sz + (long long)y * np + xl;
-                const float a = (be * psi[pEzx_r][p]) + ((ce * (h.hy_c - h.hy_xm)) / dx);
-                const float b = (be * psi[pEyx_r][p]) + ((ce * (h.hz_c - h.hz_xm)) / dx);
-                psi[pEzx_r][p] = a; psi[pEyx_r][p] = b;
-                ez += cbdx * a;
-                ey -= cbdx * b;
-            }
-        }
-        if (xin) {
-            if (ylo && y < np) {                                    // front
-                const float be = d->prof[BE_DLF][y], ce = d->prof[CE_DLF][y];
-                const long long p = (long long)z * d->fb_sz + (long long)y * sy + x;
-                const float a = (be * psi[pExy_f][p]) + ((ce * (h.hz_c - h.hz_ym)) / dx);
-                const float b = (be * psi[pEzy_f][p]) + ((ce * (h.hx_c - h.hx_ym)) / dx);
-                psi[pExy_f][p] = a; psi[pEzy_f][p] = b;
-                ex += cbdx * a;
-                ez -= cbdx * b;
-            }
-            if (y >= ny - np && yin) {                              // back
-                const int yl = y - (ny - np);
-                const float be = d->prof[BE_URB][yl], ce = d->prof[CE_URB][yl];
-                const long long p = (long long)z * d->fb_sz + (long long)yl * sy + x;
-                const float a = (be * psi[pExy_b][p]) + ((ce * (h.hz_c - h.hz_ym)) / dx);
-                const float b = (be * psi[pEzy_b][p]) + ((ce * (h.hx_c - h.hx_ym)) / dx);
-                psi[pExy_b][p] = a; psi[pEzy_b][p] = b;
-                ex += cbdx * a;
-                ez -= cbdx * b;
-            }
+        if (z < np || z >= nz - np || x < np || x >= nx - np || y < np || y >= ny - np) {
+            float ef[3] = {ex, ey, ez};
+            e_cpml(d, k, x, y, z, cb, h, ef);
+            ex = ef[0];
+            ey = ef[1];
+            ez = ef[2];
         }
     }
-    if (z == d->tfsf_z) {
-        if (d->mode == FDTD_MODE_MAIN) {
-            if (t < d->nt_inc) {
-                const long long q = ((long long)t * ny + y) * nx + x;
-                ey += d->dte0dx * d->beam[2 * d->beam_cs + q];   // hx_inc
-                ex -= d->dte0dx * d->beam[3 * d->beam_cs + q];   // hy_inc
-            }
-        } else if (t < d->n_src && ylo && yin && xin) {
-            const long long q = ((long long)t * ny + y) * nx + x;
-            ex = d->src[q];
-            ey = d->src[d->src_cs + q];
+    if (z == k.tfsf_z) {
+        const long long q = (long long)y * nx + x;
+        if (k.inj) {
+            ey += k.dte0dx * k.b2[q];   // hx_inc
+            ex -= k.dte0dx * k.b3[q];   // hy_inc
+        } else if (k.hard && ylo && yin && xin) {
+            ex = k.b0[q];
+            ey = k.b1[q];
         }
     }
 }
 
+// The six E-CPML corrections of one cell below the top plane, in the
+// reference order d,u,l,r,f,b (psi updated in place); ef = (ex, ey, ez)
+// after the Yee update.  Called only for cells inside an absorber.
+__device__ __noinline__ void e_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
+                                   float cb, HAround h, float *__restrict__ ef) {
+    const int nx = k.nx, ny = k.ny, nz = k.nz, np = k.np;
+    const long long sy = k.sy;
+    const float dx = k.dx;
+    const bool xlo = x >= 1, ylo = y >= 1, zlo = z >= 1;
+    const bool xin = x < nx - 1, yin = y < ny - 1;
+    float *const *psi = d->psi[0];
+    float ex = ef[0], ey = ef[1], ez = ef[2];
+    const float cbdx = cb * dx;
+    if (yin && xin) {
+        if (zlo && z < np) {                                    // down
+            const float be = d->prof[BE_DLF][z], ce = d->prof[CE_DLF][z];
+            const long long p = (long long)z * k.du_sz + (long long)y * sy + x;
+            const float a = (be * gp(psi[pEyz_d])[p]) + ((ce * (h.hx_c - h.hx_zm)) / dx);
+            const float b = (be * gp(psi[pExz_d])[p]) + ((ce * (h.hy_c - h.hy_zm)) / dx);
+            gp(psi[pEyz_d])[p] = a; gp(psi[pExz_d])[p] = b;
+            ey += cbdx * a;
+            ex -= cbdx * b;
+        }
+        if (z >= nz - np) {                                     // up
+            const int zl = z - (nz - np);
+            const float be = d->prof[BE_URB][zl], ce = d->prof[CE_URB][zl];
+            const long long p = (long long)zl * k.du_sz + (long long)y * sy + x;
+            const float a = (be * gp(psi[pEyz_u])[p]) + ((ce * (h.hx_c - h.hx_zm)) / dx);
+            const float b = (be * gp(psi[pExz_u])[p]) + ((ce * (h.hy_c - h.hy_zm)) / dx);
+            gp(psi[pEyz_u])[p] = a; gp(psi[pExz_u])[p] = b;
+            ey += cbdx * a;
+            ex -= cbdx * b;
+        }
+    }
+    if (yin) {
+        if (xlo && x < np) {                                    // left
+            const float be = d->prof[BE_DLF][x], ce = d->prof[CE_DLF][x];
+            const long long p = (long long)z * k.lr_sz + (long long)y * np + x;
+            const float a = (be * gp(psi[pEzx_l])[p]) + ((ce * (h.hy_c - h.hy_xm)) / dx);
+            const float b = (be * gp(psi[pEyx_l])[p]) + ((ce * (h.hz_c - h.hz_xm)) / dx);
+            gp(psi[pEzx_l])[p] = a; gp(psi[pEyx_l])[p] = b;
+            ez += cbdx * a;
+            ey -= cbdx * b;
+        }
+        if (x >= nx - np && xin) {                              // right
+            const int xl = x - (nx - np);
+            const float be = d->prof[BE_URB][xl], ce = d->prof[CE_URB][xl];
+            const long long p = (long long)z * k.lr_sz + (long long)y * np + xl;
+            const float a = (be * gp(psi[pEzx_r])[p]) + ((ce * (h.hy_c - h.hy_xm)) / dx);
+            const float b = (be * gp(psi[pEyx_r])[p]) + ((ce * (h.hz_c - h.hz_xm)) / dx);
+            gp(psi[pEzx_r])[p] = a; gp(psi[pEyx_r])[p] = b;
+            ez += cbdx * a;
+            ey -= cbdx * b;
+        }
+    }
+    if (xin) {
+        if (ylo && y < np) {                                    // front
+            const float be = d->prof[BE_DLF][y], ce = d->prof[CE_DLF][y];
+            const long long p = (long long)z * k.fb_sz + (long long)y * sy + x;
+            const float a = (be * gp(psi[pExy_f])[p]) + ((ce * (h.hz_c - h.hz_ym)) / dx);
+            const float b = (be * gp(psi[pEzy_f])[p]) + ((ce * (h.hx_c - h.hx_ym)) / dx);
+            gp(psi[pExy_f])[p] = a; gp(psi[pEzy_f])[p] = b;
+            ex += cbdx * a;
+            ez -= cbdx * b;
+        }
+        if (y >= ny - np && yin) {                              // back
+            const int yl = y - (ny - np);
+            const float be = d->prof[BE_URB][yl], ce = d->prof[CE_URB][yl];
+            const long long p = (long long)z * k.fb_sz + (long long)yl * sy + x;
+            const float a = (be * gp(psi[pExy_b])[p]) + ((ce * (h.hz_c - h.hz_ym)) / dx);
+            const float b = (be * gp(psi[pEzy_b])[p]) + ((ce * (h.hx_c - h.hx_ym)) / dx);
+            gp(psi[pExy_b])[p] = a; gp(psi[pEzy_b])[p] = b;
+            ex += cbdx * a;
+            ez -= cbdx * b;
+        }
+    }
+    ef[0] = ex;
+    ef[1] = ey;
+    ef[2] = ez;
+}
+
 // ---------------------------------------------------------------------------
-// Split stepper: H half-step kernel, then E half-step kernel, fields updated
-// in place (set 0).  Used for INCIDENT runs (E before H) and available for
-// MAIN runs.  A block of 64 x ty lanes covers a (64 V) x ty tile: each lane
-// owns V consecutive x cells of one row, loads and stores them as one
-// V-wide access (16 bytes for V = 4), and marches z upward, reusing the
-// plane above (H half: ex, ey) or below (E half: hx, hy) from registers.
-// The x +- 1 neighbours come from the lane's own cells or, at its ends, from
-// the adjacent lane by a wavefront shuffle (from memory at the tile edge).
-// Rows are padded to a multiple of 16 floats, so a lane's V cells are
-// either all inside the row or all in its padding.
+// Lane vectors: each lane owns V consecutive x cells of one row and loads
+// and stores them as one V-wide access.  Rows are padded to a multiple of
+// 16 floats, so a lane's V cells are either all inside the row or all in
+// its padding.
 // ---------------------------------------------------------------------------
+typedef float vf2 __attribute__((ext_vector_type(2)));
+typedef float vf4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) vf2 gvf2;
+typedef __attribute__((address_space(1))) vf4 gvf4;
+
 template <int V>
-__device__ __forceinline__ void vld(float (&r)[V], const float *p) {
+__device__ __forceinline__ void vld(float (&r)[V], const gfloat *p) {
     if constexpr (V == 4) {
-        const float4 q = *reinterpret_cast<const float4 *>(p);
+        const vf4 q = *reinterpret_cast<const gvf4 *>(p);
         r[0] = q.x; r[1] = q.y; r[2] = q.z; r[3] = q.w;
     } else if constexpr (V == 2) {
-        const float2 q = *reinterpret_cast<const float2 *>(p);
+        const vf2 q = *reinterpret_cast<const gvf2 *>(p);
         r[0] = q.x; r[1] = q.y;
     } else {
         r[0] = *p;
@@ -257,11 +348,11 @@ __device__ __forceinline__ void vld(float (&r)[V], const float *p) {
 }
 
 template <int V>
-__device__ __forceinline__ void vst(float *p, const float (&r)[V]) {
+__device__ __forceinline__ void vst(gfloat *p, const float (&r)[V]) {
     if constexpr (V == 4) {
-        *reinterpret_cast<float4 *>(p) = make_float4(r[0], r[1], r[2], r[3]);
+        *reinterpret_cast<gvf4 *>(p) = vf4{r[0], r[1], r[2], r[3]};
     } else if constexpr (V == 2) {
-        *reinterpret_cast<float2 *>(p) = make_float2(r[0], r[1]);
+        *reinterpret_cast<gvf2 *>(p) = vf2{r[0], r[1]};
     } else {
         *p = r[0];
     }
@@ -274,30 +365,80 @@ __device__ __forceinline__ void vset(float (&r)[V], float v) {
 }
 
 template <int V>
+__device__ __forceinline__ void vcopy(float (&r)[V], const float (&s)[V]) {
+#pragma unroll
+    for (int k = 0; k < V; ++k) r[k] = s[k];
+}
+
+// ---------------------------------------------------------------------------
+// Split stepper: H half-step kernel, then E half-step kernel, fields updated
+// in place (set 0).  Used for INCIDENT runs (E before H) and available for
+// MAIN runs.  A block of 64 x ty lanes covers a (64 V) x ty tile, one row
+// per wavefront; each lane marches z upward through the tile's z range,
+// reusing the plane above (H half: ex, ey) or below (E half: hx, hy) from
+// registers, and issues the loads of plane z + 1 before it computes plane z.
+// The x +- 1 neighbours come from the lane's own cells or, at its ends, from
+// the adjacent lane by a wavefront shuffle (from memory at the tile edge).
+// ---------------------------------------------------------------------------
+
+// What the H half-step loads per plane: H(z), E(z+1) (ex, ey), ez(z),
+// ez(y+1), ex(y+1); lane 63: ey, ez at the next tile's first cell.
+template <int V>
+struct HPlane {
+    float hx[V], hy[V], hz[V], exn[V], eyn[V], ez[V], ezyp[V], exyp[V];
+    float ey_x, ez_x;
+};
+
+template <int V>
+__device__ __forceinline__ void h_plane_load(HPlane<V> &P, const gfloat *Ex, const gfloat *Ey,
+                                             const gfloat *Ez, const gfloat *Hx, const gfloat *Hy,
+                                             const gfloat *Hz, long long o, long long sy, long long sz,
+                                             bool act, bool zin, bool yin, bool next_mem) {
+    vset(P.exn, 0.f); vset(P.eyn, 0.f); vset(P.ez, 0.f); vset(P.ezyp, 0.f); vset(P.exyp, 0.f);
+    vset(P.hx, 0.f); vset(P.hy, 0.f); vset(P.hz, 0.f);
+    P.ey_x = P.ez_x = 0.f;
+    if (act) {
+        vld(P.hx, Hx + o);
+        vld(P.hy, Hy + o);
+        vld(P.hz, Hz + o);
+        if (zin) {
+            vld(P.exn, Ex + o + sz);
+            vld(P.eyn, Ey + o + sz);
+            vld(P.ez, Ez + o);
+            if (yin) {
+                vld(P.ezyp, Ez + o + sy);
+                vld(P.exyp, Ex + o + sy);
+            }
+            if (next_mem) {
+                P.ey_x = Ey[o + V];
+                P.ez_x = Ez[o + V];
+            }
+        }
+    }
+}
+
+template <int V>
 __global__ void __launch_bounds__(256)
 k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles) {
     const int4 tl = tiles[xcd_tile(blockIdx.x, ntiles)];
     const Desc *__restrict__ d = descs + tl.x;
     const int t = load_step(d->step);
     if (t >= d->nt) return;                            // uniform per block
-    const int nx = d->nx, ny = d->ny, nz = d->nz;
+    const StepK k = step_k(d, t);
     const int y = tl.z + threadIdx.y;
-    if (y >= ny) return;                               // uniform per wavefront (one row)
-    const int xb = tl.y + V * threadIdx.x;             // the lane's first cell
-    const bool act = xb < nx;
+    if (y >= k.ny) return;                             // uniform per wavefront (one row)
+    const int lane = threadIdx.x;
+    const int xb = tl.y + V * lane;                    // the lane's first cell
+    const bool act = xb < k.nx;
     // the x + 1 neighbour of the lane's last cell is needed when that cell
     // is below nx - 1; for lane 63 it lies in the next tile
-    const bool next_mem = threadIdx.x == kWave - 1 && xb + V < nx;
+    const bool next_mem = lane == kWave - 1 && xb + V < k.nx;
     const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
-    const long long sy = d->sy, sz = d->sz;
-    const float *__restrict__ Ex = d->f[0][0];
-    const float *__restrict__ Ey = d->f[0][1];
-    const float *__restrict__ Ez = d->f[0][2];
-    float *__restrict__ Hx = d->f[0][3];
-    float *__restrict__ Hy = d->f[0][4];
-    float *__restrict__ Hz = d->f[0][5];
+    const long long sy = k.sy, sz = d->sz;
+    const gfloat *Ex = gp(d->f[0][0]), *Ey = gp(d->f[0][1]), *Ez = gp(d->f[0][2]);
+    gfloat *Hx = gp(d->f[0][3]), *Hy = gp(d->f[0][4]), *Hz = gp(d->f[0][5]);
     float *const *psi = d->psi[0];
-    const bool yin = y < ny - 1;
+    const bool yin = y < k.ny - 1;
 
     long long o = (long long)z0 * sz + (long long)y * sy + xb;
     float exc[V], eyc[V];
@@ -307,55 +448,80 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
         vld(exc, Ex + o);
         vld(eyc, Ey + o);
     }
+    HPlane<V> P;
+    h_plane_load(P, Ex, Ey, Ez, Hx, Hy, Hz, o, sy, sz, act, z0 < k.nz - 1, yin, next_mem);
     for (int z = z0; z < z1; ++z, o += sz) {
-        const bool zin = z < nz - 1;
-        float hx[V], hy[V], hz[V], exn[V], eyn[V], ezc[V], ezyp[V], exyp[V];
-        vset(exn, 0.f);
-        vset(eyn, 0.f);
-        vset(ezc, 0.f);
-        vset(ezyp, 0.f);
-        vset(exyp, 0.f);
-        if (act) {
-            vld(hx, Hx + o);
-            vld(hy, Hy + o);
-            vld(hz, Hz + o);
-            if (zin) {
-                vld(exn, Ex + o + sz);
-                vld(eyn, Ey + o + sz);
-                vld(ezc, Ez + o);
-                if (yin) {
-                    vld(ezyp, Ez + o + sy);
-                    vld(exyp, Ex + o + sy);
-                }
-            }
-        }
+        HPlane<V> N;
+        h_plane_load(N, Ex, Ey, Ez, Hx, Hy, Hz, o + sz, sy, sz, act && z + 1 < z1, z + 1 < k.nz - 1,
+                     yin, next_mem);
         float ey_next = __shfl_down(eyc[0], 1, kWave);
-        float ez_next = __shfl_down(ezc[0], 1, kWave);
-        if (next_mem && zin) {
-            ey_next = Ey[o + V];
-            ez_next = Ez[o + V];
+        float ez_next = __shfl_down(P.ez[0], 1, kWave);
+        if (lane == kWave - 1) {
+            ey_next = P.ey_x;
+            ez_next = P.ez_x;
         }
 #pragma unroll
-        for (int k = 0; k < V; ++k) {
-            if (xb + k < nx) {
+        for (int c = 0; c < V; ++c) {
+            if (xb + c < k.nx) {
                 EAround e;
-                e.ex_c = exc[k]; e.ex_n = exn[k]; e.ex_yp = exyp[k];
-                e.ey_c = eyc[k]; e.ey_n = eyn[k];
-                e.ey_xp = k + 1 < V ? eyc[k + 1 < V ? k + 1 : 0] : ey_next;
-                e.ez_c = ezc[k]; e.ez_yp = ezyp[k];
-                e.ez_xp = k + 1 < V ? ezc[k + 1 < V ? k + 1 : 0] : ez_next;
-                h_cell(d, xb + k, y, z, t, e, hx[k], hy[k], hz[k], psi, psi);
+                e.ex_c = exc[c]; e.ex_n = P.exn[c]; e.ex_yp = P.exyp[c];
+                e.ey_c = eyc[c]; e.ey_n = P.eyn[c];
+                e.ey_xp = c + 1 < V ? eyc[c + 1 < V ? c + 1 : 0] : ey_next;
+                e.ez_c = P.ez[c]; e.ez_yp = P.ezyp[c];
+                e.ez_xp = c + 1 < V ? P.ez[c + 1 < V ? c + 1 : 0] : ez_next;
+                h_cell(d, k, xb + c, y, z, e, P.hx[c], P.hy[c], P.hz[c], psi, psi);
             }
         }
         if (act) {
-            vst(Hx + o, hx);
-            vst(Hy + o, hy);
-            vst(Hz + o, hz);
+            vst(Hx + o, P.hx);
+            vst(Hy + o, P.hy);
+            vst(Hz + o, P.hz);
         }
-#pragma unroll
-        for (int k = 0; k < V; ++k) {
-            exc[k] = exn[k];
-            eyc[k] = eyn[k];
+        vcopy(exc, P.exn);
+        vcopy(eyc, P.eyn);
+        P = N;
+    }
+}
+
+// What the E half-step loads per plane: H(z) (hx, hy, hz), hz(y-1),
+// hx(y-1), E(z), Ca/Cb; lane 0: hz, hy at the previous tile's last cell.
+template <int V>
+struct EPlane {
+    float hx[V], hy[V], hz[V], hzym[V], hxym[V], ex[V], ey[V], ez[V], ca[V], cb[V];
+    float hz_p, hy_p;
+};
+
+template <int V>
+__device__ __forceinline__ void e_plane_load(EPlane<V> &Q, const gfloat *Ex, const gfloat *Ey,
+                                             const gfloat *Ez, const gfloat *Hx, const gfloat *Hy,
+                                             const gfloat *Hz, const gfloat *Ca, const gfloat *Cb,
+                                             float ca_u, float cb_u, long long o, long long sy, bool act,
+                                             bool zin, bool mat, bool ylo, bool prev_mem) {
+    vset(Q.hx, 0.f); vset(Q.hy, 0.f); vset(Q.hz, 0.f); vset(Q.hzym, 0.f); vset(Q.hxym, 0.f);
+    vset(Q.ex, 0.f); vset(Q.ey, 0.f); vset(Q.ez, 0.f);
+    vset(Q.ca, ca_u);
+    vset(Q.cb, cb_u);
+    Q.hz_p = Q.hy_p = 0.f;
+    if (act) {
+        vld(Q.hx, Hx + o);
+        vld(Q.hy, Hy + o);
+        vld(Q.ex, Ex + o);
+        vld(Q.ey, Ey + o);
+        vld(Q.ez, Ez + o);
+        if (zin) {
+            vld(Q.hz, Hz + o);
+            if (ylo) {
+                vld(Q.hzym, Hz + o - sy);
+                vld(Q.hxym, Hx + o - sy);
+            }
+            if (mat) {
+                vld(Q.ca, Ca + o);
+                vld(Q.cb, Cb + o);
+            }
+            if (prev_mem) {
+                Q.hz_p = Hz[o - 1];
+                Q.hy_p = Hy[o - 1];
+            }
         }
     }
 }
@@ -367,21 +533,18 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     const Desc *__restrict__ d = descs + tl.x;
     const int t = load_step(d->step);
     if (t >= d->nt) return;                            // uniform per block
-    const int nx = d->nx, ny = d->ny, nz = d->nz;
+    const StepK k = step_k(d, t);
     const int y = tl.z + threadIdx.y;
-    if (y >= ny) return;                               // uniform per wavefront (one row)
-    const int xb = tl.y + V * threadIdx.x;
-    const bool act = xb < nx;
-    const bool first_lane = threadIdx.x == 0;
-    const bool prev_mem = first_lane && xb >= 1 && act;   // x - 1 lies in the previous tile
+    if (y >= k.ny) return;                             // uniform per wavefront (one row)
+    const int lane = threadIdx.x;
+    const int xb = tl.y + V * lane;
+    const bool act = xb < k.nx;
+    const bool prev_mem = lane == 0 && xb >= 1 && act;    // x - 1 lies in the previous tile
     const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
-    const long long sy = d->sy, sz = d->sz;
-    float *__restrict__ Ex = d->f[0][0];
-    float *__restrict__ Ey = d->f[0][1];
-    float *__restrict__ Ez = d->f[0][2];
-    const float *__restrict__ Hx = d->f[0][3];
-    const float *__restrict__ Hy = d->f[0][4];
-    const float *__restrict__ Hz = d->f[0][5];
+    const long long sy = k.sy, sz = d->sz;
+    gfloat *Ex = gp(d->f[0][0]), *Ey = gp(d->f[0][1]), *Ez = gp(d->f[0][2]);
+    const gfloat *Hx = gp(d->f[0][3]), *Hy = gp(d->f[0][4]), *Hz = gp(d->f[0][5]);
+    const gfloat *Ca = gp(d->ca), *Cb = gp(d->cb);
     const bool ylo = y >= 1;
     const int uni_z = d->uni_z;
     const float ca_u = d->ca_u, cb_u = d->cb_u;
@@ -394,67 +557,40 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
         vld(hxzm, Hx + o - sz);
         vld(hyzm, Hy + o - sz);
     }
+    EPlane<V> P;
+    e_plane_load(P, Ex, Ey, Ez, Hx, Hy, Hz, Ca, Cb, ca_u, cb_u, o, sy, act, z0 < k.nz - 1,
+                 z0 < k.nz - 1 && z0 < uni_z, ylo, prev_mem);
     for (int z = z0; z < z1; ++z, o += sz) {
-        const bool zin = z < nz - 1;
-        const bool mat = zin && z < uni_z;             // plane holds per-cell Ca/Cb
-        float hxc[V], hyc[V], hzc[V], hzym[V], hxym[V], ex[V], ey[V], ez[V], ca[V], cb[V];
-        vset(hxc, 0.f);
-        vset(hyc, 0.f);
-        vset(hzc, 0.f);
-        vset(hzym, 0.f);
-        vset(hxym, 0.f);
-        vset(ca, ca_u);
-        vset(cb, cb_u);
-        if (act) {
-            vld(hxc, Hx + o);
-            vld(hyc, Hy + o);
-            vld(ex, Ex + o);
-            vld(ey, Ey + o);
-            vld(ez, Ez + o);
-            if (zin) {
-                vld(hzc, Hz + o);
-                if (ylo) {
-                    vld(hzym, Hz + o - sy);
-                    vld(hxym, Hx + o - sy);
-                }
-                if (mat) {
-                    vld(ca, d->ca + o);
-                    vld(cb, d->cb + o);
-                }
-            }
-        }
-        float hz_prev = __shfl_up(hzc[V - 1], 1, kWave);
-        float hy_prev = __shfl_up(hyc[V - 1], 1, kWave);
-        if (first_lane) {
-            hz_prev = 0.f;
-            hy_prev = 0.f;
-            if (prev_mem && zin) {
-                hz_prev = Hz[o - 1];
-                hy_prev = Hy[o - 1];
-            }
+        EPlane<V> N;
+        const int zn = z + 1;
+        e_plane_load(N, Ex, Ey, Ez, Hx, Hy, Hz, Ca, Cb, ca_u, cb_u, o + sz, sy, act && zn < z1,
+                     zn < k.nz - 1, zn < k.nz - 1 && zn < uni_z, ylo, prev_mem);
+        float hz_prev = __shfl_up(P.hz[V - 1], 1, kWave);
+        float hy_prev = __shfl_up(P.hy[V - 1], 1, kWave);
+        if (lane == 0) {
+            hz_prev = P.hz_p;
+            hy_prev = P.hy_p;
         }
 #pragma unroll
-        for (int k = 0; k < V; ++k) {
-            if (xb + k < nx) {
+        for (int c = 0; c < V; ++c) {
+            if (xb + c < k.nx) {
                 HAround h;
-                h.hx_c = hxc[k]; h.hy_c = hyc[k]; h.hz_c = hzc[k];
-                h.hx_zm = hxzm[k]; h.hy_zm = hyzm[k];
-                h.hz_ym = hzym[k]; h.hx_ym = hxym[k];
-                h.hz_xm = k > 0 ? hzc[k > 0 ? k - 1 : 0] : hz_prev;
-                h.hy_xm = k > 0 ? hyc[k > 0 ? k - 1 : 0] : hy_prev;
-                e_cell(d, xb + k, y, z, t, ca[k], cb[k], h, ex[k], ey[k], ez[k]);
+                h.hx_c = P.hx[c]; h.hy_c = P.hy[c]; h.hz_c = P.hz[c];
+                h.hx_zm = hxzm[c]; h.hy_zm = hyzm[c];
+                h.hz_ym = P.hzym[c]; h.hx_ym = P.hxym[c];
+                h.hz_xm = c > 0 ? P.hz[c > 0 ? c - 1 : 0] : hz_prev;
+                h.hy_xm = c > 0 ? P.hy[c > 0 ? c - 1 : 0] : hy_prev;
+                e_cell(d, k, xb + c, y, z, P.ca[c], P.cb[c], h, P.ex[c], P.ey[c], P.ez[c]);
             }
         }
         if (act) {
-            vst(Ex + o, ex);
-            vst(Ey + o, ey);
-            vst(Ez + o, ez);
+            vst(Ex + o, P.ex);
+            vst(Ey + o, P.ey);
+            vst(Ez + o, P.ez);
         }
-#pragma unroll
-        for (int k = 0; k < V; ++k) {
-            hxzm[k] = hxc[k];
-            hyzm[k] = hyc[k];
-        }
+        vcopy(hxzm, P.hx);
+        vcopy(hyzm, P.hy);
+        P = N;
     }
 }
 
@@ -467,121 +603,223 @@ template __global__ void k_e_half<4>(const Desc *, const int4 *, int);
 
 // ---------------------------------------------------------------------------
 // Fused stepper (MAIN): one kernel per time step computes H(t+1/2) and then
-// E(t+1) plane by plane, so each field value is read once and written once
-// per step (56 B/cell instead of the split stepper's 80).  Fields and H-CPML
-// state are double-buffered: step t reads set t&1 and writes set 1-(t&1).
+// E(t+1), so each field value is read once and written once per step
+// (56 B/cell instead of the split stepper's 80).  Fields and H-CPML state
+// are double-buffered: step t reads set t&1 and writes set 1-(t&1).
 //
-// A block of 64 x TY threads owns a 63 x (TY-1) tile: lane 0 and row 0 are
-// a one-cell halo whose H values (needed by E at x-1, y-1) are recomputed
-// from the old level rather than exchanged between blocks; the owner tile
-// computes the same values and is the only one that writes them.  Each
-// thread holds one (x, y) column and marches z upward: E(z+1) read at plane
-// z is E(z) at plane z+1, H(z) computed at plane z is H(z-1) at plane z+1.
-// H at x-1 comes from the neighbouring lane (wavefront shuffle); H at y-1
-// from the row below through LDS, one barrier per plane.  A tile starting
-// at z0 > 0 recomputes H at z0-1 (not stored) for its first E plane.
+// Each wavefront owns a strip of R rows of 64 V cells over a z range and
+// marches z.  Within a plane it walks its rows upward: H(t+1/2) of row y,
+// then E(t+1) of row y, which needs H(t+1/2)
+//   - at y-1: kept in registers from the previous row; for the strip's
+//     first row recomputed from the old level (not stored);
+//   - at x-1: the neighbouring lane's cell; lane 0 recomputes the last cell
+//     of the tile to its left (not stored);
+//   - at z-1: re-read from the new set, where the same lane stored it one
+//     plane earlier; a strip starting at z0 > 0 recomputes plane z0-1 (not
+//     stored) on its first plane.
+// Recomputed values are bit-identical to the owner's: same inputs, same
+// operations.  There is no LDS and no barrier: the four wavefronts of a
+// block work on independent strips.
 // ---------------------------------------------------------------------------
-template <int TY>
-__global__ void __launch_bounds__(kWave * TY)
-k_step_fused(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles) {
-    __shared__ float lds_hz[2][TY][kWave];
-    __shared__ float lds_hx[2][TY][kWave];
-    const int4 tl = tiles[xcd_tile(blockIdx.x, ntiles)];
-    const Desc *__restrict__ d = descs + tl.x;
-    const int t = load_step(d->step);
-    if (t >= d->nt) return;                            // uniform per block
-    const int nx = d->nx, ny = d->ny, nz = d->nz;
-    const int lx = threadIdx.x, ly = threadIdx.y;
-    const int x = tl.y - 1 + lx, y = tl.z - 1 + ly;    // lane 0 / row 0: halo
-    const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
-    const int p = t & 1;
-    const long long sy = d->sy, sz = d->sz;
-    const float *__restrict__ Exo = d->f[p][0];
-    const float *__restrict__ Eyo = d->f[p][1];
-    const float *__restrict__ Ezo = d->f[p][2];
-    const float *__restrict__ Hxo = d->f[p][3];
-    const float *__restrict__ Hyo = d->f[p][4];
-    const float *__restrict__ Hzo = d->f[p][5];
-    float *__restrict__ Exn = d->f[1 - p][0];
-    float *__restrict__ Eyn = d->f[1 - p][1];
-    float *__restrict__ Ezn = d->f[1 - p][2];
-    float *__restrict__ Hxn = d->f[1 - p][3];
-    float *__restrict__ Hyn = d->f[1 - p][4];
-    float *__restrict__ Hzn = d->f[1 - p][5];
-    const bool ok = x >= 0 && x < nx && y >= 0 && y < ny;
-    const bool own = ok && lx >= 1 && ly >= 1;
-    const bool xin = x < nx - 1, yin = y < ny - 1;
-    const bool last_lane = lx == kWave - 1;
-    const int zs = z0 > 0 ? z0 - 1 : 0;
 
-    long long o = (long long)zs * sz + (long long)y * sy + x;
-    EAround e;
-    e.ex_c = ok ? Exo[o] : 0.f;
-    e.ey_c = ok ? Eyo[o] : 0.f;
-    float hx_zm = 0.f, hy_zm = 0.f;
-    for (int z = zs; z < z1; ++z, o += sz) {
-        const int buf = z & 1;
-        const bool zin = z < nz - 1;
-        float hx = 0.f, hy = 0.f, hz = 0.f;
-        e.ex_n = e.ey_n = e.ez_c = e.ez_yp = e.ex_yp = 0.f;
-        if (ok) {
-            hx = Hxo[o];
-            hy = Hyo[o];
-            hz = Hzo[o];
-            e.ez_c = Ezo[o];
-            if (zin) {
-                e.ex_n = Exo[o + sz];
-                e.ey_n = Eyo[o + sz];
-                if (yin) {
-                    e.ez_yp = Ezo[o + sy];
-                    e.ex_yp = Exo[o + sy];
-                }
+// One row's H(t+1/2) and the E(t) values it was computed from.
+template <int V>
+struct HRow {
+    float hx[V], hy[V], hz[V];
+    float ex[V], ey[V], ez[V];
+};
+
+template <int V>
+__device__ __forceinline__ void h_row(const Desc *__restrict__ d, const StepK &k, const gfloat *const *fo,
+                                      int xb, int y, int z, long long o, long long sz, bool act,
+                                      bool next_mem, float *const *psi_rd, float *const *psi_wr,
+                                      HRow<V> &r) {
+    const long long sy = k.sy;
+    const bool zin = z < k.nz - 1, yin = y < k.ny - 1;
+    float exn[V], eyn[V], ezyp[V], exyp[V];
+    vset(r.hx, 0.f); vset(r.hy, 0.f); vset(r.hz, 0.f);
+    vset(r.ex, 0.f); vset(r.ey, 0.f); vset(r.ez, 0.f);
+    vset(exn, 0.f); vset(eyn, 0.f); vset(ezyp, 0.f); vset(exyp, 0.f);
+    float ey_x = 0.f, ez_x = 0.f;
+    if (act) {
+        vld(r.hx, fo[3] + o);
+        vld(r.hy, fo[4] + o);
+        vld(r.hz, fo[5] + o);
+        vld(r.ex, fo[0] + o);
+        vld(r.ey, fo[1] + o);
+        vld(r.ez, fo[2] + o);
+        if (zin) {
+            vld(exn, fo[0] + o + sz);
+            vld(eyn, fo[1] + o + sz);
+            if (yin) {
+                vld(ezyp, fo[2] + o + sy);
+                vld(exyp, fo[0] + o + sy);
+            }
+            if (next_mem) {
+                ey_x = fo[1][o + V];
+                ez_x = fo[2][o + V];
             }
         }
-        e.ey_xp = __shfl_down(e.ey_c, 1, kWave);
-        e.ez_xp = __shfl_down(e.ez_c, 1, kWave);
-        if (last_lane) {                               // x+1 belongs to the next tile
-            const bool okx = ok && xin;
-            e.ey_xp = okx ? Eyo[o + 1] : 0.f;
-            e.ez_xp = okx ? Ezo[o + 1] : 0.f;
+    }
+    float ey_next = __shfl_down(r.ey[0], 1, kWave);
+    float ez_next = __shfl_down(r.ez[0], 1, kWave);
+    if (threadIdx.x == kWave - 1) {
+        ey_next = ey_x;
+        ez_next = ez_x;
+    }
+#pragma unroll
+    for (int c = 0; c < V; ++c) {
+        if (xb + c < k.nx) {
+            EAround e;
+            e.ex_c = r.ex[c]; e.ex_n = exn[c]; e.ex_yp = exyp[c];
+            e.ey_c = r.ey[c]; e.ey_n = eyn[c];
+            e.ey_xp = c + 1 < V ? r.ey[c + 1 < V ? c + 1 : 0] : ey_next;
+            e.ez_c = r.ez[c]; e.ez_yp = ezyp[c];
+            e.ez_xp = c + 1 < V ? r.ez[c + 1 < V ? c + 1 : 0] : ez_next;
+            h_cell(d, k, xb + c, y, z, e, r.hx[c], r.hy[c], r.hz[c], psi_rd, psi_wr);
         }
-        const bool store = z >= z0;
-        if (ok) h_cell(d, x, y, z, t, e, hx, hy, hz, d->psi[p], (own && store) ? d->psi[1 - p] : nullptr);
-        lds_hz[buf][ly][lx] = hz;
-        lds_hx[buf][ly][lx] = hx;
-        const float hz_xm = __shfl_up(hz, 1, kWave);
-        const float hy_xm = __shfl_up(hy, 1, kWave);
-        __syncthreads();
-        if (own && store) {
-            HAround h;
-            h.hx_c = hx; h.hy_c = hy; h.hz_c = hz;
-            h.hx_zm = hx_zm; h.hy_zm = hy_zm;
-            h.hz_ym = lds_hz[buf][ly - 1][lx];
-            h.hx_ym = lds_hx[buf][ly - 1][lx];
-            h.hz_xm = hz_xm; h.hy_xm = hy_xm;
-            float ex = e.ex_c, ey = e.ey_c, ez = e.ez_c;
-            float ca = d->ca_u, cb = d->cb_u;
-            if (zin && z < d->uni_z) {
-                ca = d->ca[o];
-                cb = d->cb[o];
-            }
-            e_cell(d, x, y, z, t, ca, cb, h, ex, ey, ez);
-            Exn[o] = ex;
-            Eyn[o] = ey;
-            Ezn[o] = ez;
-            Hxn[o] = hx;
-            Hyn[o] = hy;
-            Hzn[o] = hz;
-        }
-        hx_zm = hx;
-        hy_zm = hy;
-        e.ex_c = e.ex_n;
-        e.ey_c = e.ey_n;
     }
 }
 
-template __global__ void k_step_fused<8>(const Desc *, const int4 *, int);
-template __global__ void k_step_fused<16>(const Desc *, const int4 *, int);
+template <int V, int R>
+__global__ void __launch_bounds__(256)
+k_step_strip(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles) {
+    const int nblk = (ntiles + 3) >> 2;
+    // one tile per wavefront: threadIdx.y is uniform across the wavefront,
+    // and saying so keeps the descriptor and its pointers in scalar registers
+    const int wt = __builtin_amdgcn_readfirstlane((xcd_tile(blockIdx.x, nblk) << 2) + (int)threadIdx.y);
+    if (wt >= ntiles) return;                          // uniform per wavefront
+    const int4 tl = tiles[wt];
+    const Desc *__restrict__ d = descs + tl.x;
+    const int t = load_step(d->step);
+    if (t >= d->nt) return;
+    const StepK k = step_k(d, t);
+    const int lane = threadIdx.x;
+    const int x0 = tl.y, y0 = tl.z;
+    const int xb = x0 + V * lane;
+    const bool act = xb < k.nx;
+    const bool next_mem = lane == kWave - 1 && xb + V < k.nx;
+    const bool halo_x = lane == 0 && x0 > 0;           // recompute H(t+1/2) at x0 - 1
+    const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
+    const int y1 = min(y0 + R, k.ny);
+    const int p = t & 1;
+    const long long sy = k.sy, sz = d->sz;
+    const gfloat *fo[6];
+    gfloat *fn[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        fo[c] = gp(d->f[p][c]);
+        fn[c] = gp(d->f[1 - p][c]);
+    }
+    float *const *psi_o = d->psi[p];
+    float *const *psi_n = d->psi[1 - p];
+    const gfloat *Ca = gp(d->ca), *Cb = gp(d->cb);
+    const int uni_z = d->uni_z;
+    const float ca_u = d->ca_u, cb_u = d->cb_u;
+
+#pragma unroll 1
+    for (int z = z0; z < z1; ++z) {
+        const bool zin = z < k.nz - 1;
+        const long long oz = (long long)z * sz + xb;
+        // H(t+1/2) of the row below the strip, this plane
+        float hzb[V], hxb[V];
+        vset(hzb, 0.f);
+        vset(hxb, 0.f);
+        if (y0 >= 1) {
+            HRow<V> b;
+            h_row(d, k, fo, xb, y0 - 1, z, oz + (long long)(y0 - 1) * sy, sz, act, next_mem, psi_o,
+                  nullptr, b);
+            vcopy(hzb, b.hz);
+            vcopy(hxb, b.hx);
+        }
+#pragma unroll 1
+        for (int y = y0; y < y1; ++y) {
+            const long long o = oz + (long long)y * sy;
+            const bool yin = y < k.ny - 1;
+            HRow<V> c;
+            h_row(d, k, fo, xb, y, z, o, sz, act, next_mem, psi_o, psi_n, c);
+            // H(t+1/2) at z - 1
+            float hxzm[V], hyzm[V];
+            vset(hxzm, 0.f);
+            vset(hyzm, 0.f);
+            if (z > z0) {
+                if (act) {
+                    vld(hxzm, fn[3] + o - sz);
+                    vld(hyzm, fn[4] + o - sz);
+                }
+            } else if (z >= 1) {
+                HRow<V> cz;
+                h_row(d, k, fo, xb, y, z - 1, o - sz, sz, act, next_mem, psi_o, nullptr, cz);
+                vcopy(hxzm, cz.hx);
+                vcopy(hyzm, cz.hy);
+            }
+            // H(t+1/2) at x - 1 of the lane's first cell
+            float hz_prev = __shfl_up(c.hz[V - 1], 1, kWave);
+            float hy_prev = __shfl_up(c.hy[V - 1], 1, kWave);
+            if (lane == 0) {
+                hz_prev = 0.f;
+                hy_prev = 0.f;
+                if (halo_x) {
+                    // only hy and hz of the cell are used: hx's inputs are left out
+                    const long long oh = o - 1;
+                    EAround e;
+                    e.ex_c = fo[0][oh];
+                    e.ey_c = fo[1][oh];
+                    e.ez_c = fo[2][oh];
+                    e.ey_xp = c.ey[0];
+                    e.ez_xp = c.ez[0];
+                    e.ex_n = zin ? fo[0][oh + sz] : 0.f;
+                    e.ex_yp = (zin && yin) ? fo[0][oh + sy] : 0.f;
+                    e.ey_n = e.ey_c;
+                    e.ez_yp = e.ez_c;
+                    float hx_h = 0.f, hy_h = fo[4][oh], hz_h = fo[5][oh];
+                    h_cell(d, k, x0 - 1, y, z, e, hx_h, hy_h, hz_h, psi_o, nullptr);
+                    hz_prev = hz_h;
+                    hy_prev = hy_h;
+                }
+            }
+            float ca[V], cb[V];
+            vset(ca, ca_u);
+            vset(cb, cb_u);
+            if (act && zin && z < uni_z) {
+                vld(ca, Ca + o);
+                vld(cb, Cb + o);
+            }
+#pragma unroll
+            for (int q = 0; q < V; ++q) {
+                if (xb + q < k.nx) {
+                    HAround h;
+                    h.hx_c = c.hx[q]; h.hy_c = c.hy[q]; h.hz_c = c.hz[q];
+                    h.hx_zm = hxzm[q]; h.hy_zm = hyzm[q];
+                    h.hz_ym = hzb[q]; h.hx_ym = hxb[q];
+                    h.hz_xm = q > 0 ? c.hz[q > 0 ? q - 1 : 0] : hz_prev;
+                    h.hy_xm = q > 0 ? c.hy[q > 0 ? q - 1 : 0] : hy_prev;
+                    e_cell(d, k, xb + q, y, z, ca[q], cb[q], h, c.ex[q], c.ey[q], c.ez[q]);
+                }
+            }
+            if (act) {
+                vst(fn[3] + o, c.hx);
+                vst(fn[4] + o, c.hy);
+                vst(fn[5] + o, c.hz);
+                vst(fn[0] + o, c.ex);
+                vst(fn[1] + o, c.ey);
+                vst(fn[2] + o, c.ez);
+            }
+            vcopy(hzb, c.hz);
+            vcopy(hxb, c.hx);
+        }
+    }
+}
+
+template __global__ void k_step_strip<1, 2>(const Desc *, const int4 *, int);
+template __global__ void k_step_strip<1, 4>(const Desc *, const int4 *, int);
+template __global__ void k_step_strip<1, 8>(const Desc *, const int4 *, int);
+template __global__ void k_step_strip<2, 2>(const Desc *, const int4 *, int);
+template __global__ void k_step_strip<2, 4>(const Desc *, const int4 *, int);
+template __global__ void k_step_strip<2, 8>(const Desc *, const int4 *, int);
+template __global__ void k_step_strip<4, 2>(const Desc *, const int4 *, int);
+template __global__ void k_step_strip<4, 4>(const Desc *, const int4 *, int);
+template __global__ void k_step_strip<4, 8>(const Desc *, const int4 *, int);
 
 // ---------------------------------------------------------------------------
 // Running single-frequency DFT on the phasor planes

@@ -77,14 +77,17 @@ def test_fields_psi_phasors_bitwise(soil60, fused):
 
 
 # (ty, zchunk, xvec): split stepper with 1, 2 and 4 x cells per lane
-# (33 cells: one partly idle tile), fused stepper with its halo tiles
+# (33 cells: one partly idle tile); fused stepper with 2, 4 and 8 rows per
+# wavefront, z ranges that start mid-column (recomputed plane below) and
+# strips whose first row is recomputed
 @pytest.mark.parametrize('fused,launch', [(False, (1, 0, 1)), (False, (2, 0, 2)),
                                           (False, (4, 0, 4)), (False, (4, 7, 4)),
                                           (False, (1, 5, 2)), (False, (2, 16, 4)),
                                           (False, (2, 3, 1)), (False, (1, 1, 4)),
-                                          (True, (8, 0, 1)), (True, (16, 0, 1)),
-                                          (True, (8, 7, 0)), (True, (16, 5, 1)),
-                                          (True, (8, 1, 0))])
+                                          (True, (2, 0, 1)), (True, (4, 0, 4)),
+                                          (True, (8, 7, 2)), (True, (4, 5, 1)),
+                                          (True, (2, 1, 4)), (True, (8, 16, 0)),
+                                          (True, (4, 3, 4))])
 def test_launch_shapes_bitwise(soil60, fused, launch):
     pc, Ca, Cb, beam, ref = soil60
     say('case: fused', fused, 'launch', launch)
@@ -127,7 +130,7 @@ def test_uneven_grid_bitwise():
     say('case: uneven 37x29x41, 40 steps')
     beam = H.synthetic_beam(pc)
     ref = H.oracle_main(pc, Ca, Cb, beam)
-    sim = make_sim(pc, Ca, Cb, beam, launch=(16, 9, 1))
+    sim = make_sim(pc, Ca, Cb, beam, launch=(8, 9, 1))
     sim.run(40)
     sim.sync()
     for n in native.FIELD_NAMES:
@@ -135,16 +138,22 @@ def test_uneven_grid_bitwise():
     sim.destroy()
 
 
+@pytest.fixture(scope='module')
+def wide30():
+    pc, Ca, Cb = H.soil_case(nx=301, ny=25, nz=27, nz_mat=10, steps=30, seed=7)
+    beam = H.synthetic_beam(pc)
+    return pc, Ca, Cb, beam, H.oracle_main(pc, Ca, Cb, beam)
+
+
+@pytest.mark.parametrize('fused', [False, True], ids=['split', 'fused'])
 @pytest.mark.parametrize('xvec', [4, 2, 1])
-def test_wide_grid_lane_widths_bitwise(xvec):
+def test_wide_grid_lane_widths_bitwise(wide30, fused, xvec):
     # 301 cells in x: two 256-cell tiles at 4 cells per lane (the second
     # with 12 active lanes), three at 2, five at 1; x +- 1 neighbours cross
-    # lanes and tiles
-    pc, Ca, Cb = H.soil_case(nx=301, ny=25, nz=27, nz_mat=10, steps=30, seed=7)
-    say('case: wide 301x25x27, 30 steps, xvec', xvec)
-    beam = H.synthetic_beam(pc)
-    ref = H.oracle_main(pc, Ca, Cb, beam)
-    sim = make_sim(pc, Ca, Cb, beam, launch=(2, 8, xvec), fused=False)
+    # lanes and tiles (the fused stepper recomputes the cell left of a tile)
+    pc, Ca, Cb, beam, ref = wide30
+    say('case: wide 301x25x27, 30 steps, fused', fused, 'xvec', xvec)
+    sim = make_sim(pc, Ca, Cb, beam, launch=(2, 8, xvec), fused=fused)
     sim.run(30)
     sim.sync()
     for n in native.FIELD_NAMES:
@@ -158,7 +167,7 @@ def test_invalid_launch_is_rejected_and_previous_kept(soil60):
     pc, Ca, Cb, beam, ref = soil60
     sim = make_sim(pc, Ca, Cb, beam, launch=(2, 0, 4), fused=False)
     with pytest.raises(native.FdtdError):
-        sim.set_launch(8, 0, 4)          # ty 8 is a fused-stepper shape
+        sim.set_launch(8, 0, 4)          # 8 rows per tile is a fused-stepper shape
     with pytest.raises(native.FdtdError):
         sim.set_launch(2, 0, 3)
     sim.run(60)
