@@ -1,10 +1,10 @@
 """Tile-shape selection for the stencil kernels.
 
 A launch shape is (ty, zchunk, xvec): blocks of 64 x ty lanes marching z.
-Split stepper: ty in {1, 2, 4}, each lane owning xvec consecutive x cells
-(1, 2 or 4; 0 = chosen by the library from the grid width).  Fused stepper:
-ty in {8, 16}, one column per lane (row 0 and lane 0 of a block are a
-recomputed halo).  zchunk: z cells per tile (0 = whole column).
+Each lane owns xvec consecutive x cells (1, 2 or 4; 0 = chosen by the
+library from the grid width).  Split stepper: ty in {1, 2, 4} rows per
+block, one per wavefront.  Fused stepper: ty in {2, 4, 8} rows walked by one
+wavefront.  zchunk: z cells per tile (0 = whole column).
 ``autotune`` measures candidates on the device and records the best per
 (stepper, arch, grid) in a JSON cache; ``launch_for`` reads it, falling back
 to a default.
@@ -14,11 +14,11 @@ import os
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'tuning_cache.json')
 CANDIDATES = {
-    'fused': [(8, 16, 1), (16, 16, 1), (8, 24, 1), (16, 24, 1), (8, 32, 1), (8, 0, 1)],
+    'fused': [(r, zc, xv) for xv in (4, 2, 1) for r in (2, 4, 8) for zc in (8, 16, 32)],
     'split': [(ty, zc, xv) for xv in (4, 2) for ty in (1, 2, 4) for zc in (8, 16, 32)]
              + [(2, 16, 1)],
 }
-DEFAULT = {'fused': (8, 16, 1), 'split': (2, 16, 0)}
+DEFAULT = {'fused': (4, 16, 0), 'split': (2, 16, 0)}
 
 
 def _key(nx, ny, nz, fused, arch='gfx950'):
@@ -45,9 +45,13 @@ def autotune(sim, nx, ny, nz, fused=True, arch='gfx950', steps=10, candidates=No
     """Time each candidate shape on ``sim`` (a ready native.Sim whose stepper
     is already chosen) and keep the fastest.  Returns ((ty, zchunk),
     {shape: ms_per_step})."""
+    from .._native import FdtdError
     results = {}
     for shape in candidates or CANDIDATES['fused' if fused else 'split']:
-        sim.set_launch(*shape)
+        try:
+            sim.set_launch(*shape)
+        except FdtdError:
+            continue                       # shape not valid for this grid / stepper
         sim.reset()
         sim.run(2)
         sim.sync()
