@@ -22,7 +22,7 @@ __global__ void k_e_half(const Desc *, const int4 *, int);
 __global__ void k_dft(const Desc *);
 __global__ void k_advance(const Desc *, int);
 template <int V, int R>
-__global__ void k_step_strip(const Desc *, const int4 *, int);
+__global__ void k_step_strip(const Desc *, const int4 *, int, float *);
 __global__ void k_phasor_out(const float4 *, long long, float2 *);
 __global__ void k_n2f_partial(const float2 *, const float2 *, const float2 *, const float2 *, int, int, int,
                               const double2 *, const double2 *, double *);
@@ -106,6 +106,7 @@ struct Group {
     hipStream_t stream = nullptr;
     Desc *d_descs = nullptr;
     int4 *d_tiles = nullptr;
+    float *d_halo = nullptr;       // fused stepper: per-tile H of the plane below the tile
     int ntiles = 0, nmodels = 0, mode = FDTD_MODE_MAIN, fused = 0;
     int xvec = 1;                  // x cells per lane of the split kernels in use
     long long max_dft_items = 0;
@@ -324,10 +325,14 @@ static int group_build(Group &G) {
     group_clear_graphs(G);
     (void)hipFree(G.d_descs);
     (void)hipFree(G.d_tiles);
+    (void)hipFree(G.d_halo);
     G.d_descs = nullptr;
     G.d_tiles = nullptr;
+    G.d_halo = nullptr;
     HIP_TRY(hipMalloc(&G.d_descs, descs.size() * sizeof(Desc)));
     HIP_TRY(hipMalloc(&G.d_tiles, tiles.size() * sizeof(int4)));
+    if (G.fused)   // hx, hy of the plane below each tile: [ntiles][2][ty][64 xvec]
+        HIP_TRY(hipMalloc(&G.d_halo, tiles.size() * 2 * (size_t)ty * kWave * G.xvec * sizeof(float)));
     HIP_TRY(hipMemcpyAsync(G.d_descs, descs.data(), descs.size() * sizeof(Desc), hipMemcpyHostToDevice,
                            G.stream));
     HIP_TRY(hipMemcpyAsync(G.d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice,
@@ -343,6 +348,7 @@ static void group_free(Group &G) {
     group_clear_graphs(G);
     (void)hipFree(G.d_descs);
     (void)hipFree(G.d_tiles);
+    (void)hipFree(G.d_halo);
     if (G.stream) (void)hipStreamDestroy(G.stream);
     G = Group();
 }
@@ -358,7 +364,7 @@ static void launch_kernel(Group &G, KernelId k) {
         if (v == 4) fn = r == 8 ? &k_step_strip<4, 8> : r == 4 ? &k_step_strip<4, 4> : &k_step_strip<4, 2>;
         else if (v == 2) fn = r == 8 ? &k_step_strip<2, 8> : r == 4 ? &k_step_strip<2, 4> : &k_step_strip<2, 2>;
         else fn = r == 8 ? &k_step_strip<1, 8> : r == 4 ? &k_step_strip<1, 4> : &k_step_strip<1, 2>;
-        hipLaunchKernelGGL(fn, grid, b4, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
+        hipLaunchKernelGGL(fn, grid, b4, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles, G.d_halo);
     } else if (k == KH) {
         if (G.xvec == 4)
             hipLaunchKernelGGL(k_h_half<4>, dim3(G.ntiles), blk, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);

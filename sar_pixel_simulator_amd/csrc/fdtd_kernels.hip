@@ -33,9 +33,19 @@ __device__ __forceinline__ int load_step(const int *p) {
 // then compile to global_load/global_store, which the waitcnt logic tracks
 // apart from the scalar loads of the descriptor.
 typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) char gchar;
 
 __device__ __forceinline__ gfloat *gp(float *p) { return (gfloat *)p; }
 __device__ __forceinline__ const gfloat *gp(const float *p) { return (const gfloat *)p; }
+
+// Element offsets are 32-bit: every array these kernels index holds fewer
+// than 2^30 floats (checked when a simulation is created), so the byte
+// offset fits the 32-bit register offset of a global access whose base
+// stays in scalar registers (one VGPR per address instead of two).
+__device__ __forceinline__ const gfloat *at(const gfloat *b, uint32_t o) {
+    return (const gfloat *)((const gchar *)b + (o << 2));
+}
+__device__ __forceinline__ gfloat *at(gfloat *b, uint32_t o) { return (gfloat *)((gchar *)b + (o << 2)); }
 
 // Scalars of one simulation at one time step, read from the descriptor once
 // per kernel so that the cell updates take them from registers.
@@ -45,9 +55,22 @@ struct StepK {
     bool inj;                      // MAIN and t < nt_inc: incident-field injection
     bool hard;                     // INCIDENT and t < n_src: hard source
     float Db, dtm0, dx, dte0dx;
-    long long sy, du_sz, lr_sz, fb_sz;
+    uint32_t sy, sz, du_sz, lr_sz, fb_sz;
+    // CPML state: each field's twelve psi arrays lie in one block in the
+    // order of enum Psi (d, u: [npml][ny][pitch]; l, r: [nz][ny][npml];
+    // f, b: [nz][npml][pitch]), so an array is its block's base plus a
+    // closed-form offset (psi_off)
+    uint32_t du_len, lr_len, fb_len;
+    gfloat *psiE;                      // E block (updated in place)
     const gfloat *b0, *b1, *b2, *b3;   // plane t of the beam (ex ey hx hy) or source (ex ey)
 };
+
+// Offset of psi array j (0..11 within one field's block, enum Psi order).
+__device__ __forceinline__ uint32_t psi_off(const StepK &k, int j) {
+    return j < 4 ? j * k.du_len
+         : j < 8 ? 4 * k.du_len + (j - 4) * k.lr_len
+                 : 4 * k.du_len + 4 * k.lr_len + (j - 8) * k.fb_len;
+}
 
 __device__ __forceinline__ StepK step_k(const Desc *__restrict__ d, int t) {
     StepK k;
@@ -56,7 +79,12 @@ __device__ __forceinline__ StepK step_k(const Desc *__restrict__ d, int t) {
     k.inj = d->mode == FDTD_MODE_MAIN && t < d->nt_inc;
     k.hard = d->mode != FDTD_MODE_MAIN && t < d->n_src;
     k.Db = d->Db; k.dtm0 = d->dtm0; k.dx = d->dx; k.dte0dx = d->dte0dx;
-    k.sy = d->sy; k.du_sz = d->du_sz; k.lr_sz = d->lr_sz; k.fb_sz = d->fb_sz;
+    k.sy = (uint32_t)d->sy; k.sz = (uint32_t)d->sz;
+    k.du_sz = (uint32_t)d->du_sz; k.lr_sz = (uint32_t)d->lr_sz; k.fb_sz = (uint32_t)d->fb_sz;
+    k.du_len = (uint32_t)(d->npml * d->du_sz);
+    k.lr_len = (uint32_t)(d->nz * d->lr_sz);
+    k.fb_len = (uint32_t)(d->nz * d->fb_sz);
+    k.psiE = gp(d->psi[0][0]);
     const long long plane = (long long)d->ny * d->nx * t;
     k.b0 = k.b1 = k.b2 = k.b3 = nullptr;
     if (k.inj) {
@@ -84,15 +112,15 @@ struct EAround {
 // cpu/fdtd_sim.py:104-133), then the E_inc injection at tfsf_z in MAIN mode
 // (cpu/main_sim.py:147-150).  Every correction reads E only, so applying
 // them in sequence per cell gives the values of the reference's separate
-// passes exactly.  psi is read from psi_rd and, when psi_wr is not null,
-// the updated value is written there (psi_wr == psi_rd updates in place).
-__device__ __noinline__ void h_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
-                                   EAround e, float *__restrict__ h, float *const *psi_rd,
-                                   float *const *psi_wr);
+// passes exactly.  psi is read from the H block hr and, when hw is not
+// null, the updated value is written to hw (hw == hr updates in place).
+__device__ __forceinline__ void h_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
+                                       const EAround &e, float &hx, float &hy, float &hz,
+                                       const gfloat *hr, gfloat *hw);
 
 __device__ __forceinline__ void h_cell(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
                                        const EAround &e, float &hx, float &hy, float &hz,
-                                       float *const *psi_rd, float *const *psi_wr) {
+                                       const gfloat *hr, gfloat *hw) {
     const int nx = k.nx, ny = k.ny, nz = k.nz, np = k.np;
     const float Db = k.Db;
     const bool xin = x < nx - 1, yin = y < ny - 1;
@@ -100,50 +128,44 @@ __device__ __forceinline__ void h_cell(const Desc *__restrict__ d, const StepK &
         if (x >= 1 && xin && yin) hx -= Db * ((e.ez_yp - e.ez_c) - (e.ey_n - e.ey_c));
         if (y >= 1 && yin && xin) hy -= Db * ((e.ex_n - e.ex_c) - (e.ez_xp - e.ez_c));
         if (z >= 1 && yin && xin) hz -= Db * ((e.ey_xp - e.ey_c) - (e.ex_yp - e.ex_c));
-        if (z < np - 1 || z >= nz - np || x < np - 1 || x >= nx - np || y < np - 1 || y >= ny - np) {
-            float h[3] = {hx, hy, hz};
-            h_cpml(d, k, x, y, z, e, h, psi_rd, psi_wr);
-            hx = h[0];
-            hy = h[1];
-            hz = h[2];
-        }
+        if (z < np - 1 || z >= nz - np || x < np - 1 || x >= nx - np || y < np - 1 || y >= ny - np)
+            h_cpml(d, k, x, y, z, e, hx, hy, hz, hr, hw);
     }
     if (k.inj && z == k.tfsf_z) {
-        const long long q = (long long)y * nx + x;
-        hx += Db * k.b1[q];   // ey_inc
-        hy -= Db * k.b0[q];   // ex_inc
+        const uint32_t q = (uint32_t)y * nx + x;
+        hx += Db * *at(k.b1, q);   // ey_inc
+        hy -= Db * *at(k.b0, q);   // ex_inc
     }
 }
 
 // The six H-CPML corrections of one cell below the top plane, in the
-// reference order d,u,l,r,f,b; h = (hx, hy, hz) after the Yee update.
-// Called only for cells inside an absorber, and not inlined, so that the
-// absorber addressing does not occupy registers in the stencil loops.
-__device__ __noinline__ void h_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
-                                   EAround e, float *__restrict__ h, float *const *psi_rd,
-                                   float *const *psi_wr) {
+// reference order d,u,l,r,f,b, after its Yee update.  psi is read from the
+// H block hr and, when hw is not null, written to the H block hw (hw == hr
+// updates in place).
+__device__ __forceinline__ void h_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
+                                       const EAround &e, float &hx, float &hy, float &hz,
+                                       const gfloat *hr, gfloat *hw) {
     const int nx = k.nx, ny = k.ny, nz = k.nz, np = k.np;
-    const long long sy = k.sy;
+    const uint32_t sy = k.sy;
     const float dtm0 = k.dtm0, dx = k.dx;
     const bool xin = x < nx - 1, yin = y < ny - 1;
-    float hx = h[0], hy = h[1], hz = h[2];
     if (yin && xin) {
         if (z < np - 1) {                                       // down
             const float bh = d->prof[BH_DLF][z], ch = d->prof[CH_DLF][z];
-            const long long p = (long long)z * k.du_sz + (long long)y * sy + x;
-            const float a = (bh * gp(psi_rd[pHyz_d])[p]) + ((ch * (e.ex_n - e.ex_c)) / dx);
-            const float b = (bh * gp(psi_rd[pHxz_d])[p]) + ((ch * (e.ey_n - e.ey_c)) / dx);
-            if (psi_wr) { gp(psi_wr[pHyz_d])[p] = a; gp(psi_wr[pHxz_d])[p] = b; }
+            const uint32_t p = (uint32_t)z * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
+            const float a = (bh * *at(hr, psi_off(k, pHyz_d - 12) + p)) + ((ch * (e.ex_n - e.ex_c)) / dx);
+            const float b = (bh * *at(hr, psi_off(k, pHxz_d - 12) + p)) + ((ch * (e.ey_n - e.ey_c)) / dx);
+            if (hw) { *at(hw, psi_off(k, pHyz_d - 12) + p) = a; *at(hw, psi_off(k, pHxz_d - 12) + p) = b; }
             hy -= dtm0 * a;
             hx += dtm0 * b;
         }
         if (z >= nz - np) {                                     // up
             const int zl = z - (nz - np);
             const float bh = d->prof[BH_URB][zl], ch = d->prof[CH_URB][zl];
-            const long long p = (long long)zl * k.du_sz + (long long)y * sy + x;
-            const float a = (bh * gp(psi_rd[pHyz_u])[p]) + ((ch * (e.ex_n - e.ex_c)) / dx);
-            const float b = (bh * gp(psi_rd[pHxz_u])[p]) + ((ch * (e.ey_n - e.ey_c)) / dx);
-            if (psi_wr) { gp(psi_wr[pHyz_u])[p] = a; gp(psi_wr[pHxz_u])[p] = b; }
+            const uint32_t p = (uint32_t)zl * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
+            const float a = (bh * *at(hr, psi_off(k, pHyz_u - 12) + p)) + ((ch * (e.ex_n - e.ex_c)) / dx);
+            const float b = (bh * *at(hr, psi_off(k, pHxz_u - 12) + p)) + ((ch * (e.ey_n - e.ey_c)) / dx);
+            if (hw) { *at(hw, psi_off(k, pHyz_u - 12) + p) = a; *at(hw, psi_off(k, pHxz_u - 12) + p) = b; }
             hy -= dtm0 * a;
             hx += dtm0 * b;
         }
@@ -151,20 +173,20 @@ __device__ __noinline__ void h_cpml(const Desc *__restrict__ d, const StepK &k, 
     if (yin) {
         if (x < np - 1) {                                       // left
             const float bh = d->prof[BH_DLF][x], ch = d->prof[CH_DLF][x];
-            const long long p = (long long)z * k.lr_sz + (long long)y * np + x;
-            const float a = (bh * gp(psi_rd[pHzx_l])[p]) + ((ch * (e.ey_xp - e.ey_c)) / dx);
-            const float b = (bh * gp(psi_rd[pHyx_l])[p]) + ((ch * (e.ez_xp - e.ez_c)) / dx);
-            if (psi_wr) { gp(psi_wr[pHzx_l])[p] = a; gp(psi_wr[pHyx_l])[p] = b; }
+            const uint32_t p = (uint32_t)z * k.lr_sz + (uint32_t)y * np + (uint32_t)x;
+            const float a = (bh * *at(hr, psi_off(k, pHzx_l - 12) + p)) + ((ch * (e.ey_xp - e.ey_c)) / dx);
+            const float b = (bh * *at(hr, psi_off(k, pHyx_l - 12) + p)) + ((ch * (e.ez_xp - e.ez_c)) / dx);
+            if (hw) { *at(hw, psi_off(k, pHzx_l - 12) + p) = a; *at(hw, psi_off(k, pHyx_l - 12) + p) = b; }
             hz -= dtm0 * a;
             hy += dtm0 * b;
         }
         if (x >= nx - np && xin) {                              // right
             const int xl = x - (nx - np);
             const float bh = d->prof[BH_URB][xl], ch = d->prof[CH_URB][xl];
-            const long long p = (long long)z * k.lr_sz + (long long)y * np + xl;
-            const float a = (bh * gp(psi_rd[pHzx_r])[p]) + ((ch * (e.ey_xp - e.ey_c)) / dx);
-            const float b = (bh * gp(psi_rd[pHyx_r])[p]) + ((ch * (e.ez_xp - e.ez_c)) / dx);
-            if (psi_wr) { gp(psi_wr[pHzx_r])[p] = a; gp(psi_wr[pHyx_r])[p] = b; }
+            const uint32_t p = (uint32_t)z * k.lr_sz + (uint32_t)y * np + (uint32_t)xl;
+            const float a = (bh * *at(hr, psi_off(k, pHzx_r - 12) + p)) + ((ch * (e.ey_xp - e.ey_c)) / dx);
+            const float b = (bh * *at(hr, psi_off(k, pHyx_r - 12) + p)) + ((ch * (e.ez_xp - e.ez_c)) / dx);
+            if (hw) { *at(hw, psi_off(k, pHzx_r - 12) + p) = a; *at(hw, psi_off(k, pHyx_r - 12) + p) = b; }
             hz -= dtm0 * a;
             hy += dtm0 * b;
         }
@@ -172,27 +194,24 @@ __device__ __noinline__ void h_cpml(const Desc *__restrict__ d, const StepK &k, 
     if (xin) {
         if (y < np - 1) {                                       // front
             const float bh = d->prof[BH_DLF][y], ch = d->prof[CH_DLF][y];
-            const long long p = (long long)z * k.fb_sz + (long long)y * sy + x;
-            const float a = (bh * gp(psi_rd[pHxy_f])[p]) + ((ch * (e.ez_yp - e.ez_c)) / dx);
-            const float b = (bh * gp(psi_rd[pHzy_f])[p]) + ((ch * (e.ex_yp - e.ex_c)) / dx);
-            if (psi_wr) { gp(psi_wr[pHxy_f])[p] = a; gp(psi_wr[pHzy_f])[p] = b; }
+            const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)y * sy + (uint32_t)x;
+            const float a = (bh * *at(hr, psi_off(k, pHxy_f - 12) + p)) + ((ch * (e.ez_yp - e.ez_c)) / dx);
+            const float b = (bh * *at(hr, psi_off(k, pHzy_f - 12) + p)) + ((ch * (e.ex_yp - e.ex_c)) / dx);
+            if (hw) { *at(hw, psi_off(k, pHxy_f - 12) + p) = a; *at(hw, psi_off(k, pHzy_f - 12) + p) = b; }
             hx -= dtm0 * a;
             hz += dtm0 * b;
         }
         if (y >= ny - np && yin) {                              // back
             const int yl = y - (ny - np);
             const float bh = d->prof[BH_URB][yl], ch = d->prof[CH_URB][yl];
-            const long long p = (long long)z * k.fb_sz + (long long)yl * sy + x;
-            const float a = (bh * gp(psi_rd[pHxy_b])[p]) + ((ch * (e.ez_yp - e.ez_c)) / dx);
-            const float b = (bh * gp(psi_rd[pHzy_b])[p]) + ((ch * (e.ex_yp - e.ex_c)) / dx);
-            if (psi_wr) { gp(psi_wr[pHxy_b])[p] = a; gp(psi_wr[pHzy_b])[p] = b; }
+            const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)yl * sy + (uint32_t)x;
+            const float a = (bh * *at(hr, psi_off(k, pHxy_b - 12) + p)) + ((ch * (e.ez_yp - e.ez_c)) / dx);
+            const float b = (bh * *at(hr, psi_off(k, pHzy_b - 12) + p)) + ((ch * (e.ex_yp - e.ex_c)) / dx);
+            if (hw) { *at(hw, psi_off(k, pHxy_b - 12) + p) = a; *at(hw, psi_off(k, pHzy_b - 12) + p) = b; }
             hx -= dtm0 * a;
             hz += dtm0 * b;
         }
     }
-    h[0] = hx;
-    h[1] = hy;
-    h[2] = hz;
 }
 
 // H values around an E node (x, y, z) at the new time level.
@@ -209,8 +228,8 @@ struct HAround {
 // cpu/main_sim.py:158-161) or the hard source at the feed-in plane
 // (INCIDENT, cpu/incident_sim.py:142-144).  ca, cb are read only below the
 // top plane (z < nz - 1).
-__device__ __noinline__ void e_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
-                                   float cb, HAround h, float *__restrict__ ef);
+__device__ __forceinline__ void e_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
+                                       float cb, const HAround &h, float &ex, float &ey, float &ez);
 
 __device__ __forceinline__ void e_cell(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
                                        float ca, float cb, const HAround &h, float &ex, float &ey,
@@ -222,56 +241,49 @@ __device__ __forceinline__ void e_cell(const Desc *__restrict__ d, const StepK &
         if (zlo && ylo && yin && xin) ex = (ex * ca) + (cb * ((h.hz_c - h.hz_ym) - (h.hy_c - h.hy_zm)));
         if (zlo && yin && xlo && xin) ey = (ey * ca) + (cb * ((h.hx_c - h.hx_zm) - (h.hz_c - h.hz_xm)));
         if (ylo && yin && xlo && xin) ez = (ez * ca) + (cb * ((h.hy_c - h.hy_xm) - (h.hx_c - h.hx_ym)));
-        if (z < np || z >= nz - np || x < np || x >= nx - np || y < np || y >= ny - np) {
-            float ef[3] = {ex, ey, ez};
-            e_cpml(d, k, x, y, z, cb, h, ef);
-            ex = ef[0];
-            ey = ef[1];
-            ez = ef[2];
-        }
+        if (z < np || z >= nz - np || x < np || x >= nx - np || y < np || y >= ny - np)
+            e_cpml(d, k, x, y, z, cb, h, ex, ey, ez);
     }
     if (z == k.tfsf_z) {
-        const long long q = (long long)y * nx + x;
+        const uint32_t q = (uint32_t)y * nx + x;
         if (k.inj) {
-            ey += k.dte0dx * k.b2[q];   // hx_inc
-            ex -= k.dte0dx * k.b3[q];   // hy_inc
+            ey += k.dte0dx * *at(k.b2, q);   // hx_inc
+            ex -= k.dte0dx * *at(k.b3, q);   // hy_inc
         } else if (k.hard && ylo && yin && xin) {
-            ex = k.b0[q];
-            ey = k.b1[q];
+            ex = *at(k.b0, q);
+            ey = *at(k.b1, q);
         }
     }
 }
 
 // The six E-CPML corrections of one cell below the top plane, in the
-// reference order d,u,l,r,f,b (psi updated in place); ef = (ex, ey, ez)
-// after the Yee update.  Called only for cells inside an absorber.
-__device__ __noinline__ void e_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
-                                   float cb, HAround h, float *__restrict__ ef) {
+// reference order d,u,l,r,f,b (psi updated in place), after its Yee update.
+__device__ __forceinline__ void e_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
+                                       float cb, const HAround &h, float &ex, float &ey, float &ez) {
     const int nx = k.nx, ny = k.ny, nz = k.nz, np = k.np;
-    const long long sy = k.sy;
+    const uint32_t sy = k.sy;
     const float dx = k.dx;
     const bool xlo = x >= 1, ylo = y >= 1, zlo = z >= 1;
     const bool xin = x < nx - 1, yin = y < ny - 1;
-    float *const *psi = d->psi[0];
-    float ex = ef[0], ey = ef[1], ez = ef[2];
+    gfloat *psi = k.psiE;
     const float cbdx = cb * dx;
     if (yin && xin) {
         if (zlo && z < np) {                                    // down
             const float be = d->prof[BE_DLF][z], ce = d->prof[CE_DLF][z];
-            const long long p = (long long)z * k.du_sz + (long long)y * sy + x;
-            const float a = (be * gp(psi[pEyz_d])[p]) + ((ce * (h.hx_c - h.hx_zm)) / dx);
-            const float b = (be * gp(psi[pExz_d])[p]) + ((ce * (h.hy_c - h.hy_zm)) / dx);
-            gp(psi[pEyz_d])[p] = a; gp(psi[pExz_d])[p] = b;
+            const uint32_t p = (uint32_t)z * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
+            const float a = (be * *at(psi, psi_off(k, pEyz_d) + p)) + ((ce * (h.hx_c - h.hx_zm)) / dx);
+            const float b = (be * *at(psi, psi_off(k, pExz_d) + p)) + ((ce * (h.hy_c - h.hy_zm)) / dx);
+            *at(psi, psi_off(k, pEyz_d) + p) = a; *at(psi, psi_off(k, pExz_d) + p) = b;
             ey += cbdx * a;
             ex -= cbdx * b;
         }
         if (z >= nz - np) {                                     // up
             const int zl = z - (nz - np);
             const float be = d->prof[BE_URB][zl], ce = d->prof[CE_URB][zl];
-            const long long p = (long long)zl * k.du_sz + (long long)y * sy + x;
-            const float a = (be * gp(psi[pEyz_u])[p]) + ((ce * (h.hx_c - h.hx_zm)) / dx);
-            const float b = (be * gp(psi[pExz_u])[p]) + ((ce * (h.hy_c - h.hy_zm)) / dx);
-            gp(psi[pEyz_u])[p] = a; gp(psi[pExz_u])[p] = b;
+            const uint32_t p = (uint32_t)zl * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
+            const float a = (be * *at(psi, psi_off(k, pEyz_u) + p)) + ((ce * (h.hx_c - h.hx_zm)) / dx);
+            const float b = (be * *at(psi, psi_off(k, pExz_u) + p)) + ((ce * (h.hy_c - h.hy_zm)) / dx);
+            *at(psi, psi_off(k, pEyz_u) + p) = a; *at(psi, psi_off(k, pExz_u) + p) = b;
             ey += cbdx * a;
             ex -= cbdx * b;
         }
@@ -279,20 +291,20 @@ __device__ __noinline__ void e_cpml(const Desc *__restrict__ d, const StepK &k, 
     if (yin) {
         if (xlo && x < np) {                                    // left
             const float be = d->prof[BE_DLF][x], ce = d->prof[CE_DLF][x];
-            const long long p = (long long)z * k.lr_sz + (long long)y * np + x;
-            const float a = (be * gp(psi[pEzx_l])[p]) + ((ce * (h.hy_c - h.hy_xm)) / dx);
-            const float b = (be * gp(psi[pEyx_l])[p]) + ((ce * (h.hz_c - h.hz_xm)) / dx);
-            gp(psi[pEzx_l])[p] = a; gp(psi[pEyx_l])[p] = b;
+            const uint32_t p = (uint32_t)z * k.lr_sz + (uint32_t)y * np + (uint32_t)x;
+            const float a = (be * *at(psi, psi_off(k, pEzx_l) + p)) + ((ce * (h.hy_c - h.hy_xm)) / dx);
+            const float b = (be * *at(psi, psi_off(k, pEyx_l) + p)) + ((ce * (h.hz_c - h.hz_xm)) / dx);
+            *at(psi, psi_off(k, pEzx_l) + p) = a; *at(psi, psi_off(k, pEyx_l) + p) = b;
             ez += cbdx * a;
             ey -= cbdx * b;
         }
         if (x >= nx - np && xin) {                              // right
             const int xl = x - (nx - np);
             const float be = d->prof[BE_URB][xl], ce = d->prof[CE_URB][xl];
-            const long long p = (long long)z * k.lr_sz + (long long)y * np + xl;
-            const float a = (be * gp(psi[pEzx_r])[p]) + ((ce * (h.hy_c - h.hy_xm)) / dx);
-            const float b = (be * gp(psi[pEyx_r])[p]) + ((ce * (h.hz_c - h.hz_xm)) / dx);
-            gp(psi[pEzx_r])[p] = a; gp(psi[pEyx_r])[p] = b;
+            const uint32_t p = (uint32_t)z * k.lr_sz + (uint32_t)y * np + (uint32_t)xl;
+            const float a = (be * *at(psi, psi_off(k, pEzx_r) + p)) + ((ce * (h.hy_c - h.hy_xm)) / dx);
+            const float b = (be * *at(psi, psi_off(k, pEyx_r) + p)) + ((ce * (h.hz_c - h.hz_xm)) / dx);
+            *at(psi, psi_off(k, pEzx_r) + p) = a; *at(psi, psi_off(k, pEyx_r) + p) = b;
             ez += cbdx * a;
             ey -= cbdx * b;
         }
@@ -300,27 +312,24 @@ __device__ __noinline__ void e_cpml(const Desc *__restrict__ d, const StepK &k, 
     if (xin) {
         if (ylo && y < np) {                                    // front
             const float be = d->prof[BE_DLF][y], ce = d->prof[CE_DLF][y];
-            const long long p = (long long)z * k.fb_sz + (long long)y * sy + x;
-            const float a = (be * gp(psi[pExy_f])[p]) + ((ce * (h.hz_c - h.hz_ym)) / dx);
-            const float b = (be * gp(psi[pEzy_f])[p]) + ((ce * (h.hx_c - h.hx_ym)) / dx);
-            gp(psi[pExy_f])[p] = a; gp(psi[pEzy_f])[p] = b;
+            const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)y * sy + (uint32_t)x;
+            const float a = (be * *at(psi, psi_off(k, pExy_f) + p)) + ((ce * (h.hz_c - h.hz_ym)) / dx);
+            const float b = (be * *at(psi, psi_off(k, pEzy_f) + p)) + ((ce * (h.hx_c - h.hx_ym)) / dx);
+            *at(psi, psi_off(k, pExy_f) + p) = a; *at(psi, psi_off(k, pEzy_f) + p) = b;
             ex += cbdx * a;
             ez -= cbdx * b;
         }
         if (y >= ny - np && yin) {                              // back
             const int yl = y - (ny - np);
             const float be = d->prof[BE_URB][yl], ce = d->prof[CE_URB][yl];
-            const long long p = (long long)z * k.fb_sz + (long long)yl * sy + x;
-            const float a = (be * gp(psi[pExy_b])[p]) + ((ce * (h.hz_c - h.hz_ym)) / dx);
-            const float b = (be * gp(psi[pEzy_b])[p]) + ((ce * (h.hx_c - h.hx_ym)) / dx);
-            gp(psi[pExy_b])[p] = a; gp(psi[pEzy_b])[p] = b;
+            const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)yl * sy + (uint32_t)x;
+            const float a = (be * *at(psi, psi_off(k, pExy_b) + p)) + ((ce * (h.hz_c - h.hz_ym)) / dx);
+            const float b = (be * *at(psi, psi_off(k, pEzy_b) + p)) + ((ce * (h.hx_c - h.hx_ym)) / dx);
+            *at(psi, psi_off(k, pExy_b) + p) = a; *at(psi, psi_off(k, pEzy_b) + p) = b;
             ex += cbdx * a;
             ez -= cbdx * b;
         }
     }
-    ef[0] = ex;
-    ef[1] = ey;
-    ef[2] = ez;
 }
 
 // ---------------------------------------------------------------------------
@@ -376,7 +385,8 @@ __device__ __forceinline__ void vcopy(float (&r)[V], const float (&s)[V]) {
 // MAIN runs.  A block of 64 x ty lanes covers a (64 V) x ty tile, one row
 // per wavefront; each lane marches z upward through the tile's z range,
 // reusing the plane above (H half: ex, ey) or below (E half: hx, hy) from
-// registers, and issues the loads of plane z + 1 before it computes plane z.
+// registers.  (Issuing plane z + 1's loads before computing plane z was
+// measured slower: the extra registers cost more occupancy than it hides.)
 // The x +- 1 neighbours come from the lane's own cells or, at its ends, from
 // the adjacent lane by a wavefront shuffle (from memory at the tile edge).
 // ---------------------------------------------------------------------------
@@ -392,26 +402,26 @@ struct HPlane {
 template <int V>
 __device__ __forceinline__ void h_plane_load(HPlane<V> &P, const gfloat *Ex, const gfloat *Ey,
                                              const gfloat *Ez, const gfloat *Hx, const gfloat *Hy,
-                                             const gfloat *Hz, long long o, long long sy, long long sz,
+                                             const gfloat *Hz, uint32_t o, uint32_t sy, uint32_t sz,
                                              bool act, bool zin, bool yin, bool next_mem) {
     vset(P.exn, 0.f); vset(P.eyn, 0.f); vset(P.ez, 0.f); vset(P.ezyp, 0.f); vset(P.exyp, 0.f);
     vset(P.hx, 0.f); vset(P.hy, 0.f); vset(P.hz, 0.f);
     P.ey_x = P.ez_x = 0.f;
     if (act) {
-        vld(P.hx, Hx + o);
-        vld(P.hy, Hy + o);
-        vld(P.hz, Hz + o);
+        vld(P.hx, at(Hx, o));
+        vld(P.hy, at(Hy, o));
+        vld(P.hz, at(Hz, o));
         if (zin) {
-            vld(P.exn, Ex + o + sz);
-            vld(P.eyn, Ey + o + sz);
-            vld(P.ez, Ez + o);
+            vld(P.exn, at(Ex, o + sz));
+            vld(P.eyn, at(Ey, o + sz));
+            vld(P.ez, at(Ez, o));
             if (yin) {
-                vld(P.ezyp, Ez + o + sy);
-                vld(P.exyp, Ex + o + sy);
+                vld(P.ezyp, at(Ez, o + sy));
+                vld(P.exyp, at(Ex, o + sy));
             }
             if (next_mem) {
-                P.ey_x = Ey[o + V];
-                P.ez_x = Ez[o + V];
+                P.ey_x = *at(Ey, o + V);
+                P.ez_x = *at(Ez, o + V);
             }
         }
     }
@@ -434,26 +444,23 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     // is below nx - 1; for lane 63 it lies in the next tile
     const bool next_mem = lane == kWave - 1 && xb + V < k.nx;
     const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
-    const long long sy = k.sy, sz = d->sz;
+    const uint32_t sy = k.sy, sz = k.sz;
     const gfloat *Ex = gp(d->f[0][0]), *Ey = gp(d->f[0][1]), *Ez = gp(d->f[0][2]);
     gfloat *Hx = gp(d->f[0][3]), *Hy = gp(d->f[0][4]), *Hz = gp(d->f[0][5]);
-    float *const *psi = d->psi[0];
+    gfloat *psiH = gp(d->psi[0][pHxz_d]);
     const bool yin = y < k.ny - 1;
 
-    long long o = (long long)z0 * sz + (long long)y * sy + xb;
+    uint32_t o = (uint32_t)z0 * sz + (uint32_t)y * sy + (uint32_t)xb;
     float exc[V], eyc[V];
     vset(exc, 0.f);
     vset(eyc, 0.f);
     if (act) {
-        vld(exc, Ex + o);
-        vld(eyc, Ey + o);
+        vld(exc, at(Ex, o));
+        vld(eyc, at(Ey, o));
     }
-    HPlane<V> P;
-    h_plane_load(P, Ex, Ey, Ez, Hx, Hy, Hz, o, sy, sz, act, z0 < k.nz - 1, yin, next_mem);
     for (int z = z0; z < z1; ++z, o += sz) {
-        HPlane<V> N;
-        h_plane_load(N, Ex, Ey, Ez, Hx, Hy, Hz, o + sz, sy, sz, act && z + 1 < z1, z + 1 < k.nz - 1,
-                     yin, next_mem);
+        HPlane<V> P;
+        h_plane_load(P, Ex, Ey, Ez, Hx, Hy, Hz, o, sy, sz, act, z < k.nz - 1, yin, next_mem);
         float ey_next = __shfl_down(eyc[0], 1, kWave);
         float ez_next = __shfl_down(P.ez[0], 1, kWave);
         if (lane == kWave - 1) {
@@ -469,17 +476,16 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
                 e.ey_xp = c + 1 < V ? eyc[c + 1 < V ? c + 1 : 0] : ey_next;
                 e.ez_c = P.ez[c]; e.ez_yp = P.ezyp[c];
                 e.ez_xp = c + 1 < V ? P.ez[c + 1 < V ? c + 1 : 0] : ez_next;
-                h_cell(d, k, xb + c, y, z, e, P.hx[c], P.hy[c], P.hz[c], psi, psi);
+                h_cell(d, k, xb + c, y, z, e, P.hx[c], P.hy[c], P.hz[c], psiH, psiH);
             }
         }
         if (act) {
-            vst(Hx + o, P.hx);
-            vst(Hy + o, P.hy);
-            vst(Hz + o, P.hz);
+            vst(at(Hx, o), P.hx);
+            vst(at(Hy, o), P.hy);
+            vst(at(Hz, o), P.hz);
         }
         vcopy(exc, P.exn);
         vcopy(eyc, P.eyn);
-        P = N;
     }
 }
 
@@ -495,7 +501,7 @@ template <int V>
 __device__ __forceinline__ void e_plane_load(EPlane<V> &Q, const gfloat *Ex, const gfloat *Ey,
                                              const gfloat *Ez, const gfloat *Hx, const gfloat *Hy,
                                              const gfloat *Hz, const gfloat *Ca, const gfloat *Cb,
-                                             float ca_u, float cb_u, long long o, long long sy, bool act,
+                                             float ca_u, float cb_u, uint32_t o, uint32_t sy, bool act,
                                              bool zin, bool mat, bool ylo, bool prev_mem) {
     vset(Q.hx, 0.f); vset(Q.hy, 0.f); vset(Q.hz, 0.f); vset(Q.hzym, 0.f); vset(Q.hxym, 0.f);
     vset(Q.ex, 0.f); vset(Q.ey, 0.f); vset(Q.ez, 0.f);
@@ -503,24 +509,24 @@ __device__ __forceinline__ void e_plane_load(EPlane<V> &Q, const gfloat *Ex, con
     vset(Q.cb, cb_u);
     Q.hz_p = Q.hy_p = 0.f;
     if (act) {
-        vld(Q.hx, Hx + o);
-        vld(Q.hy, Hy + o);
-        vld(Q.ex, Ex + o);
-        vld(Q.ey, Ey + o);
-        vld(Q.ez, Ez + o);
+        vld(Q.hx, at(Hx, o));
+        vld(Q.hy, at(Hy, o));
+        vld(Q.ex, at(Ex, o));
+        vld(Q.ey, at(Ey, o));
+        vld(Q.ez, at(Ez, o));
         if (zin) {
-            vld(Q.hz, Hz + o);
+            vld(Q.hz, at(Hz, o));
             if (ylo) {
-                vld(Q.hzym, Hz + o - sy);
-                vld(Q.hxym, Hx + o - sy);
+                vld(Q.hzym, at(Hz, o - sy));
+                vld(Q.hxym, at(Hx, o - sy));
             }
             if (mat) {
-                vld(Q.ca, Ca + o);
-                vld(Q.cb, Cb + o);
+                vld(Q.ca, at(Ca, o));
+                vld(Q.cb, at(Cb, o));
             }
             if (prev_mem) {
-                Q.hz_p = Hz[o - 1];
-                Q.hy_p = Hy[o - 1];
+                Q.hz_p = *at(Hz, o - 1);
+                Q.hy_p = *at(Hy, o - 1);
             }
         }
     }
@@ -541,7 +547,7 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     const bool act = xb < k.nx;
     const bool prev_mem = lane == 0 && xb >= 1 && act;    // x - 1 lies in the previous tile
     const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
-    const long long sy = k.sy, sz = d->sz;
+    const uint32_t sy = k.sy, sz = k.sz;
     gfloat *Ex = gp(d->f[0][0]), *Ey = gp(d->f[0][1]), *Ez = gp(d->f[0][2]);
     const gfloat *Hx = gp(d->f[0][3]), *Hy = gp(d->f[0][4]), *Hz = gp(d->f[0][5]);
     const gfloat *Ca = gp(d->ca), *Cb = gp(d->cb);
@@ -549,22 +555,18 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     const int uni_z = d->uni_z;
     const float ca_u = d->ca_u, cb_u = d->cb_u;
 
-    long long o = (long long)z0 * sz + (long long)y * sy + xb;
+    uint32_t o = (uint32_t)z0 * sz + (uint32_t)y * sy + (uint32_t)xb;
     float hxzm[V], hyzm[V];
     vset(hxzm, 0.f);
     vset(hyzm, 0.f);
     if (act && z0 >= 1) {
-        vld(hxzm, Hx + o - sz);
-        vld(hyzm, Hy + o - sz);
+        vld(hxzm, at(Hx, o - sz));
+        vld(hyzm, at(Hy, o - sz));
     }
-    EPlane<V> P;
-    e_plane_load(P, Ex, Ey, Ez, Hx, Hy, Hz, Ca, Cb, ca_u, cb_u, o, sy, act, z0 < k.nz - 1,
-                 z0 < k.nz - 1 && z0 < uni_z, ylo, prev_mem);
     for (int z = z0; z < z1; ++z, o += sz) {
-        EPlane<V> N;
-        const int zn = z + 1;
-        e_plane_load(N, Ex, Ey, Ez, Hx, Hy, Hz, Ca, Cb, ca_u, cb_u, o + sz, sy, act && zn < z1,
-                     zn < k.nz - 1, zn < k.nz - 1 && zn < uni_z, ylo, prev_mem);
+        EPlane<V> P;
+        e_plane_load(P, Ex, Ey, Ez, Hx, Hy, Hz, Ca, Cb, ca_u, cb_u, o, sy, act, z < k.nz - 1,
+                     z < k.nz - 1 && z < uni_z, ylo, prev_mem);
         float hz_prev = __shfl_up(P.hz[V - 1], 1, kWave);
         float hy_prev = __shfl_up(P.hy[V - 1], 1, kWave);
         if (lane == 0) {
@@ -584,13 +586,12 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
             }
         }
         if (act) {
-            vst(Ex + o, P.ex);
-            vst(Ey + o, P.ey);
-            vst(Ez + o, P.ez);
+            vst(at(Ex, o), P.ex);
+            vst(at(Ey, o), P.ey);
+            vst(at(Ez, o), P.ez);
         }
         vcopy(hxzm, P.hx);
         vcopy(hyzm, P.hy);
-        P = N;
     }
 }
 
@@ -631,10 +632,9 @@ struct HRow {
 
 template <int V>
 __device__ __forceinline__ void h_row(const Desc *__restrict__ d, const StepK &k, const gfloat *const *fo,
-                                      int xb, int y, int z, long long o, long long sz, bool act,
-                                      bool next_mem, float *const *psi_rd, float *const *psi_wr,
-                                      HRow<V> &r) {
-    const long long sy = k.sy;
+                                      int xb, int y, int z, uint32_t o, uint32_t sz, bool act,
+                                      bool next_mem, const gfloat *hr, gfloat *hw, HRow<V> &r) {
+    const uint32_t sy = k.sy;
     const bool zin = z < k.nz - 1, yin = y < k.ny - 1;
     float exn[V], eyn[V], ezyp[V], exyp[V];
     vset(r.hx, 0.f); vset(r.hy, 0.f); vset(r.hz, 0.f);
@@ -642,22 +642,22 @@ __device__ __forceinline__ void h_row(const Desc *__restrict__ d, const StepK &k
     vset(exn, 0.f); vset(eyn, 0.f); vset(ezyp, 0.f); vset(exyp, 0.f);
     float ey_x = 0.f, ez_x = 0.f;
     if (act) {
-        vld(r.hx, fo[3] + o);
-        vld(r.hy, fo[4] + o);
-        vld(r.hz, fo[5] + o);
-        vld(r.ex, fo[0] + o);
-        vld(r.ey, fo[1] + o);
-        vld(r.ez, fo[2] + o);
+        vld(r.hx, at(fo[3], o));
+        vld(r.hy, at(fo[4], o));
+        vld(r.hz, at(fo[5], o));
+        vld(r.ex, at(fo[0], o));
+        vld(r.ey, at(fo[1], o));
+        vld(r.ez, at(fo[2], o));
         if (zin) {
-            vld(exn, fo[0] + o + sz);
-            vld(eyn, fo[1] + o + sz);
+            vld(exn, at(fo[0], o + sz));
+            vld(eyn, at(fo[1], o + sz));
             if (yin) {
-                vld(ezyp, fo[2] + o + sy);
-                vld(exyp, fo[0] + o + sy);
+                vld(ezyp, at(fo[2], o + sy));
+                vld(exyp, at(fo[0], o + sy));
             }
             if (next_mem) {
-                ey_x = fo[1][o + V];
-                ez_x = fo[2][o + V];
+                ey_x = *at(fo[1], o + V);
+                ez_x = *at(fo[2], o + V);
             }
         }
     }
@@ -676,14 +676,15 @@ __device__ __forceinline__ void h_row(const Desc *__restrict__ d, const StepK &k
             e.ey_xp = c + 1 < V ? r.ey[c + 1 < V ? c + 1 : 0] : ey_next;
             e.ez_c = r.ez[c]; e.ez_yp = ezyp[c];
             e.ez_xp = c + 1 < V ? r.ez[c + 1 < V ? c + 1 : 0] : ez_next;
-            h_cell(d, k, xb + c, y, z, e, r.hx[c], r.hy[c], r.hz[c], psi_rd, psi_wr);
+            h_cell(d, k, xb + c, y, z, e, r.hx[c], r.hy[c], r.hz[c], hr, hw);
         }
     }
 }
 
 template <int V, int R>
 __global__ void __launch_bounds__(256)
-k_step_strip(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles) {
+k_step_strip(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles,
+             float *__restrict__ halo) {
     const int nblk = (ntiles + 3) >> 2;
     // one tile per wavefront: threadIdx.y is uniform across the wavefront,
     // and saying so keeps the descriptor and its pointers in scalar registers
@@ -701,9 +702,10 @@ k_step_strip(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int
     const bool next_mem = lane == kWave - 1 && xb + V < k.nx;
     const bool halo_x = lane == 0 && x0 > 0;           // recompute H(t+1/2) at x0 - 1
     const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
-    const int y1 = min(y0 + R, k.ny);
+    const int zs = z0 > 0 ? z0 - 1 : 0;                // plane z0 - 1: recomputed, not stored
+    const int ry = min(R, k.ny - y0);                  // rows of the strip
     const int p = t & 1;
-    const long long sy = k.sy, sz = d->sz;
+    const uint32_t sy = k.sy, sz = k.sz;
     const gfloat *fo[6];
     gfloat *fn[6];
 #pragma unroll
@@ -711,99 +713,105 @@ k_step_strip(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int
         fo[c] = gp(d->f[p][c]);
         fn[c] = gp(d->f[1 - p][c]);
     }
-    float *const *psi_o = d->psi[p];
-    float *const *psi_n = d->psi[1 - p];
+    const gfloat *psi_o = gp(d->psi[p][pHxz_d]);      // H-CPML state of this step
+    gfloat *psi_n = gp(d->psi[1 - p][pHxz_d]);        // and of the next
     const gfloat *Ca = gp(d->ca), *Cb = gp(d->cb);
     const int uni_z = d->uni_z;
     const float ca_u = d->ca_u, cb_u = d->cb_u;
+    // this tile's hx, hy of plane z0 - 1: [2][R][64 V]
+    gfloat *hs = at(gp(halo), (uint32_t)wt * (2 * R * kWave * V) + V * lane);
 
 #pragma unroll 1
-    for (int z = z0; z < z1; ++z) {
+    for (int z = zs; z < z1; ++z) {
         const bool zin = z < k.nz - 1;
-        const long long oz = (long long)z * sz + xb;
-        // H(t+1/2) of the row below the strip, this plane
+        const bool hplane = z < z0;
+        const uint32_t oz = (uint32_t)z * sz + (uint32_t)xb;
+        // H(t+1/2) of the row below, this plane (row y0 - 1: recomputed)
         float hzb[V], hxb[V];
         vset(hzb, 0.f);
         vset(hxb, 0.f);
-        if (y0 >= 1) {
-            HRow<V> b;
-            h_row(d, k, fo, xb, y0 - 1, z, oz + (long long)(y0 - 1) * sy, sz, act, next_mem, psi_o,
-                  nullptr, b);
-            vcopy(hzb, b.hz);
-            vcopy(hxb, b.hx);
-        }
+        const int r0 = (!hplane && y0 >= 1) ? -1 : 0;
 #pragma unroll 1
-        for (int y = y0; y < y1; ++y) {
-            const long long o = oz + (long long)y * sy;
-            const bool yin = y < k.ny - 1;
+        for (int r = r0; r < ry; ++r) {
+            const int y = y0 + r;
+            const uint32_t o = oz + (uint32_t)y * sy;
+            const bool own = r >= 0 && !hplane;
             HRow<V> c;
-            h_row(d, k, fo, xb, y, z, o, sz, act, next_mem, psi_o, psi_n, c);
-            // H(t+1/2) at z - 1
-            float hxzm[V], hyzm[V];
-            vset(hxzm, 0.f);
-            vset(hyzm, 0.f);
-            if (z > z0) {
+            h_row(d, k, fo, xb, y, z, o, sz, act, next_mem, psi_o, own ? psi_n : nullptr, c);
+            if (hplane) {                              // keep hx, hy for plane z0
                 if (act) {
-                    vld(hxzm, fn[3] + o - sz);
-                    vld(hyzm, fn[4] + o - sz);
+                    vst(at(hs, (uint32_t)r * kWave * V), c.hx);
+                    vst(at(hs, (uint32_t)(R + r) * kWave * V), c.hy);
                 }
-            } else if (z >= 1) {
-                HRow<V> cz;
-                h_row(d, k, fo, xb, y, z - 1, o - sz, sz, act, next_mem, psi_o, nullptr, cz);
-                vcopy(hxzm, cz.hx);
-                vcopy(hyzm, cz.hy);
+                continue;
             }
-            // H(t+1/2) at x - 1 of the lane's first cell
-            float hz_prev = __shfl_up(c.hz[V - 1], 1, kWave);
-            float hy_prev = __shfl_up(c.hy[V - 1], 1, kWave);
-            if (lane == 0) {
-                hz_prev = 0.f;
-                hy_prev = 0.f;
-                if (halo_x) {
-                    // only hy and hz of the cell are used: hx's inputs are left out
-                    const long long oh = o - 1;
-                    EAround e;
-                    e.ex_c = fo[0][oh];
-                    e.ey_c = fo[1][oh];
-                    e.ez_c = fo[2][oh];
-                    e.ey_xp = c.ey[0];
-                    e.ez_xp = c.ez[0];
-                    e.ex_n = zin ? fo[0][oh + sz] : 0.f;
-                    e.ex_yp = (zin && yin) ? fo[0][oh + sy] : 0.f;
-                    e.ey_n = e.ey_c;
-                    e.ez_yp = e.ez_c;
-                    float hx_h = 0.f, hy_h = fo[4][oh], hz_h = fo[5][oh];
-                    h_cell(d, k, x0 - 1, y, z, e, hx_h, hy_h, hz_h, psi_o, nullptr);
-                    hz_prev = hz_h;
-                    hy_prev = hy_h;
+            if (own) {
+                const bool yin = y < k.ny - 1;
+                // H(t+1/2) at z - 1: stored by this lane one plane earlier
+                float hxzm[V], hyzm[V];
+                vset(hxzm, 0.f);
+                vset(hyzm, 0.f);
+                if (act && z >= 1) {
+                    if (z > z0) {
+                        vld(hxzm, at(fn[3], o - sz));
+                        vld(hyzm, at(fn[4], o - sz));
+                    } else {
+                        vld(hxzm, at(hs, (uint32_t)r * kWave * V));
+                        vld(hyzm, at(hs, (uint32_t)(R + r) * kWave * V));
+                    }
                 }
-            }
-            float ca[V], cb[V];
-            vset(ca, ca_u);
-            vset(cb, cb_u);
-            if (act && zin && z < uni_z) {
-                vld(ca, Ca + o);
-                vld(cb, Cb + o);
-            }
+                // H(t+1/2) at x - 1 of the lane's first cell
+                float hz_prev = __shfl_up(c.hz[V - 1], 1, kWave);
+                float hy_prev = __shfl_up(c.hy[V - 1], 1, kWave);
+                if (lane == 0) {
+                    hz_prev = 0.f;
+                    hy_prev = 0.f;
+                    if (halo_x) {
+                        // only hy and hz of the cell are used: hx's inputs are left out
+                        const uint32_t oh = o - 1;
+                        EAround e;
+                        e.ex_c = *at(fo[0], oh);
+                        e.ey_c = *at(fo[1], oh);
+                        e.ez_c = *at(fo[2], oh);
+                        e.ey_xp = c.ey[0];
+                        e.ez_xp = c.ez[0];
+                        e.ex_n = zin ? *at(fo[0], oh + sz) : 0.f;
+                        e.ex_yp = (zin && yin) ? *at(fo[0], oh + sy) : 0.f;
+                        e.ey_n = e.ey_c;
+                        e.ez_yp = e.ez_c;
+                        float hx_h = 0.f, hy_h = *at(fo[4], oh), hz_h = *at(fo[5], oh);
+                        h_cell(d, k, x0 - 1, y, z, e, hx_h, hy_h, hz_h, psi_o, nullptr);
+                        hz_prev = hz_h;
+                        hy_prev = hy_h;
+                    }
+                }
+                float ca[V], cb[V];
+                vset(ca, ca_u);
+                vset(cb, cb_u);
+                if (act && zin && z < uni_z) {
+                    vld(ca, at(Ca, o));
+                    vld(cb, at(Cb, o));
+                }
 #pragma unroll
-            for (int q = 0; q < V; ++q) {
-                if (xb + q < k.nx) {
-                    HAround h;
-                    h.hx_c = c.hx[q]; h.hy_c = c.hy[q]; h.hz_c = c.hz[q];
-                    h.hx_zm = hxzm[q]; h.hy_zm = hyzm[q];
-                    h.hz_ym = hzb[q]; h.hx_ym = hxb[q];
-                    h.hz_xm = q > 0 ? c.hz[q > 0 ? q - 1 : 0] : hz_prev;
-                    h.hy_xm = q > 0 ? c.hy[q > 0 ? q - 1 : 0] : hy_prev;
-                    e_cell(d, k, xb + q, y, z, ca[q], cb[q], h, c.ex[q], c.ey[q], c.ez[q]);
+                for (int q = 0; q < V; ++q) {
+                    if (xb + q < k.nx) {
+                        HAround h;
+                        h.hx_c = c.hx[q]; h.hy_c = c.hy[q]; h.hz_c = c.hz[q];
+                        h.hx_zm = hxzm[q]; h.hy_zm = hyzm[q];
+                        h.hz_ym = hzb[q]; h.hx_ym = hxb[q];
+                        h.hz_xm = q > 0 ? c.hz[q > 0 ? q - 1 : 0] : hz_prev;
+                        h.hy_xm = q > 0 ? c.hy[q > 0 ? q - 1 : 0] : hy_prev;
+                        e_cell(d, k, xb + q, y, z, ca[q], cb[q], h, c.ex[q], c.ey[q], c.ez[q]);
+                    }
                 }
-            }
-            if (act) {
-                vst(fn[3] + o, c.hx);
-                vst(fn[4] + o, c.hy);
-                vst(fn[5] + o, c.hz);
-                vst(fn[0] + o, c.ex);
-                vst(fn[1] + o, c.ey);
-                vst(fn[2] + o, c.ez);
+                if (act) {
+                    vst(at(fn[3], o), c.hx);
+                    vst(at(fn[4], o), c.hy);
+                    vst(at(fn[5], o), c.hz);
+                    vst(at(fn[0], o), c.ex);
+                    vst(at(fn[1], o), c.ey);
+                    vst(at(fn[2], o), c.ez);
+                }
             }
             vcopy(hzb, c.hz);
             vcopy(hxb, c.hx);
@@ -811,15 +819,15 @@ k_step_strip(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int
     }
 }
 
-template __global__ void k_step_strip<1, 2>(const Desc *, const int4 *, int);
-template __global__ void k_step_strip<1, 4>(const Desc *, const int4 *, int);
-template __global__ void k_step_strip<1, 8>(const Desc *, const int4 *, int);
-template __global__ void k_step_strip<2, 2>(const Desc *, const int4 *, int);
-template __global__ void k_step_strip<2, 4>(const Desc *, const int4 *, int);
-template __global__ void k_step_strip<2, 8>(const Desc *, const int4 *, int);
-template __global__ void k_step_strip<4, 2>(const Desc *, const int4 *, int);
-template __global__ void k_step_strip<4, 4>(const Desc *, const int4 *, int);
-template __global__ void k_step_strip<4, 8>(const Desc *, const int4 *, int);
+template __global__ void k_step_strip<1, 2>(const Desc *, const int4 *, int, float *);
+template __global__ void k_step_strip<1, 4>(const Desc *, const int4 *, int, float *);
+template __global__ void k_step_strip<1, 8>(const Desc *, const int4 *, int, float *);
+template __global__ void k_step_strip<2, 2>(const Desc *, const int4 *, int, float *);
+template __global__ void k_step_strip<2, 4>(const Desc *, const int4 *, int, float *);
+template __global__ void k_step_strip<2, 8>(const Desc *, const int4 *, int, float *);
+template __global__ void k_step_strip<4, 2>(const Desc *, const int4 *, int, float *);
+template __global__ void k_step_strip<4, 4>(const Desc *, const int4 *, int, float *);
+template __global__ void k_step_strip<4, 8>(const Desc *, const int4 *, int, float *);
 
 // ---------------------------------------------------------------------------
 // Running single-frequency DFT on the phasor planes

@@ -105,7 +105,7 @@ def large_material(pc, seed=0):
 # Batched: 32 soil realizations, lateral {3,4,6,8} lambda, depth U[1,2]
 # lambda, cell 0.03 lambda, HH and VV.
 # ---------------------------------------------------------------------------
-def batched_params_list(count=32, pols=('HH', 'VV'), seed=1000):
+def batched_params_list(count=32, pols=('HH', 'VV'), seed=1000, datatype='float32'):
     rng = np.random.default_rng(seed)
     sizes = [3.0, 4.0, 6.0, 8.0]
     out = []
@@ -115,7 +115,7 @@ def batched_params_list(count=32, pols=('HH', 'VV'), seed=1000):
         for pol in pols:
             pc = base_params(polarization=pol, incident_duration=10.0,
                              illumination_width=2.0, simulation_duration=12.0,
-                             material_fraction=0.5)
+                             material_fraction=0.5, datatype=datatype)
             set_geom_params(pc, lat, lat, depth, 12.0, 0.03, 'wavelength')
             out.append((i, pc))
     return out

@@ -3,8 +3,12 @@
 * every member of a continuously-batched mix gives bit-identical phasors
   and sigma0 to the same request run alone;
 * end to end (incident beam, main run, far field, sigma0) the FP32 device
-  pipeline matches the FP64 oracle's backscatter sigma0 within 0.05 dB.
+  pipeline matches the FP64 oracle's backscatter sigma0 within 0.05 dB, for
+  the plain workload, a batched soil realization (HH and VV) and a served
+  canopy request.
 """
+import copy
+
 import numpy as np
 import pytest
 
@@ -100,6 +104,34 @@ def _db(x):
     return 10 * np.log10(x)
 
 
+def _fp64_sigma0(pc32, material):
+    """The FP64 oracle pipeline of one model: incident run, main run, top-face
+    far field and sigma (h, v) over theta = -80..80."""
+    p64 = copy.deepcopy(pc32)
+    p64.processing_params.datatype = 'float64'
+    g = p64.geom_params
+    pex, pey = H.excitation(p64)
+    rec, inc = H.oracle_incident(p64, pex, pey)
+    mm = copy.deepcopy(material)
+    mm.relative_permittivity = np.asarray(mm.relative_permittivity, np.float64)
+    mm.conductivity = np.asarray(mm.conductivity, np.float64)
+    Ca, Cb = compute_material_arrays(p64, mm)
+    sim = H.oracle_main(p64, Ca, Cb, rec)
+    n = g.size_pml
+    nxp, nyp, nzp = g.size_x - 2 * n, g.size_y - 2 * n, g.size_z - 2 * n
+    th = np.linspace(-80.0, 80.0, 161)
+    ff = oracle.far_field(sim.phasors(), nxp, nyp, nzp, g.cell_size, 2000.0, th, [0.0],
+                          p64.inc_beam_params.frequency)
+    return oracle.sigma0(ff, oracle.incident_power(inc, g.cell_size), nxp, nyp, g.cell_size)
+
+
+def _backscatter(sigmas, pc):
+    """sigma at theta = -theta_inc of the co-polarised channel."""
+    sh, sv = sigmas
+    i = 80 - int(round(pc.inc_beam_params.theta_angle))
+    return (sh if pc.inc_beam_params.polarization == 'HH' else sv)[i]
+
+
 def test_sigma0_vs_fp64_oracle():
     """Plain workload (81^3 cells, 480 steps, rough soil, HH, theta 20):
     device FP32 pipeline vs the FP64 oracle pipeline, backscatter sigma0."""
@@ -114,22 +146,7 @@ def test_sigma0_vs_fp64_oracle():
     dm.release()
     beams.release()
     got_h = s['backscatter'][0]
-
-    p64 = workloads.plain_params(datatype='float64')
-    g = p64.geom_params
-    pex, pey = H.excitation(p64)
-    rec, inc = H.oracle_incident(p64, pex, pey)
-    mm = workloads.plain_material(p64, seed=0)
-    mm.relative_permittivity = mm.relative_permittivity.astype(np.float64)
-    mm.conductivity = mm.conductivity.astype(np.float64)
-    Ca, Cb = compute_material_arrays(p64, mm)
-    sim = H.oracle_main(p64, Ca, Cb, rec)
-    n = g.size_pml
-    nxp, nyp, nzp = g.size_x - 2 * n, g.size_y - 2 * n, g.size_z - 2 * n
-    th = np.linspace(-80.0, 80.0, 161)
-    ff = oracle.far_field(sim.phasors(), nxp, nyp, nzp, g.cell_size, 2000.0, th, [0.0],
-                          p64.inc_beam_params.frequency)
-    sh, sv = oracle.sigma0(ff, oracle.incident_power(inc, g.cell_size), nxp, nyp, g.cell_size)
+    sh, sv = _fp64_sigma0(pc, m)
     want_h = sh[60]                                   # theta = -20
     say('sigma0 HH dB: device', _db(got_h), 'fp64 oracle', _db(want_h))
     assert abs(_db(got_h) - _db(want_h)) <= 0.05
@@ -137,3 +154,53 @@ def test_sigma0_vs_fp64_oracle():
     dev = _db(s['sigma_h'][20:80])
     ref = _db(sh[20:80])
     assert np.max(np.abs(dev - ref)) <= 0.05
+
+
+def test_sigma0_batched_members_vs_fp64_oracle():
+    """Batched workload: the smallest soil realization (3 x 3 lambda at 0.03
+    lambda, 800 steps), HH and VV, stepped together as one batch; each
+    member's backscatter sigma0 vs its FP64 oracle pipeline."""
+    native.init(0)
+    cases = workloads.batched_params_list(count=1)
+    mats = [workloads.batched_material(pc, i) for i, pc in cases]
+    g = cases[0][1].geom_params
+    say('case: batched members', g.size_x, g.size_y, g.size_z, 'x', g.size_t, 'steps')
+    beams = B.BeamCache()
+    models = [B.DeviceModel.from_material(pc, m, beams) for (i, pc), m in zip(cases, mats)]
+    B.run_batched(models, chunk=64)
+    got = [m.sigma0() for m in models]
+    for m in models:
+        m.release()
+    beams.release()
+    for ((i, pc), m, s) in zip(cases, mats, got):
+        want = _backscatter(_fp64_sigma0(pc, m), pc)
+        dev = s['backscatter'][0 if pc.inc_beam_params.polarization == 'HH' else 1]
+        say(pc.inc_beam_params.polarization, 'sigma0 dB: device', _db(dev), 'fp64 oracle', _db(want))
+        assert abs(_db(dev) - _db(want)) <= 0.05
+
+
+def test_sigma0_served_request_vs_fp64_oracle():
+    """Serving workload: a canopy-over-soil request (4 x 4 lambda, 2 lambda
+    high, 0.025 lambda) served in a mix with a second request; its
+    backscatter sigma0 vs the FP64 oracle pipeline."""
+    from sar_pixel_simulator_amd.hip.measure import serving_requests
+    native.init(0)
+    reqs = serving_requests(2, rate_hz=1.0, lateral=(4.0,), height=(2.0,))
+    g = reqs[0].params.geom_params
+    say('case: served request', g.size_x, g.size_y, g.size_z, 'x', g.size_t, 'steps')
+    be = HipBackend(0)
+    srv = Server(be, hbm_budget=64 << 30, max_slots=2, chunk_steps=64)
+    for r in reqs:
+        r.arrival = 0.0
+        srv.submit(r)
+    srv.drain()
+    res = {r.rid: r for r in srv.results}
+    assert res[0].error is None, res[0].error
+    be.beams.release()
+    pc = reqs[0].params
+    want = _backscatter(_fp64_sigma0(pc, reqs[0].material), pc)
+    out = res[0].output
+    dev = out['backscatter'][0 if pc.inc_beam_params.polarization == 'HH' else 1]
+    say(pc.inc_beam_params.polarization, 'theta', pc.inc_beam_params.theta_angle,
+        'sigma0 dB: device', _db(dev), 'fp64 oracle', _db(want))
+    assert abs(_db(dev) - _db(want)) <= 0.05
