@@ -156,6 +156,17 @@ static int check_geom(const FdtdGeom *g) {
     CHECK(g->mode == FDTD_MODE_MAIN || g->mode == FDTD_MODE_INCIDENT, "unknown mode %d", g->mode);
     CHECK(g->dft_accum == FDTD_DFT_COMPENSATED || g->dft_accum == FDTD_DFT_REFERENCE,
           "unknown dft_accum %d", g->dft_accum);
+    {
+        // the stencil kernels index every field, material and CPML array
+        // with 32-bit offsets: each must hold fewer than 2^30 floats
+        const long long pitch = pitch_of(g->nx), lim = 1LL << 30;
+        const long long ncell = (long long)g->nz * g->ny * pitch;
+        const long long psi_block = 4LL * g->npml * ((long long)g->ny * pitch + (long long)g->nz * g->ny +
+                                                     (long long)g->nz * pitch);
+        CHECK(ncell < lim, "grid %d x %d x %d holds %lld cells per field array; the limit is 2^30",
+              g->nx, g->ny, g->nz, ncell);
+        CHECK(psi_block < lim, "CPML state of %lld floats per field exceeds 2^30", psi_block);
+    }
     if (g->mode == FDTD_MODE_MAIN) {
         CHECK(g->tfsf_z >= 0 && g->tfsf_z < g->nz, "tfsf_z %d outside [0, %d)", g->tfsf_z, g->nz);
     } else {
@@ -331,8 +342,12 @@ static int group_build(Group &G) {
     G.d_halo = nullptr;
     HIP_TRY(hipMalloc(&G.d_descs, descs.size() * sizeof(Desc)));
     HIP_TRY(hipMalloc(&G.d_tiles, tiles.size() * sizeof(int4)));
-    if (G.fused)   // hx, hy of the plane below each tile: [ntiles][2][ty][64 xvec]
-        HIP_TRY(hipMalloc(&G.d_halo, tiles.size() * 2 * (size_t)ty * kWave * G.xvec * sizeof(float)));
+    if (G.fused) {   // hx, hy of the plane below each tile: [ntiles][2][ty][64 xvec]
+        const long long n = (long long)tiles.size() * 2 * ty * kWave * G.xvec;
+        CHECK(n < (1LL << 30), "%lld tiles of this shape need more scratch than 32-bit offsets reach; "
+              "use a larger zchunk", (long long)tiles.size());
+        HIP_TRY(hipMalloc(&G.d_halo, n * sizeof(float)));
+    }
     HIP_TRY(hipMemcpyAsync(G.d_descs, descs.data(), descs.size() * sizeof(Desc), hipMemcpyHostToDevice,
                            G.stream));
     HIP_TRY(hipMemcpyAsync(G.d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice,
