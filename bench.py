@@ -180,6 +180,57 @@ def measure_traffic(kernel, child_args, keep_dir=None):
             shutil.rmtree(out_dir, ignore_errors=True)
 
 
+def largest_leg(fused, steps=10, warmup=3, log=print):
+    """Throughput at the largest single domain (32 x 32 x 4 lambda at 0.021
+    lambda: 1523 x 1523 x 271 = 6.3e8 cells, snow over rough soil, VV theta
+    30), everything resident in HBM (the 50.9 GB beam buffer included; its
+    first planes hold an analytic beam of the real shape)."""
+    from sar_pixel_simulator_amd import _native as native, workloads
+    from sar_pixel_simulator_amd.hip import common, tuning
+    from sar_pixel_simulator_amd.materials import compute_material_arrays
+    pc = workloads.largest_params()
+    g = pc.geom_params
+    t0 = time.perf_counter()
+    mat = workloads.largest_material(pc, seed=0)
+    Ca, Cb = compute_material_arrays(pc, mat)
+    del mat
+    log(f'bench: largest {g.size_x}x{g.size_y}x{g.size_z} material in {time.perf_counter() - t0:.1f} s')
+    geom = common.main_geom(pc)
+    sim = native.Sim(geom, common.coeffs_for(pc))
+    sim.set_stepper(fused)
+    shape = tuning.launch_for(g.size_x, g.size_y, g.size_z, fused=fused)
+    sim.set_launch(*shape)
+    beam = native.Beam(g.size_t_inc, g.size_y, g.size_x)
+    beam.upload(0, *workloads.synthetic_beam_planes(pc, 0, warmup + 2 * steps + 2))
+    sim.attach_beam(beam)
+    sim.set_materials(Ca, Cb)
+    uni_z = uniform_top(Ca, Cb)
+    del Ca, Cb
+    sim.run(warmup)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.run(steps)
+    sim.sync()
+    wall = time.perf_counter() - t0
+    k = sim.time_steps(steps, per_kernel=True)
+    cells = g.size_x * g.size_y * g.size_z
+    ab = algorithmic_bytes(g.size_x, g.size_y, g.size_z, g.size_pml, uni_z)
+    step_bytes = (ab['h'] + ab['e'] - 24 * cells if fused else ab['h'] + ab['e']) + ab['dft']
+    ms = wall * 1e3 / steps
+    out = {'workload': f'largest single domain: {g.size_x}x{g.size_y}x{g.size_z} cells (32x32x4 '
+                       'lambda at 0.021 lambda), snow over rough soil, VV theta 30, '
+                       f'{g.size_t}-step run',
+           'cells': cells, 'steps': steps, 'ms_per_step': round(ms, 4),
+           'Mcell_updates_per_s': round(cells / (ms * 1e-3) / 1e6, 1),
+           'step_alg_GBs': round(step_bytes / (ms * 1e-3) / 1e9, 1),
+           'kernels_ms': {kk: round(v, 4) for kk, v in k.items()},
+           'launch': {'ty': shape[0], 'zchunk': shape[1], 'xvec': shape[2]},
+           'device_bytes': int(native.footprint(geom) + 16 * g.size_t_inc * g.size_y * g.size_x)}
+    sim.destroy()
+    beam.release()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -324,6 +375,8 @@ def main():
         extras['batched_vs_plain'] = measure.batched_vs_plain(count=8, log=log)
         print('bench: serving (Poisson canopy requests)', file=sys.stderr, flush=True)
         extras['serving'] = measure.serving(n=16, log=log)
+        print('bench: largest single domain', file=sys.stderr, flush=True)
+        extras['largest'] = largest_leg(fused, log=log)
 
     out = {'metric': METRIC, 'value': round(value, 1), 'unit': 'Mcell-updates/s',
            'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,

@@ -102,6 +102,27 @@ def large_material(pc, seed=0):
 
 
 # ---------------------------------------------------------------------------
+# Largest single domain: 32 x 32 lambda lateral, 4 lambda material depth,
+# 0.021 lambda cells, material fraction 0.7 (1523 x 1523 x 271), 21 periods,
+# VV, theta 30; 1.5 lambda of snow over rough soil.
+# ---------------------------------------------------------------------------
+def largest_params(**kw):
+    pc = base_params(polarization='VV', theta=30.0, incident_duration=10.0,
+                     illumination_width=24.0, simulation_duration=21.0,
+                     material_fraction=0.7, **kw)
+    set_geom_params(pc, 32.0, 32.0, 4.0, 21.0, 0.021, 'wavelength')
+    return pc
+
+
+def largest_material(pc, seed=0):
+    g = pc.geom_params
+    nz_mat = g.size_z_material
+    snow = int(round(1.5 / 0.021))
+    return synthetic.snow_over_soil(g.size_x, g.size_y, nz_mat, g.cell_size, lam_of(pc),
+                                    nz_mat - snow, snow, seed)
+
+
+# ---------------------------------------------------------------------------
 # Batched: 32 soil realizations, lateral {3,4,6,8} lambda, depth U[1,2]
 # lambda, cell 0.03 lambda, HH and VV.
 # ---------------------------------------------------------------------------
