@@ -22,7 +22,7 @@ __global__ void k_e_half(const Desc *, const int4 *, int);
 __global__ void k_dft(const Desc *);
 __global__ void k_advance(const Desc *, int);
 template <int V, int R>
-__global__ void k_step_strip(const Desc *, const int4 *, int, float *);
+__global__ void k_step_strip(const Desc *, const int4 *, int);
 __global__ void k_phasor_out(const float4 *, long long, float2 *);
 __global__ void k_n2f_partial(const float2 *, const float2 *, const float2 *, const float2 *, int, int, int,
                               const double2 *, const double2 *, double *);
@@ -112,7 +112,6 @@ struct Group {
     hipStream_t stream = nullptr;
     Desc *d_descs = nullptr;
     int4 *d_tiles = nullptr;
-    float *d_halo = nullptr;       // fused stepper: per-tile H of the plane below the tile
     int ntiles = 0, nmodels = 0, mode = FDTD_MODE_MAIN, fused = 0;
     std::vector<std::pair<int, int>> slabs;   // split stepper: tile ranges stepped H then E
     int xvec = 1;                  // x cells per lane of the split kernels in use
@@ -331,6 +330,13 @@ static int group_build(Group &G) {
         CHECK(ty == 1 || ty == 2 || ty == 4, "split stepper: ty must be 1, 2 or 4 (got %d)", ty);
     CHECK(xv == 0 || xv == 1 || xv == 2 || xv == 4, "xvec must be 0, 1, 2 or 4 (got %d)", xv);
     G.xvec = xv ? xv : auto_xvec(G.members);
+    if (G.fused) {
+        // a block's four wavefronts keep (4 ty + 2) x 64 xvec floats each in
+        // LDS; the built shapes stay within 64 KB per block
+        if (!xv) G.xvec = std::min(G.xvec, ty == 2 ? 4 : ty == 4 ? 2 : 1);
+        CHECK((4 * ty + 2) * G.xvec <= 64, "fused stepper: %d rows of %d-cell lanes exceed the LDS budget "
+              "(rows x cells: 2 x 4, 4 x 2, 8 x 1 at most)", ty, G.xvec);
+    }
     for (FdtdSim *s : G.members)
         CHECK(s->h.sy % G.xvec == 0, "row pitch %lld does not hold whole %d-cell lanes", s->h.sy, G.xvec);
     // tiles of (64 xvec) x ty cells: split, one row per wavefront of a
@@ -356,18 +362,10 @@ static int group_build(Group &G) {
     group_clear_graphs(G);
     (void)hipFree(G.d_descs);
     (void)hipFree(G.d_tiles);
-    (void)hipFree(G.d_halo);
     G.d_descs = nullptr;
     G.d_tiles = nullptr;
-    G.d_halo = nullptr;
     HIP_TRY(hipMalloc(&G.d_descs, descs.size() * sizeof(Desc)));
     HIP_TRY(hipMalloc(&G.d_tiles, tiles.size() * sizeof(int4)));
-    if (G.fused) {   // hx, hy of the plane below each tile: [ntiles][2][ty][64 xvec]
-        const long long n = (long long)tiles.size() * 2 * ty * kWave * G.xvec;
-        CHECK(n < (1LL << 30), "%lld tiles of this shape need more scratch than 32-bit offsets reach; "
-              "use a larger zchunk", (long long)tiles.size());
-        HIP_TRY(hipMalloc(&G.d_halo, n * sizeof(float)));
-    }
     HIP_TRY(hipMemcpyAsync(G.d_descs, descs.data(), descs.size() * sizeof(Desc), hipMemcpyHostToDevice,
                            G.stream));
     HIP_TRY(hipMemcpyAsync(G.d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice,
@@ -401,7 +399,6 @@ static void group_free(Group &G) {
     group_clear_graphs(G);
     (void)hipFree(G.d_descs);
     (void)hipFree(G.d_tiles);
-    (void)hipFree(G.d_halo);
     if (G.stream) (void)hipStreamDestroy(G.stream);
     G = Group();
 }
@@ -417,10 +414,10 @@ static void launch_kernel(Group &G, KernelId k, int t0 = 0, int t1 = -1) {
         const dim3 grid((G.ntiles + 3) / 4), b4(kWave, 4);
         auto *fn = &k_step_strip<1, 2>;
         const int r = G.launch.ty, v = G.xvec;
-        if (v == 4) fn = r == 8 ? &k_step_strip<4, 8> : r == 4 ? &k_step_strip<4, 4> : &k_step_strip<4, 2>;
-        else if (v == 2) fn = r == 8 ? &k_step_strip<2, 8> : r == 4 ? &k_step_strip<2, 4> : &k_step_strip<2, 2>;
+        if (v == 4) fn = &k_step_strip<4, 2>;
+        else if (v == 2) fn = r == 4 ? &k_step_strip<2, 4> : &k_step_strip<2, 2>;
         else fn = r == 8 ? &k_step_strip<1, 8> : r == 4 ? &k_step_strip<1, 4> : &k_step_strip<1, 2>;
-        hipLaunchKernelGGL(fn, grid, b4, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles, G.d_halo);
+        hipLaunchKernelGGL(fn, grid, b4, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
     } else if (k == KH) {
         if (G.xvec == 4)
             hipLaunchKernelGGL(k_h_half<4>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt);

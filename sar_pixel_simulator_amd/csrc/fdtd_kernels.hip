@@ -609,86 +609,54 @@ template __global__ void k_e_half<4>(const Desc *, const int4 *, int);
 // are double-buffered: step t reads set t&1 and writes set 1-(t&1).
 //
 // Each wavefront owns a strip of R rows of 64 V cells over a z range and
-// marches z.  Within a plane it walks its rows upward: H(t+1/2) of row y,
-// then E(t+1) of row y, which needs H(t+1/2)
-//   - at y-1: kept in registers from the previous row; for the strip's
-//     first row recomputed from the old level (not stored);
-//   - at x-1: the neighbouring lane's cell; lane 0 recomputes the last cell
-//     of the tile to its left (not stored);
-//   - at z-1: re-read from the new set, where the same lane stored it one
-//     plane earlier; a strip starting at z0 > 0 recomputes plane z0-1 (not
-//     stored) on its first plane.
-// Recomputed values are bit-identical to the owner's: same inputs, same
-// operations.  There is no LDS and no barrier: the four wavefronts of a
-// block work on independent strips.
+// marches z; within a plane it walks its rows upward, computing H(t+1/2) of
+// a row and then E(t+1) of the row.  What one plane hands to the next stays
+// on chip, in the wavefront's own slice of LDS (no other wavefront touches
+// it, so no barrier is needed):
+//   - E(t) at z+1 of every row (the row's E(t) at z on the next plane);
+//   - H(t+1/2) at z of the strip's rows (H at z-1 for E on the next plane).
+// Within a plane, H(t+1/2) of the row below and E(t) ez of the row above
+// pass from row to row in registers.  Halo values are recomputed from the
+// old level, bit-identical to their owner's, and never stored:
+//   - row y0-1 (H at y-1 for the strip's first row), on every plane;
+//   - plane z0-1 (H at z-1 for the first plane of a strip with z0 > 0);
+//   - cell x0-1 (H at x-1 for lane 0), by lane 0.
 // ---------------------------------------------------------------------------
 
-// One row's H(t+1/2) and the E(t) values it was computed from.
-template <int V>
-struct HRow {
-    float hx[V], hy[V], hz[V];
-    float ex[V], ey[V], ez[V];
+// LDS floats per wavefront: slots of 64 V floats (one value per lane cell):
+// ex, ey for rows 0..R (row 0 is y0-1), then hx, hy for rows 1..R.
+template <int V, int R>
+struct StripLds {
+    static constexpr int kSlot = kWave * V;
+    static constexpr int kFloats = (4 * R + 2) * kSlot;
+    __device__ static int ex(int j) { return (2 * j) * kSlot; }
+    __device__ static int ey(int j) { return (2 * j + 1) * kSlot; }
+    __device__ static int hx(int j) { return (2 * (R + 1) + 2 * (j - 1)) * kSlot; }
+    __device__ static int hy(int j) { return (2 * (R + 1) + 2 * (j - 1) + 1) * kSlot; }
 };
 
 template <int V>
-__device__ __forceinline__ void h_row(const Desc *__restrict__ d, const StepK &k, const gfloat *const *fo,
-                                      int xb, int y, int z, uint32_t o, uint32_t sz, bool act,
-                                      bool next_mem, const gfloat *hr, gfloat *hw, HRow<V> &r) {
-    const uint32_t sy = k.sy;
-    const bool zin = z < k.nz - 1, yin = y < k.ny - 1;
-    float exn[V], eyn[V], ezyp[V], exyp[V];
-    vset(r.hx, 0.f); vset(r.hy, 0.f); vset(r.hz, 0.f);
-    vset(r.ex, 0.f); vset(r.ey, 0.f); vset(r.ez, 0.f);
-    vset(exn, 0.f); vset(eyn, 0.f); vset(ezyp, 0.f); vset(exyp, 0.f);
-    float ey_x = 0.f, ez_x = 0.f;
-    if (act) {
-        vld(r.hx, at(fo[3], o));
-        vld(r.hy, at(fo[4], o));
-        vld(r.hz, at(fo[5], o));
-        vld(r.ex, at(fo[0], o));
-        vld(r.ey, at(fo[1], o));
-        vld(r.ez, at(fo[2], o));
-        if (zin) {
-            vld(exn, at(fo[0], o + sz));
-            vld(eyn, at(fo[1], o + sz));
-            if (yin) {
-                vld(ezyp, at(fo[2], o + sy));
-                vld(exyp, at(fo[0], o + sy));
-            }
-            if (next_mem) {
-                ey_x = *at(fo[1], o + V);
-                ez_x = *at(fo[2], o + V);
-            }
-        }
-    }
-    float ey_next = __shfl_down(r.ey[0], 1, kWave);
-    float ez_next = __shfl_down(r.ez[0], 1, kWave);
-    if (threadIdx.x == kWave - 1) {
-        ey_next = ey_x;
-        ez_next = ez_x;
-    }
+__device__ __forceinline__ void lds_ld(float (&r)[V], const float *p) {
 #pragma unroll
-    for (int c = 0; c < V; ++c) {
-        if (xb + c < k.nx) {
-            EAround e;
-            e.ex_c = r.ex[c]; e.ex_n = exn[c]; e.ex_yp = exyp[c];
-            e.ey_c = r.ey[c]; e.ey_n = eyn[c];
-            e.ey_xp = c + 1 < V ? r.ey[c + 1 < V ? c + 1 : 0] : ey_next;
-            e.ez_c = r.ez[c]; e.ez_yp = ezyp[c];
-            e.ez_xp = c + 1 < V ? r.ez[c + 1 < V ? c + 1 : 0] : ez_next;
-            h_cell(d, k, xb + c, y, z, e, r.hx[c], r.hy[c], r.hz[c], hr, hw);
-        }
-    }
+    for (int c = 0; c < V; ++c) r[c] = p[c];
+}
+
+template <int V>
+__device__ __forceinline__ void lds_st(float *p, const float (&r)[V]) {
+#pragma unroll
+    for (int c = 0; c < V; ++c) p[c] = r[c];
 }
 
 template <int V, int R>
 __global__ void __launch_bounds__(256)
-k_step_strip(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles,
-             float *__restrict__ halo) {
+k_step_strip(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles) {
+    using L = StripLds<V, R>;
+    __shared__ float lds_all[4][L::kFloats];
     const int nblk = (ntiles + 3) >> 2;
     // one tile per wavefront: threadIdx.y is uniform across the wavefront,
     // and saying so keeps the descriptor and its pointers in scalar registers
-    const int wt = __builtin_amdgcn_readfirstlane((xcd_tile(blockIdx.x, nblk) << 2) + (int)threadIdx.y);
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int wt = __builtin_amdgcn_readfirstlane((xcd_tile(blockIdx.x, nblk) << 2) + w);
     if (wt >= ntiles) return;                          // uniform per wavefront
     const int4 tl = tiles[wt];
     const Desc *__restrict__ d = descs + tl.x;
@@ -696,6 +664,7 @@ k_step_strip(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int
     if (t >= d->nt) return;
     const StepK k = step_k(d, t);
     const int lane = threadIdx.x;
+    float *lds = lds_all[w] + V * lane;                // this lane's cells of each slot
     const int x0 = tl.y, y0 = tl.z;
     const int xb = x0 + V * lane;
     const bool act = xb < k.nx;
@@ -718,116 +687,169 @@ k_step_strip(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int
     const gfloat *Ca = gp(d->ca), *Cb = gp(d->cb);
     const int uni_z = d->uni_z;
     const float ca_u = d->ca_u, cb_u = d->cb_u;
-    // this tile's hx, hy of plane z0 - 1: [2][R][64 V]
-    gfloat *hs = at(gp(halo), (uint32_t)wt * (2 * R * kWave * V) + V * lane);
 
 #pragma unroll 1
     for (int z = zs; z < z1; ++z) {
         const bool zin = z < k.nz - 1;
         const bool hplane = z < z0;
         const uint32_t oz = (uint32_t)z * sz + (uint32_t)xb;
-        // H(t+1/2) of the row below, this plane (row y0 - 1: recomputed)
-        float hzb[V], hxb[V];
+        const int j0 = (!hplane && y0 >= 1) ? 0 : 1;   // row 0 (y0 - 1) only where E is updated
+        float hzb[V], hxb[V];                          // H(t+1/2) of the row below
+        float ezc[V];                                  // E(t) ez of this row, from the row below
         vset(hzb, 0.f);
         vset(hxb, 0.f);
-        const int r0 = (!hplane && y0 >= 1) ? -1 : 0;
+        vset(ezc, 0.f);
 #pragma unroll 1
-        for (int r = r0; r < ry; ++r) {
-            const int y = y0 + r;
+        for (int j = j0; j <= ry; ++j) {
+            const int y = y0 - 1 + j;
             const uint32_t o = oz + (uint32_t)y * sy;
-            const bool own = r >= 0 && !hplane;
-            HRow<V> c;
-            h_row(d, k, fo, xb, y, z, o, sz, act, next_mem, psi_o, own ? psi_n : nullptr, c);
-            if (hplane) {                              // keep hx, hy for plane z0
-                if (act) {
-                    vst(at(hs, (uint32_t)r * kWave * V), c.hx);
-                    vst(at(hs, (uint32_t)(R + r) * kWave * V), c.hy);
-                }
-                continue;
+            const bool own = j >= 1 && !hplane;
+            const bool yin = y < k.ny - 1;
+            // ---- every load of the row first, so the row costs one round trip
+            // E(t) of the row at z: from LDS where this row was stepped on the
+            // plane below, else from memory; ex of the row above likewise
+            const bool have = j >= 1 ? z > zs : z > z0;
+            const bool have_up = j < ry && z > zs;
+            float ex[V], ey[V], exn[V], eyn[V], exyp[V], ezyp[V], hx[V], hy[V], hz[V];
+            float hxzm[V], hyzm[V], ca[V], cb[V];
+            vset(ex, 0.f); vset(ey, 0.f); vset(exn, 0.f); vset(eyn, 0.f);
+            vset(exyp, 0.f); vset(ezyp, 0.f); vset(hx, 0.f); vset(hy, 0.f); vset(hz, 0.f);
+            vset(hxzm, 0.f); vset(hyzm, 0.f);
+            vset(ca, ca_u); vset(cb, cb_u);
+            float ey_x = 0.f, ez_x = 0.f;
+            if (have) {
+                lds_ld(ex, lds + L::ex(j));
+                lds_ld(ey, lds + L::ey(j));
             }
-            if (own) {
-                const bool yin = y < k.ny - 1;
-                // H(t+1/2) at z - 1: stored by this lane one plane earlier
-                float hxzm[V], hyzm[V];
-                vset(hxzm, 0.f);
-                vset(hyzm, 0.f);
-                if (act && z >= 1) {
-                    if (z > z0) {
-                        vld(hxzm, at(fn[3], o - sz));
-                        vld(hyzm, at(fn[4], o - sz));
-                    } else {
-                        vld(hxzm, at(hs, (uint32_t)r * kWave * V));
-                        vld(hyzm, at(hs, (uint32_t)(R + r) * kWave * V));
+            if (zin && yin && have_up) lds_ld(exyp, lds + L::ex(j + 1));
+            if (own && z > zs) {                       // H(t+1/2) at z - 1, from the plane below
+                lds_ld(hxzm, lds + L::hx(j));
+                lds_ld(hyzm, lds + L::hy(j));
+            }
+            if (act) {
+                if (!have) {
+                    vld(ex, at(fo[0], o));
+                    vld(ey, at(fo[1], o));
+                }
+                if (j == j0) vld(ezc, at(fo[2], o));
+                vld(hx, at(fo[3], o));
+                vld(hy, at(fo[4], o));
+                vld(hz, at(fo[5], o));
+                // ez of the row above: this row's H and, passed on, that
+                // row's ez (also on the top plane, where E is only copied)
+                if (yin) vld(ezyp, at(fo[2], o + sy));
+                if (zin) {
+                    vld(exn, at(fo[0], o + sz));
+                    vld(eyn, at(fo[1], o + sz));
+                    if (yin && !have_up) vld(exyp, at(fo[0], o + sy));
+                    if (next_mem) {
+                        ey_x = *at(fo[1], o + V);
+                        ez_x = *at(fo[2], o + V);
+                    }
+                    if (own && z < uni_z) {
+                        vld(ca, at(Ca, o));
+                        vld(cb, at(Cb, o));
                     }
                 }
-                // H(t+1/2) at x - 1 of the lane's first cell
-                float hz_prev = __shfl_up(c.hz[V - 1], 1, kWave);
-                float hy_prev = __shfl_up(c.hy[V - 1], 1, kWave);
+            }
+            // lane 0: the inputs of H(t+1/2) at x0 - 1 (only its hy and hz
+            // are used, so hx's inputs are left out)
+            EAround eh;
+            float hy_h = 0.f, hz_h = 0.f;
+            const bool do_halo = own && halo_x;
+            if (do_halo) {
+                const uint32_t oh = o - 1;
+                eh.ex_c = *at(fo[0], oh);
+                eh.ey_c = *at(fo[1], oh);
+                eh.ez_c = *at(fo[2], oh);
+                eh.ex_n = zin ? *at(fo[0], oh + sz) : 0.f;
+                eh.ex_yp = (zin && yin) ? *at(fo[0], oh + sy) : 0.f;
+                hy_h = *at(fo[4], oh);
+                hz_h = *at(fo[5], oh);
+            }
+            // ---- H(t+1/2) of the row
+            float ey_next = __shfl_down(ey[0], 1, kWave);
+            float ez_next = __shfl_down(ezc[0], 1, kWave);
+            if (lane == kWave - 1) {
+                ey_next = ey_x;
+                ez_next = ez_x;
+            }
+            gfloat *hw_psi = own ? psi_n : nullptr;
+#pragma unroll
+            for (int c = 0; c < V; ++c) {
+                if (xb + c < k.nx) {
+                    EAround e;
+                    e.ex_c = ex[c]; e.ex_n = exn[c]; e.ex_yp = exyp[c];
+                    e.ey_c = ey[c]; e.ey_n = eyn[c];
+                    e.ey_xp = c + 1 < V ? ey[c + 1 < V ? c + 1 : 0] : ey_next;
+                    e.ez_c = ezc[c]; e.ez_yp = ezyp[c];
+                    e.ez_xp = c + 1 < V ? ezc[c + 1 < V ? c + 1 : 0] : ez_next;
+                    h_cell(d, k, xb + c, y, z, e, hx[c], hy[c], hz[c], psi_o, hw_psi);
+                }
+            }
+            if (own) {
+                // ---- E(t+1) of the row
+                float hz_prev = __shfl_up(hz[V - 1], 1, kWave);
+                float hy_prev = __shfl_up(hy[V - 1], 1, kWave);
                 if (lane == 0) {
                     hz_prev = 0.f;
                     hy_prev = 0.f;
-                    if (halo_x) {
-                        // only hy and hz of the cell are used: hx's inputs are left out
-                        const uint32_t oh = o - 1;
-                        EAround e;
-                        e.ex_c = *at(fo[0], oh);
-                        e.ey_c = *at(fo[1], oh);
-                        e.ez_c = *at(fo[2], oh);
-                        e.ey_xp = c.ey[0];
-                        e.ez_xp = c.ez[0];
-                        e.ex_n = zin ? *at(fo[0], oh + sz) : 0.f;
-                        e.ex_yp = (zin && yin) ? *at(fo[0], oh + sy) : 0.f;
-                        e.ey_n = e.ey_c;
-                        e.ez_yp = e.ez_c;
-                        float hx_h = 0.f, hy_h = *at(fo[4], oh), hz_h = *at(fo[5], oh);
-                        h_cell(d, k, x0 - 1, y, z, e, hx_h, hy_h, hz_h, psi_o, nullptr);
+                    if (do_halo) {
+                        eh.ey_xp = ey[0];
+                        eh.ez_xp = ezc[0];
+                        eh.ey_n = eh.ey_c;
+                        eh.ez_yp = eh.ez_c;
+                        float hx_h = 0.f;
+                        h_cell(d, k, x0 - 1, y, z, eh, hx_h, hy_h, hz_h, psi_o, nullptr);
                         hz_prev = hz_h;
                         hy_prev = hy_h;
                     }
                 }
-                float ca[V], cb[V];
-                vset(ca, ca_u);
-                vset(cb, cb_u);
-                if (act && zin && z < uni_z) {
-                    vld(ca, at(Ca, o));
-                    vld(cb, at(Cb, o));
-                }
+                float exu[V], eyu[V], ezu[V];
+                vcopy(exu, ex);
+                vcopy(eyu, ey);
+                vcopy(ezu, ezc);
 #pragma unroll
                 for (int q = 0; q < V; ++q) {
                     if (xb + q < k.nx) {
                         HAround h;
-                        h.hx_c = c.hx[q]; h.hy_c = c.hy[q]; h.hz_c = c.hz[q];
+                        h.hx_c = hx[q]; h.hy_c = hy[q]; h.hz_c = hz[q];
                         h.hx_zm = hxzm[q]; h.hy_zm = hyzm[q];
                         h.hz_ym = hzb[q]; h.hx_ym = hxb[q];
-                        h.hz_xm = q > 0 ? c.hz[q > 0 ? q - 1 : 0] : hz_prev;
-                        h.hy_xm = q > 0 ? c.hy[q > 0 ? q - 1 : 0] : hy_prev;
-                        e_cell(d, k, xb + q, y, z, ca[q], cb[q], h, c.ex[q], c.ey[q], c.ez[q]);
+                        h.hz_xm = q > 0 ? hz[q > 0 ? q - 1 : 0] : hz_prev;
+                        h.hy_xm = q > 0 ? hy[q > 0 ? q - 1 : 0] : hy_prev;
+                        e_cell(d, k, xb + q, y, z, ca[q], cb[q], h, exu[q], eyu[q], ezu[q]);
                     }
                 }
                 if (act) {
-                    vst(at(fn[3], o), c.hx);
-                    vst(at(fn[4], o), c.hy);
-                    vst(at(fn[5], o), c.hz);
-                    vst(at(fn[0], o), c.ex);
-                    vst(at(fn[1], o), c.ey);
-                    vst(at(fn[2], o), c.ez);
+                    vst(at(fn[3], o), hx);
+                    vst(at(fn[4], o), hy);
+                    vst(at(fn[5], o), hz);
+                    vst(at(fn[0], o), exu);
+                    vst(at(fn[1], o), eyu);
+                    vst(at(fn[2], o), ezu);
                 }
             }
-            vcopy(hzb, c.hz);
-            vcopy(hxb, c.hx);
+            // hand over to the plane above and to the next row
+            lds_st(lds + L::ex(j), exn);
+            lds_st(lds + L::ey(j), eyn);
+            if (j >= 1) {
+                lds_st(lds + L::hx(j), hx);
+                lds_st(lds + L::hy(j), hy);
+            }
+            vcopy(hzb, hz);
+            vcopy(hxb, hx);
+            vcopy(ezc, ezyp);
         }
     }
 }
 
-template __global__ void k_step_strip<1, 2>(const Desc *, const int4 *, int, float *);
-template __global__ void k_step_strip<1, 4>(const Desc *, const int4 *, int, float *);
-template __global__ void k_step_strip<1, 8>(const Desc *, const int4 *, int, float *);
-template __global__ void k_step_strip<2, 2>(const Desc *, const int4 *, int, float *);
-template __global__ void k_step_strip<2, 4>(const Desc *, const int4 *, int, float *);
-template __global__ void k_step_strip<2, 8>(const Desc *, const int4 *, int, float *);
-template __global__ void k_step_strip<4, 2>(const Desc *, const int4 *, int, float *);
-template __global__ void k_step_strip<4, 4>(const Desc *, const int4 *, int, float *);
-template __global__ void k_step_strip<4, 8>(const Desc *, const int4 *, int, float *);
+template __global__ void k_step_strip<1, 2>(const Desc *, const int4 *, int);
+template __global__ void k_step_strip<1, 4>(const Desc *, const int4 *, int);
+template __global__ void k_step_strip<1, 8>(const Desc *, const int4 *, int);
+template __global__ void k_step_strip<2, 2>(const Desc *, const int4 *, int);
+template __global__ void k_step_strip<2, 4>(const Desc *, const int4 *, int);
+template __global__ void k_step_strip<4, 2>(const Desc *, const int4 *, int);
 
 // ---------------------------------------------------------------------------
 // Running single-frequency DFT on the phasor planes

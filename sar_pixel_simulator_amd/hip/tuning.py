@@ -14,7 +14,8 @@ import os
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'tuning_cache.json')
 CANDIDATES = {
-    'fused': [(r, zc, xv) for xv in (4, 2, 1) for r in (2, 4, 8) for zc in (8, 16, 32)],
+    'fused': [(r, zc, xv) for xv, rows in ((1, (2, 4, 8)), (2, (2, 4)), (4, (2,)))
+              for r in rows for zc in (4, 8, 16, 32)],
     'split': [(ty, zc, xv) for xv in (4, 2) for ty in (1, 2, 4) for zc in (8, 16, 32)]
              + [(2, 16, 1)],
 }

@@ -84,10 +84,10 @@ def test_fields_psi_phasors_bitwise(soil60, fused):
                                           (False, (4, 0, 4)), (False, (4, 7, 4)),
                                           (False, (1, 5, 2)), (False, (2, 16, 4)),
                                           (False, (2, 3, 1)), (False, (1, 1, 4)),
-                                          (True, (2, 0, 1)), (True, (4, 0, 4)),
-                                          (True, (8, 7, 2)), (True, (4, 5, 1)),
+                                          (True, (2, 0, 1)), (True, (2, 0, 4)),
+                                          (True, (8, 7, 1)), (True, (4, 5, 1)),
                                           (True, (2, 1, 4)), (True, (8, 16, 0)),
-                                          (True, (4, 3, 4))])
+                                          (True, (4, 3, 2)), (True, (4, 2, 0))])
 def test_launch_shapes_bitwise(soil60, fused, launch):
     pc, Ca, Cb, beam, ref = soil60
     say('case: fused', fused, 'launch', launch)
@@ -170,6 +170,10 @@ def test_invalid_launch_is_rejected_and_previous_kept(soil60):
         sim.set_launch(8, 0, 4)          # 8 rows per tile is a fused-stepper shape
     with pytest.raises(native.FdtdError):
         sim.set_launch(2, 0, 3)
+    fsim = make_sim(pc, Ca, Cb, beam, fused=True)
+    with pytest.raises(native.FdtdError):
+        fsim.set_launch(8, 0, 2)         # 8 rows of 2-cell lanes exceed the fused LDS budget
+    fsim.destroy()
     sim.run(60)
     sim.sync()
     assert_same(sim.field('hz'), ref.fields['hz'], 'hz after rejected shapes')
