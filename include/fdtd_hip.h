@@ -132,19 +132,23 @@ int fdtd_sim_set_field(FdtdSim *s, int comp, const float *in);
 /* CPML state psi 0..23 in the reference's shapes (cpu/fdtd_sim.py:165-187):
  * E d,u: Exz Eyz; l,r: Eyx Ezx; f,b: Exy Ezy; then H likewise. */
 int fdtd_sim_get_psi(FdtdSim *s, int index, float *out);
-/* MAIN only, before the first step: 1 = fused stepper (one kernel computes
- * H then E per step; fields double-buffered), 0 = split stepper (an H
- * kernel, then an E kernel, in place).  Both give identical results. */
+/* No counterpart in the reference (its step is 26 fixed launches,
+ * gpu/main_sim.py:216-355).  MAIN only, before the first step: 1 = fused
+ * stepper (one kernel computes H then E per step; fields double-buffered),
+ * 0 = split stepper (an H kernel, then an E kernel, in place; the
+ * default).  Both give identical results. */
 int fdtd_sim_set_stepper(FdtdSim *s, int fused);
-/* Tile shape.  Split stepper: blocks of 64 x ty lanes (ty 1, 2, 4), each
- * lane owning xvec consecutive x cells (1, 2, 4; 0 = widest that leaves few
- * lanes idle).  Fused stepper: 64 x ty lanes (ty 8, 16; row 0 and lane 0 are
- * a halo), xvec 0 or 1.  zchunk: z cells per tile (0 = whole column).  An
- * invalid shape is rejected and the previous one kept. */
+/* No counterpart in the reference.  Tile shape: each lane owns xvec
+ * consecutive x cells (1, 2, 4; 0 = widest that leaves few lanes idle) and
+ * marches z over zchunk cells (0 = whole column).  Split stepper: blocks of
+ * 64 x ty lanes, ty 1, 2 or 4 rows.  Fused stepper: ty 2, 4 or 8 rows per
+ * wavefront with (4 ty + 2) xvec <= 64 (LDS).  An invalid shape is
+ * rejected and the previous one kept. */
 int fdtd_sim_set_launch(FdtdSim *s, int ty, int zchunk, int xvec);
-/* Run nsteps and report HIP-event times on the library stream: whole run and
- * per-kernel averages (ms); with the fused stepper ms_h is the fused kernel
- * and ms_e is 0.  Any pointer may be NULL. */
+/* No counterpart in the reference.  Run nsteps and report HIP-event times
+ * on the library stream: whole run and per-kernel averages (ms), the latter
+ * from whole-domain launches; with the fused stepper ms_h is the fused
+ * kernel and ms_e is 0.  Any pointer may be NULL. */
 int fdtd_sim_time_steps(FdtdSim *s, int nsteps, float *ms_total, float *ms_h,
                         float *ms_e, float *ms_dft);
 
@@ -195,8 +199,10 @@ int fdtd_beam_synthesize(const FdtdGeom *g, const FdtdCoeffs *c, const FdtdBeamS
                          FdtdBeam **out, float *inc_phasors_c64);
 
 /* ---- batches: many simulations stepped by one launch per kernel ------- */
-/* The batch references the sims (they must outlive it).  Sims may differ
- * in size and mode; each keeps its own step counter. */
+/* No counterpart in the reference, which runs one model per process
+ * (experiment.py:69-97, experiment_mpi.py:86-255).  The batch references
+ * the sims (they must outlive it).  Sims may differ in size; they share a
+ * mode and a stepper; each keeps its own step counter. */
 int fdtd_batch_create(FdtdSim *const *sims, int n, FdtdBatch **out);
 void fdtd_batch_destroy(FdtdBatch *b);
 int fdtd_batch_set_launch(FdtdBatch *b, int ty, int zchunk, int xvec);
