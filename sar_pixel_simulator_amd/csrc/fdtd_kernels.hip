@@ -145,6 +145,10 @@ __device__ __forceinline__ void h_cell(const Desc *__restrict__ d, const StepK &
 __device__ __forceinline__ void h_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
                                        const EAround &e, float &hx, float &hy, float &hz,
                                        const gfloat *hr, gfloat *hw) {
+    // the cell's x as an opaque value: the absorber's x-dependent profile
+    // loads and psi offsets are then formed here, when a cell needs them,
+    // instead of being hoisted out of the stencil loops into registers
+    asm volatile("" : "+v"(x));
     const int nx = k.nx, ny = k.ny, nz = k.nz, np = k.np;
     const uint32_t sy = k.sy;
     const float dtm0 = k.dtm0, dx = k.dx;
@@ -260,6 +264,7 @@ __device__ __forceinline__ void e_cell(const Desc *__restrict__ d, const StepK &
 // reference order d,u,l,r,f,b (psi updated in place), after its Yee update.
 __device__ __forceinline__ void e_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
                                        float cb, const HAround &h, float &ex, float &ey, float &ez) {
+    asm volatile("" : "+v"(x));   // as in h_cpml
     const int nx = k.nx, ny = k.ny, nz = k.nz, np = k.np;
     const uint32_t sy = k.sy;
     const float dx = k.dx;
