@@ -344,6 +344,10 @@ def main():
     # (and with 16-byte accesses)
     copy1_ms, copy1 = native.probe_copy(PROBE_BYTES, vec=1, reps=10)
     copy4_ms, copy4 = native.probe_copy(PROBE_BYTES, vec=4, reps=10)
+    # the same access mix as the kernel (reads + writes per cell), streamed
+    # flat: the practical ceiling for that mix
+    mix = (8, 6) if fused else ((8, 3) if kern['name'] == 'k_e_half' else (6, 3))
+    mix_ms, mix_gbs = native.probe_streams(cells, *mix, reps=5)
     traffic, tdetail = None, {}
     if not args.no_traffic:
         print('bench: PMC traffic passes (FETCH_SIZE, WRITE_SIZE)', file=sys.stderr, flush=True)
@@ -356,7 +360,8 @@ def main():
             'avg_launch_ms': round(kern['ms'], 4),
             'traffic_detail': tdetail,
             'copy_probe_GBs': {'4B_per_lane': round(copy1, 1), '16B_per_lane': round(copy4, 1)},
-            'frac_of_copy_probe': round(achieved / copy1, 4)}
+            'stream_probe': {'reads': mix[0], 'writes': mix[1], 'GBs': round(mix_gbs, 1)},
+            'frac_of_stream_probe': round(achieved / mix_gbs, 4)}
     if traffic:
         roof['traffic_GBs'] = round(traffic / (kern['ms'] * 1e-3) / 1e9, 1)
 

@@ -220,6 +220,12 @@ int fdtd_batch_time_steps(FdtdBatch *b, int nsteps, int use_graph, float *ms_tot
  * practical ceiling they are measured against, and its known byte count
  * calibrates the PMC byte counters. */
 int fdtd_probe_copy(int64_t nbytes, int vec, int reps, float *ms_per_copy);
+/* nread input and nwrite output arrays of n floats, each thread reading one
+ * element of every input and writing one of every output (4 bytes per lane
+ * per stream): the HBM access mix of a stencil kernel without its neighbour
+ * reuse or arithmetic -- 6+3 (H half), 8+3 (E half), 8+6 (fused step), and
+ * 1+1.  Reports the average HIP-event time of one pass. */
+int fdtd_probe_streams(int64_t n, int nread, int nwrite, int reps, float *ms_per_pass);
 
 #ifdef __cplusplus
 }

@@ -53,7 +53,7 @@ EXPORTS = [
     'fdtd_beam_synthesize',
     'fdtd_batch_create', 'fdtd_batch_destroy', 'fdtd_batch_set_launch',
     'fdtd_batch_run', 'fdtd_batch_sync', 'fdtd_batch_time_steps',
-    'fdtd_probe_copy',
+    'fdtd_probe_copy', 'fdtd_probe_streams',
 ]
 
 
@@ -150,6 +150,7 @@ _SIGS = {
     'fdtd_batch_sync': ([_P], _I),
     'fdtd_batch_time_steps': ([_P, _I, _I, _FP], _I),
     'fdtd_probe_copy': ([ctypes.c_int64, _I, _I, _FP], _I),
+    'fdtd_probe_streams': ([ctypes.c_int64, _I, _I, _I, _FP], _I),
 }
 
 
@@ -219,6 +220,14 @@ def probe_copy(nbytes, vec=1, reps=5):
     ms = ctypes.c_float()
     _check(lib().fdtd_probe_copy(int(nbytes), int(vec), int(reps), ctypes.byref(ms)))
     return ms.value, 2 * nbytes / (ms.value * 1e-3) / 1e9
+
+
+def probe_streams(n, nread, nwrite, reps=5):
+    """Average time (ms) of one pass of the multi-stream probe over n floats
+    per stream; returns (ms, GB/s over all streams)."""
+    ms = ctypes.c_float()
+    _check(lib().fdtd_probe_streams(int(n), int(nread), int(nwrite), int(reps), ctypes.byref(ms)))
+    return ms.value, (nread + nwrite) * 4 * n / (ms.value * 1e-3) / 1e9
 
 
 def make_geom(nx, ny, nz, npml, tfsf_z, nt, nt_inc, mode=MODE_MAIN,

@@ -30,6 +30,8 @@ __global__ void k_n2f_final(const double *, int, int, const double4 *, const dou
                             double2 *);
 template <int V>
 __global__ void k_copy_probe(const float *, float *, long long);
+template <int NR, int NW>
+__global__ void k_stream_probe(const float *, float *, long long);
 }  // namespace fdtd
 
 using namespace fdtd;
@@ -571,6 +573,56 @@ extern "C" int fdtd_probe_copy(int64_t nbytes, int vec, int reps, float *ms_per_
     if (st) (void)hipStreamDestroy(st);
     if (src) (void)hipFree(src);
     if (dst) (void)hipFree(dst);
+    return rc;
+}
+
+// nread input and nwrite output arrays of n floats each, one element of
+// each per thread; timed like fdtd_probe_copy.
+extern "C" int fdtd_probe_streams(int64_t n, int nread, int nwrite, int reps, float *ms_per_pass) {
+    const bool known = (nread == 1 && nwrite == 1) || (nread == 6 && nwrite == 3) ||
+                       (nread == 8 && nwrite == 3) || (nread == 8 && nwrite == 6);
+    CHECK(known, "stream mix %d reads + %d writes not built (1+1, 6+3, 8+3, 8+6)", nread, nwrite);
+    CHECK(n >= 256 && n <= (1LL << 31), "n outside [256, 2^31]");
+    CHECK(reps >= 1 && ms_per_pass, "bad reps / output");
+    HIP_TRY(hipSetDevice(g_device));
+    float *in = nullptr, *out = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int rc = FDTD_OK;
+    auto hip_ok = [&](hipError_t e, const char *what) {
+        if (e != hipSuccess && rc == FDTD_OK) rc = fail(FDTD_EHIP, "%s: %s", what, hipGetErrorString(e));
+        return rc == FDTD_OK;
+    };
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (hip_ok(hipMalloc(&in, (size_t)nread * n * sizeof(float)), "hipMalloc") &&
+        hip_ok(hipMalloc(&out, (size_t)nwrite * n * sizeof(float)), "hipMalloc") &&
+        hip_ok(hipStreamCreate(&st), "hipStreamCreate") && hip_ok(hipEventCreate(&ev[0]), "hipEventCreate") &&
+        hip_ok(hipEventCreate(&ev[1]), "hipEventCreate") &&
+        hip_ok(hipMemsetAsync(in, 0, (size_t)nread * n * sizeof(float), st), "hipMemsetAsync")) {
+        for (int r = 0; r <= reps && rc == FDTD_OK; ++r) {   // r = 0: warm-up
+            if (r == 1) hip_ok(hipEventRecord(ev[0], st), "hipEventRecord");
+            if (nread == 1)
+                hipLaunchKernelGGL((k_stream_probe<1, 1>), dim3(blocks), dim3(256), 0, st, in, out, (long long)n);
+            else if (nread == 6)
+                hipLaunchKernelGGL((k_stream_probe<6, 3>), dim3(blocks), dim3(256), 0, st, in, out, (long long)n);
+            else if (nwrite == 3)
+                hipLaunchKernelGGL((k_stream_probe<8, 3>), dim3(blocks), dim3(256), 0, st, in, out, (long long)n);
+            else
+                hipLaunchKernelGGL((k_stream_probe<8, 6>), dim3(blocks), dim3(256), 0, st, in, out, (long long)n);
+            hip_ok(hipGetLastError(), "k_stream_probe");
+        }
+        float ms = 0;
+        if (hip_ok(hipEventRecord(ev[1], st), "hipEventRecord") &&
+            hip_ok(hipEventSynchronize(ev[1]), "hipEventSynchronize") &&
+            hip_ok(hipEventElapsedTime(&ms, ev[0], ev[1]), "hipEventElapsedTime"))
+            *ms_per_pass = ms / reps;
+    }
+    if (st) hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize");
+    for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (st) (void)hipStreamDestroy(st);
+    if (in) (void)hipFree(in);
+    if (out) (void)hipFree(out);
     return rc;
 }
 

@@ -1019,6 +1019,28 @@ k_copy_probe(const float *__restrict__ src, float *__restrict__ dst, long long n
 template __global__ void k_copy_probe<1>(const float *, float *, long long);
 template __global__ void k_copy_probe<4>(const float *, float *, long long);
 
+// Multi-stream probe: each thread reads NR arrays and writes NW arrays of n
+// floats at its own index, 4 bytes per lane per stream -- the HBM access mix
+// of a stencil kernel (H half: 6 reads, 3 writes; E half: 8 and 3; fused
+// step: 8 and 6) without its neighbour reuse or arithmetic.  Its bandwidth
+// is the practical ceiling for that mix.
+template <int NR, int NW>
+__global__ void __launch_bounds__(256)
+k_stream_probe(const float *__restrict__ in, float *__restrict__ out, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float acc = 0.f;
+#pragma unroll
+    for (int s = 0; s < NR; ++s) acc += in[s * n + i];
+#pragma unroll
+    for (int s = 0; s < NW; ++s) out[s * n + i] = acc + (float)s;
+}
+
+template __global__ void k_stream_probe<1, 1>(const float *, float *, long long);
+template __global__ void k_stream_probe<6, 3>(const float *, float *, long long);
+template __global__ void k_stream_probe<8, 3>(const float *, float *, long long);
+template __global__ void k_stream_probe<8, 6>(const float *, float *, long long);
+
 // ---------------------------------------------------------------------------
 // Near-to-far transform over the top face (near2far.py:28-156), FP64.
 // The integrand is linear in J = (-hy_u, hx_u) and M = (ey_u, -ex_u), so
