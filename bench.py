@@ -32,6 +32,7 @@ sys.path.insert(0, ROOT)
 METRIC = 'FDTD Mcell-updates/s + material-models/hr batched vs plain; p50/p99 latency'
 HBM_PEAK_GBS = 8000.0
 PROBE_BYTES = 1 << 30          # calibration copy: 1 GiB read + 1 GiB written
+LAUNCH_KEYS = ('ty', 'zchunk', 'xvec', 'xwaves', 'xcd_order')
 PMC_COUNTERS = ('FETCH_SIZE', 'WRITE_SIZE')
 
 
@@ -224,7 +225,7 @@ def largest_leg(fused, steps=10, warmup=3, log=print):
            'Mcell_updates_per_s': round(cells / (ms * 1e-3) / 1e6, 1),
            'step_alg_GBs': round(step_bytes / (ms * 1e-3) / 1e9, 1),
            'kernels_ms': {kk: round(v, 4) for kk, v in k.items()},
-           'launch': {'ty': shape[0], 'zchunk': shape[1], 'xvec': shape[2]},
+           'launch': dict(zip(LAUNCH_KEYS, shape)),
            'device_bytes': int(native.footprint(geom) + 16 * g.size_t_inc * g.size_y * g.size_x)}
     sim.destroy()
     beam.release()
@@ -245,6 +246,10 @@ def main():
     ap.add_argument('--zchunk', type=int, default=-1)
     ap.add_argument('--xvec', type=int, default=-1,
                     help='x cells per lane of the split kernels (0 = automatic)')
+    ap.add_argument('--xwaves', type=int, default=0,
+                    help='split stepper: wavefronts side by side along x per block')
+    ap.add_argument('--xcd-order', type=int, default=-1,
+                    help='1: XCD-contiguous tile order, 0: plain order')
     ap.add_argument('--no-traffic', action='store_true',
                     help='skip the two rocprofv3 --pmc passes that measure HBM traffic')
     ap.add_argument('--pmc-dir', default=None,
@@ -276,14 +281,19 @@ def main():
     fused = common.FUSED_STEPPER if args.stepper is None else args.stepper == 'fused'
     sim = native.Sim(common.main_geom(pc), common.coeffs_for(pc))
     sim.set_stepper(fused)
-    ty, zc, xv = tuning.launch_for(g.size_x, g.size_y, g.size_z, fused=fused)
+    ty, zc, xv, xw, xo = tuning.launch_for(g.size_x, g.size_y, g.size_z, fused=fused)
     if args.ty > 0:
         ty = args.ty
     if args.zchunk >= 0:
         zc = args.zchunk
     if args.xvec >= 0:
         xv = args.xvec
-    sim.set_launch(ty, zc, xv)
+    if args.xwaves > 0:
+        xw = args.xwaves
+    if args.xcd_order >= 0:
+        xo = args.xcd_order
+    launch = (ty, zc, xv, xw, xo)
+    sim.set_launch(*launch)
     beam = native.Beam(g.size_t_inc, g.size_y, g.size_x)
     n_fill = min(g.size_t_inc, args.warmup + args.steps + 8)
     beam.upload(0, *workloads.synthetic_beam_planes(pc, 0, n_fill))
@@ -352,7 +362,8 @@ def main():
     if not args.no_traffic:
         print('bench: PMC traffic passes (FETCH_SIZE, WRITE_SIZE)', file=sys.stderr, flush=True)
         child = ['--steps', '4', '--warmup', '1', '--stepper', 'fused' if fused else 'split',
-                 '--ty', str(ty), '--zchunk', str(zc), '--xvec', str(xv)]
+                 '--ty', str(ty), '--zchunk', str(zc), '--xvec', str(xv),
+                 '--xwaves', str(xw), '--xcd-order', str(xo)]
         traffic, tdetail = measure_traffic(kern['name'], child, args.pmc_dir)
     roof = {'bound': 'hbm', 'kernel': kern['name'], 'achieved': round(achieved, 1),
             'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
@@ -391,7 +402,7 @@ def main():
                                   'lambda at 0.021 lambda, README shape), snow over rough '
                                   'soil, 5.4 GHz HH theta 20, 1714-step run',
                       'cells': cells, 'stepper': 'fused' if fused else 'split',
-                      'launch': {'ty': ty, 'zchunk': zc, 'xvec': xv},
+                      'launch': dict(zip(LAUNCH_KEYS, launch)),
                       'uniform_planes_from_z': int(uni_z),
                       'device': info['name'], 'arch': info['arch'],
                       'cu_count': info['cu_count']},

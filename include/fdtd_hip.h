@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define FDTD_ABI_VERSION 2
+#define FDTD_ABI_VERSION 3
 #define FDTD_MAX_PML 16
 
 /* error codes */
@@ -139,12 +139,16 @@ int fdtd_sim_get_psi(FdtdSim *s, int index, float *out);
  * default).  Both give identical results. */
 int fdtd_sim_set_stepper(FdtdSim *s, int fused);
 /* No counterpart in the reference.  Tile shape: each lane owns xvec
- * consecutive x cells (1, 2, 4; 0 = widest that leaves few lanes idle) and
+ * consecutive x cells (1, 2, 4) and
  * marches z over zchunk cells (0 = whole column).  Split stepper: blocks of
- * 64 x ty lanes, ty 1, 2 or 4 rows.  Fused stepper: ty 2, 4 or 8 rows per
- * wavefront with (4 ty + 2) xvec <= 64 (LDS).  An invalid shape is
- * rejected and the previous one kept. */
-int fdtd_sim_set_launch(FdtdSim *s, int ty, int zchunk, int xvec);
+ * xwaves x ty wavefronts (at most 4), xwaves (1, 2, 4) side by side along
+ * x on each of ty (1, 2, 4) rows.  Fused stepper: xwaves 1, ty 2, 4 or 8
+ * rows per wavefront with (4 ty + 2) xvec <= 64 (LDS).  0 for ty, xvec or
+ * xwaves: chosen from the grid.  xcd_order: 0 runs tiles in plain order, 1
+ * deals each XCD a contiguous range of tiles, 2 (split only) deals each XCD
+ * a band of rows within every z chunk.  An invalid shape is rejected and
+ * the previous one kept. */
+int fdtd_sim_set_launch(FdtdSim *s, int ty, int zchunk, int xvec, int xwaves, int xcd_order);
 /* No counterpart in the reference.  Run nsteps and report HIP-event times
  * on the library stream: whole run and per-kernel averages (ms), the latter
  * from whole-domain launches; with the fused stepper ms_h is the fused
@@ -205,7 +209,8 @@ int fdtd_beam_synthesize(const FdtdGeom *g, const FdtdCoeffs *c, const FdtdBeamS
  * mode and a stepper; each keeps its own step counter. */
 int fdtd_batch_create(FdtdSim *const *sims, int n, FdtdBatch **out);
 void fdtd_batch_destroy(FdtdBatch *b);
-int fdtd_batch_set_launch(FdtdBatch *b, int ty, int zchunk, int xvec);
+/* As fdtd_sim_set_launch, for every member. */
+int fdtd_batch_set_launch(FdtdBatch *b, int ty, int zchunk, int xvec, int xwaves, int xcd_order);
 /* Queue nsteps steps of every member; use_graph != 0 captures the step
  * sequence into a HIP graph on first use and replays it. */
 int fdtd_batch_run(FdtdBatch *b, int nsteps, int use_graph);

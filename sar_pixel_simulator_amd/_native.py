@@ -132,7 +132,7 @@ _SIGS = {
     'fdtd_sim_set_field': ([_P, _I, _FP], _I),
     'fdtd_sim_get_psi': ([_P, _I, _FP], _I),
     'fdtd_sim_set_stepper': ([_P, _I], _I),
-    'fdtd_sim_set_launch': ([_P, _I, _I, _I], _I),
+    'fdtd_sim_set_launch': ([_P, _I, _I, _I, _I, _I], _I),
     'fdtd_sim_time_steps': ([_P, _I, _FP, _FP, _FP, _FP], _I),
     'fdtd_sim_near2far': ([_P, _DP, _I, _D, _D, _D, _DP], _I),
     'fdtd_near2far_host': ([_FP, _FP, _FP, _FP, _I, _I, _I, _D, _DP, _I, _D,
@@ -145,7 +145,7 @@ _SIGS = {
                               ctypes.POINTER(FdtdBeamSpec), ctypes.POINTER(_P), _FP], _I),
     'fdtd_batch_create': ([ctypes.POINTER(_P), _I, ctypes.POINTER(_P)], _I),
     'fdtd_batch_destroy': ([_P], None),
-    'fdtd_batch_set_launch': ([_P, _I, _I, _I], _I),
+    'fdtd_batch_set_launch': ([_P, _I, _I, _I, _I, _I], _I),
     'fdtd_batch_run': ([_P, _I, _I], _I),
     'fdtd_batch_sync': ([_P], _I),
     'fdtd_batch_time_steps': ([_P, _I, _I, _FP], _I),
@@ -417,9 +417,10 @@ class Sim:
         _check(lib().fdtd_sim_get_psi(self._h, int(index), _fptr(out)))
         return out
 
-    def set_launch(self, ty, zchunk=0, xvec=0):
+    def set_launch(self, ty, zchunk=0, xvec=0, xwaves=1, xcd_order=1):
         """Tile shape (see fdtd_sim_set_launch); xvec 0 = automatic."""
-        _check(lib().fdtd_sim_set_launch(self._h, int(ty), int(zchunk), int(xvec)))
+        _check(lib().fdtd_sim_set_launch(self._h, int(ty), int(zchunk), int(xvec), int(xwaves),
+                                         int(xcd_order)))
 
     def set_stepper(self, fused):
         """True: fused H+E kernel per step; False: split H and E kernels."""
@@ -487,8 +488,9 @@ class Batch:
         _check(lib().fdtd_batch_create(arr, len(self.sims), ctypes.byref(h)))
         self._h = h
 
-    def set_launch(self, ty, zchunk=0, xvec=0):
-        _check(lib().fdtd_batch_set_launch(self._h, int(ty), int(zchunk), int(xvec)))
+    def set_launch(self, ty, zchunk=0, xvec=0, xwaves=1, xcd_order=1):
+        _check(lib().fdtd_batch_set_launch(self._h, int(ty), int(zchunk), int(xvec), int(xwaves),
+                                           int(xcd_order)))
 
     def run(self, nsteps, use_graph=True):
         _check(lib().fdtd_batch_run(self._h, int(nsteps), 1 if use_graph else 0))

@@ -16,13 +16,13 @@
 
 namespace fdtd {
 template <int V>
-__global__ void k_h_half(const Desc *, const int4 *, int);
+__global__ void k_h_half(const Desc *, const int4 *, int, int);
 template <int V>
-__global__ void k_e_half(const Desc *, const int4 *, int);
+__global__ void k_e_half(const Desc *, const int4 *, int, int);
 __global__ void k_dft(const Desc *);
 __global__ void k_advance(const Desc *, int);
 template <int V, int R>
-__global__ void k_step_strip(const Desc *, const int4 *, int);
+__global__ void k_step_strip(const Desc *, const int4 *, int, int);
 __global__ void k_phasor_out(const float4 *, long long, float2 *);
 __global__ void k_n2f_partial(const float2 *, const float2 *, const float2 *, const float2 *, int, int, int,
                               const double2 *, const double2 *, double *);
@@ -101,10 +101,17 @@ static void beam_unref(FdtdBeam *b) {
     }
 }
 
-// Launch shape a simulation or batch starts with, per stepper.
+// Launch shape a simulation or batch starts with, per stepper.  Split:
+// one x cell per lane, blocks of four wavefronts as wide along x as the
+// grid fills, 4-plane chunks, plain block order (measured fastest at
+// 761 x 761 x 203).  Fused: 4-row strips, 8-plane chunks.
 static Launch default_launch(int fused) {
     Launch l;
-    if (fused) l.ty = 4;
+    if (fused) {
+        l.ty = 4;
+        l.zchunk = 8;
+        l.xw = 1;
+    }
     return l;
 }
 
@@ -116,7 +123,8 @@ struct Group {
     int4 *d_tiles = nullptr;
     int ntiles = 0, nmodels = 0, mode = FDTD_MODE_MAIN, fused = 0;
     std::vector<std::pair<int, int>> slabs;   // split stepper: tile ranges stepped H then E
-    int xvec = 1;                  // x cells per lane of the split kernels in use
+    int xvec = 1;                  // x cells per lane of the kernels in use
+    int ty = 1, xw = 1;            // rows and x wavefronts per block (split), rows per strip (fused)
     long long max_dft_items = 0;
     Launch launch;
     std::vector<FdtdSim *> members;
@@ -287,19 +295,29 @@ static void group_clear_graphs(Group &G) {
 // Widest lane (4, 2 or 1 x cells) whose tiles leave at most 30 % of the
 // lanes idle over the members' rows (x padding to whole tiles), among the
 // widths every member's row pitch holds.
-static int auto_xvec(const std::vector<FdtdSim *> &members) {
-    for (int v : {4, 2}) {
-        bool fits = true;
-        double used = 0, alloc = 0;
-        for (const FdtdSim *s : members) {
-            fits = fits && s->h.sy % v == 0;
-            const long long tw = (long long)kWave * v;
-            const double rows = (double)s->g.ny * s->g.nz;
-            used += rows * s->g.nx;
-            alloc += rows * (double)((s->g.nx + tw - 1) / tw * tw);
-        }
-        if (fits && alloc <= 1.3 * used) return v;
+static bool tiles_fill(const std::vector<FdtdSim *> &members, int v, int tw) {
+    bool fits = true;
+    double used = 0, alloc = 0;
+    for (const FdtdSim *s : members) {
+        fits = fits && s->h.sy % v == 0;
+        const double rows = (double)s->g.ny * s->g.nz;
+        used += rows * s->g.nx;
+        alloc += rows * (double)((s->g.nx + tw - 1) / tw * tw);
     }
+    return fits && alloc <= 1.3 * used;
+}
+
+static int auto_xvec(const std::vector<FdtdSim *> &members, int xw) {
+    for (int v : {4, 2})
+        if (tiles_fill(members, v, kWave * v * xw)) return v;
+    return 1;
+}
+
+// Widest block (4 or 2 wavefronts along x, else 1) whose tiles of
+// 64 xvec xw cells leave at most 30 % of the lanes idle.
+static int auto_xwaves(const std::vector<FdtdSim *> &members, int v) {
+    for (int w : {4, 2})
+        if (tiles_fill(members, v, kWave * v * w)) return w;
     return 1;
 }
 
@@ -319,19 +337,32 @@ static long long slab_bytes() {
 static int group_build(Group &G) {
     CHECK(!G.members.empty(), "empty group");
     HIP_TRY(hipSetDevice(G.device));
-    const int ty = G.launch.ty;
     std::vector<int4> tiles;
     std::vector<Desc> descs;
     G.mode = G.members[0]->g.mode;
     G.fused = G.members[0]->h.fused;
     const int xv = G.launch.xvec;
-    if (G.fused)
+    int ty = G.launch.ty, xw = G.launch.xw;
+    CHECK(xv == 0 || xv == 1 || xv == 2 || xv == 4, "xvec must be 0, 1, 2 or 4 (got %d)", xv);
+    CHECK(xw == 0 || xw == 1 || xw == 2 || xw == 4, "xwaves must be 0, 1, 2 or 4 (got %d)", xw);
+    CHECK(G.launch.xcd >= 0 && G.launch.xcd <= 2, "block order must be 0, 1 or 2 (got %d)", G.launch.xcd);
+    if (G.fused) {
         CHECK(ty == 2 || ty == 4 || ty == 8, "fused stepper: ty (rows per wavefront) must be 2, 4 or 8 (got %d)",
               ty);
-    else
-        CHECK(ty == 1 || ty == 2 || ty == 4, "split stepper: ty must be 1, 2 or 4 (got %d)", ty);
-    CHECK(xv == 0 || xv == 1 || xv == 2 || xv == 4, "xvec must be 0, 1, 2 or 4 (got %d)", xv);
-    G.xvec = xv ? xv : auto_xvec(G.members);
+        CHECK(xw <= 1, "fused stepper: one wavefront per strip (xwaves must be 1)");
+        CHECK(G.launch.xcd != 2, "fused stepper: block order must be 0 (plain) or 1 (XCD-contiguous)");
+        xw = 1;
+        G.xvec = xv ? xv : auto_xvec(G.members, 1);
+    } else {
+        CHECK(ty == 0 || ty == 1 || ty == 2 || ty == 4, "split stepper: ty must be 0, 1, 2 or 4 (got %d)", ty);
+        // automatic: one cell per lane unless the block width is fixed
+        G.xvec = xv ? xv : xw ? auto_xvec(G.members, xw) : 1;
+        if (!xw) xw = auto_xwaves(G.members, G.xvec);
+        if (!ty) ty = std::max(1, 4 / xw);
+        CHECK(xw * ty <= 4, "split stepper: xwaves x ty = %d exceeds 4 wavefronts per block", xw * ty);
+    }
+    G.ty = ty;
+    G.xw = xw;
     if (G.fused) {
         // a block's four wavefronts keep (4 ty + 2) x 64 xvec floats each in
         // LDS; the built shapes stay within 64 KB per block
@@ -341,9 +372,9 @@ static int group_build(Group &G) {
     }
     for (FdtdSim *s : G.members)
         CHECK(s->h.sy % G.xvec == 0, "row pitch %lld does not hold whole %d-cell lanes", s->h.sy, G.xvec);
-    // tiles of (64 xvec) x ty cells: split, one row per wavefront of a
-    // 64 x ty block; fused, one wavefront per tile, four tiles per block
-    const int tx = kWave * G.xvec;
+    // tiles of (64 xvec xw) x ty cells: split, xw wavefronts along each of
+    // ty rows of a block; fused, one wavefront per tile, four tiles per block
+    const int tx = kWave * G.xvec * xw;
     const int tys = ty;
     G.max_dft_items = 0;
     for (size_t m = 0; m < G.members.size(); ++m) {
@@ -353,11 +384,29 @@ static int group_build(Group &G) {
         CHECK(s->device == G.device, "group members must share a device");
         descs.push_back(s->h);
         const int nz = s->g.nz, zc = G.launch.zchunk > 0 ? G.launch.zchunk : nz;
+        std::vector<int4> chunk;
         for (int z0 = 0; z0 < nz; z0 += zc) {
             const int z1 = std::min(nz, z0 + zc);
+            chunk.clear();
             for (int y0 = 0; y0 < s->g.ny; y0 += tys)
                 for (int x0 = 0; x0 < s->g.nx; x0 += tx)
-                    tiles.push_back(make_int4((int)m, x0, y0, z0 | (z1 << 16)));
+                    chunk.push_back(make_int4((int)m, x0, y0, z0 | (z1 << 16)));
+            if (G.launch.xcd != 2) {
+                tiles.insert(tiles.end(), chunk.begin(), chunk.end());
+                continue;
+            }
+            // banded order: the dispatcher deals blocks to the XCDs in
+            // turn, so block 8 k + j runs on XCD j; XCD j takes the k-th
+            // tile of band j, the j-th eighth of the chunk's rows.  The
+            // rows a tile's y +- 1 neighbours lie on then pass through one
+            // XCD's L2, while all XCDs stream through the same z chunk.
+            // Short bands are padded with empty tiles (w = 0).
+            const int T = (int)chunk.size(), B = (T + kXcds - 1) / kXcds;
+            for (int k = 0; k < B; ++k)
+                for (int j = 0; j < kXcds; ++j) {
+                    const int i = j * B + k;
+                    tiles.push_back(i < T ? chunk[i] : make_int4(0, 0, 0, 0));
+                }
         }
         G.max_dft_items = std::max(G.max_dft_items, s->h.dft_items);
     }
@@ -386,7 +435,8 @@ static int group_build(Group &G) {
     if (sb > 0 && !G.fused && G.mode == FDTD_MODE_MAIN && G.members.size() == 1) {
         const FdtdSim *m = G.members[0];
         const int zc = G.launch.zchunk > 0 ? G.launch.zchunk : m->g.nz;
-        const int per_chunk = (int)(((m->g.ny + tys - 1) / tys) * ((m->g.nx + tx - 1) / tx));
+        int per_chunk = (int)(((m->g.ny + tys - 1) / tys) * ((m->g.nx + tx - 1) / tx));
+        if (G.launch.xcd == 2) per_chunk = (per_chunk + kXcds - 1) / kXcds * kXcds;
         const long long chunk_bytes = 32LL * zc * m->g.ny * m->g.nx;
         const int chunks = (int)std::max(1LL, sb / chunk_bytes);
         G.slabs.clear();
@@ -408,32 +458,34 @@ static void group_free(Group &G) {
 enum KernelId { KH = 0, KE = 1, KD = 2, KF = 3 };
 
 static void launch_kernel(Group &G, KernelId k, int t0 = 0, int t1 = -1) {
-    const dim3 blk(kWave, G.launch.ty);
+    const dim3 blk(kWave, G.ty * G.xw);
+    const int lxw = G.xw == 4 ? 2 : G.xw == 2 ? 1 : 0;
+    const int shape = lxw | (G.launch.xcd == 1 ? kShapeXcd : 0);
     if (t1 < 0) t1 = G.ntiles;
     const int4 *tl = G.d_tiles + t0;
     const int nt = t1 - t0;
     if (k == KF) {
         const dim3 grid((G.ntiles + 3) / 4), b4(kWave, 4);
         auto *fn = &k_step_strip<1, 2>;
-        const int r = G.launch.ty, v = G.xvec;
+        const int r = G.ty, v = G.xvec;
         if (v == 4) fn = &k_step_strip<4, 2>;
         else if (v == 2) fn = r == 4 ? &k_step_strip<2, 4> : &k_step_strip<2, 2>;
         else fn = r == 8 ? &k_step_strip<1, 8> : r == 4 ? &k_step_strip<1, 4> : &k_step_strip<1, 2>;
-        hipLaunchKernelGGL(fn, grid, b4, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles);
+        hipLaunchKernelGGL(fn, grid, b4, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles, shape);
     } else if (k == KH) {
         if (G.xvec == 4)
-            hipLaunchKernelGGL(k_h_half<4>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt);
+            hipLaunchKernelGGL(k_h_half<4>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape);
         else if (G.xvec == 2)
-            hipLaunchKernelGGL(k_h_half<2>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt);
+            hipLaunchKernelGGL(k_h_half<2>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape);
         else
-            hipLaunchKernelGGL(k_h_half<1>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt);
+            hipLaunchKernelGGL(k_h_half<1>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape);
     } else if (k == KE) {
         if (G.xvec == 4)
-            hipLaunchKernelGGL(k_e_half<4>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt);
+            hipLaunchKernelGGL(k_e_half<4>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape);
         else if (G.xvec == 2)
-            hipLaunchKernelGGL(k_e_half<2>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt);
+            hipLaunchKernelGGL(k_e_half<2>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape);
         else
-            hipLaunchKernelGGL(k_e_half<1>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt);
+            hipLaunchKernelGGL(k_e_half<1>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape);
     } else {
         const unsigned gx = (unsigned)std::max(1LL, (G.max_dft_items + 255) / 256);
         hipLaunchKernelGGL(k_dft, dim3(gx, G.nmodels), dim3(256), 0, G.stream, G.d_descs);
@@ -1048,13 +1100,15 @@ extern "C" int fdtd_sim_set_stepper(FdtdSim *s, int fused) {
     return sim_refresh(s);
 }
 
-extern "C" int fdtd_sim_set_launch(FdtdSim *s, int ty, int zchunk, int xvec) {
+extern "C" int fdtd_sim_set_launch(FdtdSim *s, int ty, int zchunk, int xvec, int xwaves, int xcd_order) {
     CHECK(s, "NULL sim");
     CHECK(zchunk >= 0 && zchunk < 0x8000, "zchunk out of range");
     const Launch keep = s->self.launch;
     s->self.launch.ty = ty;
     s->self.launch.zchunk = zchunk;
     s->self.launch.xvec = xvec;
+    s->self.launch.xw = xwaves;
+    s->self.launch.xcd = xcd_order;
     int rc = group_build(s->self);
     if (rc) {
         std::string err = g_err;
@@ -1383,13 +1437,15 @@ extern "C" void fdtd_batch_destroy(FdtdBatch *b) {
     delete b;
 }
 
-extern "C" int fdtd_batch_set_launch(FdtdBatch *b, int ty, int zchunk, int xvec) {
+extern "C" int fdtd_batch_set_launch(FdtdBatch *b, int ty, int zchunk, int xvec, int xwaves, int xcd_order) {
     CHECK(b, "NULL batch");
     CHECK(zchunk >= 0 && zchunk < 0x8000, "zchunk out of range");
     const Launch keep = b->grp.launch;
     b->grp.launch.ty = ty;
     b->grp.launch.zchunk = zchunk;
     b->grp.launch.xvec = xvec;
+    b->grp.launch.xw = xwaves;
+    b->grp.launch.xcd = xcd_order;
     int rc = group_build(b->grp);
     if (rc) {
         std::string err = g_err;

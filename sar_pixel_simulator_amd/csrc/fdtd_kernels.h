@@ -75,11 +75,18 @@ struct alignas(16) Desc {
 // stepper: ty in {1, 2, 4}, one row per wavefront of a 64 x ty block.
 // Fused stepper: ty in {2, 4, 8} rows walked by one wavefront, four
 // wavefronts (tiles) per block.
+// Tile shape of the stencil kernels (0 = chosen from the grid; see
+// group_build).  The defaults are the split stepper's.
 struct Launch {
-    int ty = 2;
-    int zchunk = 16;
-    int xvec = 0;
+    int ty = 0;      // rows per block (split) or per wavefront (fused)
+    int zchunk = 4;  // z cells marched per tile (0 = whole column)
+    int xvec = 0;    // x cells per lane
+    int xw = 0;      // split stepper: wavefronts side by side along x per block
+    int xcd = 0;     // 1: XCD-contiguous block order, 0: plain block order
 };
+
+// Shape word bit of the split and strip kernels: XCD-contiguous block order.
+constexpr int kShapeXcd = 4;
 
 // Buffer set holding the fields at step t.
 __host__ __device__ inline int current_set(const Desc &d, int t) { return d.fused ? (t & 1) : 0; }

@@ -21,6 +21,17 @@ __device__ __forceinline__ int xcd_tile(int bid, int n) {
     return xcd * q + (xcd < r ? xcd : r) + k;
 }
 
+// Split-kernel launch shape word: bits 0-1 hold log2 of the wavefronts a
+// block places side by side along x (the rest of its wavefronts take the
+// following rows); bit 2 selects the XCD-contiguous block order.  In plain
+// order consecutive blocks, which the dispatcher deals out to the XCDs in
+// turn, take consecutive tiles, so the whole device streams through one
+// narrow address window.
+
+__device__ __forceinline__ int block_tile(int shape, int n) {
+    return (shape & kShapeXcd) ? xcd_tile(blockIdx.x, n) : (int)blockIdx.x;
+}
+
 __device__ __forceinline__ int load_step(const int *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -387,8 +398,12 @@ __device__ __forceinline__ void vcopy(float (&r)[V], const float (&s)[V]) {
 // ---------------------------------------------------------------------------
 // Split stepper: H half-step kernel, then E half-step kernel, fields updated
 // in place (set 0).  Used for INCIDENT runs (E before H) and available for
-// MAIN runs.  A block of 64 x ty lanes covers a (64 V) x ty tile, one row
-// per wavefront; each lane marches z upward through the tile's z range,
+// MAIN runs.  A block of 64 x (xw ty) lanes covers a (64 V xw) x ty tile:
+// xw wavefronts side by side along each of ty rows (xw ty <= 4).  Wide
+// blocks (xw = 4: 1 KB per array per block at V = 1) in plain block order
+// keep the device's concurrent accesses within a narrow window of each
+// array, which the DRAM serves faster than accesses spread over many rows
+// and planes.  Each lane marches z upward through the tile's z range,
 // reusing the plane above (H half: ex, ey) or below (E half: hx, hy) from
 // registers.  (Issuing plane z + 1's loads before computing plane z was
 // measured slower: the extra registers cost more occupancy than it hides.)
@@ -434,16 +449,19 @@ __device__ __forceinline__ void h_plane_load(HPlane<V> &P, const gfloat *Ex, con
 
 template <int V>
 __global__ void __launch_bounds__(256)
-k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles) {
-    const int4 tl = tiles[xcd_tile(blockIdx.x, ntiles)];
+k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles, int shape) {
+    const int4 tl = tiles[block_tile(shape, ntiles)];
+    if (tl.w == 0) return;                             // padding tile of the banded order
     const Desc *__restrict__ d = descs + tl.x;
     const int t = load_step(d->step);
     if (t >= d->nt) return;                            // uniform per block
     const StepK k = step_k(d, t);
-    const int y = tl.z + threadIdx.y;
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int lxw = shape & 3;
+    const int y = tl.z + (w >> lxw);
     if (y >= k.ny) return;                             // uniform per wavefront (one row)
     const int lane = threadIdx.x;
-    const int xb = tl.y + V * lane;                    // the lane's first cell
+    const int xb = tl.y + V * (((w & ((1 << lxw) - 1)) << 6) + lane);                    // the lane's first cell
     const bool act = xb < k.nx;
     // the x + 1 neighbour of the lane's last cell is needed when that cell
     // is below nx - 1; for lane 63 it lies in the next tile
@@ -539,16 +557,19 @@ __device__ __forceinline__ void e_plane_load(EPlane<V> &Q, const gfloat *Ex, con
 
 template <int V>
 __global__ void __launch_bounds__(256)
-k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles) {
-    const int4 tl = tiles[xcd_tile(blockIdx.x, ntiles)];
+k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles, int shape) {
+    const int4 tl = tiles[block_tile(shape, ntiles)];
+    if (tl.w == 0) return;                             // padding tile of the banded order
     const Desc *__restrict__ d = descs + tl.x;
     const int t = load_step(d->step);
     if (t >= d->nt) return;                            // uniform per block
     const StepK k = step_k(d, t);
-    const int y = tl.z + threadIdx.y;
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int lxw = shape & 3;
+    const int y = tl.z + (w >> lxw);
     if (y >= k.ny) return;                             // uniform per wavefront (one row)
     const int lane = threadIdx.x;
-    const int xb = tl.y + V * lane;
+    const int xb = tl.y + V * (((w & ((1 << lxw) - 1)) << 6) + lane);
     const bool act = xb < k.nx;
     const bool prev_mem = lane == 0 && xb >= 1 && act;    // x - 1 lies in the previous tile
     const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
@@ -600,12 +621,12 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     }
 }
 
-template __global__ void k_h_half<1>(const Desc *, const int4 *, int);
-template __global__ void k_h_half<2>(const Desc *, const int4 *, int);
-template __global__ void k_h_half<4>(const Desc *, const int4 *, int);
-template __global__ void k_e_half<1>(const Desc *, const int4 *, int);
-template __global__ void k_e_half<2>(const Desc *, const int4 *, int);
-template __global__ void k_e_half<4>(const Desc *, const int4 *, int);
+template __global__ void k_h_half<1>(const Desc *, const int4 *, int, int);
+template __global__ void k_h_half<2>(const Desc *, const int4 *, int, int);
+template __global__ void k_h_half<4>(const Desc *, const int4 *, int, int);
+template __global__ void k_e_half<1>(const Desc *, const int4 *, int, int);
+template __global__ void k_e_half<2>(const Desc *, const int4 *, int, int);
+template __global__ void k_e_half<4>(const Desc *, const int4 *, int, int);
 
 // ---------------------------------------------------------------------------
 // Fused stepper (MAIN): one kernel per time step computes H(t+1/2) and then
@@ -654,14 +675,14 @@ __device__ __forceinline__ void lds_st(float *p, const float (&r)[V]) {
 
 template <int V, int R>
 __global__ void __launch_bounds__(256)
-k_step_strip(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles) {
+k_step_strip(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles, int shape) {
     using L = StripLds<V, R>;
     __shared__ float lds_all[4][L::kFloats];
     const int nblk = (ntiles + 3) >> 2;
     // one tile per wavefront: threadIdx.y is uniform across the wavefront,
     // and saying so keeps the descriptor and its pointers in scalar registers
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
-    const int wt = __builtin_amdgcn_readfirstlane((xcd_tile(blockIdx.x, nblk) << 2) + w);
+    const int wt = __builtin_amdgcn_readfirstlane((block_tile(shape, nblk) << 2) + w);
     if (wt >= ntiles) return;                          // uniform per wavefront
     const int4 tl = tiles[wt];
     const Desc *__restrict__ d = descs + tl.x;
@@ -849,12 +870,12 @@ k_step_strip(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int
     }
 }
 
-template __global__ void k_step_strip<1, 2>(const Desc *, const int4 *, int);
-template __global__ void k_step_strip<1, 4>(const Desc *, const int4 *, int);
-template __global__ void k_step_strip<1, 8>(const Desc *, const int4 *, int);
-template __global__ void k_step_strip<2, 2>(const Desc *, const int4 *, int);
-template __global__ void k_step_strip<2, 4>(const Desc *, const int4 *, int);
-template __global__ void k_step_strip<4, 2>(const Desc *, const int4 *, int);
+template __global__ void k_step_strip<1, 2>(const Desc *, const int4 *, int, int);
+template __global__ void k_step_strip<1, 4>(const Desc *, const int4 *, int, int);
+template __global__ void k_step_strip<1, 8>(const Desc *, const int4 *, int, int);
+template __global__ void k_step_strip<2, 2>(const Desc *, const int4 *, int, int);
+template __global__ void k_step_strip<2, 4>(const Desc *, const int4 *, int, int);
+template __global__ void k_step_strip<4, 2>(const Desc *, const int4 *, int, int);
 
 // ---------------------------------------------------------------------------
 // Running single-frequency DFT on the phasor planes

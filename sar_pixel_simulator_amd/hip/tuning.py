@@ -1,10 +1,12 @@
 """Tile-shape selection for the stencil kernels.
 
-A launch shape is (ty, zchunk, xvec): blocks of 64 x ty lanes marching z.
-Each lane owns xvec consecutive x cells (1, 2 or 4; 0 = chosen by the
-library from the grid width).  Split stepper: ty in {1, 2, 4} rows per
-block, one per wavefront.  Fused stepper: ty in {2, 4, 8} rows walked by one
-wavefront.  zchunk: z cells per tile (0 = whole column).
+A launch shape is (ty, zchunk, xvec, xwaves, xcd_order).  Each lane owns
+xvec consecutive x cells (1, 2 or 4; 0 = chosen by the library from the
+grid width) and marches z over zchunk cells (0 = whole column).  Split
+stepper: blocks of xwaves x ty wavefronts (at most 4), xwaves side by side
+along x on each of ty rows.  Fused stepper: ty in {2, 4, 8} rows walked by
+one wavefront, xwaves 1.  xcd_order 1 deals each XCD a contiguous range of
+tiles, 0 runs them in plain order.
 ``autotune`` measures candidates on the device and records the best per
 (stepper, arch, grid) in a JSON cache; ``launch_for`` reads it, falling back
 to a default.
@@ -14,12 +16,20 @@ import os
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'tuning_cache.json')
 CANDIDATES = {
-    'fused': [(r, zc, xv) for xv, rows in ((1, (2, 4, 8)), (2, (2, 4)), (4, (2,)))
-              for r in rows for zc in (4, 8, 16, 32)],
-    'split': [(ty, zc, xv) for xv in (4, 2) for ty in (1, 2, 4) for zc in (8, 16, 32)]
-             + [(2, 16, 1)],
+    'fused': [(r, zc, xv, 1, xo) for xv, rows in ((1, (2, 4, 8)), (2, (2, 4)), (4, (2,)))
+              for r in rows for zc in (4, 8, 16, 32) for xo in (1, 0)],
+    'split': [(ty, zc, xv, 1, 1) for xv in (4, 2, 1) for ty in (1, 2, 4) for zc in (8, 16, 32)]
+             + [(ty, zc, xv, xw, xo) for xv in (1, 2) for xw, ty in ((4, 1), (2, 1), (2, 2))
+                for zc in (1, 2, 4, 8, 16) for xo in (0, 1)],
 }
-DEFAULT = {'fused': (4, 16, 0), 'split': (2, 16, 0)}
+# library defaults: 0 = chosen from the grid (split: one cell per lane,
+# four wavefronts per block, as many along x as the grid fills)
+DEFAULT = {'fused': (4, 8, 0, 1, 0), 'split': (0, 4, 0, 0, 0)}
+
+
+def _shape(hit):
+    return (int(hit['ty']), int(hit['zchunk']), int(hit.get('xvec', 0)),
+            int(hit.get('xwaves', 1)), int(hit.get('xcd_order', 1)))
 
 
 def _key(nx, ny, nz, fused, arch='gfx950'):
@@ -38,13 +48,13 @@ def launch_for(nx, ny, nz, fused=True, arch='gfx950'):
     """Cached choice for this grid, else the stepper's default."""
     hit = _load().get(_key(nx, ny, nz, fused, arch))
     if hit:
-        return int(hit['ty']), int(hit['zchunk']), int(hit.get('xvec', 0))
+        return _shape(hit)
     return DEFAULT['fused' if fused else 'split']
 
 
 def autotune(sim, nx, ny, nz, fused=True, arch='gfx950', steps=10, candidates=None, save=True):
     """Time each candidate shape on ``sim`` (a ready native.Sim whose stepper
-    is already chosen) and keep the fastest.  Returns ((ty, zchunk),
+    is already chosen) and keep the fastest.  Returns (shape,
     {shape: ms_per_step})."""
     from .._native import FdtdError
     results = {}
@@ -63,7 +73,8 @@ def autotune(sim, nx, ny, nz, fused=True, arch='gfx950', steps=10, candidates=No
     if save:
         data = _load()
         data[_key(nx, ny, nz, fused, arch)] = {'ty': best[0], 'zchunk': best[1],
-                                               'xvec': best[2], 'ms_per_step': results[best]}
+                                               'xvec': best[2], 'xwaves': best[3],
+                                               'xcd_order': best[4], 'ms_per_step': results[best]}
         tmp = CACHE + '.tmp'
         with open(tmp, 'w') as f:
             json.dump(data, f, indent=1, sort_keys=True)
@@ -107,6 +118,6 @@ if __name__ == '__main__':
             best, res = _tune_workload(name, fused)
             tag = 'fused' if fused else 'split'
             for shape, ms in sorted(res.items(), key=lambda kv: kv[1]):
-                print(f'{name} {tag} ty={shape[0]} zchunk={shape[1]} xvec={shape[2]}: '
-                      f'{ms:.4f} ms/step', flush=True)
+                print(f'{name} {tag} ty={shape[0]} zchunk={shape[1]} xvec={shape[2]} '
+                      f'xwaves={shape[3]} xcd_order={shape[4]}: {ms:.4f} ms/step', flush=True)
             print(f'{name} {tag}: best {best}', flush=True)

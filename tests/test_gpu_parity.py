@@ -87,7 +87,13 @@ def test_fields_psi_phasors_bitwise(soil60, fused):
                                           (True, (2, 0, 1)), (True, (2, 0, 4)),
                                           (True, (8, 7, 1)), (True, (4, 5, 1)),
                                           (True, (2, 1, 4)), (True, (8, 16, 0)),
-                                          (True, (4, 3, 2)), (True, (4, 2, 0))])
+                                          (True, (4, 3, 2)), (True, (4, 2, 0)),
+                                          (False, (1, 1, 1, 4, 0)), (False, (1, 2, 2, 4, 0)),
+                                          (False, (2, 4, 1, 2, 0)), (False, (1, 16, 1, 2, 1)),
+                                          (False, (1, 0, 4, 4, 1)), (False, (2, 3, 2, 2, 1)),
+                                          (True, (4, 5, 1, 1, 0)), (True, (2, 0, 4, 1, 0)),
+                                          (False, (1, 4, 1, 4, 2)), (False, (2, 3, 1, 2, 2)),
+                                          (False, (1, 0, 2, 4, 2)), (False, (4, 1, 1, 1, 2))])
 def test_launch_shapes_bitwise(soil60, fused, launch):
     pc, Ca, Cb, beam, ref = soil60
     say('case: fused', fused, 'launch', launch)
@@ -145,15 +151,17 @@ def wide30():
     return pc, Ca, Cb, beam, H.oracle_main(pc, Ca, Cb, beam)
 
 
-@pytest.mark.parametrize('fused', [False, True], ids=['split', 'fused'])
+@pytest.mark.parametrize('fused,xwaves', [(False, 1), (True, 1), (False, 4), (False, 2)],
+                         ids=['split', 'fused', 'split-4-waves', 'split-2-waves'])
 @pytest.mark.parametrize('xvec', [4, 2, 1])
-def test_wide_grid_lane_widths_bitwise(wide30, fused, xvec):
+def test_wide_grid_lane_widths_bitwise(wide30, fused, xwaves, xvec):
     # 301 cells in x: two 256-cell tiles at 4 cells per lane (the second
     # with 12 active lanes), three at 2, five at 1; x +- 1 neighbours cross
     # lanes and tiles (the fused stepper recomputes the cell left of a tile)
     pc, Ca, Cb, beam, ref = wide30
-    say('case: wide 301x25x27, 30 steps, fused', fused, 'xvec', xvec)
-    sim = make_sim(pc, Ca, Cb, beam, launch=(2, 8, xvec), fused=fused)
+    say('case: wide 301x25x27, 30 steps, fused', fused, 'xvec', xvec, 'xwaves', xwaves)
+    launch = (2, 8, xvec) if xwaves == 1 else (4 // xwaves, 2, xvec, xwaves, 2 if xvec == 1 else 0)
+    sim = make_sim(pc, Ca, Cb, beam, launch=launch, fused=fused)
     sim.run(30)
     sim.sync()
     for n in native.FIELD_NAMES:
@@ -170,9 +178,19 @@ def test_invalid_launch_is_rejected_and_previous_kept(soil60):
         sim.set_launch(8, 0, 4)          # 8 rows per tile is a fused-stepper shape
     with pytest.raises(native.FdtdError):
         sim.set_launch(2, 0, 3)
+    with pytest.raises(native.FdtdError):
+        sim.set_launch(2, 0, 1, 4)       # 8 wavefronts per block
+    with pytest.raises(native.FdtdError):
+        sim.set_launch(1, 0, 1, 3)
+    with pytest.raises(native.FdtdError):
+        sim.set_launch(1, 4, 1, 4, 3)    # block orders are 0, 1, 2
     fsim = make_sim(pc, Ca, Cb, beam, fused=True)
     with pytest.raises(native.FdtdError):
         fsim.set_launch(8, 0, 2)         # 8 rows of 2-cell lanes exceed the fused LDS budget
+    with pytest.raises(native.FdtdError):
+        fsim.set_launch(2, 0, 1, 2)      # one wavefront per strip
+    with pytest.raises(native.FdtdError):
+        fsim.set_launch(2, 0, 1, 1, 2)   # banded order is a split-stepper order
     fsim.destroy()
     sim.run(60)
     sim.sync()
@@ -216,8 +234,10 @@ def test_split_and_fused_agree_long_run():
         assert_same(out[1][1][n], out[0][1][n], n)
 
 
-@pytest.mark.parametrize('fused,launch', [(True, None), (False, None), (False, (2, 8, 4))],
-                         ids=['fused', 'split-auto', 'split-4-per-lane'])
+@pytest.mark.parametrize('fused,launch', [(True, None), (False, None), (False, (2, 8, 4)),
+                                          (False, (1, 1, 1, 4, 0)), (False, (1, 4, 1, 4, 2))],
+                         ids=['fused', 'split-auto', 'split-4-per-lane', 'split-4-waves-flat',
+                              'split-banded'])
 def test_batch_equals_solo(fused, launch):
     cases = [H.soil_case(nx=33, ny=31, nz=35, nz_mat=14, steps=30, seed=s) for s in (1, 2)]
     cases.append(H.soil_case(nx=45, ny=39, nz=37, nz_mat=15, steps=30, seed=5))
