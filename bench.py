@@ -358,6 +358,14 @@ def main():
     # flat: the practical ceiling for that mix
     mix = (8, 6) if fused else ((8, 3) if kern['name'] == 'k_e_half' else (6, 3))
     mix_ms, mix_gbs = native.probe_streams(cells, *mix, reps=5)
+    # the split kernels' own access pattern (tile shape, z march, neighbour
+    # rows) without their arithmetic: the practical ceiling of that pattern
+    stencil = None
+    if not fused:
+        kind = 1 if kern['name'] == 'k_e_half' else 0
+        st_ms, st_gbs = native.probe_stencil(g.size_x, g.size_y, g.size_z, zc if zc > 0 else g.size_z,
+                                             kind, reps=5)
+        stencil = {'half': 'E' if kind else 'H', 'zchunk': zc, 'GBs': round(st_gbs, 1)}
     traffic, tdetail = None, {}
     if not args.no_traffic:
         print('bench: PMC traffic passes (FETCH_SIZE, WRITE_SIZE)', file=sys.stderr, flush=True)
@@ -373,6 +381,9 @@ def main():
             'copy_probe_GBs': {'4B_per_lane': round(copy1, 1), '16B_per_lane': round(copy4, 1)},
             'stream_probe': {'reads': mix[0], 'writes': mix[1], 'GBs': round(mix_gbs, 1)},
             'frac_of_stream_probe': round(achieved / mix_gbs, 4)}
+    if stencil:
+        roof['stencil_probe'] = stencil
+        roof['frac_of_stencil_probe'] = round(achieved / stencil['GBs'], 4)
     if traffic:
         roof['traffic_GBs'] = round(traffic / (kern['ms'] * 1e-3) / 1e9, 1)
 

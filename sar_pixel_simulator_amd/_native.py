@@ -53,7 +53,7 @@ EXPORTS = [
     'fdtd_beam_synthesize',
     'fdtd_batch_create', 'fdtd_batch_destroy', 'fdtd_batch_set_launch',
     'fdtd_batch_run', 'fdtd_batch_sync', 'fdtd_batch_time_steps',
-    'fdtd_probe_copy', 'fdtd_probe_streams',
+    'fdtd_probe_copy', 'fdtd_probe_streams', 'fdtd_probe_stencil',
 ]
 
 
@@ -151,6 +151,7 @@ _SIGS = {
     'fdtd_batch_time_steps': ([_P, _I, _I, _FP], _I),
     'fdtd_probe_copy': ([ctypes.c_int64, _I, _I, _FP], _I),
     'fdtd_probe_streams': ([ctypes.c_int64, _I, _I, _I, _FP], _I),
+    'fdtd_probe_stencil': ([_I, _I, _I, _I, _I, _I, _FP], _I),
 }
 
 
@@ -228,6 +229,17 @@ def probe_streams(n, nread, nwrite, reps=5):
     ms = ctypes.c_float()
     _check(lib().fdtd_probe_streams(int(n), int(nread), int(nwrite), int(reps), ctypes.byref(ms)))
     return ms.value, (nread + nwrite) * 4 * n / (ms.value * 1e-3) / 1e9
+
+
+def probe_stencil(nx, ny, nz, zchunk, kind, reps=5):
+    """Average time (ms) of one pass of the stencil-pattern probe (kind 0:
+    H half, 9 streams per cell; 1: E half, 11); returns (ms, GB/s over the
+    algorithmic streams)."""
+    ms = ctypes.c_float()
+    _check(lib().fdtd_probe_stencil(int(nx), int(ny), int(nz), int(zchunk), int(kind), int(reps),
+                                    ctypes.byref(ms)))
+    streams = 9 if kind == 0 else 11
+    return ms.value, streams * 4 * nx * ny * nz / (ms.value * 1e-3) / 1e9
 
 
 def make_geom(nx, ny, nz, npml, tfsf_z, nt, nt_inc, mode=MODE_MAIN,

@@ -16,9 +16,9 @@
 
 namespace fdtd {
 template <int V>
-__global__ void k_h_half(const Desc *, const int4 *, int, int);
+__global__ void k_h_half(const Desc *, const int4 *, int, int, TileGrid, const int *);
 template <int V>
-__global__ void k_e_half(const Desc *, const int4 *, int, int);
+__global__ void k_e_half(const Desc *, const int4 *, int, int, TileGrid, const int *);
 __global__ void k_dft(const Desc *);
 __global__ void k_advance(const Desc *, int);
 template <int V, int R>
@@ -32,6 +32,8 @@ template <int V>
 __global__ void k_copy_probe(const float *, float *, long long);
 template <int NR, int NW>
 __global__ void k_stream_probe(const float *, float *, long long);
+template <int KIND>
+__global__ void k_stencil_probe(float *, long long, int, int, int, uint32_t, uint32_t, int, int);
 }  // namespace fdtd
 
 using namespace fdtd;
@@ -125,6 +127,8 @@ struct Group {
     std::vector<std::pair<int, int>> slabs;   // split stepper: tile ranges stepped H then E
     int xvec = 1;                  // x cells per lane of the kernels in use
     int ty = 1, xw = 1;            // rows and x wavefronts per block (split), rows per strip (fused)
+    TileGrid tg = {};              // split kernels, single simulation: tiles from the block index
+    const int *step1 = nullptr;    // single simulation: its step counter
     long long max_dft_items = 0;
     Launch launch;
     std::vector<FdtdSim *> members;
@@ -424,6 +428,15 @@ static int group_build(Group &G) {
     HIP_TRY(hipStreamSynchronize(G.stream));
     G.ntiles = (int)tiles.size();
     G.nmodels = (int)descs.size();
+    G.tg = TileGrid{};
+    G.step1 = nullptr;
+    if (G.members.size() == 1 && !G.fused && G.launch.xcd != 2) {
+        const FdtdSim *m = G.members[0];
+        const int zc = G.launch.zchunk > 0 ? G.launch.zchunk : m->g.nz;
+        G.tg = TileGrid{(m->g.nx + tx - 1) / tx, (m->g.ny + tys - 1) / tys, tx, tys, zc, m->g.nz, 0, 0};
+        CHECK((long long)G.tg.ntx * G.tg.nyb * ((m->g.nz + zc - 1) / zc) == G.ntiles, "tile grid mismatch");
+        G.step1 = m->h.step;
+    }
     G.slabs.assign(1, {0, G.ntiles});
     // A single MAIN simulation on the split stepper runs the H and E kernels
     // slab by slab (whole z chunks, about slab_bytes of field and material
@@ -461,6 +474,8 @@ static void launch_kernel(Group &G, KernelId k, int t0 = 0, int t1 = -1) {
     const dim3 blk(kWave, G.ty * G.xw);
     const int lxw = G.xw == 4 ? 2 : G.xw == 2 ? 1 : 0;
     const int shape = lxw | (G.launch.xcd == 1 ? kShapeXcd : 0);
+    TileGrid tg = G.tg;
+    tg.t0 = t0;
     if (t1 < 0) t1 = G.ntiles;
     const int4 *tl = G.d_tiles + t0;
     const int nt = t1 - t0;
@@ -474,18 +489,24 @@ static void launch_kernel(Group &G, KernelId k, int t0 = 0, int t1 = -1) {
         hipLaunchKernelGGL(fn, grid, b4, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles, shape);
     } else if (k == KH) {
         if (G.xvec == 4)
-            hipLaunchKernelGGL(k_h_half<4>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape);
+            hipLaunchKernelGGL(k_h_half<4>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape, tg,
+                               G.step1);
         else if (G.xvec == 2)
-            hipLaunchKernelGGL(k_h_half<2>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape);
+            hipLaunchKernelGGL(k_h_half<2>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape, tg,
+                               G.step1);
         else
-            hipLaunchKernelGGL(k_h_half<1>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape);
+            hipLaunchKernelGGL(k_h_half<1>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape, tg,
+                               G.step1);
     } else if (k == KE) {
         if (G.xvec == 4)
-            hipLaunchKernelGGL(k_e_half<4>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape);
+            hipLaunchKernelGGL(k_e_half<4>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape, tg,
+                               G.step1);
         else if (G.xvec == 2)
-            hipLaunchKernelGGL(k_e_half<2>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape);
+            hipLaunchKernelGGL(k_e_half<2>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape, tg,
+                               G.step1);
         else
-            hipLaunchKernelGGL(k_e_half<1>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape);
+            hipLaunchKernelGGL(k_e_half<1>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape, tg,
+                               G.step1);
     } else {
         const unsigned gx = (unsigned)std::max(1LL, (G.max_dft_items + 255) / 256);
         hipLaunchKernelGGL(k_dft, dim3(gx, G.nmodels), dim3(256), 0, G.stream, G.d_descs);
@@ -675,6 +696,53 @@ extern "C" int fdtd_probe_streams(int64_t n, int nread, int nwrite, int reps, fl
     if (st) (void)hipStreamDestroy(st);
     if (in) (void)hipFree(in);
     if (out) (void)hipFree(out);
+    return rc;
+}
+
+extern "C" int fdtd_probe_stencil(int nx, int ny, int nz, int zchunk, int kind, int reps, float *ms_per_pass) {
+    CHECK(kind == 0 || kind == 1, "kind must be 0 (H half) or 1 (E half)");
+    CHECK(nx >= 1 && ny >= 1 && nz >= 1 && zchunk >= 1, "bad sizes");
+    CHECK(reps >= 1 && ms_per_pass, "bad reps / output");
+    const long long sy = pitch_of(nx), sz = sy * ny, ncell = sz * nz;
+    CHECK(ncell < (1LL << 30), "grid of %lld cells exceeds 2^30", ncell);
+    const long long stride = field_stride(ncell);
+    const int ntx = (nx + 255) / 256;
+    const long long blocks = (long long)ntx * ny * ((nz + zchunk - 1) / zchunk);
+    CHECK(blocks < (1LL << 31), "too many blocks");
+    HIP_TRY(hipSetDevice(g_device));
+    float *base = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int rc = FDTD_OK;
+    auto hip_ok = [&](hipError_t e, const char *what) {
+        if (e != hipSuccess && rc == FDTD_OK) rc = fail(FDTD_EHIP, "%s: %s", what, hipGetErrorString(e));
+        return rc == FDTD_OK;
+    };
+    const size_t bytes = (size_t)8 * stride * sizeof(float);
+    if (hip_ok(hipMalloc(&base, bytes), "hipMalloc") && hip_ok(hipStreamCreate(&st), "hipStreamCreate") &&
+        hip_ok(hipEventCreate(&ev[0]), "hipEventCreate") && hip_ok(hipEventCreate(&ev[1]), "hipEventCreate") &&
+        hip_ok(hipMemsetAsync(base, 0, bytes, st), "hipMemsetAsync")) {
+        for (int r = 0; r <= reps && rc == FDTD_OK; ++r) {   // r = 0: warm-up
+            if (r == 1) hip_ok(hipEventRecord(ev[0], st), "hipEventRecord");
+            if (kind == 0)
+                hipLaunchKernelGGL(k_stencil_probe<0>, dim3((unsigned)blocks), dim3(256), 0, st, base, stride, nx,
+                                   ny, nz, (uint32_t)sy, (uint32_t)sz, zchunk, ntx);
+            else
+                hipLaunchKernelGGL(k_stencil_probe<1>, dim3((unsigned)blocks), dim3(256), 0, st, base, stride, nx,
+                                   ny, nz, (uint32_t)sy, (uint32_t)sz, zchunk, ntx);
+            hip_ok(hipGetLastError(), "k_stencil_probe");
+        }
+        float ms = 0;
+        if (hip_ok(hipEventRecord(ev[1], st), "hipEventRecord") &&
+            hip_ok(hipEventSynchronize(ev[1]), "hipEventSynchronize") &&
+            hip_ok(hipEventElapsedTime(&ms, ev[0], ev[1]), "hipEventElapsedTime"))
+            *ms_per_pass = ms / reps;
+    }
+    if (st) hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize");
+    for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (st) (void)hipStreamDestroy(st);
+    if (base) (void)hipFree(base);
     return rc;
 }
 

@@ -88,6 +88,13 @@ struct Launch {
 // Shape word bit of the split and strip kernels: XCD-contiguous block order.
 constexpr int kShapeXcd = 4;
 
+// Tile grid of a single simulation's split-kernel launch (ntx > 0): block
+// t0 + b covers x tile b % ntx of tx cells, rows (b / ntx) % nyb of ty, and
+// z chunk b / (ntx nyb) of zc planes.  ntx = 0: tiles come from the list.
+struct TileGrid {
+    int ntx, nyb, tx, ty, zc, nz, t0, pad;
+};
+
 // Buffer set holding the fields at step t.
 __host__ __device__ inline int current_set(const Desc &d, int t) { return d.fused ? (t & 1) : 0; }
 

@@ -447,31 +447,55 @@ __device__ __forceinline__ void h_plane_load(HPlane<V> &P, const gfloat *Ex, con
     }
 }
 
+// Tile of a split-kernel block: from the group's tile list, or, for a
+// single simulation (tg.ntx > 0), from the block index (no dependent load).
+// Returns false for a padding tile.
+__device__ __forceinline__ bool split_tile(const int4 *__restrict__ tiles, int ntiles, int shape,
+                                           const TileGrid &tg, int4 &tl) {
+    const int bt = block_tile(shape, ntiles);
+    if (tg.ntx > 0) {
+        const int b = bt + tg.t0;
+        const int xt = b % tg.ntx, r = b / tg.ntx;
+        const int yb = r % tg.nyb, zc = r / tg.nyb;
+        const int z0 = zc * tg.zc, z1 = min(tg.nz, z0 + tg.zc);
+        tl = make_int4(0, xt * tg.tx, yb * tg.ty, z0 | (z1 << 16));
+        return true;
+    }
+    tl = tiles[bt];
+    return tl.w != 0;                                  // padding tile of the banded order
+}
+
+// Both split kernels issue the step-counter load first (its pointer is a
+// kernel argument for a single simulation), then the loads of the tile's
+// first plane, which depend on the geometry only, and test the step after
+// that: a block's first memory round trip covers the counter, and no store
+// precedes the test.  The z loop then computes a plane, stores it and loads
+// the next one.
 template <int V>
 __global__ void __launch_bounds__(256)
-k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles, int shape) {
-    const int4 tl = tiles[block_tile(shape, ntiles)];
-    if (tl.w == 0) return;                             // padding tile of the banded order
+k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles, int shape, TileGrid tg,
+         const int *__restrict__ stepp) {
+    int4 tl;
+    if (!split_tile(tiles, ntiles, shape, tg, tl)) return;
     const Desc *__restrict__ d = descs + tl.x;
-    const int t = load_step(d->step);
-    if (t >= d->nt) return;                            // uniform per block
-    const StepK k = step_k(d, t);
+    const int t = load_step(stepp ? stepp : d->step);
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int lxw = shape & 3;
     const int y = tl.z + (w >> lxw);
-    if (y >= k.ny) return;                             // uniform per wavefront (one row)
+    const int nx = d->nx, ny = d->ny, nz = d->nz;
+    if (y >= ny) return;                               // uniform per wavefront (one row)
     const int lane = threadIdx.x;
-    const int xb = tl.y + V * (((w & ((1 << lxw) - 1)) << 6) + lane);                    // the lane's first cell
-    const bool act = xb < k.nx;
+    const int xb = tl.y + V * (((w & ((1 << lxw) - 1)) << 6) + lane);   // the lane's first cell
+    const bool act = xb < nx;
     // the x + 1 neighbour of the lane's last cell is needed when that cell
     // is below nx - 1; for lane 63 it lies in the next tile
-    const bool next_mem = lane == kWave - 1 && xb + V < k.nx;
+    const bool next_mem = lane == kWave - 1 && xb + V < nx;
     const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
-    const uint32_t sy = k.sy, sz = k.sz;
+    const uint32_t sy = (uint32_t)d->sy, sz = (uint32_t)d->sz;
     const gfloat *Ex = gp(d->f[0][0]), *Ey = gp(d->f[0][1]), *Ez = gp(d->f[0][2]);
     gfloat *Hx = gp(d->f[0][3]), *Hy = gp(d->f[0][4]), *Hz = gp(d->f[0][5]);
     gfloat *psiH = gp(d->psi[0][pHxz_d]);
-    const bool yin = y < k.ny - 1;
+    const bool yin = y < ny - 1;
 
     uint32_t o = (uint32_t)z0 * sz + (uint32_t)y * sy + (uint32_t)xb;
     float exc[V], eyc[V];
@@ -481,9 +505,11 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
         vld(exc, at(Ex, o));
         vld(eyc, at(Ey, o));
     }
-    for (int z = z0; z < z1; ++z, o += sz) {
-        HPlane<V> P;
-        h_plane_load(P, Ex, Ey, Ez, Hx, Hy, Hz, o, sy, sz, act, z < k.nz - 1, yin, next_mem);
+    HPlane<V> P;
+    h_plane_load(P, Ex, Ey, Ez, Hx, Hy, Hz, o, sy, sz, act, z0 < nz - 1, yin, next_mem);
+    if (t >= d->nt) return;                            // uniform per block
+    const StepK k = step_k(d, t);
+    for (int z = z0;;) {
         float ey_next = __shfl_down(eyc[0], 1, kWave);
         float ez_next = __shfl_down(P.ez[0], 1, kWave);
         if (lane == kWave - 1) {
@@ -507,8 +533,11 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
             vst(at(Hy, o), P.hy);
             vst(at(Hz, o), P.hz);
         }
+        if (++z >= z1) break;
+        o += sz;
         vcopy(exc, P.exn);
         vcopy(eyc, P.eyn);
+        h_plane_load(P, Ex, Ey, Ez, Hx, Hy, Hz, o, sy, sz, act, z < nz - 1, yin, next_mem);
     }
 }
 
@@ -557,23 +586,23 @@ __device__ __forceinline__ void e_plane_load(EPlane<V> &Q, const gfloat *Ex, con
 
 template <int V>
 __global__ void __launch_bounds__(256)
-k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles, int shape) {
-    const int4 tl = tiles[block_tile(shape, ntiles)];
-    if (tl.w == 0) return;                             // padding tile of the banded order
+k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles, int shape, TileGrid tg,
+         const int *__restrict__ stepp) {
+    int4 tl;
+    if (!split_tile(tiles, ntiles, shape, tg, tl)) return;
     const Desc *__restrict__ d = descs + tl.x;
-    const int t = load_step(d->step);
-    if (t >= d->nt) return;                            // uniform per block
-    const StepK k = step_k(d, t);
+    const int t = load_step(stepp ? stepp : d->step);
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int lxw = shape & 3;
     const int y = tl.z + (w >> lxw);
-    if (y >= k.ny) return;                             // uniform per wavefront (one row)
+    const int nx = d->nx, ny = d->ny, nz = d->nz;
+    if (y >= ny) return;                               // uniform per wavefront (one row)
     const int lane = threadIdx.x;
     const int xb = tl.y + V * (((w & ((1 << lxw) - 1)) << 6) + lane);
-    const bool act = xb < k.nx;
+    const bool act = xb < nx;
     const bool prev_mem = lane == 0 && xb >= 1 && act;    // x - 1 lies in the previous tile
     const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
-    const uint32_t sy = k.sy, sz = k.sz;
+    const uint32_t sy = (uint32_t)d->sy, sz = (uint32_t)d->sz;
     gfloat *Ex = gp(d->f[0][0]), *Ey = gp(d->f[0][1]), *Ez = gp(d->f[0][2]);
     const gfloat *Hx = gp(d->f[0][3]), *Hy = gp(d->f[0][4]), *Hz = gp(d->f[0][5]);
     const gfloat *Ca = gp(d->ca), *Cb = gp(d->cb);
@@ -589,10 +618,12 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
         vld(hxzm, at(Hx, o - sz));
         vld(hyzm, at(Hy, o - sz));
     }
-    for (int z = z0; z < z1; ++z, o += sz) {
-        EPlane<V> P;
-        e_plane_load(P, Ex, Ey, Ez, Hx, Hy, Hz, Ca, Cb, ca_u, cb_u, o, sy, act, z < k.nz - 1,
-                     z < k.nz - 1 && z < uni_z, ylo, prev_mem);
+    EPlane<V> P;
+    e_plane_load(P, Ex, Ey, Ez, Hx, Hy, Hz, Ca, Cb, ca_u, cb_u, o, sy, act, z0 < nz - 1,
+                 z0 < nz - 1 && z0 < uni_z, ylo, prev_mem);
+    if (t >= d->nt) return;                            // uniform per block
+    const StepK k = step_k(d, t);
+    for (int z = z0;;) {
         float hz_prev = __shfl_up(P.hz[V - 1], 1, kWave);
         float hy_prev = __shfl_up(P.hy[V - 1], 1, kWave);
         if (lane == 0) {
@@ -616,17 +647,21 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
             vst(at(Ey, o), P.ey);
             vst(at(Ez, o), P.ez);
         }
+        if (++z >= z1) break;
+        o += sz;
         vcopy(hxzm, P.hx);
         vcopy(hyzm, P.hy);
+        e_plane_load(P, Ex, Ey, Ez, Hx, Hy, Hz, Ca, Cb, ca_u, cb_u, o, sy, act, z < nz - 1,
+                     z < nz - 1 && z < uni_z, ylo, prev_mem);
     }
 }
 
-template __global__ void k_h_half<1>(const Desc *, const int4 *, int, int);
-template __global__ void k_h_half<2>(const Desc *, const int4 *, int, int);
-template __global__ void k_h_half<4>(const Desc *, const int4 *, int, int);
-template __global__ void k_e_half<1>(const Desc *, const int4 *, int, int);
-template __global__ void k_e_half<2>(const Desc *, const int4 *, int, int);
-template __global__ void k_e_half<4>(const Desc *, const int4 *, int, int);
+template __global__ void k_h_half<1>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_h_half<2>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_h_half<4>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_e_half<1>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_e_half<2>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_e_half<4>(const Desc *, const int4 *, int, int, TileGrid, const int *);
 
 // ---------------------------------------------------------------------------
 // Fused stepper (MAIN): one kernel per time step computes H(t+1/2) and then
@@ -1056,6 +1091,63 @@ k_stream_probe(const float *__restrict__ in, float *__restrict__ out, long long 
 #pragma unroll
     for (int s = 0; s < NW; ++s) out[s * n + i] = acc + (float)s;
 }
+
+// The split kernels' access pattern without their arithmetic, for the
+// practical ceiling of that pattern: the same arrays (field stride apart),
+// blocks of 256 lanes along x (1 KB per array), tiles in plain order, a z
+// march over zchunk planes with the plane-to-plane carry in registers, and
+// the y +- 1 row loads.  KIND 0 is the H half (reads ex ey ez hx hy hz, ex
+// and ez of row y+1, ex ey of plane z+1; writes hx hy hz), KIND 1 the E half
+// (reads ex ey ez hx hy hz ca cb, hz and hx of row y-1, hx hy of plane z-1;
+// writes ex ey ez).  Algorithmic streams: 9 and 11 floats per cell.
+template <int KIND>
+__global__ void __launch_bounds__(256)
+k_stencil_probe(float *__restrict__ base, long long stride, int nx, int ny, int nz, uint32_t sy, uint32_t sz,
+                int zchunk, int ntx) {
+    const int b = blockIdx.x;
+    const int xt = b % ntx, rest = b / ntx;
+    const int y = rest % ny, zc = rest / ny;
+    const int x = xt * 256 + (int)threadIdx.x;
+    if (x >= nx) return;
+    const int z0 = zc * zchunk, z1 = min(nz, z0 + zchunk);
+    gfloat *A0 = gp(base), *A1 = gp(base + stride), *A2 = gp(base + 2 * stride);
+    gfloat *A3 = gp(base + 3 * stride), *A4 = gp(base + 4 * stride), *A5 = gp(base + 5 * stride);
+    uint32_t o = (uint32_t)z0 * sz + (uint32_t)y * sy + (uint32_t)x;
+    if (KIND == 0) {
+        const bool yin = y + 1 < ny;
+        float c0 = *at(A0, o), c1 = *at(A1, o);
+        for (int z = z0; z < z1; ++z, o += sz) {
+            const bool zin = z + 1 < nz;
+            const float n0 = zin ? *at(A0, o + sz) : 0.f, n1 = zin ? *at(A1, o + sz) : 0.f;
+            const float e2 = *at(A2, o), h3 = *at(A3, o), h4 = *at(A4, o), h5 = *at(A5, o);
+            const float p0 = yin ? *at(A0, o + sy) : 0.f, p2 = yin ? *at(A2, o + sy) : 0.f;
+            const float acc = ((c0 - n0) + (c1 - n1)) + ((p0 - c0) + (p2 - e2));
+            *at(A3, o) = h3 - acc;
+            *at(A4, o) = h4 + acc;
+            *at(A5, o) = h5 * acc;
+            c0 = n0;
+            c1 = n1;
+        }
+    } else {
+        const gfloat *A6 = gp(base + 6 * stride), *A7 = gp(base + 7 * stride);
+        const bool ylo = y >= 1;
+        float m3 = z0 >= 1 ? *at(A3, o - sz) : 0.f, m4 = z0 >= 1 ? *at(A4, o - sz) : 0.f;
+        for (int z = z0; z < z1; ++z, o += sz) {
+            const float h3 = *at(A3, o), h4 = *at(A4, o), h5 = *at(A5, o);
+            const float e0 = *at(A0, o), e1 = *at(A1, o), e2 = *at(A2, o);
+            const float ca = *at(A6, o), cb = *at(A7, o);
+            const float q5 = ylo ? *at(A5, o - sy) : 0.f, q3 = ylo ? *at(A3, o - sy) : 0.f;
+            *at(A0, o) = ca * e0 + cb * ((h5 - q5) - (h4 - m4));
+            *at(A1, o) = ca * e1 + cb * ((h3 - m3) - h5);
+            *at(A2, o) = ca * e2 + cb * ((h4 - h3) - (h3 - q3));
+            m3 = h3;
+            m4 = h4;
+        }
+    }
+}
+
+template __global__ void k_stencil_probe<0>(float *, long long, int, int, int, uint32_t, uint32_t, int, int);
+template __global__ void k_stencil_probe<1>(float *, long long, int, int, int, uint32_t, uint32_t, int, int);
 
 template __global__ void k_stream_probe<1, 1>(const float *, float *, long long);
 template __global__ void k_stream_probe<6, 3>(const float *, float *, long long);

@@ -388,3 +388,23 @@ def test_device_beam_synthesis_matches_host(pol, theta):
         assert H.rel_l2(dev_ph[n], host_ph[n]) < 1e-4, n
     host_beam.release()
     dev_beam.release()
+
+
+def test_bandwidth_probes_run_and_reject_bad_arguments():
+    """The probes bench.py reports as practical ceilings run on small sizes
+    and reject unbuilt shapes with an error, not a launch."""
+    say('case: probes, small sizes')
+    ms, gbs = native.probe_copy(1 << 20, vec=1, reps=2)
+    assert ms > 0 and gbs > 0
+    ms, gbs = native.probe_streams(1 << 16, 6, 3, reps=2)
+    assert ms > 0 and gbs > 0
+    for kind in (0, 1):
+        for zc in (1, 4, 64):
+            ms, gbs = native.probe_stencil(301, 17, 9, zc, kind, reps=2)
+            assert ms > 0 and gbs > 0
+    with pytest.raises(native.FdtdError):
+        native.probe_stencil(64, 8, 8, 4, 2)
+    with pytest.raises(native.FdtdError):
+        native.probe_stencil(64, 8, 8, 0, 0)
+    with pytest.raises(native.FdtdError):
+        native.probe_streams(1 << 16, 5, 3)
