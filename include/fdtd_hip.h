@@ -107,6 +107,63 @@ int fdtd_sim_set_materials(FdtdSim *s, const float *Ca, const float *Cb);
 /* Same value in every cell (the incident simulation's vacuum,
  * cpu/incident_sim.py:115-116). */
 int fdtd_sim_set_uniform_material(FdtdSim *s, float ca, float cb);
+/* ---- material models built on the device ------------------------------ */
+/* No counterpart in the reference, whose generators fill (eps_r, sigma)
+ * volumes on the host (materials/soilmodels.py:16-298,
+ * materials/materials.py:112-160) and ship Ca, Cb to the device
+ * (gpu/main_sim.py:203-204).  Here a model is described by its layers,
+ * inclusions and int32 surface maps; the device fills eps_r and sigma per
+ * cell and forms Ca, Cb with compute_material_arrays' float32 arithmetic
+ * (materials/materials.py:23-68), so a request ships O(nx ny) bytes. */
+#define FDTD_MAX_LAYERS 4
+enum {
+    FDTD_LAYER_CONST = 0,     /* constant (eps, sigma) */
+    FDTD_LAYER_RANDOM = 1,    /* eps ~ N(eps, eps_std^2) clipped at 1, sigma ~ U(sigma, sigma_hi) per cell */
+    FDTD_LAYER_GRADIENT = 2   /* Hallikainen soil, moisture linear in z between the layer below's
+                                 surface and this layer's (soilmodels.py:233-247) */
+};
+
+typedef struct FdtdMatLayer {
+    int32_t kind;
+    uint32_t seed;            /* RANDOM: stream key of the per-cell draws */
+    double eps, sigma;        /* CONST: eps_r, sigma [S/m]; RANDOM: eps mean, sigma lower bound */
+    double eps_std, sigma_hi; /* RANDOM */
+    double mv_bottom, mv_top; /* GRADIENT: moisture at the lower and at this surface */
+    double freq;              /* GRADIENT: frequency [Hz] */
+    double eps0;              /* GRADIENT: eps0 of sigma = |eps''| eps0 2 pi f (scipy's value) */
+    double f0, f1;            /* GRADIENT: Hallikainen table frequencies around freq */
+    double poly[2][2][3];     /* GRADIENT: [f0, f1][real, imaginary][a, b, c] of eps = a + b mv + c mv^2 */
+} FdtdMatLayer;
+
+typedef struct FdtdMatCylinder {
+    double c[3], u[3];        /* centre [m] and unit axis, (z, y, x) */
+    double radius, length, eps, sigma;
+    int32_t lo[3], hi[3];     /* cell box [lo, hi) that is tested, (z, y, x) */
+} FdtdMatCylinder;
+
+typedef struct FdtdMatSpec {
+    int32_t nz_mat;           /* material planes; vacuum above (materials.py:42-47) */
+    int32_t nlayers;          /* 0..FDTD_MAX_LAYERS */
+    int32_t ncyl;
+    int32_t vox_z1;           /* Bernoulli(vox_frac) inclusions in planes z < vox_z1 */
+    double vox_frac, vox_eps, vox_sigma;
+    uint32_t vox_seed, pad0;
+    float dt, dx, eps0, pad1; /* float32 time step, cell size and eps0 of compute_material_arrays */
+    FdtdMatLayer layer[FDTD_MAX_LAYERS];
+} FdtdMatSpec;
+
+/* Build Ca, Cb of a MAIN simulation on the device.  Per cell, in order:
+ * vacuum; a voxel inclusion where z < vox_z1 and its draw is below
+ * vox_frac; the last cylinder whose axis segment lies within length/2 and
+ * radius of the cell centre (z dx, y dx, x dx); the first layer k with
+ * z <= surfaces[k][y][x].  Per-cell draws are SplitMix64 hashes of
+ * (seed, cell index, draw), normals by Box-Muller.  surfaces: nlayers
+ * (ny, nx) int32 maps; cyl: ncyl cylinders (host arrays, copied). */
+int fdtd_sim_build_materials(FdtdSim *s, const FdtdMatSpec *spec, const int32_t *surfaces,
+                             const FdtdMatCylinder *cyl);
+/* No counterpart in the reference (its arrays stay on the host).  Copy the
+ * simulation's Ca, Cb, each (nz, ny, nx) float32, to the host. */
+int fdtd_sim_get_materials(FdtdSim *s, float *Ca, float *Cb);
 /* INCIDENT: hard-source planes, ex and ey each (n_src, ny, nx) float32
  * (imaginary parts of the excitation, cpu/incident_sim.py:110-111,142-144). */
 int fdtd_sim_set_source(FdtdSim *s, const float *src_ex, const float *src_ey, int n_src);

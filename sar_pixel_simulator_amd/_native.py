@@ -42,7 +42,7 @@ FIELD_NAMES = ['ex', 'ey', 'ez', 'hx', 'hy', 'hz']
 EXPORTS = [
     'fdtd_abi_version', 'fdtd_last_error', 'fdtd_init', 'fdtd_device_info',
     'fdtd_sim_footprint', 'fdtd_sim_create', 'fdtd_sim_destroy',
-    'fdtd_sim_set_materials', 'fdtd_sim_set_uniform_material',
+    'fdtd_sim_set_materials', 'fdtd_sim_set_uniform_material', 'fdtd_sim_build_materials', 'fdtd_sim_get_materials',
     'fdtd_sim_set_source', 'fdtd_sim_attach_beam', 'fdtd_sim_reset',
     'fdtd_sim_run', 'fdtd_sim_sync', 'fdtd_sim_step_count',
     'fdtd_sim_get_phasor', 'fdtd_sim_get_field', 'fdtd_sim_set_field',
@@ -83,6 +83,37 @@ class FdtdCoeffs(ctypes.Structure):
                 ('prof', (ctypes.c_float * FDTD_MAX_PML) * 8)]
 
 
+MAX_LAYERS = 4
+LAYER_CONST, LAYER_RANDOM, LAYER_GRADIENT = 0, 1, 2
+
+
+class FdtdMatLayer(ctypes.Structure):
+    _fields_ = [('kind', ctypes.c_int32), ('seed', ctypes.c_uint32),
+                ('eps', ctypes.c_double), ('sigma', ctypes.c_double),
+                ('eps_std', ctypes.c_double), ('sigma_hi', ctypes.c_double),
+                ('mv_bottom', ctypes.c_double), ('mv_top', ctypes.c_double),
+                ('freq', ctypes.c_double), ('eps0', ctypes.c_double),
+                ('f0', ctypes.c_double), ('f1', ctypes.c_double),
+                ('poly', ((ctypes.c_double * 3) * 2) * 2)]
+
+
+class FdtdMatCylinder(ctypes.Structure):
+    _fields_ = [('c', ctypes.c_double * 3), ('u', ctypes.c_double * 3),
+                ('radius', ctypes.c_double), ('length', ctypes.c_double),
+                ('eps', ctypes.c_double), ('sigma', ctypes.c_double),
+                ('lo', ctypes.c_int32 * 3), ('hi', ctypes.c_int32 * 3)]
+
+
+class FdtdMatSpec(ctypes.Structure):
+    _fields_ = [('nz_mat', ctypes.c_int32), ('nlayers', ctypes.c_int32),
+                ('ncyl', ctypes.c_int32), ('vox_z1', ctypes.c_int32),
+                ('vox_frac', ctypes.c_double), ('vox_eps', ctypes.c_double),
+                ('vox_sigma', ctypes.c_double), ('vox_seed', ctypes.c_uint32),
+                ('pad0', ctypes.c_uint32), ('dt', ctypes.c_float), ('dx', ctypes.c_float),
+                ('eps0', ctypes.c_float), ('pad1', ctypes.c_float),
+                ('layer', FdtdMatLayer * MAX_LAYERS)]
+
+
 class FdtdBeamSpec(ctypes.Structure):
     _fields_ = [('polarization', ctypes.c_int32), ('cb_vacuum', ctypes.c_float),
                 ('theta_deg', ctypes.c_double), ('phi_deg', ctypes.c_double),
@@ -120,6 +151,9 @@ _SIGS = {
                          ctypes.POINTER(_P)], _I),
     'fdtd_sim_destroy': ([_P], None),
     'fdtd_sim_set_materials': ([_P, _FP, _FP], _I),
+    'fdtd_sim_get_materials': ([_P, _FP, _FP], _I),
+    'fdtd_sim_build_materials': ([_P, ctypes.POINTER(FdtdMatSpec), ctypes.POINTER(ctypes.c_int32),
+                                  ctypes.POINTER(FdtdMatCylinder)], _I),
     'fdtd_sim_set_uniform_material': ([_P, _F, _F], _I),
     'fdtd_sim_set_source': ([_P, _FP, _FP, _I], _I),
     'fdtd_sim_attach_beam': ([_P, _P], _I),
@@ -371,6 +405,31 @@ class Sim:
     def set_materials(self, Ca, Cb):
         Ca, Cb = _f32(Ca, self.shape), _f32(Cb, self.shape)
         _check(lib().fdtd_sim_set_materials(self._h, _fptr(Ca), _fptr(Cb)))
+
+    def build_materials(self, spec, surfaces=None, cylinders=None):
+        """Form Ca, Cb on the device (fdtd_sim_build_materials) from an
+        FdtdMatSpec, its int32 (nlayers, ny, nx) surface maps and an array
+        of FdtdMatCylinder."""
+        surf_p = ctypes.POINTER(ctypes.c_int32)()
+        if spec.nlayers > 0:
+            surfaces = np.ascontiguousarray(surfaces, dtype=np.int32)
+            if surfaces.shape != (spec.nlayers,) + self.shape[1:]:
+                raise ValueError(f'surfaces of shape {surfaces.shape}, expected '
+                                 f'{(spec.nlayers,) + self.shape[1:]}')
+            surf_p = surfaces.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        cyl_p = ctypes.POINTER(FdtdMatCylinder)()
+        if spec.ncyl > 0:
+            if cylinders is None or len(cylinders) != spec.ncyl:
+                raise ValueError('cylinder list does not match spec.ncyl')
+            cyl_p = cylinders
+        _check(lib().fdtd_sim_build_materials(self._h, ctypes.byref(spec), surf_p, cyl_p))
+
+    def materials(self):
+        """The device's (Ca, Cb), each (nz, ny, nx) float32."""
+        Ca = np.empty(self.shape, np.float32)
+        Cb = np.empty(self.shape, np.float32)
+        _check(lib().fdtd_sim_get_materials(self._h, _fptr(Ca), _fptr(Cb)))
+        return Ca, Cb
 
     def set_uniform_material(self, ca, cb):
         _check(lib().fdtd_sim_set_uniform_material(self._h, float(ca), float(cb)))

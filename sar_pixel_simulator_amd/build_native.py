@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB_NAME = 'libfdtd_hip.so'
 LIB_PATH = os.path.join(HERE, LIB_NAME)
-SOURCES = ['fdtd_api.hip', 'fdtd_kernels.hip', 'fdtd_beam.hip']
+SOURCES = ['fdtd_api.hip', 'fdtd_kernels.hip', 'fdtd_beam.hip', 'fdtd_materials.hip']
 LIBS = ['-L/opt/rocm/lib', '-lhipfft', '-Wl,-rpath,/opt/rocm/lib']
 ARCH = os.environ.get('FDTD_OFFLOAD_ARCH', 'gfx950')
 

@@ -34,6 +34,9 @@ template <int NR, int NW>
 __global__ void k_stream_probe(const float *, float *, long long);
 template <int KIND>
 __global__ void k_stencil_probe(float *, long long, int, int, int, uint32_t, uint32_t, int, int);
+__global__ void k_mat_cylinders(const FdtdMatCylinder *, double, int *, long long, long long);
+__global__ void k_mat_build(FdtdMatSpec, const int32_t *, const FdtdMatCylinder *, int, int, int, long long,
+                            long long, float *, float *);
 }  // namespace fdtd
 
 using namespace fdtd;
@@ -973,6 +976,78 @@ extern "C" int fdtd_sim_set_materials(FdtdSim *s, const float *Ca, const float *
     return sim_refresh(s);
 }
 
+extern "C" int fdtd_sim_build_materials(FdtdSim *s, const FdtdMatSpec *spec, const int32_t *surfaces,
+                                        const FdtdMatCylinder *cyl) {
+    CHECK(s && spec, "NULL argument");
+    CHECK(s->g.mode == FDTD_MODE_MAIN, "per-cell materials are for MAIN simulations");
+    const FdtdMatSpec &m = *spec;
+    const int nx = s->g.nx, ny = s->g.ny, nz = s->g.nz;
+    CHECK(m.nz_mat >= 0 && m.nz_mat <= nz, "nz_mat %d outside [0, %d]", m.nz_mat, nz);
+    CHECK(m.nlayers >= 0 && m.nlayers <= FDTD_MAX_LAYERS, "nlayers %d outside [0, %d]", m.nlayers,
+          FDTD_MAX_LAYERS);
+    CHECK(m.nlayers == 0 || surfaces, "surfaces is NULL");
+    CHECK(m.ncyl >= 0 && (m.ncyl == 0 || cyl), "bad cylinder list");
+    CHECK(m.vox_z1 >= 0 && m.vox_z1 <= m.nz_mat, "vox_z1 %d outside [0, nz_mat]", m.vox_z1);
+    CHECK(m.vox_frac >= 0.0 && m.vox_frac <= 1.0, "vox_frac outside [0, 1]");
+    CHECK(m.dt > 0.0f && m.dx > 0.0f && m.eps0 > 0.0f, "dt, dx and eps0 must be positive");
+    for (int k = 0; k < m.nlayers; ++k) {
+        const FdtdMatLayer &L = m.layer[k];
+        CHECK(L.kind == FDTD_LAYER_CONST || L.kind == FDTD_LAYER_RANDOM || L.kind == FDTD_LAYER_GRADIENT,
+              "layer %d: unknown kind %d", k, L.kind);
+        CHECK(L.kind != FDTD_LAYER_GRADIENT || k >= 1, "layer %d: a gradient layer needs a layer below", k);
+        CHECK(L.kind != FDTD_LAYER_GRADIENT || (L.f0 < L.f1 && L.freq >= L.f0 && L.freq <= L.f1),
+              "layer %d: frequency outside its table interval", k);
+    }
+    for (int c = 0; c < m.ncyl; ++c)
+        CHECK(cyl[c].lo[0] >= 0 && cyl[c].lo[1] >= 0 && cyl[c].lo[2] >= 0 && cyl[c].hi[0] <= m.nz_mat &&
+                  cyl[c].hi[1] <= ny && cyl[c].hi[2] <= nx && cyl[c].lo[0] <= cyl[c].hi[0] &&
+                  cyl[c].lo[1] <= cyl[c].hi[1] && cyl[c].lo[2] <= cyl[c].hi[2],
+              "cylinder %d: cell box outside the material volume", c);
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->self.stream));
+    const size_t plane = (size_t)nx * ny;
+    int32_t *d_surf = nullptr;
+    FdtdMatCylinder *d_cyl = nullptr;
+    float *Ca = s->mat, *Cb = s->mat + s->ncell;
+    int rc = FDTD_OK;
+    auto hip_ok = [&](hipError_t e, const char *what) {
+        if (e != hipSuccess && rc == FDTD_OK) rc = fail(FDTD_EHIP, "%s: %s", what, hipGetErrorString(e));
+        return rc == FDTD_OK;
+    };
+    if (m.nlayers > 0 && hip_ok(hipMalloc(&d_surf, m.nlayers * plane * sizeof(int32_t)), "hipMalloc"))
+        hip_ok(hipMemcpyAsync(d_surf, surfaces, m.nlayers * plane * sizeof(int32_t), hipMemcpyHostToDevice,
+                              s->self.stream), "hipMemcpyAsync");
+    if (rc == FDTD_OK && m.ncyl > 0 &&
+        hip_ok(hipMalloc(&d_cyl, m.ncyl * sizeof(FdtdMatCylinder)), "hipMalloc") &&
+        hip_ok(hipMemcpyAsync(d_cyl, cyl, m.ncyl * sizeof(FdtdMatCylinder), hipMemcpyHostToDevice,
+                              s->self.stream), "hipMemcpyAsync") &&
+        hip_ok(hipMemsetAsync(Cb, 0, s->ncell * sizeof(float), s->self.stream), "hipMemsetAsync")) {
+        // owners of the cylinder cells, in Cb's storage
+        hipLaunchKernelGGL(k_mat_cylinders, dim3(m.ncyl, 64), dim3(256), 0, s->self.stream, d_cyl,
+                           (double)s->c.cell_size, reinterpret_cast<int *>(Cb), (long long)s->pitch,
+                           (long long)s->pitch * ny);
+        hip_ok(hipGetLastError(), "k_mat_cylinders");
+    }
+    if (rc == FDTD_OK) {
+        const long long n = (long long)plane * nz;
+        hipLaunchKernelGGL(k_mat_build, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->self.stream, m,
+                           d_surf, d_cyl, nx, ny, nz, (long long)s->pitch, (long long)s->pitch * ny, Ca, Cb);
+        hip_ok(hipGetLastError(), "k_mat_build");
+    }
+    hip_ok(hipStreamSynchronize(s->self.stream), "hipStreamSynchronize");
+    if (d_surf) (void)hipFree(d_surf);
+    if (d_cyl) (void)hipFree(d_cyl);
+    if (rc) return rc;
+    // planes from nz_mat up are the vacuum compute_material_arrays appends;
+    // its Ca, Cb by the same float32 operations
+    const float perm = 1.0f * m.eps0, loss = (0.0f * m.dt) / (2.0f * perm);
+    s->h.uni_z = m.nz_mat;
+    s->h.ca_u = (1.0f - loss) / (1.0f + loss);
+    s->h.cb_u = (m.dt / (perm * m.dx)) / (1.0f + loss);
+    s->mat_set = true;
+    return sim_refresh(s);
+}
+
 extern "C" int fdtd_sim_set_uniform_material(FdtdSim *s, float ca, float cb) {
     CHECK(s, "NULL sim");
     s->h.uni_z = 0;
@@ -1109,6 +1184,16 @@ static int host_current_set(FdtdSim *s, int *set) {
     if (rc) return rc;
     *set = current_set(s->h, t);
     return FDTD_OK;
+}
+
+extern "C" int fdtd_sim_get_materials(FdtdSim *s, float *Ca, float *Cb) {
+    CHECK(s && Ca && Cb, "NULL argument");
+    CHECK(s->g.mode == FDTD_MODE_MAIN, "per-cell materials are for MAIN simulations");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->self.stream));
+    int rc = copy_volume_out(s, Ca, s->mat);
+    if (!rc) rc = copy_volume_out(s, Cb, s->mat + s->ncell);
+    return rc;
 }
 
 extern "C" int fdtd_sim_get_field(FdtdSim *s, int comp, float *out) {

@@ -20,7 +20,7 @@ CANDIDATES = {
               for r in rows for zc in (4, 8, 16, 32) for xo in (1, 0)],
     'split': [(ty, zc, xv, 1, 1) for xv in (4, 2, 1) for ty in (1, 2, 4) for zc in (8, 16, 32)]
              + [(ty, zc, xv, xw, xo) for xv in (1, 2) for xw, ty in ((4, 1), (2, 1), (2, 2))
-                for zc in (1, 2, 4, 8, 16) for xo in (0, 1)],
+                for zc in (1, 2, 3, 4, 6, 8, 16) for xo in (0, 1)],
 }
 # library defaults: 0 = chosen from the grid (split: one cell per lane,
 # four wavefronts per block, as many along x as the grid fills)

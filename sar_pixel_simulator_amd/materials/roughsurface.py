@@ -40,3 +40,38 @@ def gaussian_correlated_surface(nx, ny, dx, corr_len_x, corr_len_y,
     surf /= np.std(surf)
     surf *= height_rms
     return surf
+
+
+def two_layer_gaussian_correlated_surface(nx, ny, dx, corr_len_x, corr_len_y, height_rms,
+                                          inter_layer_corr, seed=None, rabus_compat=False,
+                                          debug_plots=False):
+    """Two surfaces [m], shape (2, ny, nx), each with Gaussian
+    autocorrelation, correlated with each other by ``inter_layer_corr``
+    (materials/roughsurface.py:116-224).
+
+    White noise of shape (ny + ky - 1, nx + kx - 1, 2) from
+    ``default_rng(seed).normal`` is mixed per pixel by the Cholesky factor
+    of [[1, rho], [rho, 1]] (rho capped at 1 - 1e-5), each layer is convolved
+    with the Gaussian kernel ('valid', direct convolution), shifted to zero
+    mean, scaled to unit standard deviation and multiplied by the rms height.
+    """
+    if rabus_compat:
+        corr_len_x /= np.sqrt(2.0)
+        corr_len_y /= np.sqrt(2.0)
+    rho = min(inter_layer_corr, 1.0 - 1e-5)
+    kernel = gaussian_kernel(dx, corr_len_x, corr_len_y)
+    ky, kx = kernel.shape
+    rng = np.random.default_rng(seed=seed)
+    noise = rng.normal(0, 1, size=(ny + ky - 1, nx + kx - 1, 2))
+    chol = np.linalg.cholesky(np.array([[1.0, rho], [rho, 1.0]]))
+    # per pixel: chol @ (n0, n1); chol is lower triangular
+    layers = np.empty_like(noise)
+    layers[..., 0] = chol[0, 0] * noise[..., 0] + chol[0, 1] * noise[..., 1]
+    layers[..., 1] = chol[1, 0] * noise[..., 0] + chol[1, 1] * noise[..., 1]
+    surf = np.zeros((2, ny, nx))
+    for i in range(2):
+        corr = scipy.signal.convolve2d(layers[..., i], kernel, mode='valid')
+        surf[i] = corr - np.mean(corr)
+        surf[i] /= np.std(surf[i])
+        surf[i] *= height_rms
+    return surf
