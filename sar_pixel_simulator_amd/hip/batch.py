@@ -66,7 +66,7 @@ def beam_key(params):
 class DeviceModel:
     """One material model's simulation resident on the device."""
 
-    def __init__(self, params, Ca, Cb, beams: BeamCache, name='model', meta=None):
+    def __init__(self, params, Ca, Cb, beams: BeamCache, name='model', meta=None, spec=None):
         common.require_float32(params)
         self.params = params
         self.name = name
@@ -79,13 +79,23 @@ class DeviceModel:
         if g.size_t_inc > 0:
             self.beam, self.inc_phasors, self.p_inc = beams.get(params)
             self.sim.attach_beam(self.beam)
-        self.sim.set_materials(Ca, Cb)
+        if spec is not None:
+            from ..materials import device
+            self.sim.build_materials(*device.to_c(spec, params))
+        else:
+            self.sim.set_materials(Ca, Cb)
         self.steps_done = 0
 
     @classmethod
     def from_material(cls, params, material, beams, name=None):
         Ca, Cb = compute_material_arrays(params, material)
         return cls(params, Ca, Cb, beams, name or material.name, material.metadata)
+
+    @classmethod
+    def from_spec(cls, params, spec, beams, name=None):
+        """A model whose Ca, Cb are formed on the device from a
+        ``materials.device.MaterialSpec``."""
+        return cls(params, None, None, beams, name or spec.name, spec.metadata, spec=spec)
 
     @staticmethod
     def footprint(params):

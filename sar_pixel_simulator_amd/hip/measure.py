@@ -16,6 +16,7 @@ from .. import _native as native
 from .. import workloads
 from ..materials import compute_material_arrays, synthetic
 from . import batch as B
+from . import common
 from .serve import HipBackend, Request, Server, latency_stats
 
 
@@ -86,11 +87,15 @@ def batched_vs_plain(count=8, chunk=64, device=0, log=print):
 
 
 def serving_requests(n, rate_hz, seed=7, lateral=(4.0, 6.0), height=(2.0, 3.0),
-                     thetas=(20.0, 30.0, 40.0), pols=('HH', 'VV'), cell=0.025):
+                     thetas=(20.0, 30.0, 40.0), pols=('HH', 'VV'), cell=0.025,
+                     device_materials=False):
     """Poisson arrivals of vegetation-canopy-over-soil requests.  Lateral
     sizes and heights are drawn from a few discrete values so that beams are
     shared between requests (a beam is keyed by grid size, angle and
-    polarisation)."""
+    polarisation).  With ``device_materials`` a request carries a
+    MaterialSpec (surface map and cylinder list) that the device expands,
+    instead of the (nz, ny, nx) permittivity and conductivity volumes."""
+    from ..materials import device
     rng = np.random.default_rng(seed)
     t = 0.0
     reqs = []
@@ -105,18 +110,63 @@ def serving_requests(n, rate_hz, seed=7, lateral=(4.0, 6.0), height=(2.0, 3.0),
         workloads.set_geom_params(pc, lat, lat, h, 12.0, cell, 'wavelength')
         g = pc.geom_params
         soil = max(2, g.size_z_material // 3)
-        m = synthetic.canopy_over_soil(g.size_x, g.size_y, g.size_z_material, g.cell_size,
-                                       workloads.lam_of(pc), soil, seed=1000 + k,
-                                       density=float(rng.uniform(2.0, 6.0)))
-        reqs.append(Request(rid=k, params=pc, material=m, name=f'req{k}', arrival=t))
+        density = float(rng.uniform(2.0, 6.0))
+        if device_materials:
+            spec = device.canopy_over_soil_spec(pc, soil, seed=1000 + k, density=density)
+            reqs.append(Request(rid=k, params=pc, spec=spec, name=f'req{k}', arrival=t))
+        else:
+            m = synthetic.canopy_over_soil(g.size_x, g.size_y, g.size_z_material, g.cell_size,
+                                           workloads.lam_of(pc), soil, seed=1000 + k,
+                                           density=density)
+            reqs.append(Request(rid=k, params=pc, material=m, name=f'req{k}', arrival=t))
     return reqs
 
 
-def serving(n=24, rate_hz=None, budget_gb=None, max_slots=16, chunk=48, device=0, log=print):
+def material_setup(n=4, lateral=8.0, height=4.0, cell=0.025, device=0, log=print):
+    """Time to have a canopy-over-soil model's Ca, Cb resident, per model:
+    host path (generate the volumes, compute_material_arrays, upload) vs
+    device path (surface map and cylinder list to the device builder), and
+    the bytes each ships."""
+    from ..materials import compute_material_arrays, device as dev
+    native.init(device)
+    pc = workloads.base_params(polarization='VV', theta=30.0, material_fraction=0.6)
+    workloads.set_geom_params(pc, lateral, lateral, height, 12.0, cell, 'wavelength')
+    g = pc.geom_params
+    soil = max(2, g.size_z_material // 3)
+    sim = native.Sim(common.main_geom(pc), common.coeffs_for(pc))
+    t_host = t_dev = 0.0
+    host_bytes = dev_bytes = 0
+    for k in range(n):
+        t0 = time.perf_counter()
+        m = synthetic.canopy_over_soil(g.size_x, g.size_y, g.size_z_material, g.cell_size,
+                                       workloads.lam_of(pc), soil, seed=2000 + k, density=4.0)
+        Ca, Cb = compute_material_arrays(pc, m)
+        sim.set_materials(Ca, Cb)
+        sim.sync()
+        t_host += time.perf_counter() - t0
+        host_bytes = Ca.nbytes + Cb.nbytes
+        t0 = time.perf_counter()
+        spec = dev.canopy_over_soil_spec(pc, soil, seed=2000 + k, density=4.0)
+        sim.build_materials(*dev.to_c(spec, pc))
+        sim.sync()
+        t_dev += time.perf_counter() - t0
+        dev_bytes = spec.payload_bytes()
+    sim.destroy()
+    out = {'grid': [g.size_x, g.size_y, g.size_z], 'models': n,
+           'host_path_s_per_model': round(t_host / n, 4),
+           'device_path_s_per_model': round(t_dev / n, 4),
+           'speedup': round(t_host / t_dev, 2),
+           'host_bytes_shipped': int(host_bytes), 'device_bytes_shipped': int(dev_bytes)}
+    log(f'material_setup: {out}')
+    return out
+
+
+def serving(n=24, rate_hz=None, budget_gb=None, max_slots=16, chunk=48, device=0, log=print,
+            device_materials=False):
     """Serve ``n`` Poisson requests; the rate defaults to 1.5x the measured
     one-at-a-time service rate so admission binds."""
     native.init(device)
-    reqs = serving_requests(n, rate_hz=1.0)
+    reqs = serving_requests(n, rate_hz=1.0, device_materials=device_materials)
     be = HipBackend(device)
     for r in reqs:                         # beams first: reported separately
         be.beams.get(r.params)
@@ -145,6 +195,7 @@ def serving(n=24, rate_hz=None, budget_gb=None, max_slots=16, chunk=48, device=0
     be.beams.release()
     st = latency_stats(res)
     st.update({'requests': n, 'errors': sum(r.error is not None for r in res),
+               'materials': 'built on the device' if device_materials else 'host volumes uploaded',
                'arrival_rate_hz': round(rate_hz, 3), 'hbm_budget_bytes': budget,
                'peak_hbm_bytes': srv.peak_used, 'peak_members': srv.peak_slots,
                'chunks': srv.chunks, 'wall_s': round(wall, 2)})

@@ -32,7 +32,8 @@ import numpy as np
 class Request:
     rid: int
     params: Any                      # ParameterCollection with geometry set
-    material: Any = None             # MaterialModel, or None when Ca/Cb given
+    material: Any = None             # MaterialModel, or None when Ca/Cb or spec given
+    spec: Any = None                 # materials.device.MaterialSpec: built on the device
     Ca: Optional[np.ndarray] = None
     Cb: Optional[np.ndarray] = None
     name: str = ''
@@ -80,6 +81,8 @@ class HipBackend:
     def admit(self, req):
         from .batch import DeviceModel
         from ..materials import compute_material_arrays
+        if req.spec is not None:
+            return DeviceModel.from_spec(req.params, req.spec, self.beams, req.name)
         Ca, Cb = (req.Ca, req.Cb) if req.Ca is not None else \
             compute_material_arrays(req.params, req.material)
         return DeviceModel(req.params, Ca, Cb, self.beams, req.name)

@@ -41,6 +41,39 @@ def _request_params(k):
     return pc, m
 
 
+def test_serving_device_built_materials_mix_equals_alone():
+    """Requests that carry MaterialSpec's (built on the device at
+    admission): each result in the mix equals the same spec run alone."""
+    from sar_pixel_simulator_amd.materials import device
+    say('case: 3 spec requests alone, then served together')
+    native.init(0)
+    beams = B.BeamCache()
+    specs = []
+    for k in range(3):
+        pc, _ = _request_params(k)
+        g = pc.geom_params
+        specs.append((pc, device.canopy_over_soil_spec(pc, g.size_z_material // 2, seed=50 + k,
+                                                       density=20.0)))
+    alone = {}
+    for k, (pc, spec) in enumerate(specs):
+        dm = B.DeviceModel.from_spec(pc, spec, beams, name=f's{k}')
+        B.run_plain([dm])
+        alone[k] = dm.sigma0()
+        dm.release()
+    be = HipBackend(0)
+    be.beams = beams
+    srv = Server(be, hbm_budget=64 << 30, max_slots=2, chunk_steps=16)
+    for k, (pc, spec) in enumerate(specs):
+        srv.submit(Request(rid=k, params=pc, spec=spec, name=f's{k}', arrival=0.0))
+    srv.drain()
+    assert len(srv.results) == 3
+    for r in srv.results:
+        assert r.error is None, r.error
+        for key in ('sigma_h', 'sigma_v'):
+            assert np.array_equal(r.output[key], alone[r.rid][key]), (r.rid, key)
+    beams.release()
+
+
 def test_serving_mix_equals_alone():
     say('case: 4 requests alone, then served together')
     native.init(0)
