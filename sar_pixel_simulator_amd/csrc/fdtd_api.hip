@@ -108,7 +108,7 @@ static void beam_unref(FdtdBeam *b) {
 
 // Launch shape a simulation or batch starts with, per stepper.  Split:
 // one x cell per lane, blocks of four wavefronts as wide along x as the
-// grid fills, 4-plane chunks, plain block order (measured fastest at
+// grid fills, 3-plane chunks, plain block order (measured fastest at
 // 761 x 761 x 203).  Fused: 4-row strips, 8-plane chunks.
 static Launch default_launch(int fused) {
     Launch l;

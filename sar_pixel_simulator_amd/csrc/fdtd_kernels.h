@@ -79,7 +79,7 @@ struct alignas(16) Desc {
 // group_build).  The defaults are the split stepper's.
 struct Launch {
     int ty = 0;      // rows per block (split) or per wavefront (fused)
-    int zchunk = 4;  // z cells marched per tile (0 = whole column)
+    int zchunk = 3;  // z cells marched per tile (0 = whole column)
     int xvec = 0;    // x cells per lane
     int xw = 0;      // split stepper: wavefronts side by side along x per block
     int xcd = 0;     // 1: XCD-contiguous block order, 0: plain block order

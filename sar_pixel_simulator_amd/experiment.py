@@ -35,7 +35,10 @@ def get_fdtd_simulator(params: ParameterCollection):
 def run_experiment(params: ParameterCollection, materials: Iterable[MaterialModel]):
     """Simulate every material model, then post-process all results
     (experiment.py:41-97).  When the geometry is not set, it is taken from
-    the first model: shape (nz_mat, ny, nx) and its cell size."""
+    the first model: shape (nz_mat, ny, nx) and its cell size.  A model may
+    also be a ``materials.device.MaterialSpec`` (the geometry must then be
+    set): its Ca, Cb are formed on the device."""
+    from .materials.device import MaterialSpec
     defaultLogSetup()
     g = params.geom_params
     geom_specified = not (g.size_x == 0 and g.size_y == 0 and g.size_z == 0)
@@ -44,6 +47,11 @@ def run_experiment(params: ParameterCollection, materials: Iterable[MaterialMode
         sim = get_fdtd_simulator(params)(params)
         IncidentBeam(params).setup()
     for material in materials:
+        if isinstance(material, MaterialSpec):
+            if sim is None:
+                raise ValueError('a MaterialSpec needs the geometry set in the parameters')
+            sim.run_spec(material)
+            continue
         if sim is None:
             nz_mat, ny, nx = material.conductivity.shape
             dx = material.cell_size

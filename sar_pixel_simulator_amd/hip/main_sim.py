@@ -100,6 +100,29 @@ class MainSimHip:
         self.sim.sync()
         return inc
 
+    def simulate_spec(self, spec):
+        """As ``simulate``, with Ca/Cb formed on the device from a
+        ``materials.device.MaterialSpec``."""
+        from ..materials import device
+        inc = self._attach_beam()
+        self.sim.reset()
+        self.sim.build_materials(*device.to_c(spec, self.params))
+        self.sim.run(self.geom.nt)
+        self.sim.sync()
+        return inc
+
+    def run_spec(self, spec):
+        """``run`` for a model described by a MaterialSpec."""
+        t0 = time.monotonic()
+        out_name = results_filename(self.params, spec.name)
+        if os.path.exists(out_name):
+            MainSimHip.logger.info(f'{out_name} already exists, skipping')
+            return
+        inc = self.simulate_spec(spec)
+        self.save_results(spec.name, spec.metadata, inc)
+        MainSimHip.logger.info(f'Simulation of {spec.name} on GPU {self.gpu_num} '
+                               f'is complete ({time.monotonic() - t0:.3f}s)')
+
     def run(self, Ca: np.ndarray, Cb: np.ndarray, material_name: str, material_meta: dict):
         t0 = time.monotonic()
         out_name = results_filename(self.params, material_name)

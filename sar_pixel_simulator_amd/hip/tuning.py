@@ -24,7 +24,7 @@ CANDIDATES = {
 }
 # library defaults: 0 = chosen from the grid (split: one cell per lane,
 # four wavefronts per block, as many along x as the grid fills)
-DEFAULT = {'fused': (4, 8, 0, 1, 0), 'split': (0, 4, 0, 0, 0)}
+DEFAULT = {'fused': (4, 8, 0, 1, 0), 'split': (0, 3, 0, 0, 0)}
 
 
 def _shape(hit):
