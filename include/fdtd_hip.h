@@ -190,10 +190,12 @@ int fdtd_sim_set_field(FdtdSim *s, int comp, const float *in);
  * E d,u: Exz Eyz; l,r: Eyx Ezx; f,b: Exy Ezy; then H likewise. */
 int fdtd_sim_get_psi(FdtdSim *s, int index, float *out);
 /* No counterpart in the reference (its step is 26 fixed launches,
- * gpu/main_sim.py:216-355).  MAIN only, before the first step: 1 = fused
- * stepper (one kernel computes H then E per step; fields double-buffered),
- * 0 = split stepper (an H kernel, then an E kernel, in place; the
- * default).  Both give identical results. */
+ * gpu/main_sim.py:216-355).  MAIN only, before the first step: 0 = split
+ * stepper (an H kernel, then an E kernel, in place; the default); 1 = fused
+ * stepper on row strips (one kernel computes H then E per step, one
+ * wavefront walks a strip's rows; fields double-buffered); 2 = fused
+ * stepper with one row per wavefront (ty = 3 or 7 rows per block, xvec 1 or
+ * 2).  All give identical results. */
 int fdtd_sim_set_stepper(FdtdSim *s, int fused);
 /* No counterpart in the reference.  Tile shape: each lane owns xvec
  * consecutive x cells (1, 2, 4) and

@@ -494,8 +494,10 @@ class Sim:
                                          int(xcd_order)))
 
     def set_stepper(self, fused):
-        """True: fused H+E kernel per step; False: split H and E kernels."""
-        _check(lib().fdtd_sim_set_stepper(self._h, 1 if fused else 0))
+        """0 / False: split H and E kernels; 1 / True: fused H+E kernel per
+        step on row strips; 2: fused H+E kernel with one row per
+        wavefront."""
+        _check(lib().fdtd_sim_set_stepper(self._h, int(fused)))
 
     def time_steps(self, nsteps, per_kernel=True):
         """Run ``nsteps`` and return HIP-event times (ms) measured on the

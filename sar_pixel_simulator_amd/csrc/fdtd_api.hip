@@ -23,6 +23,8 @@ __global__ void k_dft(const Desc *);
 __global__ void k_advance(const Desc *, int);
 template <int V, int R>
 __global__ void k_step_strip(const Desc *, const int4 *, int, int);
+template <int V, int R>
+__global__ void k_step_rows(const Desc *, const int4 *, int, int);
 __global__ void k_phasor_out(const float4 *, long long, float2 *);
 __global__ void k_n2f_partial(const float2 *, const float2 *, const float2 *, const float2 *, int, int, int,
                               const double2 *, const double2 *, double *);
@@ -112,7 +114,12 @@ static void beam_unref(FdtdBeam *b) {
 // 761 x 761 x 203).  Fused: 4-row strips, 8-plane chunks.
 static Launch default_launch(int fused) {
     Launch l;
-    if (fused) {
+    if (fused == 2) {
+        l.ty = 7;
+        l.zchunk = 8;
+        l.xvec = 1;
+        l.xw = 1;
+    } else if (fused) {
         l.ty = 4;
         l.zchunk = 8;
         l.xw = 1;
@@ -353,7 +360,14 @@ static int group_build(Group &G) {
     CHECK(xv == 0 || xv == 1 || xv == 2 || xv == 4, "xvec must be 0, 1, 2 or 4 (got %d)", xv);
     CHECK(xw == 0 || xw == 1 || xw == 2 || xw == 4, "xwaves must be 0, 1, 2 or 4 (got %d)", xw);
     CHECK(G.launch.xcd >= 0 && G.launch.xcd <= 2, "block order must be 0, 1 or 2 (got %d)", G.launch.xcd);
-    if (G.fused) {
+    if (G.fused == 2) {
+        CHECK(ty == 3 || ty == 7, "row-parallel stepper: ty (rows per block) must be 3 or 7 (got %d)", ty);
+        CHECK(xw <= 1, "row-parallel stepper: xwaves must be 1");
+        CHECK(xv == 0 || xv == 1 || xv == 2, "row-parallel stepper: xvec must be 1 or 2 (got %d)", xv);
+        CHECK(G.launch.xcd != 2, "row-parallel stepper: block order must be 0 (plain) or 1 (XCD-contiguous)");
+        xw = 1;
+        G.xvec = xv ? xv : 1;
+    } else if (G.fused) {
         CHECK(ty == 2 || ty == 4 || ty == 8, "fused stepper: ty (rows per wavefront) must be 2, 4 or 8 (got %d)",
               ty);
         CHECK(xw <= 1, "fused stepper: one wavefront per strip (xwaves must be 1)");
@@ -370,7 +384,7 @@ static int group_build(Group &G) {
     }
     G.ty = ty;
     G.xw = xw;
-    if (G.fused) {
+    if (G.fused == 1) {
         // a block's four wavefronts keep (4 ty + 2) x 64 xvec floats each in
         // LDS; the built shapes stay within 64 KB per block
         if (!xv) G.xvec = std::min(G.xvec, ty == 2 ? 4 : ty == 4 ? 2 : 1);
@@ -482,7 +496,13 @@ static void launch_kernel(Group &G, KernelId k, int t0 = 0, int t1 = -1) {
     if (t1 < 0) t1 = G.ntiles;
     const int4 *tl = G.d_tiles + t0;
     const int nt = t1 - t0;
-    if (k == KF) {
+    if (k == KF && G.fused == 2) {
+        const int r = G.ty, v = G.xvec;
+        auto *fn = v == 2 ? (r == 7 ? &k_step_rows<2, 7> : &k_step_rows<2, 3>)
+                          : (r == 7 ? &k_step_rows<1, 7> : &k_step_rows<1, 3>);
+        hipLaunchKernelGGL(fn, dim3(G.ntiles), dim3(kWave, r + 1), 0, G.stream, G.d_descs, G.d_tiles, G.ntiles,
+                           shape);
+    } else if (k == KF) {
         const dim3 grid((G.ntiles + 3) / 4), b4(kWave, 4);
         auto *fn = &k_step_strip<1, 2>;
         const int r = G.ty, v = G.xvec;
@@ -1241,7 +1261,7 @@ extern "C" int fdtd_sim_get_psi(FdtdSim *s, int index, float *out) {
 
 extern "C" int fdtd_sim_set_stepper(FdtdSim *s, int fused) {
     CHECK(s, "NULL sim");
-    CHECK(fused == 0 || fused == 1, "fused must be 0 or 1");
+    CHECK(fused == 0 || fused == 1 || fused == 2, "stepper must be 0 (split), 1 (fused strips) or 2 (fused rows)");
     CHECK(!fused || s->g.mode == FDTD_MODE_MAIN, "the fused stepper runs MAIN simulations");
     int t = 0;
     int rc = fdtd_sim_step_count(s, &t);

@@ -905,6 +905,184 @@ k_step_strip(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused stepper, row-parallel form (stepper 2).  A block of R + 1
+// wavefronts owns R rows y0 .. y0 + R - 1 of 64 V cells over a z range;
+// wavefront w holds row y0 - 1 + w, so wavefront 0 recomputes the halo row
+// y0 - 1 (its H(t+1/2) is never stored).  Every wavefront marches z with its
+// row's E(t) at z+1 and H(t+1/2) at z in registers, as the split kernels
+// carry their planes; H(t+1/2) of the row below reaches the E update through
+// LDS, double-buffered by plane parity, with one barrier per plane.  Lane 0
+// recomputes H(t+1/2) at x0 - 1.  Waves whose row lies outside the grid keep
+// to the barriers and touch no memory.
+// ---------------------------------------------------------------------------
+template <int V, int R>
+__global__ void __launch_bounds__(64 * (R + 1))
+k_step_rows(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles, int shape) {
+    __shared__ float lds_h[2][R][2][kWave * V];        // [plane parity][row 0..R-1][hx, hz][cells]
+    const int4 tl = tiles[block_tile(shape, ntiles)];
+    const Desc *__restrict__ d = descs + tl.x;
+    const int t = load_step(d->step);
+    if (t >= d->nt) return;                            // uniform per block
+    const StepK k = step_k(d, t);
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int lane = threadIdx.x;
+    const int x0 = tl.y, y0 = tl.z;
+    const int y = y0 - 1 + w;
+    const bool row = y >= 0 && y < k.ny;               // uniform per wavefront
+    const bool own = w >= 1 && row;
+    const int xb = x0 + V * lane;
+    const bool act = row && xb < k.nx;
+    const bool next_mem = row && lane == kWave - 1 && xb + V < k.nx;
+    const bool halo_x = own && lane == 0 && x0 > 0;     // recompute H(t+1/2) at x0 - 1
+    const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
+    const int zs = z0 > 0 ? z0 - 1 : 0;                // plane z0 - 1: H only, not stored
+    const int p = t & 1;
+    const uint32_t sy = k.sy, sz = k.sz;
+    const gfloat *fo[6];
+    gfloat *fn[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        fo[c] = gp(d->f[p][c]);
+        fn[c] = gp(d->f[1 - p][c]);
+    }
+    const gfloat *psi_o = gp(d->psi[p][pHxz_d]);
+    gfloat *psi_n = gp(d->psi[1 - p][pHxz_d]);
+    const gfloat *Ca = gp(d->ca), *Cb = gp(d->cb);
+    const int uni_z = d->uni_z;
+    const float ca_u = d->ca_u, cb_u = d->cb_u;
+    const bool yin = y < k.ny - 1;
+    const bool ylo = y >= 1;
+
+    uint32_t o = (uint32_t)zs * sz + (uint32_t)(row ? y : 0) * sy + (uint32_t)xb;
+    float exc[V], eyc[V], hxzm[V], hyzm[V];            // E(t) at z; H(t+1/2) at z - 1
+    vset(exc, 0.f); vset(eyc, 0.f); vset(hxzm, 0.f); vset(hyzm, 0.f);
+    if (act) {
+        vld(exc, at(fo[0], o));
+        vld(eyc, at(fo[1], o));
+    }
+#pragma unroll 1
+    for (int z = zs; z < z1; ++z, o += sz) {
+        const bool zin = z < k.nz - 1;
+        const bool hplane = z < z0;
+        const bool upd = own && !hplane;               // this row's cells are updated and stored
+        // ---- loads: H half of the row, then what the E half adds
+        HPlane<V> P;
+        h_plane_load(P, fo[0], fo[1], fo[2], fo[3], fo[4], fo[5], o, sy, sz, act, zin, yin, next_mem);
+        float ezt[V], ca[V], cb[V];
+        vcopy(ezt, P.ez);
+        vset(ca, ca_u);
+        vset(cb, cb_u);
+        if (upd && act) {
+            if (!zin) vld(ezt, at(fo[2], o));          // top plane: E is copied
+            if (zin && z < uni_z) {
+                vld(ca, at(Ca, o));
+                vld(cb, at(Cb, o));
+            }
+        }
+        EAround eh;
+        float hy_h = 0.f, hz_h = 0.f;
+        if (halo_x && !hplane) {
+            const uint32_t oh = o - 1;
+            eh.ex_c = *at(fo[0], oh);
+            eh.ey_c = *at(fo[1], oh);
+            eh.ez_c = *at(fo[2], oh);
+            eh.ex_n = zin ? *at(fo[0], oh + sz) : 0.f;
+            eh.ex_yp = (zin && yin) ? *at(fo[0], oh + sy) : 0.f;
+            hy_h = *at(fo[4], oh);
+            hz_h = *at(fo[5], oh);
+        }
+        // ---- H(t+1/2) of the row
+        float ey_next = __shfl_down(eyc[0], 1, kWave);
+        float ez_next = __shfl_down(P.ez[0], 1, kWave);
+        if (lane == kWave - 1) {
+            ey_next = P.ey_x;
+            ez_next = P.ez_x;
+        }
+        gfloat *hw_psi = upd ? psi_n : nullptr;
+        if (row) {
+#pragma unroll
+            for (int c = 0; c < V; ++c) {
+                if (xb + c < k.nx) {
+                    EAround e;
+                    e.ex_c = exc[c]; e.ex_n = P.exn[c]; e.ex_yp = P.exyp[c];
+                    e.ey_c = eyc[c]; e.ey_n = P.eyn[c];
+                    e.ey_xp = c + 1 < V ? eyc[c + 1 < V ? c + 1 : 0] : ey_next;
+                    e.ez_c = P.ez[c]; e.ez_yp = P.ezyp[c];
+                    e.ez_xp = c + 1 < V ? P.ez[c + 1 < V ? c + 1 : 0] : ez_next;
+                    h_cell(d, k, xb + c, y, z, e, P.hx[c], P.hy[c], P.hz[c], psi_o, hw_psi);
+                }
+            }
+        }
+        const int par = z & 1;
+        if (w < R) {                                   // the row above reads hx, hz of this row
+            lds_st(&lds_h[par][w][0][V * lane], P.hx);
+            lds_st(&lds_h[par][w][1][V * lane], P.hz);
+        }
+        __syncthreads();
+        if (upd) {
+            // ---- E(t+1) of the row
+            float hxym[V], hzym[V];
+            vset(hxym, 0.f);
+            vset(hzym, 0.f);
+            if (ylo) {
+                lds_ld(hxym, &lds_h[par][w - 1][0][V * lane]);
+                lds_ld(hzym, &lds_h[par][w - 1][1][V * lane]);
+            }
+            float hz_prev = __shfl_up(P.hz[V - 1], 1, kWave);
+            float hy_prev = __shfl_up(P.hy[V - 1], 1, kWave);
+            if (lane == 0) {
+                hz_prev = 0.f;
+                hy_prev = 0.f;
+                if (halo_x) {
+                    eh.ey_xp = eyc[0];
+                    eh.ez_xp = P.ez[0];
+                    eh.ey_n = eh.ey_c;
+                    eh.ez_yp = eh.ez_c;
+                    float hx_h = 0.f;
+                    h_cell(d, k, x0 - 1, y, z, eh, hx_h, hy_h, hz_h, psi_o, nullptr);
+                    hz_prev = hz_h;
+                    hy_prev = hy_h;
+                }
+            }
+            float exu[V], eyu[V], ezu[V];
+            vcopy(exu, exc);
+            vcopy(eyu, eyc);
+            vcopy(ezu, ezt);
+#pragma unroll
+            for (int q = 0; q < V; ++q) {
+                if (xb + q < k.nx) {
+                    HAround h;
+                    h.hx_c = P.hx[q]; h.hy_c = P.hy[q]; h.hz_c = P.hz[q];
+                    h.hx_zm = hxzm[q]; h.hy_zm = hyzm[q];
+                    h.hz_ym = hzym[q]; h.hx_ym = hxym[q];
+                    h.hz_xm = q > 0 ? P.hz[q > 0 ? q - 1 : 0] : hz_prev;
+                    h.hy_xm = q > 0 ? P.hy[q > 0 ? q - 1 : 0] : hy_prev;
+                    e_cell(d, k, xb + q, y, z, ca[q], cb[q], h, exu[q], eyu[q], ezu[q]);
+                }
+            }
+            if (act) {
+                vst(at(fn[3], o), P.hx);
+                vst(at(fn[4], o), P.hy);
+                vst(at(fn[5], o), P.hz);
+                vst(at(fn[0], o), exu);
+                vst(at(fn[1], o), eyu);
+                vst(at(fn[2], o), ezu);
+            }
+        }
+        // hand over to the plane above
+        vcopy(exc, P.exn);
+        vcopy(eyc, P.eyn);
+        vcopy(hxzm, P.hx);
+        vcopy(hyzm, P.hy);
+    }
+}
+
+template __global__ void k_step_rows<1, 3>(const Desc *, const int4 *, int, int);
+template __global__ void k_step_rows<1, 7>(const Desc *, const int4 *, int, int);
+template __global__ void k_step_rows<2, 3>(const Desc *, const int4 *, int, int);
+template __global__ void k_step_rows<2, 7>(const Desc *, const int4 *, int, int);
+
 template __global__ void k_step_strip<1, 2>(const Desc *, const int4 *, int, int);
 template __global__ void k_step_strip<1, 4>(const Desc *, const int4 *, int, int);
 template __global__ void k_step_strip<1, 8>(const Desc *, const int4 *, int, int);

@@ -56,7 +56,7 @@ def soil60():
     return pc, Ca, Cb, beam, ref
 
 
-@pytest.mark.parametrize('fused', [False, True], ids=['split', 'fused'])
+@pytest.mark.parametrize('fused', [False, True, 2], ids=['split', 'fused', 'rows'])
 def test_fields_psi_phasors_bitwise(soil60, fused):
     pc, Ca, Cb, beam, ref = soil60
     say('case: soil 33x31x35, 60 steps, reference DFT, fused', fused)
@@ -93,7 +93,10 @@ def test_fields_psi_phasors_bitwise(soil60, fused):
                                           (False, (1, 0, 4, 4, 1)), (False, (2, 3, 2, 2, 1)),
                                           (True, (4, 5, 1, 1, 0)), (True, (2, 0, 4, 1, 0)),
                                           (False, (1, 4, 1, 4, 2)), (False, (2, 3, 1, 2, 2)),
-                                          (False, (1, 0, 2, 4, 2)), (False, (4, 1, 1, 1, 2))])
+                                          (False, (1, 0, 2, 4, 2)), (False, (4, 1, 1, 1, 2)),
+                                          (2, (3, 0, 1, 1, 0)), (2, (7, 5, 1, 1, 0)),
+                                          (2, (3, 1, 2, 1, 1)), (2, (7, 8, 2, 1, 0)),
+                                          (2, (3, 16, 0, 1, 0))])
 def test_launch_shapes_bitwise(soil60, fused, launch):
     pc, Ca, Cb, beam, ref = soil60
     say('case: fused', fused, 'launch', launch)
@@ -151,8 +154,8 @@ def wide30():
     return pc, Ca, Cb, beam, H.oracle_main(pc, Ca, Cb, beam)
 
 
-@pytest.mark.parametrize('fused,xwaves', [(False, 1), (True, 1), (False, 4), (False, 2)],
-                         ids=['split', 'fused', 'split-4-waves', 'split-2-waves'])
+@pytest.mark.parametrize('fused,xwaves', [(False, 1), (True, 1), (False, 4), (False, 2), (2, 1)],
+                         ids=['split', 'fused', 'split-4-waves', 'split-2-waves', 'rows'])
 @pytest.mark.parametrize('xvec', [4, 2, 1])
 def test_wide_grid_lane_widths_bitwise(wide30, fused, xwaves, xvec):
     # 301 cells in x: two 256-cell tiles at 4 cells per lane (the second
@@ -161,6 +164,10 @@ def test_wide_grid_lane_widths_bitwise(wide30, fused, xwaves, xvec):
     pc, Ca, Cb, beam, ref = wide30
     say('case: wide 301x25x27, 30 steps, fused', fused, 'xvec', xvec, 'xwaves', xwaves)
     launch = (2, 8, xvec) if xwaves == 1 else (4 // xwaves, 2, xvec, xwaves, 2 if xvec == 1 else 0)
+    if fused == 2:
+        if xvec == 4:
+            pytest.skip('the row-parallel stepper takes 1 or 2 cells per lane')
+        launch = (3, 4, xvec, 1, 0)
     sim = make_sim(pc, Ca, Cb, beam, launch=launch, fused=fused)
     sim.run(30)
     sim.sync()
@@ -220,24 +227,26 @@ def test_split_and_fused_agree_long_run():
     pc = workloads.plain_params()
     Ca, Cb = compute_material_arrays(pc, workloads.plain_material(pc, 2))
     beam = H.synthetic_beam(pc)
-    say('case: 81^3, 300 steps, split vs fused')
+    say('case: 81^3, 300 steps, split vs both fused steppers')
     out = []
-    for fused in (False, True):
+    for fused in (False, True, 2):
         sim = make_sim(pc, Ca, Cb, beam, fused=fused)
         sim.run(300)
         sim.sync()
         out.append(({n: sim.field(n) for n in native.FIELD_NAMES}, sim.phasors()))
         sim.destroy()
-    for n in native.FIELD_NAMES:
-        assert_same(out[1][0][n], out[0][0][n], n)
-    for n in native.PHASOR_NAMES:
-        assert_same(out[1][1][n], out[0][1][n], n)
+    for other in out[1:]:
+        for n in native.FIELD_NAMES:
+            assert_same(other[0][n], out[0][0][n], n)
+        for n in native.PHASOR_NAMES:
+            assert_same(other[1][n], out[0][1][n], n)
 
 
 @pytest.mark.parametrize('fused,launch', [(True, None), (False, None), (False, (2, 8, 4)),
-                                          (False, (1, 1, 1, 4, 0)), (False, (1, 4, 1, 4, 2))],
+                                          (False, (1, 1, 1, 4, 0)), (False, (1, 4, 1, 4, 2)),
+                                          (2, None)],
                          ids=['fused', 'split-auto', 'split-4-per-lane', 'split-4-waves-flat',
-                              'split-banded'])
+                              'split-banded', 'rows'])
 def test_batch_equals_solo(fused, launch):
     cases = [H.soil_case(nx=33, ny=31, nz=35, nz_mat=14, steps=30, seed=s) for s in (1, 2)]
     cases.append(H.soil_case(nx=45, ny=39, nz=37, nz_mat=15, steps=30, seed=5))
