@@ -240,7 +240,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-extras', action='store_true',
                     help='skip the batched-vs-plain and serving legs')
-    ap.add_argument('--stepper', choices=['split', 'fused'], default=None,
+    ap.add_argument('--stepper', choices=['split', 'fused', 'rows'], default=None,
                     help='time-step kernels (default: the library default)')
     ap.add_argument('--ty', type=int, default=0)
     ap.add_argument('--zchunk', type=int, default=-1)
@@ -278,7 +278,10 @@ def main():
 
     native.init(0)
     info = native.device_info()
-    fused = common.FUSED_STEPPER if args.stepper is None else args.stepper == 'fused'
+    fused = (int(common.FUSED_STEPPER) if args.stepper is None
+             else {'split': 0, 'fused': 1, 'rows': 2}[args.stepper])
+    stepper_name = tuning.STEPPERS[fused]
+    fused_kernel = 'k_step_rows' if fused == 2 else 'k_step_strip'
     sim = native.Sim(common.main_geom(pc), common.coeffs_for(pc))
     sim.set_stepper(fused)
     ty, zc, xv, xw, xo = tuning.launch_for(g.size_x, g.size_y, g.size_z, fused=fused)
@@ -335,8 +338,8 @@ def main():
         # one kernel does both halves: E and H each read once and written
         # once (the H-half reads of E and E-half reads of H are gone)
         fb = ab['h'] + ab['e'] - 24 * ab['cells']
-        kern = {'name': 'k_step_strip', 'bytes': fb, 'ms': k['h']}
-        per_kernel = {'k_step_strip_ms': round(k['h'], 4), 'k_dft_ms': round(k['dft'], 4),
+        kern = {'name': fused_kernel, 'bytes': fb, 'ms': k['h']}
+        per_kernel = {fused_kernel + '_ms': round(k['h'], 4), 'k_dft_ms': round(k['dft'], 4),
                       'fused_GBs': gbs(fb, k['h']), 'dft_GBs': gbs(ab['dft'], k['dft'])}
         step_alg = fb + ab['dft']
     else:
@@ -369,7 +372,7 @@ def main():
     traffic, tdetail = None, {}
     if not args.no_traffic:
         print('bench: PMC traffic passes (FETCH_SIZE, WRITE_SIZE)', file=sys.stderr, flush=True)
-        child = ['--steps', '4', '--warmup', '1', '--stepper', 'fused' if fused else 'split',
+        child = ['--steps', '4', '--warmup', '1', '--stepper', stepper_name,
                  '--ty', str(ty), '--zchunk', str(zc), '--xvec', str(xv),
                  '--xwaves', str(xw), '--xcd-order', str(xo)]
         traffic, tdetail = measure_traffic(kern['name'], child, args.pmc_dir)
@@ -416,7 +419,7 @@ def main():
            'config': {'workload': 'single large plain run: 761x761x203 cells (16x16x3 '
                                   'lambda at 0.021 lambda, README shape), snow over rough '
                                   'soil, 5.4 GHz HH theta 20, 1714-step run',
-                      'cells': cells, 'stepper': 'fused' if fused else 'split',
+                      'cells': cells, 'stepper': stepper_name,
                       'launch': dict(zip(LAUNCH_KEYS, launch)),
                       'uniform_planes_from_z': int(uni_z),
                       'device': info['name'], 'arch': info['arch'],

@@ -21,10 +21,17 @@ CANDIDATES = {
     'split': [(ty, zc, xv, 1, 1) for xv in (4, 2, 1) for ty in (1, 2, 4) for zc in (8, 16, 32)]
              + [(ty, zc, xv, xw, xo) for xv in (1, 2) for xw, ty in ((4, 1), (2, 1), (2, 2))
                 for zc in (1, 2, 3, 4, 6, 8, 16) for xo in (0, 1)],
+    'rows': [(r, zc, xv, 1, xo) for r in (3, 7) for zc in (4, 8, 16, 32) for xv in (1, 2)
+             for xo in (0, 1)],
 }
 # library defaults: 0 = chosen from the grid (split: one cell per lane,
 # four wavefronts per block, as many along x as the grid fills)
-DEFAULT = {'fused': (4, 8, 0, 1, 0), 'split': (0, 3, 0, 0, 0)}
+DEFAULT = {'fused': (4, 8, 0, 1, 0), 'split': (0, 3, 0, 0, 0), 'rows': (7, 16, 2, 1, 0)}
+STEPPERS = {0: 'split', 1: 'fused', 2: 'rows'}
+
+
+def _stepper(fused):
+    return STEPPERS[int(fused)]
 
 
 def _shape(hit):
@@ -33,7 +40,7 @@ def _shape(hit):
 
 
 def _key(nx, ny, nz, fused, arch='gfx950'):
-    return f'{"fused" if fused else "split"}:{arch}:{nx}x{ny}x{nz}'
+    return f'{_stepper(fused)}:{arch}:{nx}x{ny}x{nz}'
 
 
 def _load():
@@ -49,7 +56,7 @@ def launch_for(nx, ny, nz, fused=True, arch='gfx950'):
     hit = _load().get(_key(nx, ny, nz, fused, arch))
     if hit:
         return _shape(hit)
-    return DEFAULT['fused' if fused else 'split']
+    return DEFAULT[_stepper(fused)]
 
 
 def autotune(sim, nx, ny, nz, fused=True, arch='gfx950', steps=10, candidates=None, save=True):
@@ -58,7 +65,7 @@ def autotune(sim, nx, ny, nz, fused=True, arch='gfx950', steps=10, candidates=No
     {shape: ms_per_step})."""
     from .._native import FdtdError
     results = {}
-    for shape in candidates or CANDIDATES['fused' if fused else 'split']:
+    for shape in candidates or CANDIDATES[_stepper(fused)]:
         try:
             sim.set_launch(*shape)
         except FdtdError:
@@ -114,9 +121,9 @@ if __name__ == '__main__':
     import sys
     args = sys.argv[1:] or ['large']
     for name in args:
-        for fused in (True, False):
+        for fused in (1, 0, 2):
             best, res = _tune_workload(name, fused)
-            tag = 'fused' if fused else 'split'
+            tag = _stepper(fused)
             for shape, ms in sorted(res.items(), key=lambda kv: kv[1]):
                 print(f'{name} {tag} ty={shape[0]} zchunk={shape[1]} xvec={shape[2]} '
                       f'xwaves={shape[3]} xcd_order={shape[4]}: {ms:.4f} ms/step', flush=True)
