@@ -15,9 +15,9 @@
 #include "fdtd_kernels.h"
 
 namespace fdtd {
-template <int V>
+template <int V, bool DIRECT>
 __global__ void k_h_half(const Desc *, const int4 *, int, int, TileGrid, const int *);
-template <int V>
+template <int V, bool DIRECT>
 __global__ void k_e_half(const Desc *, const int4 *, int, int, TileGrid, const int *);
 __global__ void k_dft(const Desc *);
 __global__ void k_advance(const Desc *, int);
@@ -110,7 +110,7 @@ static void beam_unref(FdtdBeam *b) {
 
 // Launch shape a simulation or batch starts with, per stepper.  Split:
 // one x cell per lane, blocks of four wavefronts as wide along x as the
-// grid fills, 3-plane chunks, plain block order (measured fastest at
+// grid fills, 2-plane chunks, plain block order (measured fastest at
 // 761 x 761 x 203).  Fused: 4-row strips, 8-plane chunks.
 static Launch default_launch(int fused) {
     Launch l;
@@ -341,11 +341,8 @@ static int auto_xwaves(const std::vector<FdtdSim *> &members, int v) {
 // about 4 %; other sizes lose more to the extra launches than the
 // memory-side cache saves.
 static long long slab_bytes() {
-    static const long long v = [] {
-        const char *e = getenv("FDTD_SLAB_BYTES");
-        return e ? std::max(0LL, atoll(e)) : 0LL;
-    }();
-    return v;
+    const char *e = getenv("FDTD_SLAB_BYTES");   // read at each group build
+    return e ? std::max(0LL, atoll(e)) : 0LL;
 }
 
 static int group_build(Group &G) {
@@ -447,7 +444,7 @@ static int group_build(Group &G) {
     G.nmodels = (int)descs.size();
     G.tg = TileGrid{};
     G.step1 = nullptr;
-    if (G.members.size() == 1 && !G.fused && G.launch.xcd != 2) {
+    if (G.members.size() == 1 && !G.fused && G.launch.xcd == 0) {
         const FdtdSim *m = G.members[0];
         const int zc = G.launch.zchunk > 0 ? G.launch.zchunk : m->g.nz;
         G.tg = TileGrid{(m->g.nx + tx - 1) / tx, (m->g.ny + tys - 1) / tys, tx, tys, zc, m->g.nz, 0, 0};
@@ -487,15 +484,28 @@ static void group_free(Group &G) {
 
 enum KernelId { KH = 0, KE = 1, KD = 2, KF = 3 };
 
+template <typename K>
+static void launch_split(K direct, K listed, bool is_direct, dim3 grid, dim3 blk, Group &G, const int4 *tl,
+                         int nt, int shape, const TileGrid &tg) {
+    hipLaunchKernelGGL(is_direct ? direct : listed, grid, blk, 0, G.stream, G.d_descs, tl, nt, shape, tg, G.step1);
+}
+
 static void launch_kernel(Group &G, KernelId k, int t0 = 0, int t1 = -1) {
     const dim3 blk(kWave, G.ty * G.xw);
     const int lxw = G.xw == 4 ? 2 : G.xw == 2 ? 1 : 0;
     const int shape = lxw | (G.launch.xcd == 1 ? kShapeXcd : 0);
-    TileGrid tg = G.tg;
-    tg.t0 = t0;
     if (t1 < 0) t1 = G.ntiles;
     const int4 *tl = G.d_tiles + t0;
     const int nt = t1 - t0;
+    // split kernels of a single simulation: a 3-D grid over tiles [t0, t1),
+    // which are whole z chunks
+    TileGrid tg = G.tg;
+    dim3 sgrid((unsigned)nt);
+    if (tg.ntx > 0) {
+        const int per = tg.ntx * tg.nyb;
+        tg.t0 = t0 / per;
+        sgrid = dim3((unsigned)tg.ntx, (unsigned)tg.nyb, (unsigned)(nt / per));
+    }
     if (k == KF && G.fused == 2) {
         const int r = G.ty, v = G.xvec;
         auto *fn = v == 2 ? (r == 7 ? &k_step_rows<2, 7> : &k_step_rows<2, 3>)
@@ -512,24 +522,18 @@ static void launch_kernel(Group &G, KernelId k, int t0 = 0, int t1 = -1) {
         hipLaunchKernelGGL(fn, grid, b4, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles, shape);
     } else if (k == KH) {
         if (G.xvec == 4)
-            hipLaunchKernelGGL(k_h_half<4>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape, tg,
-                               G.step1);
+            launch_split(k_h_half<4, true>, k_h_half<4, false>, tg.ntx > 0, sgrid, blk, G, tl, nt, shape, tg);
         else if (G.xvec == 2)
-            hipLaunchKernelGGL(k_h_half<2>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape, tg,
-                               G.step1);
+            launch_split(k_h_half<2, true>, k_h_half<2, false>, tg.ntx > 0, sgrid, blk, G, tl, nt, shape, tg);
         else
-            hipLaunchKernelGGL(k_h_half<1>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape, tg,
-                               G.step1);
+            launch_split(k_h_half<1, true>, k_h_half<1, false>, tg.ntx > 0, sgrid, blk, G, tl, nt, shape, tg);
     } else if (k == KE) {
         if (G.xvec == 4)
-            hipLaunchKernelGGL(k_e_half<4>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape, tg,
-                               G.step1);
+            launch_split(k_e_half<4, true>, k_e_half<4, false>, tg.ntx > 0, sgrid, blk, G, tl, nt, shape, tg);
         else if (G.xvec == 2)
-            hipLaunchKernelGGL(k_e_half<2>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape, tg,
-                               G.step1);
+            launch_split(k_e_half<2, true>, k_e_half<2, false>, tg.ntx > 0, sgrid, blk, G, tl, nt, shape, tg);
         else
-            hipLaunchKernelGGL(k_e_half<1>, dim3(nt), blk, 0, G.stream, G.d_descs, tl, nt, shape, tg,
-                               G.step1);
+            launch_split(k_e_half<1, true>, k_e_half<1, false>, tg.ntx > 0, sgrid, blk, G, tl, nt, shape, tg);
     } else {
         const unsigned gx = (unsigned)std::max(1LL, (G.max_dft_items + 255) / 256);
         hipLaunchKernelGGL(k_dft, dim3(gx, G.nmodels), dim3(256), 0, G.stream, G.d_descs);

@@ -79,7 +79,7 @@ struct alignas(16) Desc {
 // group_build).  The defaults are the split stepper's.
 struct Launch {
     int ty = 0;      // rows per block (split) or per wavefront (fused)
-    int zchunk = 3;  // z cells marched per tile (0 = whole column)
+    int zchunk = 2;  // z cells marched per tile (0 = whole column)
     int xvec = 0;    // x cells per lane
     int xw = 0;      // split stepper: wavefronts side by side along x per block
     int xcd = 0;     // 1: XCD-contiguous block order, 0: plain block order
@@ -88,9 +88,10 @@ struct Launch {
 // Shape word bit of the split and strip kernels: XCD-contiguous block order.
 constexpr int kShapeXcd = 4;
 
-// Tile grid of a single simulation's split-kernel launch (ntx > 0): block
-// t0 + b covers x tile b % ntx of tx cells, rows (b / ntx) % nyb of ty, and
-// z chunk b / (ntx nyb) of zc planes.  ntx = 0: tiles come from the list.
+// Tile grid of a single simulation's split-kernel launch (ntx > 0), a 3-D
+// grid of ntx x nyb x (z chunks) blocks: block (i, j, k) covers x tile i of
+// tx cells, rows j ty .. j ty + ty - 1 and z chunk t0 + k of zc planes.
+// ntx = 0: tiles come from the group's list.
 struct TileGrid {
     int ntx, nyb, tx, ty, zc, nz, t0, pad;
 };

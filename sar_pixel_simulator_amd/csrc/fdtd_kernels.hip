@@ -83,12 +83,12 @@ __device__ __forceinline__ uint32_t psi_off(const StepK &k, int j) {
                  : 4 * k.du_len + 4 * k.lr_len + (j - 8) * k.fb_len;
 }
 
-__device__ __forceinline__ StepK step_k(const Desc *__restrict__ d, int t) {
+// The time-independent part of StepK.
+__device__ __forceinline__ StepK step_geom(const Desc *__restrict__ d) {
     StepK k;
     k.nx = d->nx; k.ny = d->ny; k.nz = d->nz; k.np = d->npml;
     k.tfsf_z = d->tfsf_z;
-    k.inj = d->mode == FDTD_MODE_MAIN && t < d->nt_inc;
-    k.hard = d->mode != FDTD_MODE_MAIN && t < d->n_src;
+    k.inj = k.hard = false;
     k.Db = d->Db; k.dtm0 = d->dtm0; k.dx = d->dx; k.dte0dx = d->dte0dx;
     k.sy = (uint32_t)d->sy; k.sz = (uint32_t)d->sz;
     k.du_sz = (uint32_t)d->du_sz; k.lr_sz = (uint32_t)d->lr_sz; k.fb_sz = (uint32_t)d->fb_sz;
@@ -96,19 +96,58 @@ __device__ __forceinline__ StepK step_k(const Desc *__restrict__ d, int t) {
     k.lr_len = (uint32_t)(d->nz * d->lr_sz);
     k.fb_len = (uint32_t)(d->nz * d->fb_sz);
     k.psiE = gp(d->psi[0][0]);
-    const long long plane = (long long)d->ny * d->nx * t;
     k.b0 = k.b1 = k.b2 = k.b3 = nullptr;
+    return k;
+}
+
+// What the step-dependent part of StepK is formed from.
+struct StepTime {
+    int nt, nt_inc, n_src, main;
+    const gfloat *beam, *src;
+    long long beam_cs, src_cs, plane;
+};
+
+__device__ __forceinline__ StepTime step_time(const Desc *__restrict__ d) {
+    StepTime s;
+    s.nt = d->nt; s.nt_inc = d->nt_inc; s.n_src = d->n_src;
+    s.main = d->mode == FDTD_MODE_MAIN;
+    s.beam = gp(d->beam); s.src = gp(d->src);
+    s.beam_cs = d->beam_cs; s.src_cs = d->src_cs;
+    s.plane = (long long)d->ny * d->nx;
+    return s;
+}
+
+// StepK of step t: the injection flags and the beam (or source) plane.
+__device__ __forceinline__ StepK step_at(StepK k, const StepTime &s, int t) {
+    k.inj = s.main && t < s.nt_inc;
+    k.hard = !s.main && t < s.n_src;
+    const long long plane = s.plane * t;
     if (k.inj) {
-        const gfloat *b = gp(d->beam) + plane;
+        const gfloat *b = s.beam + plane;
         k.b0 = b;
-        k.b1 = b + d->beam_cs;
-        k.b2 = b + 2 * d->beam_cs;
-        k.b3 = b + 3 * d->beam_cs;
+        k.b1 = b + s.beam_cs;
+        k.b2 = b + 2 * s.beam_cs;
+        k.b3 = b + 3 * s.beam_cs;
     } else if (k.hard) {
-        k.b0 = gp(d->src) + plane;
-        k.b1 = gp(d->src) + d->src_cs + plane;
+        k.b0 = s.src + plane;
+        k.b1 = s.src + s.src_cs + plane;
     }
     return k;
+}
+
+__device__ __forceinline__ StepK step_k(const Desc *__restrict__ d, int t) {
+    return step_at(step_geom(d), step_time(d), t);
+}
+
+// Holds the tile's descriptor scalars in SGPRs at this point, so that their
+// loads are issued as one batch ahead of the field loads instead of one
+// dependent round trip each after them.
+__device__ __forceinline__ void scalars_resident(const StepK &k, const StepTime &s) {
+    asm volatile("" ::"s"(k.nx), "s"(k.ny), "s"(k.nz), "s"(k.np), "s"(k.tfsf_z), "s"(k.sy), "s"(k.sz),
+                 "s"(k.du_sz), "s"(k.lr_sz), "s"(k.fb_sz), "s"(k.psiE));
+    asm volatile("" ::"s"(k.Db), "s"(k.dtm0), "s"(k.dx), "s"(k.dte0dx), "s"(s.nt), "s"(s.nt_inc),
+                 "s"(s.n_src), "s"(s.main), "s"(s.beam), "s"(s.src), "s"(s.beam_cs), "s"(s.src_cs),
+                 "s"(s.plane));
 }
 
 // E values around an H node (x, y, z) at the old time level.
@@ -447,38 +486,41 @@ __device__ __forceinline__ void h_plane_load(HPlane<V> &P, const gfloat *Ex, con
     }
 }
 
-// Tile of a split-kernel block: from the group's tile list, or, for a
-// single simulation (tg.ntx > 0), from the block index (no dependent load).
-// Returns false for a padding tile.
+// Tile of a split-kernel block: for a single simulation (DIRECT), from
+// the 3-D grid (x tile, row block, z chunk: x fastest, as the plain order;
+// no dependent load); else from the group's tile list.  Returns false for a
+// padding tile.
+template <bool DIRECT>
 __device__ __forceinline__ bool split_tile(const int4 *__restrict__ tiles, int ntiles, int shape,
                                            const TileGrid &tg, int4 &tl) {
-    const int bt = block_tile(shape, ntiles);
-    if (tg.ntx > 0) {
-        const int b = bt + tg.t0;
-        const int xt = b % tg.ntx, r = b / tg.ntx;
-        const int yb = r % tg.nyb, zc = r / tg.nyb;
-        const int z0 = zc * tg.zc, z1 = min(tg.nz, z0 + tg.zc);
-        tl = make_int4(0, xt * tg.tx, yb * tg.ty, z0 | (z1 << 16));
+    if (DIRECT) {
+        const int z0 = (int)(blockIdx.z + tg.t0) * tg.zc, z1 = min(tg.nz, z0 + tg.zc);
+        tl = make_int4(0, (int)blockIdx.x * tg.tx, (int)blockIdx.y * tg.ty, z0 | (z1 << 16));
         return true;
     }
-    tl = tiles[bt];
+    tl = tiles[block_tile(shape, ntiles)];
     return tl.w != 0;                                  // padding tile of the banded order
 }
 
-// Both split kernels issue the step-counter load first (its pointer is a
-// kernel argument for a single simulation), then the loads of the tile's
-// first plane, which depend on the geometry only, and test the step after
-// that: a block's first memory round trip covers the counter, and no store
-// precedes the test.  The z loop then computes a plane, stores it and loads
-// the next one.
-template <int V>
+// Both split kernels read the descriptor's scalars and, for a single
+// simulation, the step counter in one batch of scalar loads, then issue the
+// tile's first plane loads, and test the step after that (no store precedes
+// the test).  The z loop then computes a plane, stores it and loads the
+// next one.
+template <int V, bool DIRECT>
 __global__ void __launch_bounds__(256)
 k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles, int shape, TileGrid tg,
          const int *__restrict__ stepp) {
     int4 tl;
-    if (!split_tile(tiles, ntiles, shape, tg, tl)) return;
+    if (!split_tile<DIRECT>(tiles, ntiles, shape, tg, tl)) return;
     const Desc *__restrict__ d = descs + tl.x;
-    const int t = load_step(stepp ? stepp : d->step);
+    // a single simulation's counter is a plain load through a read-only
+    // kernel argument, a scalar load in the descriptor's batch: the
+    // previous kernel of the stream wrote it, and nothing writes it now
+    const int t = DIRECT ? *stepp : load_step(d->step);
+    const StepK kg = step_geom(d);
+    const StepTime st = step_time(d);
+    scalars_resident(kg, st);
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int lxw = shape & 3;
     const int y = tl.z + (w >> lxw);
@@ -507,8 +549,8 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     }
     HPlane<V> P;
     h_plane_load(P, Ex, Ey, Ez, Hx, Hy, Hz, o, sy, sz, act, z0 < nz - 1, yin, next_mem);
-    if (t >= d->nt) return;                            // uniform per block
-    const StepK k = step_k(d, t);
+    if (t >= st.nt) return;                            // uniform per block
+    const StepK k = step_at(kg, st, t);
     for (int z = z0;;) {
         float ey_next = __shfl_down(eyc[0], 1, kWave);
         float ez_next = __shfl_down(P.ez[0], 1, kWave);
@@ -584,14 +626,20 @@ __device__ __forceinline__ void e_plane_load(EPlane<V> &Q, const gfloat *Ex, con
     }
 }
 
-template <int V>
+template <int V, bool DIRECT>
 __global__ void __launch_bounds__(256)
 k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles, int shape, TileGrid tg,
          const int *__restrict__ stepp) {
     int4 tl;
-    if (!split_tile(tiles, ntiles, shape, tg, tl)) return;
+    if (!split_tile<DIRECT>(tiles, ntiles, shape, tg, tl)) return;
     const Desc *__restrict__ d = descs + tl.x;
-    const int t = load_step(stepp ? stepp : d->step);
+    // a single simulation's counter is a plain load through a read-only
+    // kernel argument, a scalar load in the descriptor's batch: the
+    // previous kernel of the stream wrote it, and nothing writes it now
+    const int t = DIRECT ? *stepp : load_step(d->step);
+    const StepK kg = step_geom(d);
+    const StepTime st = step_time(d);
+    scalars_resident(kg, st);
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int lxw = shape & 3;
     const int y = tl.z + (w >> lxw);
@@ -621,8 +669,8 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     EPlane<V> P;
     e_plane_load(P, Ex, Ey, Ez, Hx, Hy, Hz, Ca, Cb, ca_u, cb_u, o, sy, act, z0 < nz - 1,
                  z0 < nz - 1 && z0 < uni_z, ylo, prev_mem);
-    if (t >= d->nt) return;                            // uniform per block
-    const StepK k = step_k(d, t);
+    if (t >= st.nt) return;                            // uniform per block
+    const StepK k = step_at(kg, st, t);
     for (int z = z0;;) {
         float hz_prev = __shfl_up(P.hz[V - 1], 1, kWave);
         float hy_prev = __shfl_up(P.hy[V - 1], 1, kWave);
@@ -656,12 +704,18 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     }
 }
 
-template __global__ void k_h_half<1>(const Desc *, const int4 *, int, int, TileGrid, const int *);
-template __global__ void k_h_half<2>(const Desc *, const int4 *, int, int, TileGrid, const int *);
-template __global__ void k_h_half<4>(const Desc *, const int4 *, int, int, TileGrid, const int *);
-template __global__ void k_e_half<1>(const Desc *, const int4 *, int, int, TileGrid, const int *);
-template __global__ void k_e_half<2>(const Desc *, const int4 *, int, int, TileGrid, const int *);
-template __global__ void k_e_half<4>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_h_half<1, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_h_half<1, true>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_h_half<2, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_h_half<2, true>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_h_half<4, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_h_half<4, true>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_e_half<1, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_e_half<1, true>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_e_half<2, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_e_half<2, true>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_e_half<4, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_e_half<4, true>(const Desc *, const int4 *, int, int, TileGrid, const int *);
 
 // ---------------------------------------------------------------------------
 // Fused stepper (MAIN): one kernel per time step computes H(t+1/2) and then

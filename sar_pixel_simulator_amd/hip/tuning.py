@@ -26,7 +26,7 @@ CANDIDATES = {
 }
 # library defaults: 0 = chosen from the grid (split: one cell per lane,
 # four wavefronts per block, as many along x as the grid fills)
-DEFAULT = {'fused': (4, 8, 0, 1, 0), 'split': (0, 3, 0, 0, 0), 'rows': (7, 16, 2, 1, 0)}
+DEFAULT = {'fused': (4, 8, 0, 1, 0), 'split': (0, 2, 0, 0, 0), 'rows': (7, 16, 2, 1, 0)}
 STEPPERS = {0: 'split', 1: 'fused', 2: 'rows'}
 
 

@@ -134,6 +134,27 @@ def test_steps_past_end_are_noops(soil60):
     sim.destroy()
 
 
+@pytest.mark.parametrize('launch', [(1, 2, 1, 4, 0), (2, 3, 1, 1, 1), (1, 2, 1, 4, 2)],
+                         ids=['direct-grid', 'xcd-order', 'banded'])
+def test_slabs_bitwise(soil60, launch):
+    """H and E launched slab by slab (FDTD_SLAB_BYTES), several z chunks a
+    slab: the same fields as the oracle."""
+    import os
+    pc, Ca, Cb, beam, ref = soil60
+    g = pc.geom_params
+    say('case: slabs of 3 chunks, launch', launch)
+    os.environ['FDTD_SLAB_BYTES'] = str(32 * 3 * launch[1] * g.size_y * g.size_x)
+    try:
+        sim = make_sim(pc, Ca, Cb, beam, launch=launch, fused=False)
+    finally:
+        del os.environ['FDTD_SLAB_BYTES']
+    sim.run(60)
+    sim.sync()
+    for n in native.FIELD_NAMES:
+        assert_same(sim.field(n), ref.fields[n], n)
+    sim.destroy()
+
+
 def test_uneven_grid_bitwise():
     pc, Ca, Cb = H.soil_case(nx=37, ny=29, nz=41, nz_mat=17, steps=40, seed=11)
     say('case: uneven 37x29x41, 40 steps')
