@@ -157,6 +157,7 @@ struct FdtdSim {
     float *psi = nullptr;        // all 24 arrays
     long long psi_len[24] = {};
     float4 *acc = nullptr;
+    float4 *tapx = nullptr;      // MAIN: x-face column copies, 2 * 2 * nz * ny
     double4 *tw64 = nullptr;
     float4 *tw32 = nullptr;
     float *src = nullptr;
@@ -257,11 +258,16 @@ static long long psi_h_len(const Sizes &s) {
     return n;
 }
 
+// float4s of a simulation's x-face taps (Desc::tapx).
+static long long tap_len(const FdtdGeom *g) {
+    return g->mode == FDTD_MODE_MAIN ? 2LL * 2 * g->nz * g->ny : 0;
+}
+
 static long long footprint_bytes(const FdtdGeom *g) {
     const Sizes s = sizes_of(g);
     const int ns = nsets_of(g);
     long long b = 6 * field_stride(s.ncell) * 4 * ns + (s.psi_total + (ns - 1) * psi_h_len(s)) * 4 + s.acc_total * 16;
-    if (g->mode == FDTD_MODE_MAIN) b += 2 * s.ncell * 4;
+    if (g->mode == FDTD_MODE_MAIN) b += 2 * s.ncell * 4 + tap_len(g) * 16;
     b += (long long)(g->nt + 1) * (32 + 32) + 4096;
     return b;
 }
@@ -795,6 +801,10 @@ static int sim_alloc(FdtdSim *s) {
     if (g->mode == FDTD_MODE_MAIN) HIP_TRY(hipMalloc(&s->mat, 2 * z.ncell * sizeof(float)));
     HIP_TRY(hipMalloc(&s->psi, psi_alloc * sizeof(float)));
     HIP_TRY(hipMalloc(&s->acc, z.acc_total * sizeof(float4)));
+    if (tap_len(g)) {
+        HIP_TRY(hipMalloc(&s->tapx, tap_len(g) * sizeof(float4)));
+        HIP_TRY(hipMemset(s->tapx, 0, tap_len(g) * sizeof(float4)));
+    }
     std::vector<double4> w64;
     std::vector<float4> w32;
     make_twiddles(g, &s->c, w64, w32);
@@ -860,6 +870,7 @@ static int sim_alloc(FdtdSim *s) {
     d.tw64 = s->tw64;
     d.tw32 = s->tw32;
     d.step = s->step;
+    d.tapx = s->tapx;
     return FDTD_OK;
 }
 
@@ -870,6 +881,7 @@ static void sim_free(FdtdSim *s) {
     (void)hipFree(s->mat);
     (void)hipFree(s->psi);
     (void)hipFree(s->acc);
+    (void)hipFree(s->tapx);
     (void)hipFree(s->tw64);
     (void)hipFree(s->tw32);
     (void)hipFree(s->src);

@@ -67,6 +67,10 @@ struct alignas(16) Desc {
     const double4 *tw64;           // [nt]
     const float4 *tw32;            // [2 * nt]: E then H, scaled by 1/2 and 1/4, hi/lo split
     int *step;                     // device step counter
+    // MAIN: copies of the x-face columns, [2][nz][ny] records of two float4,
+    // (ey, ez, hy(x-1), hy), (hz(x-1), hz, 0, 0) at x = npml + 1 (face 0)
+    // and nx - npml - 1 (face 1), written by the split E kernel
+    float4 *tapx;
 };
 
 // Launch shape of the stencil kernels.  Each lane owns xvec consecutive x

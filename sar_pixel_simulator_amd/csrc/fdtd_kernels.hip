@@ -486,6 +486,43 @@ __device__ __forceinline__ void h_plane_load(HPlane<V> &P, const gfloat *Ex, con
     }
 }
 
+// x-face taps.  The DFT's x faces lie at x = np + 1 and x = nx - np - 1;
+// their samples need ey, ez at the face column and hy, hz at it and at the
+// column below, for every (z, y).  Read from the field arrays these are
+// one cache line per value; the E half-step kernel, which holds all six
+// (the new H at x and x - 1 are its curl operands), therefore copies them
+// into Desc::tapx, where the DFT reads them with unit stride in y.  Only
+// the wavefronts whose cells include a face column take the branch.  The
+// copies hold every (z, y) with z < nz - 1, all the DFT reads.
+template <int V>
+__device__ __forceinline__ bool wave_taps(int x0, int nx, int np) {
+    const int x1 = x0 + kWave * V;
+    const int p0 = np + 1, p1 = nx - np - 1;
+    return (p0 >= x0 && p0 < x1) || (p1 >= x0 && p1 < x1);
+}
+
+// hy_xm, hz_xm: the new hy, hz at the x - 1 neighbour of the lane's first
+// cell.  One 32-byte record per (face, z, y): Desc::tapx.
+template <int V>
+__device__ __forceinline__ void e_tap(gchar *tap, int xb, int nx, int np, int nz, int ny, int z, int y,
+                                      const float (&ey)[V], const float (&ez)[V], const float (&hy)[V],
+                                      const float (&hz)[V], float hy_xm, float hz_xm) {
+#pragma unroll
+    for (int c = 0; c < V; ++c) {
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+            if (xb + c == (f ? nx - np - 1 : np + 1)) {
+                const uint32_t cell = (uint32_t)((f * nz + z) * ny + y);
+                gvf4 *p = (gvf4 *)(tap + (cell << 5));
+                const float hy0 = c > 0 ? hy[c > 0 ? c - 1 : 0] : hy_xm;
+                const float hz0 = c > 0 ? hz[c > 0 ? c - 1 : 0] : hz_xm;
+                p[0] = vf4{ey[c], ez[c], hy0, hy[c]};
+                p[1] = vf4{hz0, hz[c], 0.f, 0.f};
+            }
+        }
+    }
+}
+
 // Tile of a split-kernel block: for a single simulation (DIRECT), from
 // the 3-D grid (x tile, row block, z chunk: x fastest, as the plain order;
 // no dependent load); else from the group's tile list.  Returns false for a
@@ -656,6 +693,8 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     const gfloat *Ca = gp(d->ca), *Cb = gp(d->cb);
     const bool ylo = y >= 1;
     const int uni_z = d->uni_z;
+    const int x0w = tl.y + V * ((w & ((1 << lxw) - 1)) << 6);   // the wavefront's first cell
+    gchar *const tap = d->tapx && wave_taps<V>(x0w, nx, d->npml) ? (gchar *)d->tapx : nullptr;
     const float ca_u = d->ca_u, cb_u = d->cb_u;
 
     uint32_t o = (uint32_t)z0 * sz + (uint32_t)y * sy + (uint32_t)xb;
@@ -695,6 +734,7 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
             vst(at(Ey, o), P.ey);
             vst(at(Ez, o), P.ez);
         }
+        if (tap) e_tap<V>(tap, xb, nx, k.np, nz, ny, z, y, P.ey, P.ez, P.hy, P.hz, hy_prev, hz_prev);
         if (++z >= z1) break;
         o += sz;
         vcopy(hxzm, P.hx);
@@ -1218,6 +1258,21 @@ k_dft(const Desc *__restrict__ descs) {
             break;
         }
         case kFaceX: {   // (a, b) = (z, y); ey ez hy hz
+            if (d->tapx && !d->fused) {
+                // the split E kernel's copies of the face columns:
+                // (ey, ez, hy(x-1), hy), (hz(x-1), hz, -, -) per (z, y)
+                const int ny = d->ny;
+                const float4 *T = d->tapx + 2 * (pos == np + 1 ? 0 : (long long)d->nz * ny);
+                const long long ti = (long long)(np + a) * ny + (np + b);
+                const float4 c0 = T[2 * ti], c1 = T[2 * ti + 1];
+                const float4 y0 = T[2 * (ti - 1)], y1 = T[2 * (ti - 1) + 1];
+                const float4 z0 = T[2 * (ti - ny)];
+                s[0] = c0.x + y0.x;
+                s[1] = c0.y + z0.y;
+                s[2] = c0.w + c0.z + z0.w + z0.z;
+                s[3] = c1.y + c1.x + y1.y + y1.x;
+                break;
+            }
             const long long o = (long long)(np + a) * sz + (long long)(np + b) * sy + pos;
             s[0] = Ey[o] + Ey[o - sy];
             s[1] = Ez[o] + Ez[o - sz];
