@@ -55,15 +55,17 @@ def uniform_top(Ca, Cb):
     return z
 
 
-def algorithmic_bytes(nx, ny, nz, np_, uni_z=None):
+def algorithmic_bytes(nx, ny, nz, np_, uni_z=None, taps=True):
     """Compulsory HBM bytes per launch of each kernel.
 
     H half: read E(3) + H(3), write H(3) = 36 B/cell; each CPML face cell
     reads and writes two psi values = 16 B; E_inc injection reads 2 floats
     per TFSF-plane cell.  E half: read H(3) + E(3) + Ca + Cb, write E(3) =
     44 B/cell (36 B on the uniform top planes z >= uni_z, where Ca/Cb are
-    scalars), same CPML and injection terms.  DFT: per plane sample, four
-    float4 running sums read and written (128 B) plus four field values.
+    scalars), same CPML and injection terms, and (split stepper, taps) the
+    six x-face column values per (face, z, y) it copies for the DFT, 24 B.
+    DFT: per plane sample, four float4 running sums read and written
+    (128 B) plus four field values.
     """
     n = nx * ny * nz
     nc = cpml_cells(nx, ny, nz, np_)
@@ -72,7 +74,7 @@ def algorithmic_bytes(nx, ny, nz, np_, uni_z=None):
     nxp, nyp, nzp = nx - 2 * np_, ny - 2 * np_, nz - 2 * np_
     samples = 2 * nyp * nxp + 2 * nzp * nyp + 2 * nzp * nxp
     return {'h': 36 * n + 16 * nc + 8 * plane,
-            'e': 44 * n - 8 * n_uni + 16 * nc + 8 * plane,
+            'e': 44 * n - 8 * n_uni + 16 * nc + 8 * plane + (48 * ny * nz if taps else 0),
             'dft': 144 * samples,
             'cells': n, 'cpml_cells': nc, 'dft_samples': samples}
 
@@ -215,7 +217,7 @@ def largest_leg(fused, steps=10, warmup=3, log=print):
     wall = time.perf_counter() - t0
     k = sim.time_steps(steps, per_kernel=True)
     cells = g.size_x * g.size_y * g.size_z
-    ab = algorithmic_bytes(g.size_x, g.size_y, g.size_z, g.size_pml, uni_z)
+    ab = algorithmic_bytes(g.size_x, g.size_y, g.size_z, g.size_pml, uni_z, taps=not fused)
     step_bytes = (ab['h'] + ab['e'] - 24 * cells if fused else ab['h'] + ab['e']) + ab['dft']
     ms = wall * 1e3 / steps
     out = {'workload': f'largest single domain: {g.size_x}x{g.size_y}x{g.size_z} cells (32x32x4 '
@@ -329,7 +331,7 @@ def main():
     # per-kernel HIP-event times on the library's stream
     k = sim.time_steps(min(args.steps, 50), per_kernel=True)
     uni_z = uniform_top(Ca, Cb)
-    ab = algorithmic_bytes(g.size_x, g.size_y, g.size_z, g.size_pml, uni_z)
+    ab = algorithmic_bytes(g.size_x, g.size_y, g.size_z, g.size_pml, uni_z, taps=not fused)
 
     def gbs(nbytes, ms):
         return round(nbytes / (ms * 1e-3) / 1e9, 1) if ms > 0 else None

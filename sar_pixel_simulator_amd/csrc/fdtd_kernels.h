@@ -11,6 +11,8 @@
 //   twiddles                   per step: double4 (cos,sin E; cos,sin H) and the
 //                               scaled float hi/lo split of the same values
 //   beam                       float32 [4][nt_beam][ny][nx] (ex ey hx hy)
+//   x-face copies (MAIN)       two float4 per (face, z, y): the DFT's x-face
+//                               columns as the split E kernel wrote them
 //
 // Every kernel reads a per-simulation descriptor (Desc) from device memory,
 // so one launch can step many simulations of different sizes (a batch), and

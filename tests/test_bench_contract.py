@@ -22,14 +22,18 @@ def test_more_than_one_gpu_is_refused():
 
 def test_algorithmic_bytes_without_cpml_or_uniform_planes():
     # npml 1: no CPML cells (every face range is empty)
-    ab = bench.algorithmic_bytes(10, 12, 14, 1, uni_z=None)
+    ab = bench.algorithmic_bytes(10, 12, 14, 1, uni_z=None, taps=False)
     n = 10 * 12 * 14
     assert ab['cpml_cells'] == 0
     assert ab['h'] == 36 * n + 8 * 10 * 12
     assert ab['e'] == 44 * n + 8 * 10 * 12
     # all planes uniform: the E half reads no Ca/Cb
-    ab0 = bench.algorithmic_bytes(10, 12, 14, 1, uni_z=0)
+    ab0 = bench.algorithmic_bytes(10, 12, 14, 1, uni_z=0, taps=False)
     assert ab0['e'] == 36 * n + 8 * 10 * 12
+    # split stepper: six x-face column values per (face, z, y) written
+    abt = bench.algorithmic_bytes(10, 12, 14, 1, uni_z=None)
+    assert abt['e'] - ab['e'] == 2 * 14 * 12 * 6 * 4
+    assert abt['h'] == ab['h']
 
 
 def test_cpml_cells_brute_force():

@@ -105,6 +105,9 @@ def test_launch_shapes_bitwise(soil60, fused, launch):
     sim.sync()
     for n in native.FIELD_NAMES:
         assert_same(sim.field(n), ref.fields[n], f'{n} {launch}')
+    ph, oph = sim.phasors(), ref.phasors()
+    for n in native.PHASOR_NAMES:
+        assert_same(ph[n], oph[n], f'{n} {launch}')
     sim.destroy()
 
 
@@ -197,6 +200,33 @@ def test_wide_grid_lane_widths_bitwise(wide30, fused, xwaves, xvec):
     for i, n in enumerate(native.PSI_NAMES):
         assert_same(sim.psi(i), ref.psi[n], f'{n} xvec {xvec}')
     sim.destroy()
+
+
+@pytest.mark.parametrize('nx', [139, 77])
+def test_x_face_taps_bitwise(nx):
+    # the split stepper's DFT reads the x faces (x = 11 and nx - 11) from
+    # the E kernel's copies of the face columns; hy, hz at x - 1 come from
+    # the lane itself (x = 11 at 2 and 4 cells per lane), the lane below
+    # (x = 66 at 2 cells per lane) or memory (x = 128: a wavefront's first
+    # lane at 1 and 2 cells per lane)
+    pc, Ca, Cb = H.soil_case(nx=nx, ny=27, nz=29, nz_mat=11, steps=150, seed=13,
+                             dft_accumulation='reference')
+    say('case: x-face taps', nx, 'x 27 x 29, 150 steps')
+    beam = H.synthetic_beam(pc)
+    ref = H.oracle_main(pc, Ca, Cb, beam)
+    oph = ref.phasors()
+    for n in ('ey_ph_l', 'hy_ph_l', 'ez_ph_r', 'hz_ph_r'):
+        assert np.abs(oph[n]).max() > 1e-4, n
+    for launch in ((1, 2, 1, 4, 0), (1, 3, 1, 1, 0), (2, 2, 2, 2, 0), (1, 4, 2, 4, 0),
+                   (4, 0, 4, 1, 0), (2, 5, 4, 2, 1), (1, 1, 1, 4, 2)):
+        say('  launch', launch)
+        sim = make_sim(pc, Ca, Cb, beam, launch=launch, fused=False)
+        sim.run(150)
+        sim.sync()
+        ph = sim.phasors()
+        for n in native.PHASOR_NAMES:
+            assert_same(ph[n], oph[n], f'{n} {launch}')
+        sim.destroy()
 
 
 def test_invalid_launch_is_rejected_and_previous_kept(soil60):
