@@ -20,7 +20,6 @@ __global__ void k_h_half(const Desc *, const int4 *, int, int, TileGrid, const i
 template <int V, bool DIRECT>
 __global__ void k_e_half(const Desc *, const int4 *, int, int, TileGrid, const int *);
 __global__ void k_dft(const Desc *);
-__global__ void k_advance(const Desc *, int);
 template <int V, int R>
 __global__ void k_step_strip(const Desc *, const int4 *, int, int);
 template <int V, int R>
@@ -543,7 +542,6 @@ static void launch_kernel(Group &G, KernelId k, int t0 = 0, int t1 = -1) {
     } else {
         const unsigned gx = (unsigned)std::max(1LL, (G.max_dft_items + 255) / 256);
         hipLaunchKernelGGL(k_dft, dim3(gx, G.nmodels), dim3(256), 0, G.stream, G.d_descs);
-        hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, G.stream, G.d_descs, G.nmodels);
     }
 }
 
@@ -812,7 +810,7 @@ static int sim_alloc(FdtdSim *s) {
     HIP_TRY(hipMalloc(&s->tw32, w32.size() * sizeof(float4)));
     HIP_TRY(hipMemcpy(s->tw64, w64.data(), w64.size() * sizeof(double4), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->tw32, w32.data(), w32.size() * sizeof(float4), hipMemcpyHostToDevice));
-    HIP_TRY(hipMalloc(&s->step, sizeof(int)));
+    HIP_TRY(hipMalloc(&s->step, 2 * sizeof(int)));   // counter, step of the DFT
     // the field arrays' padding columns are read as x+1 neighbours never,
     // but keep all device memory defined
     HIP_TRY(hipMemset(s->fields, 0, 6 * field_stride(z.ncell) * ns * sizeof(float)));
@@ -1149,7 +1147,7 @@ extern "C" int fdtd_sim_reset(FdtdSim *s) {
     HIP_TRY(hipMemsetAsync(s->fields, 0, 6 * field_stride(z.ncell) * ns * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(s->psi, 0, (z.psi_total + (ns - 1) * psi_h_len(z)) * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(s->acc, 0, z.acc_total * sizeof(float4), st));
-    HIP_TRY(hipMemsetAsync(s->step, 0, sizeof(int), st));
+    HIP_TRY(hipMemsetAsync(s->step, 0, 2 * sizeof(int), st));
     HIP_TRY(hipStreamSynchronize(st));
     return FDTD_OK;
 }

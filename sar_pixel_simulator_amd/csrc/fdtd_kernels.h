@@ -68,7 +68,7 @@ struct alignas(16) Desc {
     long long dft_items;           // samples of all faces (+ record items, INCIDENT)
     const double4 *tw64;           // [nt]
     const float4 *tw32;            // [2 * nt]: E then H, scaled by 1/2 and 1/4, hi/lo split
-    int *step;                     // device step counter
+    int *step;                     // [0] step counter, [1] the step k_dft runs
     // MAIN: copies of the x-face columns, [2][nz][ny] records of two float4,
     // (ey, ez, hy(x-1), hy), (hz(x-1), hz, 0, 0) at x = npml + 1 (face 0)
     // and nx - npml - 1 (face 1), written by the split E kernel

@@ -36,6 +36,17 @@ __device__ __forceinline__ int load_step(const int *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// A simulation's step counter is step[0]; step[1] is the step its k_dft
+// runs.  The H-half (or fused) kernel copies step[0] into step[1] from one
+// lane of the tile at (0, 0, 0), and k_dft, which reads only step[1], has
+// its first block advance step[0]: no launch of a step reads a counter
+// another launch of the same step changes, and no separate launch is
+// needed to advance it.
+__device__ __forceinline__ void note_step(const Desc *__restrict__ d, const int4 &tl, int w, int t) {
+    if (tl.y == 0 && tl.z == 0 && (tl.w & 0xffff) == 0 && w == 0 && threadIdx.x == 0)
+        __hip_atomic_store(d->step + 1, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ---------------------------------------------------------------------------
 // Per-cell updates shared by the split and the fused steppers.
 // ---------------------------------------------------------------------------
@@ -559,6 +570,7 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     const StepTime st = step_time(d);
     scalars_resident(kg, st);
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    note_step(d, tl, w, t);
     const int lxw = shape & 3;
     const int y = tl.z + (w >> lxw);
     const int nx = d->nx, ny = d->ny, nz = d->nz;
@@ -816,6 +828,7 @@ k_step_strip(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int
     const int4 tl = tiles[wt];
     const Desc *__restrict__ d = descs + tl.x;
     const int t = load_step(d->step);
+    note_step(d, tl, 0, t);                            // tiles are per wavefront
     if (t >= d->nt) return;
     const StepK k = step_k(d, t);
     const int lane = threadIdx.x;
@@ -1017,9 +1030,10 @@ k_step_rows(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int 
     const int4 tl = tiles[block_tile(shape, ntiles)];
     const Desc *__restrict__ d = descs + tl.x;
     const int t = load_step(d->step);
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    note_step(d, tl, w, t);
     if (t >= d->nt) return;                            // uniform per block
     const StepK k = step_k(d, t);
-    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int lane = threadIdx.x;
     const int x0 = tl.y, y0 = tl.z;
     const int y = y0 - 1 + w;
@@ -1197,7 +1211,7 @@ template __global__ void k_step_strip<4, 2>(const Desc *, const int4 *, int, int
 // averaging factor) and added with TwoSum, so the accumulated value carries
 // ~44 significant bits in float32 arithmetic only.
 //
-// k_advance then moves every member's step counter to the next step.
+// The first block then moves the step counter to the next step (note_step).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void acc_ref(float4 &a, float s, double half, double c, double sn) {
     const double avg = half * (double)s;
@@ -1226,8 +1240,9 @@ __device__ __forceinline__ void acc_comp(float4 &a, float s, float4 w) {
 __global__ void __launch_bounds__(256)
 k_dft(const Desc *__restrict__ descs) {
     const Desc *__restrict__ d = descs + blockIdx.y;
-    const int t = load_step(d->step);
+    const int t = load_step(d->step + 1);
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i == 0 && t < d->nt) __hip_atomic_store(d->step, t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (t >= d->nt || i >= d->dft_items) return;
     const int nsamp = (int)d->face_off[d->nfaces];
     const long long sy = d->sy, sz = d->sz;
@@ -1319,17 +1334,6 @@ k_dft(const Desc *__restrict__ descs) {
         const int src_comp[4] = {0, 1, 3, 4};
         const float v = d->f[cs][src_comp[c]][(long long)d->read_out * sz + (long long)yy * sy + xx];
         d->rec[c * d->beam_cs + (long long)t * plane + r] = v;
-    }
-}
-
-// Advance every member's step counter after the step's kernels (a separate
-// launch, so no kernel of the step reads a counter another has changed).
-__global__ void __launch_bounds__(64)
-k_advance(const Desc *__restrict__ descs, int nmodels) {
-    for (int m = threadIdx.x; m < nmodels; m += blockDim.x) {
-        int *sp = descs[m].step;
-        const int t = load_step(sp);
-        if (t < descs[m].nt) __hip_atomic_store(sp, t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 

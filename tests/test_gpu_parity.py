@@ -155,6 +155,9 @@ def test_slabs_bitwise(soil60, launch):
     sim.sync()
     for n in native.FIELD_NAMES:
         assert_same(sim.field(n), ref.fields[n], n)
+    ph, oph = sim.phasors(), ref.phasors()
+    for n in native.PHASOR_NAMES:
+        assert_same(ph[n], oph[n], n)
     sim.destroy()
 
 
@@ -309,6 +312,7 @@ def test_batch_equals_solo(fused, launch):
         s.run(30)
         s.sync()
         solo.append({n: s.field(n) for n in native.FIELD_NAMES})
+        solo[-1].update(s.phasors())
         s.destroy()
         batch_sims.append(make_sim(pc, Ca, Cb, beam, fused=fused))
     for use_graph in (False, True):
@@ -324,6 +328,9 @@ def test_batch_equals_solo(fused, launch):
             assert s.step_count() == 30
             for n in native.FIELD_NAMES:
                 assert_same(s.field(n), solo[k][n], f'member {k} {n} graph={use_graph}')
+            ph = s.phasors()
+            for n in native.PHASOR_NAMES:
+                assert_same(ph[n], solo[k][n], f'member {k} {n} graph={use_graph}')
         b.destroy()
     for s in batch_sims:
         s.destroy()
