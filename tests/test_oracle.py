@@ -6,7 +6,8 @@ break:
 * constant fields + zero update coefficients: every phasor equals the field
   interpolated to the common Yee node times a geometric series -> pins the
   plane positions, sample indexing, 2-/4-point averaging, the twiddle sign
-  and the H half-step offset (closed form)
+  and the H half-step offset (closed form); the same for the incident run
+  (E at t - 1/2, H at t) and its recorded read-out planes
 * brute-force Python loops of the reference kernels on a tiny grid -> pins
   the C transcription of every pass (index ranges, signs, operation order)
 * zero stays zero; doubling the input doubles every value bit for bit
@@ -108,6 +109,39 @@ def test_dft_closed_form():
                 else:
                     X, Y, Z = npml + b, pos[face], npml + a
                 want[a, b] = _node_value(comp, X, Y, Z) * series[comp[0]]
+        assert np.all(arr[0, :] == 0) and np.all(arr[:, 0] == 0), name
+        err = np.abs(arr - want).max() / np.abs(want).max()
+        assert err < 1e-5, (name, err)
+
+
+def test_incident_dft_and_recording_closed_form():
+    # the incident run (cpu/incident_sim.py:145-161) with zero update
+    # coefficients and no source: the recorded planes are ex, ey, hx, hy at
+    # the read-out plane on every step, and the incident phasors
+    # (cpu/field_updates.py:464-491) are the node values at that plane
+    # times a geometric series, E at t - 1/2 and H at t (the reverse of the
+    # main run)
+    nx, ny, nz, npml, nt, read_out = 27, 25, 27, 10, 23, 13
+    geo = _tiny_geom(nx=nx, ny=ny, nz=nz, npml=npml, nt_inc=0, read_out=read_out,
+                     Db=np.float32(0), dtm0=np.float32(0), dte0dx=np.float32(0))
+    ones = np.ones((nz, ny, nx), np.float32)
+    sim = oracle.OracleSim(geo, ones, 0 * ones, np.full(8 * npml, 0.5, np.float32), incident=True)
+    for n in oracle.FIELD_NAMES:
+        sim.fields[n][...] = _linear_field(n, nz, ny, nx)
+    rec = sim.run_incident(nt)
+    for c, n in enumerate(('ex', 'ey', 'hx', 'hy')):
+        assert np.array_equal(rec[c], np.broadcast_to(_linear_field(n, nz, ny, nx)[read_out], (nt, ny, nx)))
+    w = geo['omega'] * geo['dt']
+    geo_sum = (1 - np.exp(-1j * w * nt)) / (1 - np.exp(-1j * w))
+    series = {'e': np.exp(0.5j * w) * geo_sum, 'h': geo_sum}
+    for name, arr in sim.incident_phasors().items():
+        comp = name[:2]
+        A, B = arr.shape
+        assert (A, B) == (ny - 2 * npml, nx - 2 * npml)
+        want = np.zeros((A, B), np.complex128)
+        for a in range(1, A):
+            for b in range(1, B):
+                want[a, b] = _node_value(comp, npml + b, npml + a, read_out) * series[comp[0]]
         assert np.all(arr[0, :] == 0) and np.all(arr[:, 0] == 0), name
         err = np.abs(arr - want).max() / np.abs(want).max()
         assert err < 1e-5, (name, err)
