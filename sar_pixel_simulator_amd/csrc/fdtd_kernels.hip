@@ -422,12 +422,14 @@ __device__ __forceinline__ void vld(float (&r)[V], const gfloat *p) {
     }
 }
 
-// Non-temporal load, for the material coefficients Ca, Cb.  They are read
-// once per step and never written while stepping; marked as streaming they
-// do not displace the field lines just written by the previous kernel,
-// which the stencil re-reads through L2 and the memory-side cache (E half
-// at 761 x 761 x 203: 0.938 -> 0.920 ms).  The same hint on field loads
-// (lines the kernel reads back or writes) was measured slower.
+// Non-temporal ("streaming") loads and stores.  At 761 x 761 x 203 the
+// split kernels run fastest with these streamed: the material coefficients
+// Ca, Cb (read once per step, never written while stepping), the E
+// half-step's loads of its own ex, ey, ez, both half-steps' field stores,
+// and the DFT's running sums.  Streaming the other loads (the H half-step's
+// own fields, the E half-step's hy, ey in the H half-step, the neighbour
+// rows) was measured slower.  Together: E half 0.941 -> 0.900 ms, step
+// 1.820 -> 1.765 ms.
 template <int V>
 __device__ __forceinline__ void vld_stream(float (&r)[V], const gfloat *p) {
     if constexpr (V == 4) {
@@ -449,6 +451,17 @@ __device__ __forceinline__ void vst(gfloat *p, const float (&r)[V]) {
         *reinterpret_cast<gvf2 *>(p) = vf2{r[0], r[1]};
     } else {
         *p = r[0];
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void vst_stream(gfloat *p, const float (&r)[V]) {
+    if constexpr (V == 4) {
+        __builtin_nontemporal_store(vf4{r[0], r[1], r[2], r[3]}, reinterpret_cast<gvf4 *>(p));
+    } else if constexpr (V == 2) {
+        __builtin_nontemporal_store(vf2{r[0], r[1]}, reinterpret_cast<gvf2 *>(p));
+    } else {
+        __builtin_nontemporal_store(r[0], p);
     }
 }
 
@@ -639,9 +652,9 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
             }
         }
         if (act) {
-            vst(at(Hx, o), P.hx);
-            vst(at(Hy, o), P.hy);
-            vst(at(Hz, o), P.hz);
+            vst_stream(at(Hx, o), P.hx);
+            vst_stream(at(Hy, o), P.hy);
+            vst_stream(at(Hz, o), P.hz);
         }
         if (++z >= z1) break;
         o += sz;
@@ -673,9 +686,9 @@ __device__ __forceinline__ void e_plane_load(EPlane<V> &Q, const gfloat *Ex, con
     if (act) {
         vld(Q.hx, at(Hx, o));
         vld(Q.hy, at(Hy, o));
-        vld(Q.ex, at(Ex, o));
-        vld(Q.ey, at(Ey, o));
-        vld(Q.ez, at(Ez, o));
+        vld_stream(Q.ex, at(Ex, o));
+        vld_stream(Q.ey, at(Ey, o));
+        vld_stream(Q.ez, at(Ez, o));
         if (zin) {
             vld(Q.hz, at(Hz, o));
             if (ylo) {
@@ -761,9 +774,9 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
             }
         }
         if (act) {
-            vst(at(Ex, o), P.ex);
-            vst(at(Ey, o), P.ey);
-            vst(at(Ez, o), P.ez);
+            vst_stream(at(Ex, o), P.ex);
+            vst_stream(at(Ey, o), P.ey);
+            vst_stream(at(Ez, o), P.ez);
         }
         if (tap) e_tap<V>(tap, xb, nx, k.np, nz, ny, z, y, P.ey, P.ez, P.hy, P.hz, hy_prev, hz_prev);
         if (++z >= z1) break;
@@ -1232,6 +1245,16 @@ template __global__ void k_step_strip<4, 2>(const Desc *, const int4 *, int, int
 //
 // The first block then moves the step counter to the next step (note_step).
 // ---------------------------------------------------------------------------
+// Running sums are read and written once per step, streamed (vld_stream).
+__device__ __forceinline__ float4 acc_load(const float4 &v) {
+    const vf4 q = __builtin_nontemporal_load(reinterpret_cast<const vf4 *>(&v));
+    return make_float4(q.x, q.y, q.z, q.w);
+}
+
+__device__ __forceinline__ void acc_store(float4 &v, const float4 &r) {
+    __builtin_nontemporal_store(vf4{r.x, r.y, r.z, r.w}, reinterpret_cast<vf4 *>(&v));
+}
+
 __device__ __forceinline__ void acc_ref(float4 &a, float s, double half, double c, double sn) {
     const double avg = half * (double)s;
     a.x = (float)((double)a.x + avg * c);
@@ -1328,18 +1351,18 @@ k_dft(const Desc *__restrict__ descs) {
             const double4 w = d->tw64[t];
             for (int c = 0; c < 4; ++c) {
                 float4 &v = acc[d->acc_off[4 * f + c] + j];
-                float4 r = v;
+                float4 r = acc_load(v);
                 if (c < 2) acc_ref(r, s[c], 0.5, w.x, w.y);
                 else acc_ref(r, s[c], 0.25, w.z, w.w);
-                v = r;
+                acc_store(v, r);
             }
         } else {
             const float4 we = d->tw32[2 * t], wh = d->tw32[2 * t + 1];
             for (int c = 0; c < 4; ++c) {
                 float4 &v = acc[d->acc_off[4 * f + c] + j];
-                float4 r = v;
+                float4 r = acc_load(v);
                 acc_comp(r, s[c], c < 2 ? we : wh);
-                v = r;
+                acc_store(v, r);
             }
         }
     } else if (d->rec) {
