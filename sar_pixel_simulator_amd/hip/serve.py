@@ -10,8 +10,11 @@ at every chunk boundary the loop
 2. admits queued requests in arrival order while a slot is free and the
    request's HBM footprint (fields, coefficients, CPML state, phasor sums,
    plus its incident beam unless already resident) fits the budget;
-3. rebuilds the batch when membership changed and runs the next chunk of
-   time steps for every member in one launch per kernel per step.
+3. rebuilds the batch when membership changed and starts the next chunk of
+   time steps for every member, one launch per kernel per step;
+4. while that chunk runs on the device, prepares queued requests on the
+   host (the Yee coefficients of requests that carry a MaterialModel), so
+   that their admission at the next boundary does not stall the device.
 
 Requests that arrive while others run join at the next boundary, so their
 steps overlap on the device.  Each member's arithmetic does not depend on the
@@ -78,6 +81,13 @@ class HipBackend:
     def steps(self, req):
         return req.params.geom_params.size_t
 
+    def prepare(self, req):
+        """Host-side part of admit() that needs no device memory: the Yee
+        coefficients of a request given as a MaterialModel."""
+        from ..materials import compute_material_arrays
+        if req.spec is None and req.Ca is None and req.material is not None:
+            req.Ca, req.Cb = compute_material_arrays(req.params, req.material)
+
     def admit(self, req):
         from .batch import DeviceModel
         from ..materials import compute_material_arrays
@@ -121,6 +131,7 @@ class Server:
         self.active = []                 # (request, member, footprint, admitted, steps_done, nt)
         self.used = 0
         self.batch = None
+        self.inflight = 0                # steps of the chunk running on the device
         self.results = []
         self.chunks = 0
         self.peak_used = 0
@@ -180,8 +191,29 @@ class Server:
         self.peak_slots = max(self.peak_slots, len(self.active))
         return changed
 
+    def _finish_chunk(self):
+        if self.inflight:
+            self.backend.sync(self.batch)
+            for e in self.active:
+                e[4] = min(e[5], e[4] + self.inflight)
+            self.inflight = 0
+
+    def _prepare_queued(self):
+        prepare = getattr(self.backend, 'prepare', None)
+        if prepare is None:
+            return
+        free = max(1, self.max_slots - len(self.active))
+        for req in list(self.queue)[:free]:
+            try:
+                prepare(req)
+            except Exception:               # admit() repeats the work and reports it
+                pass
+
     def step(self):
-        """Retire, admit, run one chunk.  Returns the number of active members."""
+        """Finish the chunk in flight, retire, admit, start the next chunk and
+        prepare queued requests while it runs.  Returns the number of active
+        members."""
+        self._finish_chunk()
         changed = self._retire()
         changed = self._admit() or changed
         if changed and self.batch is not None:
@@ -192,11 +224,10 @@ class Server:
         if self.batch is None:
             self.batch = self.backend.make_batch([e[1] for e in self.active])
         n = min(self.chunk, max(e[5] - e[4] for e in self.active))
-        self.backend.run(self.batch, n)
-        self.backend.sync(self.batch)
-        for e in self.active:
-            e[4] = min(e[5], e[4] + n)
+        self.backend.run(self.batch, n)        # asynchronous
+        self.inflight = n
         self.chunks += 1
+        self._prepare_queued()
         return len(self.active)
 
     def drain(self):
@@ -205,6 +236,7 @@ class Server:
         self.close()
 
     def close(self):
+        self._finish_chunk()
         if self.batch is not None:
             self.backend.drop_batch(self.batch)
             self.batch = None

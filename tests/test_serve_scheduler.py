@@ -153,3 +153,54 @@ def test_latency_stats():
     assert st['count'] == 5
     assert st['p50_s'] <= st['p95_s'] <= st['p99_s']
     assert st['models_per_hour'] > 0
+
+
+class LoggingBackend(FakeBackend):
+    """FakeBackend that logs device launches, syncs and host preparation."""
+
+    def __init__(self, *a, fail_prepare=(), **kw):
+        super().__init__(*a, **kw)
+        self.log, self.fail_prepare = [], set(fail_prepare)
+
+    def run(self, batch, n):
+        self.log.append(('run', tuple(m.req.rid for m in batch.members)))
+        super().run(batch, n)
+
+    def sync(self, batch):
+        self.log.append(('sync',))
+
+    def prepare(self, req):
+        self.log.append(('prepare', req.rid))
+        if req.rid in self.fail_prepare:
+            raise ValueError('bad material')
+        req.prepared = True
+
+
+def test_queued_requests_are_prepared_while_a_chunk_runs():
+    # one slot: request 1 waits while 0 runs, and is prepared between the
+    # launch of 0's chunk and its sync
+    be = LoggingBackend({0: 10, 1: 10}, {0: 16, 1: 8})
+    s = Server(be, hbm_budget=100, max_slots=1, chunk_steps=8, clock=Clock())
+    for r in _reqs(2):
+        s.submit(r)
+    s.drain()
+    log = be.log
+    first_run = log.index(('run', (0,)))
+    assert log[first_run + 1] == ('prepare', 1) and log[first_run + 2] == ('sync',)
+    assert log.index(('prepare', 1)) < log.index(('run', (1,)))
+    assert [r.rid for r in sorted(s.results, key=lambda r: r.finished)] == [0, 1]
+    assert all(r.error is None for r in s.results)
+    assert log[-1] == ('sync',)                      # nothing left in flight
+    assert s.inflight == 0
+
+
+def test_failed_preparation_is_left_to_admission():
+    be = LoggingBackend({0: 10, 1: 10, 2: 10}, {0: 8, 1: 8, 2: 8}, fail_prepare={1},
+                        fail_admit={1})
+    s = Server(be, hbm_budget=100, max_slots=1, chunk_steps=8, clock=Clock())
+    for r in _reqs(3):
+        s.submit(r)
+    s.drain()
+    res = {r.rid: r for r in s.results}
+    assert res[1].error.startswith('MemoryError')     # reported by admit, not prepare
+    assert res[0].error is None and res[2].error is None
