@@ -41,6 +41,7 @@ class Request:
     Cb: Optional[np.ndarray] = None
     name: str = ''
     arrival: float = 0.0             # seconds on the server clock
+    prepared: bool = False           # Ca, Cb computed by the backend while queued
 
 
 @dataclasses.dataclass
@@ -87,6 +88,7 @@ class HipBackend:
         from ..materials import compute_material_arrays
         if req.spec is None and req.Ca is None and req.material is not None:
             req.Ca, req.Cb = compute_material_arrays(req.params, req.material)
+            req.prepared = True
 
     def admit(self, req):
         from .batch import DeviceModel
@@ -95,7 +97,11 @@ class HipBackend:
             return DeviceModel.from_spec(req.params, req.spec, self.beams, req.name)
         Ca, Cb = (req.Ca, req.Cb) if req.Ca is not None else \
             compute_material_arrays(req.params, req.material)
-        return DeviceModel(req.params, Ca, Cb, self.beams, req.name)
+        model = DeviceModel(req.params, Ca, Cb, self.beams, req.name)
+        if req.prepared:                    # uploaded: the host copy is no longer needed
+            req.Ca = req.Cb = None
+            req.prepared = False
+        return model
 
     def make_batch(self, members):
         from .batch import DeviceBatch

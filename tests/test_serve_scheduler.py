@@ -204,3 +204,26 @@ def test_failed_preparation_is_left_to_admission():
     res = {r.rid: r for r in s.results}
     assert res[1].error.startswith('MemoryError')     # reported by admit, not prepare
     assert res[0].error is None and res[2].error is None
+
+
+def test_hip_backend_prepare_computes_host_coefficients():
+    # prepare() needs no device: it is called unbound here
+    import numpy as np
+    from sar_pixel_simulator_amd import workloads
+    from sar_pixel_simulator_amd.hip.serve import HipBackend
+    from sar_pixel_simulator_amd.materials import compute_material_arrays, synthetic
+    pc = workloads.small_params(nx=25, ny=23, nz=27, nz_mat=9)
+    g = pc.geom_params
+    m = synthetic.rough_soil(g.size_x, g.size_y, g.size_z_material, g.cell_size,
+                             workloads.lam_of(pc), 5, 3)
+    req = Request(rid=0, params=pc, material=m)
+    HipBackend.prepare(None, req)
+    Ca, Cb = compute_material_arrays(pc, m)
+    assert req.prepared and np.array_equal(req.Ca, Ca) and np.array_equal(req.Cb, Cb)
+    # arrays the caller gave, and device specs, are left alone
+    given = Request(rid=1, params=pc, Ca=Ca, Cb=Cb)
+    HipBackend.prepare(None, given)
+    assert not given.prepared and given.Ca is Ca
+    spec = Request(rid=2, params=pc, spec=object())
+    HipBackend.prepare(None, spec)
+    assert not spec.prepared and spec.Ca is None
