@@ -261,8 +261,9 @@ def main():
     if args.pmc_child:
         args.no_cpu = args.no_extras = args.no_traffic = True
     if args.gpus != 1:
-        print(json.dumps({'error': 'this build runs on one GPU; multi-GPU runs are not '
-                                   'part of it (see DESIGN.md, sharding)',
+        print(json.dumps({'error': 'the engine runs on one GPU; independent models shard '
+                                   'across GPUs as separate processes, with no device '
+                                   'communication, so there is no multi-GPU run to measure',
                           'n_gpus': args.gpus}))
         sys.exit(2)
 
