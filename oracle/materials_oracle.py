@@ -1,7 +1,8 @@
 """ORACLE -- test infrastructure only.
 
 numpy restatement of the device material builder (fdtd_sim_build_materials)
-used by tests/ as the checker; the product never imports this module.
+that tests/ compare the device results against; the product never imports
+this module.
 
 Per cell, in order: vacuum; a voxel inclusion where z < vox_z1 and the
 cell's draw 3 is below vox_frac; the last cylinder containing the cell

@@ -1,8 +1,9 @@
 """ORACLE — test infrastructure only.
 
-CPU restatement of the reference's hot path, used by tests/, the smoke
-check and bench.py's cpu_baseline leg as the checker.  The product never
-imports this module.
+CPU restatement of the reference's hot path.  Only tests/, the smoke
+check and bench.py's cpu_baseline leg use it, to compare the HIP library's
+results against (and, in bench.py, to time the CPU path).  The product
+never imports this module.
 
 * ``OracleSim`` drives the C loops in fdtd_oracle_impl.h (FDTD time loop of
   fdtdsarsim/cpu/main_sim.py:85-195 and cpu/incident_sim.py:67-171).
