@@ -232,6 +232,32 @@ def test_x_face_taps_bitwise(nx):
         sim.destroy()
 
 
+@pytest.mark.parametrize('nx', [22, 23])
+def test_minimum_interior_bitwise(nx):
+    # npml 10: nx = 22 is the smallest grid accepted (2 interior columns),
+    # where both x faces lie on one column (x = 11); at 23 they are adjacent
+    pc, Ca, Cb = H.soil_case(nx=nx, ny=24, nz=26, nz_mat=8, steps=60, seed=17,
+                             dft_accumulation='reference')
+    say('case: minimum interior', nx, 'x 24 x 26, 60 steps')
+    beam = H.synthetic_beam(pc)
+    ref = H.oracle_main(pc, Ca, Cb, beam)
+    oph = ref.phasors()
+    for fused, launch in ((False, None), (False, (1, 2, 2, 2, 0)), (False, (2, 3, 4, 1, 1)),
+                          (True, None), (2, None)):
+        say('  stepper', fused, 'launch', launch)
+        sim = make_sim(pc, Ca, Cb, beam, launch=launch, fused=fused)
+        sim.run(60)
+        sim.sync()
+        for n in native.FIELD_NAMES:
+            assert_same(sim.field(n), ref.fields[n], f'{n} {fused} {launch}')
+        for i, n in enumerate(native.PSI_NAMES):
+            assert_same(sim.psi(i), ref.psi[n], f'{n} {fused} {launch}')
+        ph = sim.phasors()
+        for n in native.PHASOR_NAMES:
+            assert_same(ph[n], oph[n], f'{n} {fused} {launch}')
+        sim.destroy()
+
+
 def test_invalid_launch_is_rejected_and_previous_kept(soil60):
     pc, Ca, Cb, beam, ref = soil60
     sim = make_sim(pc, Ca, Cb, beam, launch=(2, 0, 4), fused=False)
