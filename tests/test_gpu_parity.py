@@ -322,6 +322,49 @@ def test_split_and_fused_agree_long_run():
             assert_same(other[1][n], out[0][1][n], n)
 
 
+def test_split_and_rows_agree_at_largest_size():
+    """The largest single domain, 1523 x 1523 x 271 (6.3e8 cells), 240 steps
+    on device-built snow over rough soil: the split stepper and the
+    row-parallel fused stepper, two independent implementations of the
+    step, give bit-identical fields and phasors."""
+    from sar_pixel_simulator_amd import workloads
+    from sar_pixel_simulator_amd.materials import device
+    nsteps = 240
+    pc = workloads.largest_params()
+    g = pc.geom_params
+    g.size_t = g.size_t_inc = nsteps
+    snow = int(round(1.5 / 0.021))
+    spec = device.snow_over_soil_spec(pc, g.size_z_material - snow, snow, seed=3)
+    beam = workloads.synthetic_beam_planes(pc, 0, nsteps)
+    say('case: largest 1523x1523x271, 240 steps, split vs row-parallel')
+
+    def run(fused):
+        native.init(0)
+        sim = native.Sim(common.main_geom(pc), common.coeffs_for(pc))
+        sim.set_stepper(fused)
+        b = native.Beam(g.size_t_inc, g.size_y, g.size_x, *beam)
+        sim.attach_beam(b)
+        sim.build_materials(*device.to_c(spec, pc))
+        sim.run(nsteps)
+        sim.sync()
+        return sim, b
+
+    sim, b = run(False)
+    ref = {n: sim.field(n) for n in native.FIELD_NAMES}
+    ref_ph = sim.phasors()
+    sim.destroy()
+    b.release()
+    assert np.abs(ref['ey']).max() > 0 and np.abs(ref_ph['ey_ph_l']).max() > 0
+    sim, b = run(2)
+    for n in native.FIELD_NAMES:
+        assert_same(sim.field(n), ref.pop(n), n)
+    ph = sim.phasors()
+    for n in native.PHASOR_NAMES:
+        assert_same(ph[n], ref_ph[n], n)
+    sim.destroy()
+    b.release()
+
+
 @pytest.mark.parametrize('fused,launch', [(True, None), (False, None), (False, (2, 8, 4)),
                                           (False, (1, 1, 1, 4, 0)), (False, (1, 4, 1, 4, 2)),
                                           (2, None)],
