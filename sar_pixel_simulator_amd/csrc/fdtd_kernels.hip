@@ -168,6 +168,132 @@ struct EAround {
     float ez_c, ez_yp, ez_xp;  // ez(z), ez(y+1), ez(x+1)
 };
 
+// What the CPML corrections of one cell read, loaded ahead by the split
+// kernels: the psi values (at most one face per axis applies, d or u, l or
+// r, f or b, each with two psi arrays), with the cell's plane loads, and
+// the x-face profile coefficients, once per lane (its x is fixed as it
+// marches z).  A CPML cell then waits for no dependent load after its Yee
+// update.  The loads below mirror h_cpml / e_cpml face for face
+// (conditions, slots, offsets).  761 x 761 x 203: step -0.4 %; 16 small
+// soil models as one batch (35 - 52 % CPML cells): -3.4 %.
+struct PsiPre {
+    float z[2], x[2], y[2];
+    float xpb, xpc;        // x-face profile coefficients (b, c) of the cell
+};
+
+__device__ __forceinline__ void h_xprof_load(PsiPre &q, const Desc *__restrict__ d, const StepK &k, int x) {
+    const int nx = k.nx, np = k.np;
+    q.xpb = q.xpc = 0.f;
+    if (x < np - 1) {
+        q.xpb = d->prof[BH_DLF][x];
+        q.xpc = d->prof[CH_DLF][x];
+    } else if (x >= nx - np && x < nx) {
+        q.xpb = d->prof[BH_URB][x - (nx - np)];
+        q.xpc = d->prof[CH_URB][x - (nx - np)];
+    }
+}
+
+__device__ __forceinline__ void e_xprof_load(PsiPre &q, const Desc *__restrict__ d, const StepK &k, int x) {
+    const int nx = k.nx, np = k.np;
+    q.xpb = q.xpc = 0.f;
+    if (x < np) {
+        q.xpb = d->prof[BE_DLF][x];
+        q.xpc = d->prof[CE_DLF][x];
+    } else if (x >= nx - np && x < nx) {
+        q.xpb = d->prof[BE_URB][x - (nx - np)];
+        q.xpc = d->prof[CE_URB][x - (nx - np)];
+    }
+}
+
+__device__ __forceinline__ void h_psi_load(PsiPre &q, const StepK &k, const gfloat *hr, int x, int y, int z) {
+    q.z[0] = q.z[1] = q.x[0] = q.x[1] = q.y[0] = q.y[1] = 0.f;
+    const int nx = k.nx, ny = k.ny, nz = k.nz, np = k.np;
+    const uint32_t sy = k.sy;
+    const bool xin = x < nx - 1, yin = y < ny - 1;
+    if (z >= nz - 1) return;                                // h_cpml runs below the top plane
+    if (yin && xin) {
+        if (z < np - 1) {
+            const uint32_t p = (uint32_t)z * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
+            q.z[0] = *at(hr, psi_off(k, pHyz_d - 12) + p);
+            q.z[1] = *at(hr, psi_off(k, pHxz_d - 12) + p);
+        }
+        if (z >= nz - np) {
+            const uint32_t p = (uint32_t)(z - (nz - np)) * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
+            q.z[0] = *at(hr, psi_off(k, pHyz_u - 12) + p);
+            q.z[1] = *at(hr, psi_off(k, pHxz_u - 12) + p);
+        }
+    }
+    if (yin) {
+        if (x < np - 1) {
+            const uint32_t p = (uint32_t)z * k.lr_sz + (uint32_t)y * np + (uint32_t)x;
+            q.x[0] = *at(hr, psi_off(k, pHzx_l - 12) + p);
+            q.x[1] = *at(hr, psi_off(k, pHyx_l - 12) + p);
+        }
+        if (x >= nx - np && xin) {
+            const uint32_t p = (uint32_t)z * k.lr_sz + (uint32_t)y * np + (uint32_t)(x - (nx - np));
+            q.x[0] = *at(hr, psi_off(k, pHzx_r - 12) + p);
+            q.x[1] = *at(hr, psi_off(k, pHyx_r - 12) + p);
+        }
+    }
+    if (xin) {
+        if (y < np - 1) {
+            const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)y * sy + (uint32_t)x;
+            q.y[0] = *at(hr, psi_off(k, pHxy_f - 12) + p);
+            q.y[1] = *at(hr, psi_off(k, pHzy_f - 12) + p);
+        }
+        if (y >= ny - np && yin) {
+            const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)(y - (ny - np)) * sy + (uint32_t)x;
+            q.y[0] = *at(hr, psi_off(k, pHxy_b - 12) + p);
+            q.y[1] = *at(hr, psi_off(k, pHzy_b - 12) + p);
+        }
+    }
+}
+
+__device__ __forceinline__ void e_psi_load(PsiPre &q, const StepK &k, const gfloat *psi, int x, int y, int z) {
+    q.z[0] = q.z[1] = q.x[0] = q.x[1] = q.y[0] = q.y[1] = 0.f;
+    const int nx = k.nx, ny = k.ny, nz = k.nz, np = k.np;
+    const uint32_t sy = k.sy;
+    const bool xlo = x >= 1, ylo = y >= 1, zlo = z >= 1;
+    const bool xin = x < nx - 1, yin = y < ny - 1;
+    if (z >= nz - 1) return;                                // e_cpml runs below the top plane
+    if (yin && xin) {
+        if (zlo && z < np) {
+            const uint32_t p = (uint32_t)z * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
+            q.z[0] = *at(psi, psi_off(k, pEyz_d) + p);
+            q.z[1] = *at(psi, psi_off(k, pExz_d) + p);
+        }
+        if (z >= nz - np) {
+            const uint32_t p = (uint32_t)(z - (nz - np)) * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
+            q.z[0] = *at(psi, psi_off(k, pEyz_u) + p);
+            q.z[1] = *at(psi, psi_off(k, pExz_u) + p);
+        }
+    }
+    if (yin) {
+        if (xlo && x < np) {
+            const uint32_t p = (uint32_t)z * k.lr_sz + (uint32_t)y * np + (uint32_t)x;
+            q.x[0] = *at(psi, psi_off(k, pEzx_l) + p);
+            q.x[1] = *at(psi, psi_off(k, pEyx_l) + p);
+        }
+        if (x >= nx - np && xin) {
+            const uint32_t p = (uint32_t)z * k.lr_sz + (uint32_t)y * np + (uint32_t)(x - (nx - np));
+            q.x[0] = *at(psi, psi_off(k, pEzx_r) + p);
+            q.x[1] = *at(psi, psi_off(k, pEyx_r) + p);
+        }
+    }
+    if (xin) {
+        if (ylo && y < np) {
+            const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)y * sy + (uint32_t)x;
+            q.y[0] = *at(psi, psi_off(k, pExy_f) + p);
+            q.y[1] = *at(psi, psi_off(k, pEzy_f) + p);
+        }
+        if (y >= ny - np && yin) {
+            const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)(y - (ny - np)) * sy + (uint32_t)x;
+            q.y[0] = *at(psi, psi_off(k, pExy_b) + p);
+            q.y[1] = *at(psi, psi_off(k, pEzy_b) + p);
+        }
+    }
+}
+
 // H half-step of one cell: Yee update (cpu/field_updates.py:198-240), the
 // six H-CPML corrections in the reference order d,u,l,r,f,b (:243-366,
 // cpu/fdtd_sim.py:104-133), then the E_inc injection at tfsf_z in MAIN mode
@@ -177,11 +303,11 @@ struct EAround {
 // null, the updated value is written to hw (hw == hr updates in place).
 __device__ __forceinline__ void h_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
                                        const EAround &e, float &hx, float &hy, float &hz,
-                                       const gfloat *hr, gfloat *hw);
+                                       const gfloat *hr, gfloat *hw, const PsiPre *pre);
 
 __device__ __forceinline__ void h_cell(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
                                        const EAround &e, float &hx, float &hy, float &hz,
-                                       const gfloat *hr, gfloat *hw) {
+                                       const gfloat *hr, gfloat *hw, const PsiPre *pre = nullptr) {
     const int nx = k.nx, ny = k.ny, nz = k.nz, np = k.np;
     const float Db = k.Db;
     const bool xin = x < nx - 1, yin = y < ny - 1;
@@ -190,7 +316,7 @@ __device__ __forceinline__ void h_cell(const Desc *__restrict__ d, const StepK &
         if (y >= 1 && yin && xin) hy -= Db * ((e.ex_n - e.ex_c) - (e.ez_xp - e.ez_c));
         if (z >= 1 && yin && xin) hz -= Db * ((e.ey_xp - e.ey_c) - (e.ex_yp - e.ex_c));
         if (z < np - 1 || z >= nz - np || x < np - 1 || x >= nx - np || y < np - 1 || y >= ny - np)
-            h_cpml(d, k, x, y, z, e, hx, hy, hz, hr, hw);
+            h_cpml(d, k, x, y, z, e, hx, hy, hz, hr, hw, pre);
     }
     if (k.inj && z == k.tfsf_z) {
         const uint32_t q = (uint32_t)y * nx + x;
@@ -202,10 +328,12 @@ __device__ __forceinline__ void h_cell(const Desc *__restrict__ d, const StepK &
 // The six H-CPML corrections of one cell below the top plane, in the
 // reference order d,u,l,r,f,b, after its Yee update.  psi is read from the
 // H block hr and, when hw is not null, written to the H block hw (hw == hr
-// updates in place).
+// updates in place).  With pre, the psi values and the x-face profile
+// coefficients come from the split kernel's loads ahead (PsiPre); without,
+// they are loaded here (fused steppers).
 __device__ __forceinline__ void h_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
                                        const EAround &e, float &hx, float &hy, float &hz,
-                                       const gfloat *hr, gfloat *hw) {
+                                       const gfloat *hr, gfloat *hw, const PsiPre *pre) {
     // the cell's x as an opaque value: the absorber's x-dependent profile
     // loads and psi offsets are then formed here, when a cell needs them,
     // instead of being hoisted out of the stencil loops into registers
@@ -218,8 +346,10 @@ __device__ __forceinline__ void h_cpml(const Desc *__restrict__ d, const StepK &
         if (z < np - 1) {                                       // down
             const float bh = d->prof[BH_DLF][z], ch = d->prof[CH_DLF][z];
             const uint32_t p = (uint32_t)z * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
-            const float a = (bh * *at(hr, psi_off(k, pHyz_d - 12) + p)) + ((ch * (e.ex_n - e.ex_c)) / dx);
-            const float b = (bh * *at(hr, psi_off(k, pHxz_d - 12) + p)) + ((ch * (e.ey_n - e.ey_c)) / dx);
+            const float pa = pre ? pre->z[0] : *at(hr, psi_off(k, pHyz_d - 12) + p);
+            const float a = (bh * pa) + ((ch * (e.ex_n - e.ex_c)) / dx);
+            const float pb = pre ? pre->z[1] : *at(hr, psi_off(k, pHxz_d - 12) + p);
+            const float b = (bh * pb) + ((ch * (e.ey_n - e.ey_c)) / dx);
             if (hw) { *at(hw, psi_off(k, pHyz_d - 12) + p) = a; *at(hw, psi_off(k, pHxz_d - 12) + p) = b; }
             hy -= dtm0 * a;
             hx += dtm0 * b;
@@ -228,8 +358,10 @@ __device__ __forceinline__ void h_cpml(const Desc *__restrict__ d, const StepK &
             const int zl = z - (nz - np);
             const float bh = d->prof[BH_URB][zl], ch = d->prof[CH_URB][zl];
             const uint32_t p = (uint32_t)zl * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
-            const float a = (bh * *at(hr, psi_off(k, pHyz_u - 12) + p)) + ((ch * (e.ex_n - e.ex_c)) / dx);
-            const float b = (bh * *at(hr, psi_off(k, pHxz_u - 12) + p)) + ((ch * (e.ey_n - e.ey_c)) / dx);
+            const float pa = pre ? pre->z[0] : *at(hr, psi_off(k, pHyz_u - 12) + p);
+            const float a = (bh * pa) + ((ch * (e.ex_n - e.ex_c)) / dx);
+            const float pb = pre ? pre->z[1] : *at(hr, psi_off(k, pHxz_u - 12) + p);
+            const float b = (bh * pb) + ((ch * (e.ey_n - e.ey_c)) / dx);
             if (hw) { *at(hw, psi_off(k, pHyz_u - 12) + p) = a; *at(hw, psi_off(k, pHxz_u - 12) + p) = b; }
             hy -= dtm0 * a;
             hx += dtm0 * b;
@@ -237,20 +369,24 @@ __device__ __forceinline__ void h_cpml(const Desc *__restrict__ d, const StepK &
     }
     if (yin) {
         if (x < np - 1) {                                       // left
-            const float bh = d->prof[BH_DLF][x], ch = d->prof[CH_DLF][x];
+            const float bh = pre ? pre->xpb : d->prof[BH_DLF][x], ch = pre ? pre->xpc : d->prof[CH_DLF][x];
             const uint32_t p = (uint32_t)z * k.lr_sz + (uint32_t)y * np + (uint32_t)x;
-            const float a = (bh * *at(hr, psi_off(k, pHzx_l - 12) + p)) + ((ch * (e.ey_xp - e.ey_c)) / dx);
-            const float b = (bh * *at(hr, psi_off(k, pHyx_l - 12) + p)) + ((ch * (e.ez_xp - e.ez_c)) / dx);
+            const float pa = pre ? pre->x[0] : *at(hr, psi_off(k, pHzx_l - 12) + p);
+            const float a = (bh * pa) + ((ch * (e.ey_xp - e.ey_c)) / dx);
+            const float pb = pre ? pre->x[1] : *at(hr, psi_off(k, pHyx_l - 12) + p);
+            const float b = (bh * pb) + ((ch * (e.ez_xp - e.ez_c)) / dx);
             if (hw) { *at(hw, psi_off(k, pHzx_l - 12) + p) = a; *at(hw, psi_off(k, pHyx_l - 12) + p) = b; }
             hz -= dtm0 * a;
             hy += dtm0 * b;
         }
         if (x >= nx - np && xin) {                              // right
             const int xl = x - (nx - np);
-            const float bh = d->prof[BH_URB][xl], ch = d->prof[CH_URB][xl];
+            const float bh = pre ? pre->xpb : d->prof[BH_URB][xl], ch = pre ? pre->xpc : d->prof[CH_URB][xl];
             const uint32_t p = (uint32_t)z * k.lr_sz + (uint32_t)y * np + (uint32_t)xl;
-            const float a = (bh * *at(hr, psi_off(k, pHzx_r - 12) + p)) + ((ch * (e.ey_xp - e.ey_c)) / dx);
-            const float b = (bh * *at(hr, psi_off(k, pHyx_r - 12) + p)) + ((ch * (e.ez_xp - e.ez_c)) / dx);
+            const float pa = pre ? pre->x[0] : *at(hr, psi_off(k, pHzx_r - 12) + p);
+            const float a = (bh * pa) + ((ch * (e.ey_xp - e.ey_c)) / dx);
+            const float pb = pre ? pre->x[1] : *at(hr, psi_off(k, pHyx_r - 12) + p);
+            const float b = (bh * pb) + ((ch * (e.ez_xp - e.ez_c)) / dx);
             if (hw) { *at(hw, psi_off(k, pHzx_r - 12) + p) = a; *at(hw, psi_off(k, pHyx_r - 12) + p) = b; }
             hz -= dtm0 * a;
             hy += dtm0 * b;
@@ -260,8 +396,10 @@ __device__ __forceinline__ void h_cpml(const Desc *__restrict__ d, const StepK &
         if (y < np - 1) {                                       // front
             const float bh = d->prof[BH_DLF][y], ch = d->prof[CH_DLF][y];
             const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)y * sy + (uint32_t)x;
-            const float a = (bh * *at(hr, psi_off(k, pHxy_f - 12) + p)) + ((ch * (e.ez_yp - e.ez_c)) / dx);
-            const float b = (bh * *at(hr, psi_off(k, pHzy_f - 12) + p)) + ((ch * (e.ex_yp - e.ex_c)) / dx);
+            const float pa = pre ? pre->y[0] : *at(hr, psi_off(k, pHxy_f - 12) + p);
+            const float a = (bh * pa) + ((ch * (e.ez_yp - e.ez_c)) / dx);
+            const float pb = pre ? pre->y[1] : *at(hr, psi_off(k, pHzy_f - 12) + p);
+            const float b = (bh * pb) + ((ch * (e.ex_yp - e.ex_c)) / dx);
             if (hw) { *at(hw, psi_off(k, pHxy_f - 12) + p) = a; *at(hw, psi_off(k, pHzy_f - 12) + p) = b; }
             hx -= dtm0 * a;
             hz += dtm0 * b;
@@ -270,8 +408,10 @@ __device__ __forceinline__ void h_cpml(const Desc *__restrict__ d, const StepK &
             const int yl = y - (ny - np);
             const float bh = d->prof[BH_URB][yl], ch = d->prof[CH_URB][yl];
             const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)yl * sy + (uint32_t)x;
-            const float a = (bh * *at(hr, psi_off(k, pHxy_b - 12) + p)) + ((ch * (e.ez_yp - e.ez_c)) / dx);
-            const float b = (bh * *at(hr, psi_off(k, pHzy_b - 12) + p)) + ((ch * (e.ex_yp - e.ex_c)) / dx);
+            const float pa = pre ? pre->y[0] : *at(hr, psi_off(k, pHxy_b - 12) + p);
+            const float a = (bh * pa) + ((ch * (e.ez_yp - e.ez_c)) / dx);
+            const float pb = pre ? pre->y[1] : *at(hr, psi_off(k, pHzy_b - 12) + p);
+            const float b = (bh * pb) + ((ch * (e.ex_yp - e.ex_c)) / dx);
             if (hw) { *at(hw, psi_off(k, pHxy_b - 12) + p) = a; *at(hw, psi_off(k, pHzy_b - 12) + p) = b; }
             hx -= dtm0 * a;
             hz += dtm0 * b;
@@ -294,11 +434,12 @@ struct HAround {
 // (INCIDENT, cpu/incident_sim.py:142-144).  ca, cb are read only below the
 // top plane (z < nz - 1).
 __device__ __forceinline__ void e_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
-                                       float cb, const HAround &h, float &ex, float &ey, float &ez);
+                                       float cb, const HAround &h, float &ex, float &ey, float &ez,
+                                       const PsiPre *pre);
 
 __device__ __forceinline__ void e_cell(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
                                        float ca, float cb, const HAround &h, float &ex, float &ey,
-                                       float &ez) {
+                                       float &ez, const PsiPre *pre = nullptr) {
     const int nx = k.nx, ny = k.ny, nz = k.nz, np = k.np;
     const bool xlo = x >= 1, ylo = y >= 1, zlo = z >= 1;
     const bool xin = x < nx - 1, yin = y < ny - 1;
@@ -307,7 +448,7 @@ __device__ __forceinline__ void e_cell(const Desc *__restrict__ d, const StepK &
         if (zlo && yin && xlo && xin) ey = (ey * ca) + (cb * ((h.hx_c - h.hx_zm) - (h.hz_c - h.hz_xm)));
         if (ylo && yin && xlo && xin) ez = (ez * ca) + (cb * ((h.hy_c - h.hy_xm) - (h.hx_c - h.hx_ym)));
         if (z < np || z >= nz - np || x < np || x >= nx - np || y < np || y >= ny - np)
-            e_cpml(d, k, x, y, z, cb, h, ex, ey, ez);
+            e_cpml(d, k, x, y, z, cb, h, ex, ey, ez, pre);
     }
     if (z == k.tfsf_z) {
         const uint32_t q = (uint32_t)y * nx + x;
@@ -322,9 +463,11 @@ __device__ __forceinline__ void e_cell(const Desc *__restrict__ d, const StepK &
 }
 
 // The six E-CPML corrections of one cell below the top plane, in the
-// reference order d,u,l,r,f,b (psi updated in place), after its Yee update.
+// reference order d,u,l,r,f,b (psi updated in place), after its Yee update;
+// psi values and x-face profile coefficients from pre as in h_cpml.
 __device__ __forceinline__ void e_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
-                                       float cb, const HAround &h, float &ex, float &ey, float &ez) {
+                                       float cb, const HAround &h, float &ex, float &ey, float &ez,
+                                       const PsiPre *pre) {
     asm volatile("" : "+v"(x));   // as in h_cpml
     const int nx = k.nx, ny = k.ny, nz = k.nz, np = k.np;
     const uint32_t sy = k.sy;
@@ -337,8 +480,10 @@ __device__ __forceinline__ void e_cpml(const Desc *__restrict__ d, const StepK &
         if (zlo && z < np) {                                    // down
             const float be = d->prof[BE_DLF][z], ce = d->prof[CE_DLF][z];
             const uint32_t p = (uint32_t)z * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
-            const float a = (be * *at(psi, psi_off(k, pEyz_d) + p)) + ((ce * (h.hx_c - h.hx_zm)) / dx);
-            const float b = (be * *at(psi, psi_off(k, pExz_d) + p)) + ((ce * (h.hy_c - h.hy_zm)) / dx);
+            const float pa = pre ? pre->z[0] : *at(psi, psi_off(k, pEyz_d) + p);
+            const float a = (be * pa) + ((ce * (h.hx_c - h.hx_zm)) / dx);
+            const float pb = pre ? pre->z[1] : *at(psi, psi_off(k, pExz_d) + p);
+            const float b = (be * pb) + ((ce * (h.hy_c - h.hy_zm)) / dx);
             *at(psi, psi_off(k, pEyz_d) + p) = a; *at(psi, psi_off(k, pExz_d) + p) = b;
             ey += cbdx * a;
             ex -= cbdx * b;
@@ -347,8 +492,10 @@ __device__ __forceinline__ void e_cpml(const Desc *__restrict__ d, const StepK &
             const int zl = z - (nz - np);
             const float be = d->prof[BE_URB][zl], ce = d->prof[CE_URB][zl];
             const uint32_t p = (uint32_t)zl * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
-            const float a = (be * *at(psi, psi_off(k, pEyz_u) + p)) + ((ce * (h.hx_c - h.hx_zm)) / dx);
-            const float b = (be * *at(psi, psi_off(k, pExz_u) + p)) + ((ce * (h.hy_c - h.hy_zm)) / dx);
+            const float pa = pre ? pre->z[0] : *at(psi, psi_off(k, pEyz_u) + p);
+            const float a = (be * pa) + ((ce * (h.hx_c - h.hx_zm)) / dx);
+            const float pb = pre ? pre->z[1] : *at(psi, psi_off(k, pExz_u) + p);
+            const float b = (be * pb) + ((ce * (h.hy_c - h.hy_zm)) / dx);
             *at(psi, psi_off(k, pEyz_u) + p) = a; *at(psi, psi_off(k, pExz_u) + p) = b;
             ey += cbdx * a;
             ex -= cbdx * b;
@@ -356,20 +503,24 @@ __device__ __forceinline__ void e_cpml(const Desc *__restrict__ d, const StepK &
     }
     if (yin) {
         if (xlo && x < np) {                                    // left
-            const float be = d->prof[BE_DLF][x], ce = d->prof[CE_DLF][x];
+            const float be = pre ? pre->xpb : d->prof[BE_DLF][x], ce = pre ? pre->xpc : d->prof[CE_DLF][x];
             const uint32_t p = (uint32_t)z * k.lr_sz + (uint32_t)y * np + (uint32_t)x;
-            const float a = (be * *at(psi, psi_off(k, pEzx_l) + p)) + ((ce * (h.hy_c - h.hy_xm)) / dx);
-            const float b = (be * *at(psi, psi_off(k, pEyx_l) + p)) + ((ce * (h.hz_c - h.hz_xm)) / dx);
+            const float pa = pre ? pre->x[0] : *at(psi, psi_off(k, pEzx_l) + p);
+            const float a = (be * pa) + ((ce * (h.hy_c - h.hy_xm)) / dx);
+            const float pb = pre ? pre->x[1] : *at(psi, psi_off(k, pEyx_l) + p);
+            const float b = (be * pb) + ((ce * (h.hz_c - h.hz_xm)) / dx);
             *at(psi, psi_off(k, pEzx_l) + p) = a; *at(psi, psi_off(k, pEyx_l) + p) = b;
             ez += cbdx * a;
             ey -= cbdx * b;
         }
         if (x >= nx - np && xin) {                              // right
             const int xl = x - (nx - np);
-            const float be = d->prof[BE_URB][xl], ce = d->prof[CE_URB][xl];
+            const float be = pre ? pre->xpb : d->prof[BE_URB][xl], ce = pre ? pre->xpc : d->prof[CE_URB][xl];
             const uint32_t p = (uint32_t)z * k.lr_sz + (uint32_t)y * np + (uint32_t)xl;
-            const float a = (be * *at(psi, psi_off(k, pEzx_r) + p)) + ((ce * (h.hy_c - h.hy_xm)) / dx);
-            const float b = (be * *at(psi, psi_off(k, pEyx_r) + p)) + ((ce * (h.hz_c - h.hz_xm)) / dx);
+            const float pa = pre ? pre->x[0] : *at(psi, psi_off(k, pEzx_r) + p);
+            const float a = (be * pa) + ((ce * (h.hy_c - h.hy_xm)) / dx);
+            const float pb = pre ? pre->x[1] : *at(psi, psi_off(k, pEyx_r) + p);
+            const float b = (be * pb) + ((ce * (h.hz_c - h.hz_xm)) / dx);
             *at(psi, psi_off(k, pEzx_r) + p) = a; *at(psi, psi_off(k, pEyx_r) + p) = b;
             ez += cbdx * a;
             ey -= cbdx * b;
@@ -379,8 +530,10 @@ __device__ __forceinline__ void e_cpml(const Desc *__restrict__ d, const StepK &
         if (ylo && y < np) {                                    // front
             const float be = d->prof[BE_DLF][y], ce = d->prof[CE_DLF][y];
             const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)y * sy + (uint32_t)x;
-            const float a = (be * *at(psi, psi_off(k, pExy_f) + p)) + ((ce * (h.hz_c - h.hz_ym)) / dx);
-            const float b = (be * *at(psi, psi_off(k, pEzy_f) + p)) + ((ce * (h.hx_c - h.hx_ym)) / dx);
+            const float pa = pre ? pre->y[0] : *at(psi, psi_off(k, pExy_f) + p);
+            const float a = (be * pa) + ((ce * (h.hz_c - h.hz_ym)) / dx);
+            const float pb = pre ? pre->y[1] : *at(psi, psi_off(k, pEzy_f) + p);
+            const float b = (be * pb) + ((ce * (h.hx_c - h.hx_ym)) / dx);
             *at(psi, psi_off(k, pExy_f) + p) = a; *at(psi, psi_off(k, pEzy_f) + p) = b;
             ex += cbdx * a;
             ez -= cbdx * b;
@@ -389,8 +542,10 @@ __device__ __forceinline__ void e_cpml(const Desc *__restrict__ d, const StepK &
             const int yl = y - (ny - np);
             const float be = d->prof[BE_URB][yl], ce = d->prof[CE_URB][yl];
             const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)yl * sy + (uint32_t)x;
-            const float a = (be * *at(psi, psi_off(k, pExy_b) + p)) + ((ce * (h.hz_c - h.hz_ym)) / dx);
-            const float b = (be * *at(psi, psi_off(k, pEzy_b) + p)) + ((ce * (h.hx_c - h.hx_ym)) / dx);
+            const float pa = pre ? pre->y[0] : *at(psi, psi_off(k, pExy_b) + p);
+            const float a = (be * pa) + ((ce * (h.hz_c - h.hz_ym)) / dx);
+            const float pb = pre ? pre->y[1] : *at(psi, psi_off(k, pEzy_b) + p);
+            const float b = (be * pb) + ((ce * (h.hx_c - h.hx_ym)) / dx);
             *at(psi, psi_off(k, pExy_b) + p) = a; *at(psi, psi_off(k, pEzy_b) + p) = b;
             ex += cbdx * a;
             ez -= cbdx * b;
@@ -630,6 +785,12 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     }
     HPlane<V> P;
     h_plane_load(P, Ex, Ey, Ez, Hx, Hy, Hz, o, sy, sz, act, z0 < nz - 1, yin, next_mem);
+    PsiPre pq[V] = {};
+#pragma unroll
+    for (int c = 0; c < V; ++c) {
+        h_xprof_load(pq[c], d, kg, xb + c);
+        if (xb + c < nx) h_psi_load(pq[c], kg, psiH, xb + c, y, z0);
+    }
     if (t >= st.nt) return;                            // uniform per block
     const StepK k = step_at(kg, st, t);
     for (int z = z0;;) {
@@ -648,7 +809,7 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
                 e.ey_xp = c + 1 < V ? eyc[c + 1 < V ? c + 1 : 0] : ey_next;
                 e.ez_c = P.ez[c]; e.ez_yp = P.ezyp[c];
                 e.ez_xp = c + 1 < V ? P.ez[c + 1 < V ? c + 1 : 0] : ez_next;
-                h_cell(d, k, xb + c, y, z, e, P.hx[c], P.hy[c], P.hz[c], psiH, psiH);
+                h_cell(d, k, xb + c, y, z, e, P.hx[c], P.hy[c], P.hz[c], psiH, psiH, &pq[c]);
             }
         }
         if (act) {
@@ -661,6 +822,9 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
         vcopy(exc, P.exn);
         vcopy(eyc, P.eyn);
         h_plane_load(P, Ex, Ey, Ez, Hx, Hy, Hz, o, sy, sz, act, z < nz - 1, yin, next_mem);
+#pragma unroll
+        for (int c = 0; c < V; ++c)
+            if (xb + c < nx) h_psi_load(pq[c], k, psiH, xb + c, y, z);
     }
 }
 
@@ -752,6 +916,12 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     EPlane<V> P;
     e_plane_load(P, Ex, Ey, Ez, Hx, Hy, Hz, Ca, Cb, ca_u, cb_u, o, sy, act, z0 < nz - 1,
                  z0 < nz - 1 && z0 < uni_z, ylo, prev_mem);
+    PsiPre pq[V] = {};
+#pragma unroll
+    for (int c = 0; c < V; ++c) {
+        e_xprof_load(pq[c], d, kg, xb + c);
+        if (xb + c < nx) e_psi_load(pq[c], kg, kg.psiE, xb + c, y, z0);
+    }
     if (t >= st.nt) return;                            // uniform per block
     const StepK k = step_at(kg, st, t);
     for (int z = z0;;) {
@@ -770,7 +940,7 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
                 h.hz_ym = P.hzym[c]; h.hx_ym = P.hxym[c];
                 h.hz_xm = c > 0 ? P.hz[c > 0 ? c - 1 : 0] : hz_prev;
                 h.hy_xm = c > 0 ? P.hy[c > 0 ? c - 1 : 0] : hy_prev;
-                e_cell(d, k, xb + c, y, z, P.ca[c], P.cb[c], h, P.ex[c], P.ey[c], P.ez[c]);
+                e_cell(d, k, xb + c, y, z, P.ca[c], P.cb[c], h, P.ex[c], P.ey[c], P.ez[c], &pq[c]);
             }
         }
         if (act) {
@@ -785,6 +955,9 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
         vcopy(hyzm, P.hy);
         e_plane_load(P, Ex, Ey, Ez, Hx, Hy, Hz, Ca, Cb, ca_u, cb_u, o, sy, act, z < nz - 1,
                      z < nz - 1 && z < uni_z, ylo, prev_mem);
+#pragma unroll
+        for (int c = 0; c < V; ++c)
+            if (xb + c < nx) e_psi_load(pq[c], k, k.psiE, xb + c, y, z);
     }
 }
 
