@@ -168,16 +168,16 @@ struct EAround {
     float ez_c, ez_yp, ez_xp;  // ez(z), ez(y+1), ez(x+1)
 };
 
-// What the CPML corrections of one cell read, loaded ahead by the split
-// kernels: the psi values (at most one face per axis applies, d or u, l or
-// r, f or b, each with two psi arrays), with the cell's plane loads, and
-// the x-face profile coefficients, once per lane (its x is fixed as it
-// marches z).  A CPML cell then waits for no dependent load after its Yee
-// update.  The loads below mirror h_cpml / e_cpml face for face
-// (conditions, slots, offsets).  761 x 761 x 203: step -0.4 %; 16 small
-// soil models as one batch (35 - 52 % CPML cells): -3.4 %.
+// What the x-face CPML corrections of one cell read, loaded ahead by the
+// split kernels: the two psi values of its l or r face with the cell's
+// plane loads, and the face's profile coefficients once per lane (its x is
+// fixed as it marches z).  An x-face cell then waits for no dependent load
+// after its Yee update.  The loads mirror h_cpml / e_cpml (conditions,
+// slots, offsets).  761 x 761 x 203: step -0.6 %; 16 small soil models as
+// one batch: -5 %.  The z- and y-face values loaded ahead as well raised
+// the register count past 64 (fewer wavefronts per SIMD) and gained less.
 struct PsiPre {
-    float z[2], x[2], y[2];
+    float x[2];
     float xpb, xpc;        // x-face profile coefficients (b, c) of the cell
 };
 
@@ -206,23 +206,10 @@ __device__ __forceinline__ void e_xprof_load(PsiPre &q, const Desc *__restrict__
 }
 
 __device__ __forceinline__ void h_psi_load(PsiPre &q, const StepK &k, const gfloat *hr, int x, int y, int z) {
-    q.z[0] = q.z[1] = q.x[0] = q.x[1] = q.y[0] = q.y[1] = 0.f;
+    q.x[0] = q.x[1] = 0.f;
     const int nx = k.nx, ny = k.ny, nz = k.nz, np = k.np;
-    const uint32_t sy = k.sy;
     const bool xin = x < nx - 1, yin = y < ny - 1;
     if (z >= nz - 1) return;                                // h_cpml runs below the top plane
-    if (yin && xin) {
-        if (z < np - 1) {
-            const uint32_t p = (uint32_t)z * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
-            q.z[0] = *at(hr, psi_off(k, pHyz_d - 12) + p);
-            q.z[1] = *at(hr, psi_off(k, pHxz_d - 12) + p);
-        }
-        if (z >= nz - np) {
-            const uint32_t p = (uint32_t)(z - (nz - np)) * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
-            q.z[0] = *at(hr, psi_off(k, pHyz_u - 12) + p);
-            q.z[1] = *at(hr, psi_off(k, pHxz_u - 12) + p);
-        }
-    }
     if (yin) {
         if (x < np - 1) {
             const uint32_t p = (uint32_t)z * k.lr_sz + (uint32_t)y * np + (uint32_t)x;
@@ -235,39 +222,14 @@ __device__ __forceinline__ void h_psi_load(PsiPre &q, const StepK &k, const gflo
             q.x[1] = *at(hr, psi_off(k, pHyx_r - 12) + p);
         }
     }
-    if (xin) {
-        if (y < np - 1) {
-            const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)y * sy + (uint32_t)x;
-            q.y[0] = *at(hr, psi_off(k, pHxy_f - 12) + p);
-            q.y[1] = *at(hr, psi_off(k, pHzy_f - 12) + p);
-        }
-        if (y >= ny - np && yin) {
-            const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)(y - (ny - np)) * sy + (uint32_t)x;
-            q.y[0] = *at(hr, psi_off(k, pHxy_b - 12) + p);
-            q.y[1] = *at(hr, psi_off(k, pHzy_b - 12) + p);
-        }
-    }
 }
 
 __device__ __forceinline__ void e_psi_load(PsiPre &q, const StepK &k, const gfloat *psi, int x, int y, int z) {
-    q.z[0] = q.z[1] = q.x[0] = q.x[1] = q.y[0] = q.y[1] = 0.f;
+    q.x[0] = q.x[1] = 0.f;
     const int nx = k.nx, ny = k.ny, nz = k.nz, np = k.np;
-    const uint32_t sy = k.sy;
-    const bool xlo = x >= 1, ylo = y >= 1, zlo = z >= 1;
+    const bool xlo = x >= 1;
     const bool xin = x < nx - 1, yin = y < ny - 1;
     if (z >= nz - 1) return;                                // e_cpml runs below the top plane
-    if (yin && xin) {
-        if (zlo && z < np) {
-            const uint32_t p = (uint32_t)z * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
-            q.z[0] = *at(psi, psi_off(k, pEyz_d) + p);
-            q.z[1] = *at(psi, psi_off(k, pExz_d) + p);
-        }
-        if (z >= nz - np) {
-            const uint32_t p = (uint32_t)(z - (nz - np)) * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
-            q.z[0] = *at(psi, psi_off(k, pEyz_u) + p);
-            q.z[1] = *at(psi, psi_off(k, pExz_u) + p);
-        }
-    }
     if (yin) {
         if (xlo && x < np) {
             const uint32_t p = (uint32_t)z * k.lr_sz + (uint32_t)y * np + (uint32_t)x;
@@ -278,18 +240,6 @@ __device__ __forceinline__ void e_psi_load(PsiPre &q, const StepK &k, const gflo
             const uint32_t p = (uint32_t)z * k.lr_sz + (uint32_t)y * np + (uint32_t)(x - (nx - np));
             q.x[0] = *at(psi, psi_off(k, pEzx_r) + p);
             q.x[1] = *at(psi, psi_off(k, pEyx_r) + p);
-        }
-    }
-    if (xin) {
-        if (ylo && y < np) {
-            const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)y * sy + (uint32_t)x;
-            q.y[0] = *at(psi, psi_off(k, pExy_f) + p);
-            q.y[1] = *at(psi, psi_off(k, pEzy_f) + p);
-        }
-        if (y >= ny - np && yin) {
-            const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)(y - (ny - np)) * sy + (uint32_t)x;
-            q.y[0] = *at(psi, psi_off(k, pExy_b) + p);
-            q.y[1] = *at(psi, psi_off(k, pEzy_b) + p);
         }
     }
 }
@@ -329,8 +279,8 @@ __device__ __forceinline__ void h_cell(const Desc *__restrict__ d, const StepK &
 // reference order d,u,l,r,f,b, after its Yee update.  psi is read from the
 // H block hr and, when hw is not null, written to the H block hw (hw == hr
 // updates in place).  With pre, the psi values and the x-face profile
-// coefficients come from the split kernel's loads ahead (PsiPre); without,
-// they are loaded here (fused steppers).
+// coefficients of the x faces come from the split kernel's loads ahead
+// (PsiPre); without, they are loaded here (fused steppers).
 __device__ __forceinline__ void h_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
                                        const EAround &e, float &hx, float &hy, float &hz,
                                        const gfloat *hr, gfloat *hw, const PsiPre *pre) {
@@ -346,9 +296,9 @@ __device__ __forceinline__ void h_cpml(const Desc *__restrict__ d, const StepK &
         if (z < np - 1) {                                       // down
             const float bh = d->prof[BH_DLF][z], ch = d->prof[CH_DLF][z];
             const uint32_t p = (uint32_t)z * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
-            const float pa = pre ? pre->z[0] : *at(hr, psi_off(k, pHyz_d - 12) + p);
+            const float pa = *at(hr, psi_off(k, pHyz_d - 12) + p);
             const float a = (bh * pa) + ((ch * (e.ex_n - e.ex_c)) / dx);
-            const float pb = pre ? pre->z[1] : *at(hr, psi_off(k, pHxz_d - 12) + p);
+            const float pb = *at(hr, psi_off(k, pHxz_d - 12) + p);
             const float b = (bh * pb) + ((ch * (e.ey_n - e.ey_c)) / dx);
             if (hw) { *at(hw, psi_off(k, pHyz_d - 12) + p) = a; *at(hw, psi_off(k, pHxz_d - 12) + p) = b; }
             hy -= dtm0 * a;
@@ -358,9 +308,9 @@ __device__ __forceinline__ void h_cpml(const Desc *__restrict__ d, const StepK &
             const int zl = z - (nz - np);
             const float bh = d->prof[BH_URB][zl], ch = d->prof[CH_URB][zl];
             const uint32_t p = (uint32_t)zl * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
-            const float pa = pre ? pre->z[0] : *at(hr, psi_off(k, pHyz_u - 12) + p);
+            const float pa = *at(hr, psi_off(k, pHyz_u - 12) + p);
             const float a = (bh * pa) + ((ch * (e.ex_n - e.ex_c)) / dx);
-            const float pb = pre ? pre->z[1] : *at(hr, psi_off(k, pHxz_u - 12) + p);
+            const float pb = *at(hr, psi_off(k, pHxz_u - 12) + p);
             const float b = (bh * pb) + ((ch * (e.ey_n - e.ey_c)) / dx);
             if (hw) { *at(hw, psi_off(k, pHyz_u - 12) + p) = a; *at(hw, psi_off(k, pHxz_u - 12) + p) = b; }
             hy -= dtm0 * a;
@@ -396,9 +346,9 @@ __device__ __forceinline__ void h_cpml(const Desc *__restrict__ d, const StepK &
         if (y < np - 1) {                                       // front
             const float bh = d->prof[BH_DLF][y], ch = d->prof[CH_DLF][y];
             const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)y * sy + (uint32_t)x;
-            const float pa = pre ? pre->y[0] : *at(hr, psi_off(k, pHxy_f - 12) + p);
+            const float pa = *at(hr, psi_off(k, pHxy_f - 12) + p);
             const float a = (bh * pa) + ((ch * (e.ez_yp - e.ez_c)) / dx);
-            const float pb = pre ? pre->y[1] : *at(hr, psi_off(k, pHzy_f - 12) + p);
+            const float pb = *at(hr, psi_off(k, pHzy_f - 12) + p);
             const float b = (bh * pb) + ((ch * (e.ex_yp - e.ex_c)) / dx);
             if (hw) { *at(hw, psi_off(k, pHxy_f - 12) + p) = a; *at(hw, psi_off(k, pHzy_f - 12) + p) = b; }
             hx -= dtm0 * a;
@@ -408,9 +358,9 @@ __device__ __forceinline__ void h_cpml(const Desc *__restrict__ d, const StepK &
             const int yl = y - (ny - np);
             const float bh = d->prof[BH_URB][yl], ch = d->prof[CH_URB][yl];
             const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)yl * sy + (uint32_t)x;
-            const float pa = pre ? pre->y[0] : *at(hr, psi_off(k, pHxy_b - 12) + p);
+            const float pa = *at(hr, psi_off(k, pHxy_b - 12) + p);
             const float a = (bh * pa) + ((ch * (e.ez_yp - e.ez_c)) / dx);
-            const float pb = pre ? pre->y[1] : *at(hr, psi_off(k, pHzy_b - 12) + p);
+            const float pb = *at(hr, psi_off(k, pHzy_b - 12) + p);
             const float b = (bh * pb) + ((ch * (e.ex_yp - e.ex_c)) / dx);
             if (hw) { *at(hw, psi_off(k, pHxy_b - 12) + p) = a; *at(hw, psi_off(k, pHzy_b - 12) + p) = b; }
             hx -= dtm0 * a;
@@ -464,7 +414,7 @@ __device__ __forceinline__ void e_cell(const Desc *__restrict__ d, const StepK &
 
 // The six E-CPML corrections of one cell below the top plane, in the
 // reference order d,u,l,r,f,b (psi updated in place), after its Yee update;
-// psi values and x-face profile coefficients from pre as in h_cpml.
+// x-face psi values and profile coefficients from pre as in h_cpml.
 __device__ __forceinline__ void e_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
                                        float cb, const HAround &h, float &ex, float &ey, float &ez,
                                        const PsiPre *pre) {
@@ -480,9 +430,9 @@ __device__ __forceinline__ void e_cpml(const Desc *__restrict__ d, const StepK &
         if (zlo && z < np) {                                    // down
             const float be = d->prof[BE_DLF][z], ce = d->prof[CE_DLF][z];
             const uint32_t p = (uint32_t)z * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
-            const float pa = pre ? pre->z[0] : *at(psi, psi_off(k, pEyz_d) + p);
+            const float pa = *at(psi, psi_off(k, pEyz_d) + p);
             const float a = (be * pa) + ((ce * (h.hx_c - h.hx_zm)) / dx);
-            const float pb = pre ? pre->z[1] : *at(psi, psi_off(k, pExz_d) + p);
+            const float pb = *at(psi, psi_off(k, pExz_d) + p);
             const float b = (be * pb) + ((ce * (h.hy_c - h.hy_zm)) / dx);
             *at(psi, psi_off(k, pEyz_d) + p) = a; *at(psi, psi_off(k, pExz_d) + p) = b;
             ey += cbdx * a;
@@ -492,9 +442,9 @@ __device__ __forceinline__ void e_cpml(const Desc *__restrict__ d, const StepK &
             const int zl = z - (nz - np);
             const float be = d->prof[BE_URB][zl], ce = d->prof[CE_URB][zl];
             const uint32_t p = (uint32_t)zl * k.du_sz + (uint32_t)y * sy + (uint32_t)x;
-            const float pa = pre ? pre->z[0] : *at(psi, psi_off(k, pEyz_u) + p);
+            const float pa = *at(psi, psi_off(k, pEyz_u) + p);
             const float a = (be * pa) + ((ce * (h.hx_c - h.hx_zm)) / dx);
-            const float pb = pre ? pre->z[1] : *at(psi, psi_off(k, pExz_u) + p);
+            const float pb = *at(psi, psi_off(k, pExz_u) + p);
             const float b = (be * pb) + ((ce * (h.hy_c - h.hy_zm)) / dx);
             *at(psi, psi_off(k, pEyz_u) + p) = a; *at(psi, psi_off(k, pExz_u) + p) = b;
             ey += cbdx * a;
@@ -530,9 +480,9 @@ __device__ __forceinline__ void e_cpml(const Desc *__restrict__ d, const StepK &
         if (ylo && y < np) {                                    // front
             const float be = d->prof[BE_DLF][y], ce = d->prof[CE_DLF][y];
             const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)y * sy + (uint32_t)x;
-            const float pa = pre ? pre->y[0] : *at(psi, psi_off(k, pExy_f) + p);
+            const float pa = *at(psi, psi_off(k, pExy_f) + p);
             const float a = (be * pa) + ((ce * (h.hz_c - h.hz_ym)) / dx);
-            const float pb = pre ? pre->y[1] : *at(psi, psi_off(k, pEzy_f) + p);
+            const float pb = *at(psi, psi_off(k, pEzy_f) + p);
             const float b = (be * pb) + ((ce * (h.hx_c - h.hx_ym)) / dx);
             *at(psi, psi_off(k, pExy_f) + p) = a; *at(psi, psi_off(k, pEzy_f) + p) = b;
             ex += cbdx * a;
@@ -542,9 +492,9 @@ __device__ __forceinline__ void e_cpml(const Desc *__restrict__ d, const StepK &
             const int yl = y - (ny - np);
             const float be = d->prof[BE_URB][yl], ce = d->prof[CE_URB][yl];
             const uint32_t p = (uint32_t)z * k.fb_sz + (uint32_t)yl * sy + (uint32_t)x;
-            const float pa = pre ? pre->y[0] : *at(psi, psi_off(k, pExy_b) + p);
+            const float pa = *at(psi, psi_off(k, pExy_b) + p);
             const float a = (be * pa) + ((ce * (h.hz_c - h.hz_ym)) / dx);
-            const float pb = pre ? pre->y[1] : *at(psi, psi_off(k, pEzy_b) + p);
+            const float pb = *at(psi, psi_off(k, pEzy_b) + p);
             const float b = (be * pb) + ((ce * (h.hx_c - h.hx_ym)) / dx);
             *at(psi, psi_off(k, pExy_b) + p) = a; *at(psi, psi_off(k, pEzy_b) + p) = b;
             ex += cbdx * a;
