@@ -161,7 +161,7 @@ def material_setup(n=4, lateral=8.0, height=4.0, cell=0.025, device=0, log=print
     return out
 
 
-def serving(n=24, rate_hz=None, budget_gb=None, max_slots=16, chunk=48, device=0, log=print,
+def serving(n=24, rate_hz=None, budget_gb=None, max_slots=16, chunk=96, device=0, log=print,
             device_materials=False):
     """Serve ``n`` Poisson requests; the rate defaults to 1.5x the measured
     one-at-a-time service rate so admission binds."""
