@@ -689,9 +689,10 @@ __device__ __forceinline__ bool split_tile(const int4 *__restrict__ tiles, int n
 
 // Both split kernels read the descriptor's scalars and, for a single
 // simulation, the step counter in one batch of scalar loads, then issue the
-// tile's first plane loads, and test the step after that (no store precedes
-// the test).  The z loop then computes a plane, stores it and loads the
-// next one.
+// tile's first plane loads (and its x-face CPML values), and test the step
+// after that; no field store precedes the test (the H kernel's one-lane copy
+// of the counter for k_dft does, note_step).  The z loop then computes a
+// plane, stores it and loads the next one.
 template <int V, bool DIRECT>
 __global__ void __launch_bounds__(256)
 k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles, int shape, TileGrid tg,
