@@ -227,6 +227,8 @@ def largest_leg(fused, steps=10, warmup=3, log=print):
            'Mcell_updates_per_s': round(cells / (ms * 1e-3) / 1e6, 1),
            'step_alg_GBs': round(step_bytes / (ms * 1e-3) / 1e9, 1),
            'kernels_ms': {kk: round(v, 4) for kk, v in k.items()},
+           'kernels_alg_GBs': ({kk: round(ab[kk] / (k[kk] * 1e-3) / 1e9, 1) for kk in ('h', 'e', 'dft')}
+                               if not fused and min(k['h'], k['e'], k['dft']) > 0 else None),
            'launch': dict(zip(LAUNCH_KEYS, shape)),
            'device_bytes': int(native.footprint(geom) + 16 * g.size_t_inc * g.size_y * g.size_x)}
     sim.destroy()
@@ -257,6 +259,8 @@ def main():
     ap.add_argument('--pmc-dir', default=None,
                     help='keep the PMC databases in this directory')
     ap.add_argument('--pmc-child', action='store_true', help=argparse.SUPPRESS)
+    ap.add_argument('--largest-only', action='store_true',
+                    help='run only the largest-domain leg and print its JSON (for profiling it)')
     args = ap.parse_args()
     if args.pmc_child:
         args.no_cpu = args.no_extras = args.no_traffic = True
@@ -270,6 +274,15 @@ def main():
     from sar_pixel_simulator_amd import _native as native, workloads
     from sar_pixel_simulator_amd.hip import common, tuning
     from sar_pixel_simulator_amd.materials import compute_material_arrays
+
+    if args.largest_only:
+        native.init(0)
+        fused = (int(common.FUSED_STEPPER) if args.stepper is None
+                 else {'split': 0, 'fused': 1, 'rows': 2}[args.stepper])
+        print(json.dumps(largest_leg(fused, steps=args.steps, warmup=args.warmup,
+                                     log=lambda m: print(m, file=sys.stderr, flush=True))),
+              flush=True)
+        return
 
     pc = workloads.large_params()
     g = pc.geom_params
