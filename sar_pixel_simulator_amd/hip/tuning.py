@@ -95,7 +95,8 @@ def _tune_workload(name, fused):
     import numpy as np
     from .. import _native as native, workloads
     from . import common
-    pc = {'large': workloads.large_params, 'plain': workloads.plain_params}[name]()
+    pc = {'large': workloads.large_params, 'plain': workloads.plain_params,
+          'largest': workloads.largest_params}[name]()
     g = pc.geom_params
     native.init(0)
     sim = native.Sim(common.main_geom(pc), common.coeffs_for(pc))
