@@ -300,6 +300,43 @@ def test_incident_run_bitwise():
     beam.release()
 
 
+@pytest.fixture(scope='module')
+def incident31():
+    pc = H.soil_case(nx=31, ny=33, nz=35, nz_mat=14, dft_accumulation='reference')[0]
+    pex, pey = H.excitation(pc)
+    rec, oph = H.oracle_incident(pc, pex, pey)
+    return pc, pex, pey, rec, oph
+
+
+@pytest.mark.parametrize('launch', [(1, 2, 1, 4, 0), (2, 3, 2, 2, 1), (1, 1, 4, 1, 2), (4, 0, 1, 1, 0)],
+                         ids=['wide', 'xcd-2-cells', 'banded-4-cells', 'whole-column'])
+def test_incident_launch_shapes_bitwise(incident31, launch):
+    # the incident run steps E before H (hard source, recorded read-out
+    # plane, incident phasors) on the split kernels in any tile shape
+    pc, pex, pey, rec, oph = incident31
+    g = pc.geom_params
+    say('case: incident run, launch', launch)
+    native.init(0)
+    geom = common.incident_geom(pc)
+    sim = native.Sim(geom, common.coeffs_for(pc))
+    sim.set_launch(*launch)
+    sim.set_uniform_material(1.0, common.incident_cb(pc))
+    n = geom.nt_inc
+    sim.set_source(np.ascontiguousarray(np.imag(pex[:n]), dtype=np.float32),
+                   np.ascontiguousarray(np.imag(pey[:n]), dtype=np.float32))
+    beam = native.Beam(g.size_t_inc, g.size_y, g.size_x)
+    sim.attach_beam(beam)
+    sim.run(g.size_t_inc)
+    sim.sync()
+    ph = sim.phasors()
+    sim.destroy()
+    for c in range(4):
+        assert_same(beam.component(c), rec[c], f'beam component {c} {launch}')
+    for nm in native.INC_PHASOR_NAMES:
+        assert_same(ph[nm], oph[nm], f'{nm} {launch}')
+    beam.release()
+
+
 def test_split_and_fused_agree_long_run():
     """300 steps on the plain workload's grid: both steppers bit-identical."""
     from sar_pixel_simulator_amd import workloads
