@@ -29,19 +29,13 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from sar_pixel_simulator_amd.hip.roofline import algorithmic_bytes, cpml_cells  # noqa: E402
+
 METRIC = 'FDTD Mcell-updates/s + material-models/hr batched vs plain; p50/p99 latency'
 HBM_PEAK_GBS = 8000.0
 PROBE_BYTES = 1 << 30          # calibration copy: 1 GiB read + 1 GiB written
 LAUNCH_KEYS = ('ty', 'zchunk', 'xvec', 'xwaves', 'xcd_order')
 PMC_COUNTERS = ('FETCH_SIZE', 'WRITE_SIZE')
-
-
-def cpml_cells(nx, ny, nz, np_):
-    """Cells each half-step's CPML passes touch (cpu/field_updates.py:66-191,
-    243-366 index ranges): d/u (np-1)(ny-1)(nx-1), l/r (nz-1)(ny-1)(np-1),
-    f/b (nz-1)(np-1)(nx-1), two of each."""
-    return 2 * ((np_ - 1) * (ny - 1) * (nx - 1) + (nz - 1) * (ny - 1) * (np_ - 1)
-                + (nz - 1) * (np_ - 1) * (nx - 1))
 
 
 def uniform_top(Ca, Cb):
@@ -53,30 +47,6 @@ def uniform_top(Ca, Cb):
     while z > 0 and np.all(Ca[z - 1] == a0) and np.all(Cb[z - 1] == b0):
         z -= 1
     return z
-
-
-def algorithmic_bytes(nx, ny, nz, np_, uni_z=None, taps=True):
-    """Compulsory HBM bytes per launch of each kernel.
-
-    H half: read E(3) + H(3), write H(3) = 36 B/cell; each CPML face cell
-    reads and writes two psi values = 16 B; E_inc injection reads 2 floats
-    per TFSF-plane cell.  E half: read H(3) + E(3) + Ca + Cb, write E(3) =
-    44 B/cell (36 B on the uniform top planes z >= uni_z, where Ca/Cb are
-    scalars), same CPML and injection terms, and (split stepper, taps) the
-    six x-face column values per (face, z, y) it copies for the DFT, 24 B.
-    DFT: per plane sample, four float4 running sums read and written
-    (128 B) plus four field values.
-    """
-    n = nx * ny * nz
-    nc = cpml_cells(nx, ny, nz, np_)
-    plane = nx * ny
-    n_uni = (nz - (nz if uni_z is None else uni_z)) * plane
-    nxp, nyp, nzp = nx - 2 * np_, ny - 2 * np_, nz - 2 * np_
-    samples = 2 * nyp * nxp + 2 * nzp * nyp + 2 * nzp * nxp
-    return {'h': 36 * n + 16 * nc + 8 * plane,
-            'e': 44 * n - 8 * n_uni + 16 * nc + 8 * plane + (48 * ny * nz if taps else 0),
-            'dft': 144 * samples,
-            'cells': n, 'cpml_cells': nc, 'dft_samples': samples}
 
 
 def cpu_threads():

@@ -203,6 +203,37 @@ def serving(n=24, rate_hz=None, budget_gb=None, max_slots=16, chunk=96, device=0
     return st
 
 
+def batch_step_rate(count=8, steps=30, launch=None, device=0):
+    """Steady-state step rate of the batched workload as one batch: all
+    members resident, ``steps`` graph-replayed steps after a short warm-up.
+    Returns ms per step, Gcell-updates/s and the algorithmic rate (the
+    split kernels' and the DFT's compulsory bytes per step over the step)."""
+    from .roofline import algorithmic_bytes
+    native.init(device)
+    cases = batched_workload(count)
+    beams = B.BeamCache()
+    models = _models_for(cases, beams)
+    alg, cells = 0, 0
+    for pc, _, _, _ in cases:
+        g = pc.geom_params
+        a = algorithmic_bytes(g.size_x, g.size_y, g.size_z, g.size_pml, None)
+        alg += a['h'] + a['e'] + a['dft']
+        cells += g.size_x * g.size_y * g.size_z
+    b = native.Batch([m.sim for m in models])
+    if launch:
+        b.set_launch(*launch)
+    b.run(5, False)
+    b.sync()
+    ms = b.time_steps(steps, True) / steps
+    b.destroy()
+    for m in models:
+        m.release()
+    beams.release()
+    return {'models': len(cases), 'cells': cells, 'steps': steps, 'ms_per_step': round(ms, 4),
+            'Gcell_updates_per_s': round(cells / ms / 1e6, 1),
+            'alg_TBs': round(alg / ms / 1e9, 2)}
+
+
 if __name__ == '__main__':
     import json
     import sys
@@ -212,4 +243,6 @@ if __name__ == '__main__':
         out['batched'] = batched_vs_plain()
     if 'serving' in what:
         out['serving'] = serving()
+    if 'batch_step' in what:
+        out['batch_step'] = batch_step_rate()
     print(json.dumps(out))
