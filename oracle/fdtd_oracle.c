@@ -1,5 +1,5 @@
 /*
- * ORACLE — test infrastructure only (parity checker and CPU baseline).
+ * ORACLE — test infrastructure only (the parity reference of the tests and the CPU baseline).
  * CPU restatement of the reference's FDTD time loop; see
  * fdtd_oracle_impl.h for the per-function reference citations.
  *
