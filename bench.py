@@ -390,9 +390,9 @@ def main():
         print('bench: batched vs plain (uneven soil models)', file=sys.stderr, flush=True)
         extras['batched_vs_plain'] = measure.batched_vs_plain(count=8, log=log)
         print('bench: serving (Poisson canopy requests)', file=sys.stderr, flush=True)
-        extras['serving'] = measure.serving(n=16, log=log)
+        extras['serving'] = measure.serving(n=48, log=log)
         print('bench: serving, materials built on the device', file=sys.stderr, flush=True)
-        extras['serving_device_materials'] = measure.serving(n=16, log=log, device_materials=True)
+        extras['serving_device_materials'] = measure.serving(n=48, log=log, device_materials=True)
         print('bench: material setup, host vs device', file=sys.stderr, flush=True)
         extras['material_setup'] = measure.material_setup(log=log)
         print('bench: largest single domain', file=sys.stderr, flush=True)

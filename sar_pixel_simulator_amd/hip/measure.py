@@ -162,19 +162,21 @@ def material_setup(n=4, lateral=8.0, height=4.0, cell=0.025, device=0, log=print
 
 
 def serving(n=24, rate_hz=None, budget_gb=None, max_slots=16, chunk=96, device=0, log=print,
-            device_materials=False):
-    """Serve ``n`` Poisson requests; the rate defaults to 1.5x the measured
-    one-at-a-time service rate so admission binds."""
+            device_materials=False, use_graph=True):
+    """Serve ``n`` Poisson requests; the rate defaults to 1.5x the service
+    rate of the same requests run one at a time, so admission binds."""
     native.init(device)
     reqs = serving_requests(n, rate_hz=1.0, device_materials=device_materials)
-    be = HipBackend(device)
+    be = HipBackend(device, use_graph=use_graph)
     for r in reqs:                         # beams first: reported separately
         be.beams.get(r.params)
     fps = [be.footprint(r) for r in reqs]
-    # one-at-a-time service time of the first few requests sets the rate
+    # the same requests run one at a time (admit, run, sigma0, release) give
+    # the plain service rate, which sets the arrival rate
+    svc = None
     if rate_hz is None:
         t0 = time.perf_counter()
-        k = min(3, n)
+        k = n
         for r in reqs[:k]:
             m = be.admit(r)
             B.run_plain([m])
@@ -198,7 +200,13 @@ def serving(n=24, rate_hz=None, budget_gb=None, max_slots=16, chunk=96, device=0
                'materials': 'built on the device' if device_materials else 'host volumes uploaded',
                'arrival_rate_hz': round(rate_hz, 3), 'hbm_budget_bytes': budget,
                'peak_hbm_bytes': srv.peak_used, 'peak_members': srv.peak_slots,
-               'chunks': srv.chunks, 'wall_s': round(wall, 2)})
+               'chunks': srv.chunks, 'wall_s': round(wall, 2),
+               'boundary_host_s': {k: round(v, 3) for k, v in srv.phase_s.items()}})
+    if svc:
+        # the same requests run one at a time (admit, run, sigma0, release)
+        st['plain_models_per_hour'] = round(3600.0 / svc, 1)
+        if st.get('models_per_hour'):
+            st['served_vs_plain'] = round(st['models_per_hour'] * svc / 3600.0, 3)
     log(f'serving: {st}')
     return st
 

@@ -67,13 +67,14 @@ class AdmissionError(RuntimeError):
 class HipBackend:
     """Device operations the loop needs, on libfdtd_hip."""
 
-    def __init__(self, device=0, launch=None):
+    def __init__(self, device=0, launch=None, use_graph=True):
         from .. import _native as native
         from .batch import BeamCache
         native.init(device)
         self.native = native
         self.beams = BeamCache()
         self.launch = launch
+        self.use_graph = use_graph
 
     def footprint(self, req):
         from .batch import DeviceModel
@@ -108,7 +109,7 @@ class HipBackend:
         return DeviceBatch(members, self.launch)
 
     def run(self, batch, nsteps):
-        batch.run(nsteps, use_graph=True)
+        batch.run(nsteps, use_graph=self.use_graph)
 
     def sync(self, batch):
         batch.sync()
@@ -142,6 +143,8 @@ class Server:
         self.chunks = 0
         self.peak_used = 0
         self.peak_slots = 0
+        # host seconds spent at chunk boundaries, by phase
+        self.phase_s = dict.fromkeys(('wait', 'retire', 'admit', 'batch', 'prepare'), 0.0)
 
     # -- queue ----------------------------------------------------------------
     def submit(self, req: Request):
@@ -219,13 +222,22 @@ class Server:
         """Finish the chunk in flight, retire, admit, start the next chunk and
         prepare queued requests while it runs.  Returns the number of active
         members."""
+        ph = self.phase_s
+        t0 = time.perf_counter()
         self._finish_chunk()
+        t1 = time.perf_counter()
         changed = self._retire()
+        t2 = time.perf_counter()
         changed = self._admit() or changed
+        t3 = time.perf_counter()
+        ph['wait'] += t1 - t0
+        ph['retire'] += t2 - t1
+        ph['admit'] += t3 - t2
         if changed and self.batch is not None:
             self.backend.drop_batch(self.batch)
             self.batch = None
         if not self.active:
+            ph['batch'] += time.perf_counter() - t3
             return 0
         if self.batch is None:
             self.batch = self.backend.make_batch([e[1] for e in self.active])
@@ -233,7 +245,10 @@ class Server:
         self.backend.run(self.batch, n)        # asynchronous
         self.inflight = n
         self.chunks += 1
+        t4 = time.perf_counter()
+        ph['batch'] += t4 - t3
         self._prepare_queued()
+        ph['prepare'] += time.perf_counter() - t4
         return len(self.active)
 
     def drain(self):
