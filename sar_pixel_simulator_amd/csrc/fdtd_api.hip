@@ -359,6 +359,8 @@ static int group_build(Group &G) {
     G.fused = G.members[0]->h.fused;
     const int xv = G.launch.xvec;
     int ty = G.launch.ty, xw = G.launch.xw;
+    bool per_member = false;
+    CHECK(G.members.size() < (1u << 16), "a group holds at most 65535 simulations");
     CHECK(xv == 0 || xv == 1 || xv == 2 || xv == 4, "xvec must be 0, 1, 2 or 4 (got %d)", xv);
     CHECK(xw == 0 || xw == 1 || xw == 2 || xw == 4, "xwaves must be 0, 1, 2 or 4 (got %d)", xw);
     CHECK(G.launch.xcd >= 0 && G.launch.xcd <= 2, "block order must be 0, 1 or 2 (got %d)", G.launch.xcd);
@@ -380,6 +382,13 @@ static int group_build(Group &G) {
         CHECK(ty == 0 || ty == 1 || ty == 2 || ty == 4, "split stepper: ty must be 0, 1, 2 or 4 (got %d)", ty);
         // automatic: one cell per lane unless the block width is fixed
         G.xvec = xv ? xv : xw ? auto_xvec(G.members, xw) : 1;
+        // a batch with a fully automatic shape gives each member its own
+        // block width (4-wavefront blocks, 4 / xw rows); its tiles carry it
+        per_member = !xw && !ty && !xv && G.members.size() > 1;
+        if (per_member) {
+            xw = 4;
+            ty = 1;
+        }
         if (!xw) xw = auto_xwaves(G.members, G.xvec);
         if (!ty) ty = std::max(1, 4 / xw);
         CHECK(xw * ty <= 4, "split stepper: xwaves x ty = %d exceeds 4 wavefronts per block", xw * ty);
@@ -402,6 +411,11 @@ static int group_build(Group &G) {
     G.max_dft_items = 0;
     for (size_t m = 0; m < G.members.size(); ++m) {
         FdtdSim *s = G.members[m];
+        // split tiles carry log2(xwaves) above the member index
+        const int xw_m = per_member ? auto_xwaves(std::vector<FdtdSim *>{s}, G.xvec) : xw;
+        const int lxw_m = G.fused ? 0 : xw_m == 4 ? 2 : xw_m == 2 ? 1 : 0;
+        const int tx_m = kWave * G.xvec * xw_m, ty_m = per_member ? 4 / xw_m : tys;
+        const int tag = (int)m | (lxw_m << 16);
         CHECK(s->g.mode == G.mode, "a group cannot mix MAIN and INCIDENT simulations");
         CHECK(s->h.fused == G.fused, "a group cannot mix fused and split steppers");
         CHECK(s->device == G.device, "group members must share a device");
@@ -411,9 +425,9 @@ static int group_build(Group &G) {
         for (int z0 = 0; z0 < nz; z0 += zc) {
             const int z1 = std::min(nz, z0 + zc);
             chunk.clear();
-            for (int y0 = 0; y0 < s->g.ny; y0 += tys)
-                for (int x0 = 0; x0 < s->g.nx; x0 += tx)
-                    chunk.push_back(make_int4((int)m, x0, y0, z0 | (z1 << 16)));
+            for (int y0 = 0; y0 < s->g.ny; y0 += ty_m)
+                for (int x0 = 0; x0 < s->g.nx; x0 += tx_m)
+                    chunk.push_back(make_int4(tag, x0, y0, z0 | (z1 << 16)));
             if (G.launch.xcd != 2) {
                 tiles.insert(tiles.end(), chunk.begin(), chunk.end());
                 continue;

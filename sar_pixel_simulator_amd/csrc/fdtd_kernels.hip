@@ -699,7 +699,7 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
          const int *__restrict__ stepp) {
     int4 tl;
     if (!split_tile<DIRECT>(tiles, ntiles, shape, tg, tl)) return;
-    const Desc *__restrict__ d = descs + tl.x;
+    const Desc *__restrict__ d = descs + (tl.x & 0xffff);
     // a single simulation's counter is a plain load through a read-only
     // kernel argument, a scalar load in the descriptor's batch: the
     // previous kernel of the stream wrote it, and nothing writes it now
@@ -709,7 +709,7 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     scalars_resident(kg, st);
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     note_step(d, tl, w, t);
-    const int lxw = shape & 3;
+    const int lxw = DIRECT ? shape & 3 : tl.x >> 16;   // listed tiles carry their block width
     const int y = tl.z + (w >> lxw);
     const int nx = d->nx, ny = d->ny, nz = d->nz;
     if (y >= ny) return;                               // uniform per wavefront (one row)
@@ -828,7 +828,7 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
          const int *__restrict__ stepp) {
     int4 tl;
     if (!split_tile<DIRECT>(tiles, ntiles, shape, tg, tl)) return;
-    const Desc *__restrict__ d = descs + tl.x;
+    const Desc *__restrict__ d = descs + (tl.x & 0xffff);
     // a single simulation's counter is a plain load through a read-only
     // kernel argument, a scalar load in the descriptor's batch: the
     // previous kernel of the stream wrote it, and nothing writes it now
@@ -837,7 +837,7 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     const StepTime st = step_time(d);
     scalars_resident(kg, st);
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
-    const int lxw = shape & 3;
+    const int lxw = DIRECT ? shape & 3 : tl.x >> 16;   // listed tiles carry their block width
     const int y = tl.z + (w >> lxw);
     const int nx = d->nx, ny = d->ny, nz = d->nz;
     if (y >= ny) return;                               // uniform per wavefront (one row)
