@@ -268,7 +268,9 @@ int fdtd_beam_synthesize(const FdtdGeom *g, const FdtdCoeffs *c, const FdtdBeamS
  * mode and a stepper; each keeps its own step counter. */
 int fdtd_batch_create(FdtdSim *const *sims, int n, FdtdBatch **out);
 void fdtd_batch_destroy(FdtdBatch *b);
-/* As fdtd_sim_set_launch, for every member. */
+/* As fdtd_sim_set_launch, for every member.  With ty, xvec and xwaves all 0
+ * (automatic, the default) the split stepper gives each member its own
+ * block width. */
 int fdtd_batch_set_launch(FdtdBatch *b, int ty, int zchunk, int xvec, int xwaves, int xcd_order);
 /* Queue nsteps steps of every member; use_graph != 0 captures the step
  * sequence into a HIP graph on first use and replays it. */
