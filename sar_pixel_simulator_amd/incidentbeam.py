@@ -1,11 +1,13 @@
 """Incident beam: temporal pulse, tapered-wave excitation and the vacuum
 FDTD run that records the beam injected at the TFSF plane.
 
-Follows fdtdsarsim/incidentbeam.py:25-433 (pulse, Braunisch tapered-wave
-spectrum, FFT synthesis, 10.0 peak normalisation, metadata validation) and
-cpu/incident_sim.py:22-224 (the recording run).  The recording run executes
-on the HIP library in its INCIDENT mode, and the recorded beam stays resident
-in HBM for the main simulations that share its key.
+Follows fdtdsarsim/incidentbeam.py:25-204 (pulse, metadata validation) and
+cpu/incident_sim.py:22-224 (the recording run).  The tapered-wave spectra,
+their FFT synthesis and the 10.0 peak normalisation (incidentbeam.py:206-433)
+run on the device (``synthesize_on_device``, csrc/fdtd_beam.hip); the
+recording run executes on the HIP library in its INCIDENT mode, and the
+recorded beam stays resident in HBM for the main simulations that share its
+key.
 """
 import json
 import os.path
@@ -29,59 +31,6 @@ META_KEYS = ('datatype', 'frequency', 'polarization', 'theta_angle',
 
 # beams recorded in this process, keyed by cache file name
 _RESIDENT = {}
-
-
-def taper_k(k, k_x, k_y, theta, phi, radius, pol):
-    """Tapered-wave spectrum (Braunisch), incidentbeam.py:316-433.
-
-    Returns the (x, y) spectral field components for wavenumber ``k`` on the
-    (ny, nx) spatial-frequency grid.  Same dtype flow as the reference: k_z
-    is formed in complex64, the rest in float64/complex128.
-    """
-    k_p = np.hypot(k_x, k_y)
-    k_z = np.conj(np.sqrt((k ** 2 - k_p ** 2).astype(np.complex64)))
-    theta = np.radians(theta)
-    phi = np.radians(phi)
-    k_ix = k * np.sin(theta) * np.cos(phi)
-    k_iy = k * np.sin(theta) * np.sin(phi)
-    k_ip = np.hypot(k_ix, k_iy)
-    k_iz = np.sqrt(k ** 2 - k_ip ** 2)
-    gauss = ((radius ** 2) / (4 * np.pi)) * np.exp(
-        -(radius ** 2) * (((k_x - k_ix) ** 2) + ((k_y - k_iy) ** 2)) / 4)
-
-    pos = k_p > 0.0
-    kp = np.where(pos, k_p, 1.0)
-    hor_x = np.where(pos, k_y / kp, np.sin(phi)).astype(np.complex128)
-    hor_y = np.where(pos, -k_x / kp, -np.cos(phi)).astype(np.complex128)
-    ver_x = np.where(pos, (k_z * k_x) / (k * kp), np.cos(phi)).astype(np.complex128)
-    ver_y = np.where(pos, (k_z * k_y) / (k * kp), np.sin(phi)).astype(np.complex128)
-    ver_z = np.where(pos, k_p / k, 0.0).astype(np.complex128)
-
-    if k_ip > 0:
-        hor_ix, hor_iy = k_iy / k_ip, -k_ix / k_ip
-        ver_ix, ver_iy = (k_iz * k_ix) / (k * k_ip), (k_iz * k_iy) / (k * k_ip)
-        ver_iz = k_ip / k
-    else:
-        hor_ix, hor_iy = np.sin(phi), -np.cos(phi)
-        ver_ix, ver_iy, ver_iz = np.cos(phi), np.sin(phi), 0.0
-
-    if pol == 'HH':
-        Eh, Ev = 1, 0
-    elif pol == 'VV':
-        Eh, Ev = 0, 1
-    elif pol == 'RHCP':
-        Eh, Ev = -1j, 1
-    elif pol == 'LHCP':
-        Eh, Ev = 1j, 1
-    else:
-        raise ValueError('Invalid polarization value')
-
-    eh = Eh * ((hor_ix * hor_x) + (hor_iy * hor_y)) \
-        + Ev * ((ver_ix * hor_x) + (ver_iy * hor_y))
-    ev = Eh * ((hor_ix * ver_x) + (hor_iy * ver_y)) \
-        + Ev * ((ver_ix * ver_x) + (ver_iy * ver_y) + (ver_iz * ver_z))
-    return gauss * ((eh * hor_x) + (ev * ver_x)), \
-        gauss * ((eh * hor_y) + (ev * ver_y))
 
 
 class IncidentBeam:
@@ -162,41 +111,6 @@ class IncidentBeam:
         self.pulse_fft = np.fft.fft(self.pulse)
         self.pulse_freqs = np.fft.fftfreq(self.pulse.size, dt)
 
-    def create_inc_excitation(self):
-        """Ex, Ey excitation of the incident simulation, (nt_inc, ny, nx)
-        complex64 (incidentbeam.py:206-313)."""
-        g, b = self.geom_params, self.inc_beam_params
-        dtype_c = {'float32': np.complex64, 'float64': np.complex128}[
-            self.processing_params.datatype]
-        width = b.illumination_width * C0 / b.frequency
-        gauss_radius = width / (2.0 * np.sqrt(2.0 * np.log(10.0)))
-        nt_inc, nx, ny, dx = g.size_t_inc, g.size_x, g.size_y, g.cell_size
-        kx = 2 * np.pi * np.repeat(np.atleast_2d(np.fft.fftfreq(nx, dx)), ny, axis=0)
-        kx = np.fft.fftshift(kx.reshape((ny, nx)), axes=1)
-        ky = 2 * np.pi * np.repeat(np.atleast_2d(np.fft.fftfreq(ny, dx)).transpose(),
-                                   nx, axis=1)
-        ky = np.fft.fftshift(ky.reshape((ny, nx)), axes=0)
-        pulse_ex = np.zeros((nt_inc, ny, nx), dtype_c)
-        pulse_ey = np.zeros((nt_inc, ny, nx), dtype_c)
-        all_k = 2 * np.pi * self.pulse_freqs / C0
-        sh, ish, fft2 = np.fft.fftshift, np.fft.ifftshift, np.fft.fft2
-        for k_idx in range(nt_inc):
-            k_val = all_k[k_idx]
-            if k_val == 0.0:
-                continue
-            sx, sy = taper_k(k_val, kx, ky, b.theta_angle, b.phi_angle,
-                             gauss_radius, b.polarization)
-            w = 1.0 * self.pulse_fft[k_idx]
-            pulse_ex[k_idx] = w * sh(fft2(ish(sx)))
-            pulse_ey[k_idx] = w * sh(fft2(ish(sy)))
-        pulse_ex = np.fft.ifft(pulse_ex, axis=0).astype(dtype_c, copy=False)
-        pulse_ey = np.fft.ifft(pulse_ey, axis=0).astype(dtype_c, copy=False)
-        maxval = float(max(np.abs(pulse_ex).max(), np.abs(pulse_ey).max()))
-        scale = 10.0 / maxval
-        pulse_ex = (pulse_ex.astype(np.complex128) * scale).astype(dtype_c)
-        pulse_ey = (pulse_ey.astype(np.complex128) * scale).astype(dtype_c)
-        return pulse_ex, pulse_ey
-
     # -- storage ---------------------------------------------------------------
     def save(self):
         """Write the beam, its phasors and metadata (the incident HDF5 layout,
@@ -270,8 +184,9 @@ def beam_spectrum(params: ParameterCollection):
 def synthesize_on_device(params: ParameterCollection, device=None):
     """Excitation synthesis and incident simulation both on the device
     (libfdtd_hip fdtd_beam_synthesize).  Returns (native.Beam, incident
-    phasor dict); the same beam as ``create_inc_excitation`` followed by
-    ``run_incident`` up to FFT rounding."""
+    phasor dict); the same beam as the host synthesis of the excitation
+    (incidentbeam.py:206-313) followed by ``run_incident``, up to FFT
+    rounding."""
     common.require_float32(params)
     native.init(params_device(params) if device is None else device)
     b = params.inc_beam_params

@@ -23,7 +23,9 @@ def say(*a):
     print(*a, flush=True)
 
 
-def make_sim(pc, Ca=None, Cb=None, beam_planes=None, launch=None, fused=True):
+def make_sim(pc, Ca=None, Cb=None, beam_planes=None, launch=None, fused=False):
+    """A simulation on device 0; the default stepper is the library's
+    default (split), the others are 1 (fused strips) and 2 (row-parallel)."""
     native.init(0)
     sim = native.Sim(common.main_geom(pc), common.coeffs_for(pc))
     sim.set_stepper(fused)
@@ -111,12 +113,16 @@ def test_launch_shapes_bitwise(soil60, fused, launch):
     sim.destroy()
 
 
-def test_compensated_phasors_close(soil60):
+STEPPERS = pytest.mark.parametrize('fused', [False, True, 2], ids=['split', 'fused', 'rows'])
+
+
+@STEPPERS
+def test_compensated_phasors_close(soil60, fused):
     pc, Ca, Cb, beam, ref = soil60
-    say('case: compensated DFT')
+    say('case: compensated DFT, stepper', fused)
     pc.processing_params.dft_accumulation = 'compensated'
     try:
-        sim = make_sim(pc, Ca, Cb, beam)
+        sim = make_sim(pc, Ca, Cb, beam, fused=fused)
     finally:
         pc.processing_params.dft_accumulation = 'reference'
     sim.run(60)
@@ -127,9 +133,52 @@ def test_compensated_phasors_close(soil60):
     sim.destroy()
 
 
-def test_steps_past_end_are_noops(soil60):
+@pytest.fixture(scope='module')
+def taps139():
+    """139 x 27 x 29, 150 steps: the right x face (x = 128) is a wavefront's
+    first lane at 1 and 2 cells per lane, so its hy, hz at x - 1 come from
+    memory in the split stepper's copies; the oracle's float32 phasors and
+    the exact FP64 running sums of six face components."""
+    pc, Ca, Cb = H.soil_case(nx=139, ny=27, nz=29, nz_mat=11, steps=150, seed=13,
+                             dft_accumulation='reference')
+    beam = H.synthetic_beam(pc)
+    ref = H.oracle_main(pc, Ca, Cb, beam)
+    exact = _exact_dft(pc, Ca, Cb, beam, 150)
+    return pc, Ca, Cb, beam, ref.phasors(), exact
+
+
+@pytest.mark.parametrize('fused,launch', [(False, None), (False, (1, 2, 1, 4, 0)),
+                                          (False, (1, 3, 2, 2, 0)), (False, (2, 5, 4, 2, 1)),
+                                          (True, None), (2, None)],
+                         ids=['split-auto', 'split-1-cell', 'split-2-cells', 'split-4-cells-xcd',
+                              'fused', 'rows'])
+def test_compensated_x_faces_close_and_closer_to_exact(taps139, fused, launch):
+    """The shipped default (split stepper, compensated DFT) at phasor level
+    where the x-face taps sit on wavefront edges: every phasor within 1e-5
+    relative L2 of the oracle's complex64 running sum, and the x- and z-face
+    components at least as close to the exact FP64 sum as the oracle's."""
+    pc, Ca, Cb, beam, oph, exact = taps139
+    say('case: compensated DFT, 139 x 27 x 29, 150 steps, stepper', fused, 'launch', launch)
+    pc.processing_params.dft_accumulation = 'compensated'
+    try:
+        sim = make_sim(pc, Ca, Cb, beam, launch=launch, fused=fused)
+    finally:
+        pc.processing_params.dft_accumulation = 'reference'
+    sim.run(150)
+    sim.sync()
+    ph = sim.phasors()
+    sim.destroy()
+    for n in native.PHASOR_NAMES:
+        assert H.rel_l2(ph[n], oph[n]) <= 1e-5, n
+    for n, want in exact.items():
+        e_comp, e_ref = H.rel_l2(ph[n], want), H.rel_l2(oph[n], want)
+        assert e_comp <= e_ref and e_comp < 1e-6, (n, e_comp, e_ref)
+
+
+@STEPPERS
+def test_steps_past_end_are_noops(soil60, fused):
     pc, Ca, Cb, beam, ref = soil60
-    sim = make_sim(pc, Ca, Cb, beam)
+    sim = make_sim(pc, Ca, Cb, beam, fused=fused)
     sim.run(75)
     sim.sync()
     assert sim.step_count() == 60
@@ -166,7 +215,7 @@ def test_uneven_grid_bitwise():
     say('case: uneven 37x29x41, 40 steps')
     beam = H.synthetic_beam(pc)
     ref = H.oracle_main(pc, Ca, Cb, beam)
-    sim = make_sim(pc, Ca, Cb, beam, launch=(8, 9, 1))
+    sim = make_sim(pc, Ca, Cb, beam, launch=(8, 9, 1), fused=True)
     sim.run(40)
     sim.sync()
     for n in native.FIELD_NAMES:
@@ -504,44 +553,68 @@ def test_linearity_at_large_size():
     assert_same(out[1], 2.0 * out[0], 'ey(2 beam) vs 2 ey(beam)')
 
 
-def _exact_dft_z_face(pc, Ca, Cb, beam, steps, face_z):
-    """FP64 running DFT of the top-face ex samples, accumulated from the
-    oracle's float32 field history step by step (the exact sum the two
-    accumulation modes approximate)."""
+def _exact_dft(pc, Ca, Cb, beam, steps):
+    """FP64 running DFT of six face components (top face ex, hy; left face
+    ey, hz; right face hy, hz), accumulated from the oracle's float32 field
+    history step by step, as cpu/field_updates.py:373-461 form each term
+    (float32 neighbour sums, E at phase t, H at t - 1/2): the exact sum the
+    two accumulation modes approximate."""
     g = pc.geom_params
     n = g.size_pml
     s = H.scalars(pc)
     sim = oracle.OracleSim(H.oracle_geom(pc), Ca, Cb, H.profiles_of(pc), inc=beam)
-    nyp, nxp = g.size_y - 2 * n, g.size_x - 2 * n
-    acc = np.zeros((nyp, nxp), np.complex128)
+    nzp, nyp, nxp = g.size_z - 2 * n, g.size_y - 2 * n, g.size_x - 2 * n
+    zt = g.size_z - n - 1
+    xs = {'l': n + 1, 'r': g.size_x - n - 1}
+    acc = {k: np.zeros((nyp, nxp), np.complex128) for k in ('ex_ph_u', 'hy_ph_u')}
+    acc.update({k: np.zeros((nzp, nyp), np.complex128)
+                for k in ('ey_ph_l', 'hz_ph_l', 'hy_ph_r', 'hz_ph_r')})
+    Y, X = slice(n, n + nyp), slice(n, n + nxp)
+    Y1, X1 = slice(n - 1, n - 1 + nyp), slice(n - 1, n - 1 + nxp)
+    Z, Z1 = slice(n, n + nzp), slice(n - 1, n - 1 + nzp)
     for t in range(steps):
         sim.run(1)
-        ex = sim.fields['ex'][face_z, n:n + nyp, :]
-        a = (ex[:, n:n + nxp] + ex[:, n - 1:n - 1 + nxp]).astype(np.float64)   # float32 sums
-        th = (-s['omega'] * t) * s['dt']
-        acc += 0.5 * a * np.exp(1j * th)
-    acc[0, :] = 0
-    acc[:, 0] = 0
+        F = sim.fields
+        we = np.exp(1j * ((-s['omega'] * t) * s['dt']))
+        wh = np.exp(1j * ((-s['omega'] * (t - 0.5)) * s['dt']))
+        ex, hy = F['ex'][zt], F['hy']
+        acc['ex_ph_u'] += 0.5 * (ex[Y, X] + ex[Y, X1]).astype(np.float64) * we
+        acc['hy_ph_u'] += 0.25 * (((hy[zt, Y, X] + hy[zt, Y, X1]) + hy[zt - 1, Y, X])
+                                  + hy[zt - 1, Y, X1]).astype(np.float64) * wh
+        for f, x in xs.items():
+            ey, hz = F['ey'][:, :, x], F['hz']
+            hzx = (((hz[Z, Y, x] + hz[Z, Y, x - 1]) + hz[Z, Y1, x]) + hz[Z, Y1, x - 1])
+            acc['hz_ph_' + f] += 0.25 * hzx.astype(np.float64) * wh
+            if f == 'l':
+                acc['ey_ph_l'] += 0.5 * (ey[Z, Y] + ey[Z, Y1]).astype(np.float64) * we
+            else:
+                hyx = (((hy[Z, Y, x] + hy[Z, Y, x - 1]) + hy[Z1, Y, x]) + hy[Z1, Y, x - 1])
+                acc['hy_ph_r'] += 0.25 * hyx.astype(np.float64) * wh
+    for a in acc.values():
+        a[0, :] = 0
+        a[:, 0] = 0
     return acc
 
 
-def test_compensated_dft_is_closer_to_exact_sum():
+@STEPPERS
+def test_compensated_dft_is_closer_to_exact_sum(fused):
     pc, Ca, Cb = H.soil_case(steps=120, seed=9)
     beam = H.synthetic_beam(pc)
-    g = pc.geom_params
-    say('case: DFT accumulation modes vs exact sum, 120 steps')
-    exact = _exact_dft_z_face(pc, Ca, Cb, beam, 120, g.size_z - g.size_pml - 1)
+    say('case: DFT accumulation modes vs exact sum, 120 steps, stepper', fused)
+    exact = _exact_dft(pc, Ca, Cb, beam, 120)
     err = {}
     for mode in ('reference', 'compensated'):
         pc.processing_params.dft_accumulation = mode
-        sim = make_sim(pc, Ca, Cb, beam)
+        sim = make_sim(pc, Ca, Cb, beam, fused=fused)
         sim.run(120)
         sim.sync()
-        err[mode] = H.rel_l2(sim.phasor(4), exact)      # ex_ph_u
+        ph = sim.phasors()
+        err[mode] = {k: H.rel_l2(ph[k], want) for k, want in exact.items()}
         sim.destroy()
     say('relative L2 error vs exact sum', err)
-    assert err['compensated'] <= err['reference']
-    assert err['compensated'] < 1e-6 and err['reference'] < 1e-5
+    for k in exact:
+        assert err['compensated'][k] <= err['reference'][k], k
+        assert err['compensated'][k] < 1e-6 and err['reference'][k] < 1e-5, k
 
 
 @pytest.mark.parametrize('pol,theta', [('HH', 20.0), ('VV', 35.0), ('RHCP', 0.0)])
