@@ -220,6 +220,21 @@ int fdtd_sim_time_steps(FdtdSim *s, int nsteps, float *ms_total, float *ms_h,
  * each ntheta complex128 (interleaved), i.e. 8*ntheta doubles. */
 int fdtd_sim_near2far(FdtdSim *s, const double *thetas_deg, int ntheta,
                       double phi_deg, double r, double freq, double *out);
+/* No counterpart in the reference.  fdtd_sim_near2far without any device
+ * allocation, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): its work
+ * area (angle list, phase tables, partial sums, output staging) is a
+ * preallocated per-device array of FDTD_N2F_SCRATCH_BYTES, held by each GPU
+ * the library runs on until the process ends.  Same arguments and results
+ * (bit for bit) as fdtd_sim_near2far; rejects ntheta > FDTD_N2F_MAX_THETA or
+ * a phasor plane wider than FDTD_N2F_MAX_SIDE samples with FDTD_EINVAL.
+ * Every call on one GPU shares that GPU's work area: calls that overlap on
+ * one GPU (several host threads) race and must be serialised by the caller. */
+#define FDTD_N2F_MAX_THETA 181
+#define FDTD_N2F_MAX_SIDE 2047
+#define FDTD_N2F_SCRATCH_BYTES 12620032
+int fdtd_sim_near2far_noalloc(FdtdSim *s, const double *thetas_deg, int ntheta,
+                              double phi_deg, double r, double freq, double *out);
 /* Same transform of caller-supplied complex64 top-face phasors, each
  * (ny_ph, nx_ph): ex_u, ey_u, hx_u, hy_u. */
 int fdtd_near2far_host(const float *ex_u, const float *ey_u, const float *hx_u,

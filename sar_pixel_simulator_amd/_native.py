@@ -48,7 +48,7 @@ EXPORTS = [
     'fdtd_sim_get_phasor', 'fdtd_sim_get_field', 'fdtd_sim_set_field',
     'fdtd_sim_get_psi', 'fdtd_sim_set_stepper', 'fdtd_sim_set_launch',
     'fdtd_sim_time_steps',
-    'fdtd_sim_near2far', 'fdtd_near2far_host',
+    'fdtd_sim_near2far', 'fdtd_sim_near2far_noalloc', 'fdtd_near2far_host',
     'fdtd_beam_create', 'fdtd_beam_upload', 'fdtd_beam_get', 'fdtd_beam_release',
     'fdtd_beam_synthesize',
     'fdtd_batch_create', 'fdtd_batch_destroy', 'fdtd_batch_set_launch',
@@ -169,6 +169,7 @@ _SIGS = {
     'fdtd_sim_set_launch': ([_P, _I, _I, _I, _I, _I], _I),
     'fdtd_sim_time_steps': ([_P, _I, _FP, _FP, _FP, _FP], _I),
     'fdtd_sim_near2far': ([_P, _DP, _I, _D, _D, _D, _DP], _I),
+    'fdtd_sim_near2far_noalloc': ([_P, _DP, _I, _D, _D, _D, _DP], _I),
     'fdtd_near2far_host': ([_FP, _FP, _FP, _FP, _I, _I, _I, _D, _DP, _I, _D,
                             _D, _D, _DP], _I),
     'fdtd_beam_create': ([_I, _I, _I, _FP, _FP, _FP, _FP, ctypes.POINTER(_P)], _I),

@@ -2,6 +2,7 @@
 // HIP graphs around the kernels in fdtd_kernels.hip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdarg>
@@ -9,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -25,8 +27,11 @@ __global__ void k_step_strip(const Desc *, const int4 *, int, int);
 template <int V, int R>
 __global__ void k_step_rows(const Desc *, const int4 *, int, int);
 __global__ void k_phasor_out(const float4 *, long long, float2 *);
-__global__ void k_n2f_partial(const float2 *, const float2 *, const float2 *, const float2 *, int, int, int,
-                              const double2 *, const double2 *, double *);
+__global__ void k_n2f_tables(const double *, int, int, double, double, double, double, double, double2 *,
+                             double2 *, double2 *, double4 *);
+template <typename P>
+__global__ void k_n2f_partial(const P *, const P *, const P *, const P *, int, int, int, const double2 *,
+                              const double2 *, double *);
 __global__ void k_n2f_final(const double *, int, int, const double4 *, const double2 *, double2, double,
                             double2 *);
 template <int V>
@@ -620,13 +625,16 @@ static int group_run(Group &G, int nsteps, bool use_graph) {
 extern "C" int fdtd_abi_version(void) { return FDTD_ABI_VERSION; }
 extern "C" const char *fdtd_last_error(void) { return g_err.c_str(); }
 
+static int n2f_scratch_ptr(int device, void **out);
+
 extern "C" int fdtd_init(int device) {
     int n = 0;
     HIP_TRY(hipGetDeviceCount(&n));
     CHECK(device >= 0 && device < n, "device %d not present (%d devices)", device, n);
     HIP_TRY(hipSetDevice(device));
     g_device = device;
-    return FDTD_OK;
+    void *scratch_ptr = nullptr;   // the no-allocation entry point's work area on this device
+    return n2f_scratch_ptr(device, &scratch_ptr);
 }
 
 extern "C" int fdtd_device_info(FdtdDeviceInfo *out) {
@@ -1379,9 +1387,91 @@ extern "C" int fdtd_sim_time_steps(FdtdSim *s, int nsteps, float *ms_total, floa
 // ---------------------------------------------------------------------------
 // near-to-far
 // ---------------------------------------------------------------------------
-static int near2far_run(int device, hipStream_t st, const float2 *ex, const float2 *ey, const float2 *hx,
-                        const float2 *hy, int nxp, int nyp, int nzp, double dx, const double *thetas_deg,
-                        int ntheta, double phi_deg, double r, double freq, double *out) {
+// Device work area of one near-to-far call, carved from one buffer in
+// 256-byte aligned pieces: the theta list, the column and row phase tables,
+// the top-face factors, the per-(theta, split) partial sums and the output.
+struct N2fWork {
+    double *thetas;
+    double2 *phx, *phy, *zph;
+    double4 *trig;
+    double *part;
+    double2 *out;
+};
+
+static constexpr size_t n2f_round(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static int n2f_splits(int nyp) { return std::max(1, std::min(64, nyp / 8)); }
+
+static size_t n2f_carve(char *base, int ntheta, int nxp, int nyp, N2fWork *w) {
+    const size_t T = (size_t)ntheta;
+    const size_t sizes[7] = {T * sizeof(double), T * nxp * sizeof(double2), T * nyp * sizeof(double2),
+                             T * sizeof(double2), T * sizeof(double4),
+                             T * n2f_splits(nyp) * 8 * sizeof(double), 4 * T * sizeof(double2)};
+    void *ptrs[7];
+    size_t off = 0;
+    for (int i = 0; i < 7; ++i) {
+        ptrs[i] = base ? base + off : nullptr;
+        off += n2f_round(sizes[i]);
+    }
+    if (w) {
+        w->thetas = (double *)ptrs[0];
+        w->phx = (double2 *)ptrs[1];
+        w->phy = (double2 *)ptrs[2];
+        w->zph = (double2 *)ptrs[3];
+        w->trig = (double4 *)ptrs[4];
+        w->part = (double *)ptrs[5];
+        w->out = (double2 *)ptrs[6];
+    }
+    return off;
+}
+
+// Preallocated work area of fdtd_sim_near2far_noalloc: one per device (each
+// GPU the library runs on holds its own copy until the process ends), sized
+// for FDTD_N2F_MAX_THETA angles and FDTD_N2F_MAX_SIDE samples per side.
+// Host code reaches it only through the address hipGetSymbolAddress returns
+// for the current device (n2f_scratch_ptr), never through its name.
+static constexpr size_t kN2fScratchBytes =
+    n2f_round(FDTD_N2F_MAX_THETA * sizeof(double)) +
+    2 * n2f_round((size_t)FDTD_N2F_MAX_THETA * FDTD_N2F_MAX_SIDE * sizeof(double2)) +
+    n2f_round(FDTD_N2F_MAX_THETA * sizeof(double2)) + n2f_round(FDTD_N2F_MAX_THETA * sizeof(double4)) +
+    n2f_round((size_t)FDTD_N2F_MAX_THETA * 64 * 8 * sizeof(double)) +
+    n2f_round(4 * FDTD_N2F_MAX_THETA * sizeof(double2));
+static_assert(kN2fScratchBytes == FDTD_N2F_SCRATCH_BYTES, "header states the scratch size");
+static __device__ __align__(256) char scratch[kN2fScratchBytes];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+// Device address of `scratch` on `device`, looked up once per device with
+// that device current.
+static int n2f_scratch_ptr(int device, void **out) {
+    CHECK(device >= 0 && device < kMaxDevices, "device %d beyond the library's %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        HIP_TRY(hipGetDevice(&cur));
+        HIP_TRY(hipSetDevice(device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return fail(FDTD_EHIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", device,
+                        hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return FDTD_OK;
+}
+
+// One near-to-far transform on stream st.  work: a device area of at least
+// n2f_carve(nullptr, ...) bytes, or NULL to allocate one for the call.
+// Every piece of the area is written before it is read, so nothing needs
+// clearing between calls.
+template <typename P>
+static int near2far_run(int device, hipStream_t st, const P *ex, const P *ey, const P *hx, const P *hy,
+                        int nxp, int nyp, int nzp, double dx, const double *thetas_deg, int ntheta,
+                        double phi_deg, double r, double freq, double *out, void *work) {
     CHECK(nxp >= 3 && nyp >= 3, "phasor plane %d x %d too small", nyp, nxp);
     CHECK(nxp % 2 == 1 && nyp % 2 == 1, "composite Simpson needs odd sample counts (got %d x %d)", nyp, nxp);
     CHECK(ntheta > 0 && thetas_deg && out, "bad theta arguments");
@@ -1389,81 +1479,73 @@ static int near2far_run(int device, hipStream_t st, const float2 *ex, const floa
     const double eta0 = 376.730313668, c0 = 299792458.0;
     const double lam = c0 / freq, k = 2.0 * pi / lam;
     const double phi = phi_deg * (pi / 180.0);
-    // x_loc = arange(-(n-1)//2, (n-1)//2 + 1) * dx  (near2far.py:71-73)
-    const int x0 = -((nxp - 1) / 2), y0 = -((nyp - 1) / 2);   // n odd: (n-1) even
-    const double z_top = (double)((nzp - 1) / 2) * dx;
-    auto simpson_w = [](int n, int i) { return (i == 0 || i == n - 1) ? 1.0 : (i % 2 ? 4.0 : 2.0); };
-    std::vector<double2> phx((size_t)ntheta * nxp), phy((size_t)ntheta * nyp), zph(ntheta);
-    std::vector<double4> trig(ntheta);
-    const double cp = std::cos(phi), sp = std::sin(phi);
-    for (int t = 0; t < ntheta; ++t) {
-        const double th = thetas_deg[t] * (pi / 180.0);
-        const double st = std::sin(th), ct = std::cos(th);
-        const double a = st * cp, b = st * sp;
-        for (int i = 0; i < nxp; ++i) {
-            const double xv = (double)(x0 + i) * dx, w = simpson_w(nxp, i) * dx / 3.0;
-            const double p = k * (xv * a);
-            phx[(size_t)t * nxp + i] = make_double2(w * std::cos(p), w * std::sin(p));
-        }
-        for (int j = 0; j < nyp; ++j) {
-            const double yv = (double)(y0 + j) * dx, w = simpson_w(nyp, j) * dx / 3.0;
-            const double p = k * (yv * b);
-            phy[(size_t)t * nyp + j] = make_double2(w * std::cos(p), w * std::sin(p));
-        }
-        const double pz = k * (z_top * ct);
-        zph[t] = make_double2(std::cos(pz), std::sin(pz));
-        trig[t] = make_double4(ct, st, cp, sp);
-    }
+    const double z_top = (double)((nzp - 1) / 2) * dx;   // z_loc[-1] (near2far.py:73, :97)
     // ff_weight = 1j * k * exp(-1j * k * r) / (4 pi r)   (near2far.py:76-77)
     const double kr = k * r;
     const double2 wgt = make_double2(k * std::sin(kr) / (4.0 * pi * r), k * std::cos(kr) / (4.0 * pi * r));
-    const int nsplit = std::max(1, std::min(64, nyp / 8));
-    double2 *d_phx, *d_phy, *d_zph, *d_out;
-    double4 *d_trig;
-    double *d_part;
+    const int nsplit = n2f_splits(nyp);
     HIP_TRY(hipSetDevice(device));
-    HIP_TRY(hipMalloc(&d_phx, phx.size() * sizeof(double2)));
-    HIP_TRY(hipMalloc(&d_phy, phy.size() * sizeof(double2)));
-    HIP_TRY(hipMalloc(&d_zph, zph.size() * sizeof(double2)));
-    HIP_TRY(hipMalloc(&d_trig, trig.size() * sizeof(double4)));
-    HIP_TRY(hipMalloc(&d_part, (size_t)ntheta * nsplit * 8 * sizeof(double)));
-    HIP_TRY(hipMalloc(&d_out, (size_t)4 * ntheta * sizeof(double2)));
-    HIP_TRY(hipMemcpyAsync(d_phx, phx.data(), phx.size() * sizeof(double2), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(d_phy, phy.data(), phy.size() * sizeof(double2), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(d_zph, zph.data(), zph.size() * sizeof(double2), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(d_trig, trig.data(), trig.size() * sizeof(double4), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_n2f_partial, dim3(ntheta, nsplit), dim3(256), 0, st, ex, ey, hx, hy, nxp, nyp, nsplit,
-                       d_phx, d_phy, d_part);
-    hipLaunchKernelGGL(k_n2f_final, dim3((ntheta + 63) / 64), dim3(64), 0, st, d_part, ntheta, nsplit, d_trig,
-                       d_zph, wgt, eta0, d_out);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(out, d_out, (size_t)4 * ntheta * sizeof(double2), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    (void)hipFree(d_phx);
-    (void)hipFree(d_phy);
-    (void)hipFree(d_zph);
-    (void)hipFree(d_trig);
-    (void)hipFree(d_part);
-    (void)hipFree(d_out);
-    return FDTD_OK;
+    char *own = nullptr;
+    if (!work) {
+        HIP_TRY(hipMalloc(&own, n2f_carve(nullptr, ntheta, nxp, nyp, nullptr)));
+        work = own;
+    }
+    N2fWork w;
+    n2f_carve((char *)work, ntheta, nxp, nyp, &w);
+    int rc = FDTD_OK;
+    hipError_t e = hipMemcpyAsync(w.thetas, thetas_deg, (size_t)ntheta * sizeof(double),
+                                  hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_n2f_tables, dim3(ntheta, (std::max(nxp, nyp) + 255) / 256), dim3(256), 0, st,
+                           w.thetas, nxp, nyp, dx, k, std::cos(phi), std::sin(phi), z_top, w.phx, w.phy,
+                           w.zph, w.trig);
+        hipLaunchKernelGGL(k_n2f_partial<P>, dim3(ntheta, nsplit), dim3(256), 0, st, ex, ey, hx, hy, nxp, nyp,
+                           nsplit, w.phx, w.phy, w.part);
+        hipLaunchKernelGGL(k_n2f_final, dim3((ntheta + 63) / 64), dim3(64), 0, st, w.part, ntheta, nsplit,
+                           w.trig, w.zph, wgt, eta0, w.out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, w.out, (size_t)4 * ntheta * sizeof(double2), hipMemcpyDeviceToHost, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = es;
+    if (e != hipSuccess) rc = fail(FDTD_EHIP, "near-to-far on device %d failed: %s", device, hipGetErrorString(e));
+    if (own) (void)hipFree(own);
+    return rc;
+}
+
+static int near2far_of_sim(FdtdSim *s, const double *thetas_deg, int ntheta, double phi_deg, double r,
+                           double freq, double *out, void *work) {
+    HIP_TRY(hipSetDevice(s->device));
+    const float4 *u = s->acc;   // ex_ph_u ey_ph_u hx_ph_u hy_ph_u = phasors 4..7
+    const int np = s->g.npml;
+    return near2far_run<float4>(s->device, s->self.stream, u + s->h.acc_off[4], u + s->h.acc_off[5],
+                                u + s->h.acc_off[6], u + s->h.acc_off[7], s->g.nx - 2 * np, s->g.ny - 2 * np,
+                                s->g.nz - 2 * np, s->c.cell_size, thetas_deg, ntheta, phi_deg, r, freq, out,
+                                work);
 }
 
 extern "C" int fdtd_sim_near2far(FdtdSim *s, const double *thetas_deg, int ntheta, double phi_deg, double r,
                                  double freq, double *out) {
     CHECK(s, "NULL sim");
     CHECK(s->g.mode == FDTD_MODE_MAIN, "near-to-far needs a MAIN simulation");
-    HIP_TRY(hipSetDevice(s->device));
-    const long long n = s->h.face_off[2] - s->h.face_off[1];   // top (u) face
-    int rc = ensure_scratch(s, 4 * n);
+    return near2far_of_sim(s, thetas_deg, ntheta, phi_deg, r, freq, out, nullptr);
+}
+
+extern "C" int fdtd_sim_near2far_noalloc(FdtdSim *s, const double *thetas_deg, int ntheta, double phi_deg,
+                                         double r, double freq, double *out) {
+    CHECK(s, "NULL sim");
+    CHECK(s->g.mode == FDTD_MODE_MAIN, "near-to-far needs a MAIN simulation");
+    CHECK(ntheta >= 1 && ntheta <= FDTD_N2F_MAX_THETA, "ntheta %d outside [1, %d] (preallocated work area)",
+          ntheta, FDTD_N2F_MAX_THETA);
+    const int nxp = s->g.nx - 2 * s->g.npml, nyp = s->g.ny - 2 * s->g.npml;
+    CHECK(nxp <= FDTD_N2F_MAX_SIDE && nyp <= FDTD_N2F_MAX_SIDE,
+          "phasor plane %d x %d exceeds %d samples per side (preallocated work area)", nyp, nxp,
+          FDTD_N2F_MAX_SIDE);
+    void *work = nullptr;
+    int rc = n2f_scratch_ptr(s->device, &work);
     if (rc) return rc;
-    for (int c = 0; c < 4; ++c)   // ex_ph_u ey_ph_u hx_ph_u hy_ph_u = phasors 4..7
-        hipLaunchKernelGGL(k_phasor_out, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->self.stream,
-                           s->acc + s->h.acc_off[4 + c], n, s->scratch_c64 + c * n);
-    HIP_TRY(hipGetLastError());
-    const int np = s->g.npml;
-    return near2far_run(s->device, s->self.stream, s->scratch_c64, s->scratch_c64 + n, s->scratch_c64 + 2 * n,
-                        s->scratch_c64 + 3 * n, s->g.nx - 2 * np, s->g.ny - 2 * np, s->g.nz - 2 * np,
-                        s->c.cell_size, thetas_deg, ntheta, phi_deg, r, freq, out);
+    return near2far_of_sim(s, thetas_deg, ntheta, phi_deg, r, freq, out, work);
 }
 
 extern "C" int fdtd_near2far_host(const float *ex_u, const float *ey_u, const float *hx_u, const float *hy_u,
@@ -1480,8 +1562,8 @@ extern "C" int fdtd_near2far_host(const float *ex_u, const float *ey_u, const fl
     const float *src[4] = {ex_u, ey_u, hx_u, hy_u};
     for (int c = 0; c < 4; ++c)
         HIP_TRY(hipMemcpyAsync(d + c * n, src[c], n * sizeof(float2), hipMemcpyHostToDevice, st));
-    int rc = near2far_run(g_device, st, d, d + n, d + 2 * n, d + 3 * n, nx_ph, ny_ph, nz_ph, dx, thetas_deg,
-                          ntheta, phi_deg, r, freq, out);
+    int rc = near2far_run<float2>(g_device, st, d, d + n, d + 2 * n, d + 3 * n, nx_ph, ny_ph, nz_ph, dx,
+                                  thetas_deg, ntheta, phi_deg, r, freq, out, nullptr);
     (void)hipStreamSynchronize(st);
     (void)hipStreamDestroy(st);
     (void)hipFree(d);

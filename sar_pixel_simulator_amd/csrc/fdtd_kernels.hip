@@ -1616,7 +1616,7 @@ template __global__ void k_stream_probe<8, 6>(const float *, float *, long long)
 // The integrand is linear in J = (-hy_u, hx_u) and M = (ey_u, -ex_u), so
 // each theta needs the four Simpson-weighted sums of J_x, J_y, M_x, M_y
 // times exp(jk(x sin th cos ph + y sin th sin ph + z_top cos th)); the phase
-// factorises into per-column and per-row tables built on the host.
+// factorises into per-column and per-row tables (k_n2f_tables).
 // Block (theta, split) reduces its rows with wavefront shuffles, then LDS.
 // ---------------------------------------------------------------------------
 struct cd { double re, im; };
@@ -1627,9 +1627,57 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Phase tables of one call: per theta, the Simpson-weighted column factors
+// w_i exp(jk x_i sin th cos ph) and row factors w_j exp(jk y_j sin th sin ph)
+// with x_i = (i - (nxp-1)/2) dx (near2far.py:71-73, :126-131), the top-face
+// factor exp(jk z_top cos th) and (cos th, sin th, cos ph, sin ph).
+// Grid (ntheta, ceil(max(nxp, nyp) / 256)).
+__device__ __forceinline__ double simpson_w(int n, int i) {
+    return (i == 0 || i == n - 1) ? 1.0 : ((i & 1) ? 4.0 : 2.0);
+}
+
 __global__ void __launch_bounds__(256)
-k_n2f_partial(const float2 *__restrict__ ex_u, const float2 *__restrict__ ey_u,
-              const float2 *__restrict__ hx_u, const float2 *__restrict__ hy_u,
+k_n2f_tables(const double *__restrict__ thetas_deg, int nxp, int nyp, double dx, double k,
+             double cp, double sp, double z_top, double2 *__restrict__ phx,
+             double2 *__restrict__ phy, double2 *__restrict__ zph, double4 *__restrict__ trig) {
+    const int t = blockIdx.x;
+    const int i = blockIdx.y * blockDim.x + threadIdx.x;
+    const double th = thetas_deg[t] * (3.141592653589793 / 180.0);
+    double st, ct;
+    sincos(th, &st, &ct);
+    if (i < nxp) {
+        const double xv = (double)(i - (nxp - 1) / 2) * dx, w = simpson_w(nxp, i) * dx / 3.0;
+        double s, c;
+        sincos(k * (xv * (st * cp)), &s, &c);
+        phx[(long long)t * nxp + i] = make_double2(w * c, w * s);
+    }
+    if (i < nyp) {
+        const double yv = (double)(i - (nyp - 1) / 2) * dx, w = simpson_w(nyp, i) * dx / 3.0;
+        double s, c;
+        sincos(k * (yv * (st * sp)), &s, &c);
+        phy[(long long)t * nyp + i] = make_double2(w * c, w * s);
+    }
+    if (i == 0) {
+        double s, c;
+        sincos(k * (z_top * ct), &s, &c);
+        zph[t] = make_double2(c, s);
+        trig[t] = make_double4(ct, st, cp, sp);
+    }
+}
+
+// A top-face sample as complex64: from a staged complex64 plane, or from
+// the running-sum accumulators (hi.re, hi.im, lo.re, lo.im) rounded as
+// k_phasor_out rounds them, so both sources give the same values.
+__device__ __forceinline__ float2 n2f_sample(const float2 *p, long long q) { return p[q]; }
+__device__ __forceinline__ float2 n2f_sample(const float4 *p, long long q) {
+    const float4 v = p[q];
+    return make_float2((float)((double)v.x + (double)v.z), (float)((double)v.y + (double)v.w));
+}
+
+template <typename P>
+__global__ void __launch_bounds__(256)
+k_n2f_partial(const P *__restrict__ ex_u, const P *__restrict__ ey_u,
+              const P *__restrict__ hx_u, const P *__restrict__ hy_u,
               int nxp, int nyp, int nsplit,
               const double2 *__restrict__ phx,   // [ntheta][nxp]  w_i exp(jk x_i a)
               const double2 *__restrict__ phy,   // [ntheta][nyp]  w_j exp(jk y_j b)
@@ -1646,7 +1694,8 @@ k_n2f_partial(const float2 *__restrict__ ex_u, const float2 *__restrict__ ey_u,
         for (int i = threadIdx.x; i < nxp; i += blockDim.x) {
             const long long q = (long long)j * nxp + i;
             const cd w = {px[i].x, px[i].y};
-            const float2 a = hy_u[q], b = hx_u[q], c = ey_u[q], e = ex_u[q];
+            const float2 a = n2f_sample(hy_u, q), b = n2f_sample(hx_u, q);
+            const float2 c = n2f_sample(ey_u, q), e = n2f_sample(ex_u, q);
             const cd jx = cmul(w, cd{-(double)a.x, -(double)a.y});
             const cd jy = cmul(w, cd{(double)b.x, (double)b.y});
             const cd mx = cmul(w, cd{(double)c.x, (double)c.y});
@@ -1673,6 +1722,13 @@ k_n2f_partial(const float2 *__restrict__ ex_u, const float2 *__restrict__ ey_u,
         partial[((long long)th * nsplit + sp) * 8 + threadIdx.x] = v;
     }
 }
+
+template __global__ void k_n2f_partial<float2>(const float2 *, const float2 *, const float2 *,
+                                              const float2 *, int, int, int, const double2 *,
+                                              const double2 *, double *);
+template __global__ void k_n2f_partial<float4>(const float4 *, const float4 *, const float4 *,
+                                              const float4 *, int, int, int, const double2 *,
+                                              const double2 *, double *);
 
 // Sum the splits and assemble E/H far fields (near2far.py:94-106).
 // trig: per theta (cos th, sin th, cos ph, sin ph); zph: exp(jk z_top cos th);
