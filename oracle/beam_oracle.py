@@ -70,10 +70,15 @@ def taper_k(k, k_x, k_y, theta, phi, radius, pol):
         gauss * ((eh * hor_y) + (ev * ver_y))
 
 
-def create_inc_excitation(params):
+def create_inc_excitation(params, workdir=None, rows=16):
     """Ex, Ey excitation of the incident simulation, (nt_inc, ny, nx)
-    complex64 (incidentbeam.py:206-313), from the pulse of
-    IncidentBeam.create_pulse (incidentbeam.py:174-204)."""
+    complex64 (complex128 for datatype float64) (incidentbeam.py:206-313),
+    from the pulse of IncidentBeam.create_pulse (incidentbeam.py:174-204).
+
+    With ``workdir`` the spectra and the result live in memory-mapped files
+    there and the inverse FFT over time runs ``rows`` grid rows at a time:
+    the same values per pixel, for grids whose (nt_inc, ny, nx) complex
+    arrays do not fit in memory several times over."""
     ib = IncidentBeam(params)
     ib.create_pulse()
     g, b = params.geom_params, params.inc_beam_params
@@ -87,24 +92,45 @@ def create_inc_excitation(params):
     ky = 2 * np.pi * np.repeat(np.atleast_2d(np.fft.fftfreq(ny, dx)).transpose(),
                                nx, axis=1)
     ky = np.fft.fftshift(ky.reshape((ny, nx)), axes=0)
-    pulse_ex = np.zeros((nt_inc, ny, nx), dtype_c)
-    pulse_ey = np.zeros((nt_inc, ny, nx), dtype_c)
+    shape = (nt_inc, ny, nx)
+    if workdir is None:
+        pulse_ex = np.zeros(shape, dtype_c)
+        pulse_ey = np.zeros(shape, dtype_c)
+    else:
+        import os
+        pulse_ex, pulse_ey = (np.lib.format.open_memmap(os.path.join(workdir, f'pulse_{c}.npy'),
+                                                        mode='w+', dtype=dtype_c, shape=shape)
+                              for c in ('ex', 'ey'))
     all_k = 2 * np.pi * ib.pulse_freqs / C0
     sh, ish, fft2 = np.fft.fftshift, np.fft.ifftshift, np.fft.fft2
     for k_idx in range(nt_inc):
         k_val = all_k[k_idx]
         if k_val == 0.0:
+            pulse_ex[k_idx] = 0
+            pulse_ey[k_idx] = 0
             continue
         sx, sy = taper_k(k_val, kx, ky, b.theta_angle, b.phi_angle,
                          gauss_radius, b.polarization)
         w = 1.0 * ib.pulse_fft[k_idx]
         pulse_ex[k_idx] = w * sh(fft2(ish(sx)))
         pulse_ey[k_idx] = w * sh(fft2(ish(sy)))
-    pulse_ex = np.fft.ifft(pulse_ex, axis=0).astype(dtype_c, copy=False)
-    pulse_ey = np.fft.ifft(pulse_ey, axis=0).astype(dtype_c, copy=False)
-    maxval = float(max(np.abs(pulse_ex).max(), np.abs(pulse_ey).max()))
+    if workdir is None:
+        pulse_ex = np.fft.ifft(pulse_ex, axis=0).astype(dtype_c, copy=False)
+        pulse_ey = np.fft.ifft(pulse_ey, axis=0).astype(dtype_c, copy=False)
+        maxval = float(max(np.abs(pulse_ex).max(), np.abs(pulse_ey).max()))
+        scale = 10.0 / maxval
+        pulse_ex = (pulse_ex.astype(np.complex128) * scale).astype(dtype_c)
+        pulse_ey = (pulse_ey.astype(np.complex128) * scale).astype(dtype_c)
+        return pulse_ex, pulse_ey
+    maxval = 0.0
+    for arr in (pulse_ex, pulse_ey):
+        for y0 in range(0, ny, rows):
+            v = np.fft.ifft(arr[:, y0:y0 + rows], axis=0).astype(dtype_c, copy=False)
+            arr[:, y0:y0 + rows] = v
+            maxval = max(maxval, float(np.abs(v).max()))
     scale = 10.0 / maxval
-    pulse_ex = (pulse_ex.astype(np.complex128) * scale).astype(dtype_c)
-    pulse_ey = (pulse_ey.astype(np.complex128) * scale).astype(dtype_c)
+    for arr in (pulse_ex, pulse_ey):
+        for y0 in range(0, ny, rows):
+            arr[:, y0:y0 + rows] = (arr[:, y0:y0 + rows].astype(np.complex128) * scale).astype(dtype_c)
+        arr.flush()
     return pulse_ex, pulse_ey
-

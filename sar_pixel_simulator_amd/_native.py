@@ -489,8 +489,10 @@ class Sim:
         _check(lib().fdtd_sim_get_psi(self._h, int(index), _fptr(out)))
         return out
 
-    def set_launch(self, ty, zchunk=0, xvec=0, xwaves=1, xcd_order=1):
-        """Tile shape (see fdtd_sim_set_launch); xvec 0 = automatic."""
+    def set_launch(self, ty, zchunk=2, xvec=0, xwaves=0, xcd_order=0):
+        """Tile shape (fdtd_sim_set_launch).  The defaults are the library's
+        own: 2-plane z chunks, lane width and block width chosen from the
+        grid (0), plain block order; ty 0 chooses the rows per block."""
         _check(lib().fdtd_sim_set_launch(self._h, int(ty), int(zchunk), int(xvec), int(xwaves),
                                          int(xcd_order)))
 
@@ -562,7 +564,10 @@ class Batch:
         _check(lib().fdtd_batch_create(arr, len(self.sims), ctypes.byref(h)))
         self._h = h
 
-    def set_launch(self, ty, zchunk=0, xvec=0, xwaves=1, xcd_order=1):
+    def set_launch(self, ty, zchunk=2, xvec=0, xwaves=0, xcd_order=0):
+        """Tile shape of the batch (fdtd_batch_set_launch), defaults as
+        Sim.set_launch; ty, xvec and xwaves all 0 give each member its own
+        block width."""
         _check(lib().fdtd_batch_set_launch(self._h, int(ty), int(zchunk), int(xvec), int(xwaves),
                                            int(xcd_order)))
 

@@ -982,7 +982,7 @@ k_step_strip(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int
     const int wt = __builtin_amdgcn_readfirstlane((block_tile(shape, nblk) << 2) + w);
     if (wt >= ntiles) return;                          // uniform per wavefront
     const int4 tl = tiles[wt];
-    const Desc *__restrict__ d = descs + tl.x;
+    const Desc *__restrict__ d = descs + (tl.x & 0xffff);   // low 16 bits: member index
     const int t = load_step(d->step);
     note_step(d, tl, 0, t);                            // tiles are per wavefront
     if (t >= d->nt) return;
@@ -1184,7 +1184,7 @@ __global__ void __launch_bounds__(64 * (R + 1))
 k_step_rows(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles, int shape) {
     __shared__ float lds_h[2][R][2][kWave * V];        // [plane parity][row 0..R-1][hx, hz][cells]
     const int4 tl = tiles[block_tile(shape, ntiles)];
-    const Desc *__restrict__ d = descs + tl.x;
+    const Desc *__restrict__ d = descs + (tl.x & 0xffff);   // low 16 bits: member index
     const int t = load_step(d->step);
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     note_step(d, tl, w, t);

@@ -23,37 +23,91 @@ class BeamCache:
     reference's incident-file key, parameters.py:308-324, plus the values its
     metadata check compares, incidentbeam.py:105-165).  A beam is generated on
     first use, entirely on the device (excitation synthesis and incident
-    FDTD)."""
+    FDTD).  Each model holding a beam counts as a reference; a beam no model
+    references stays cached for later models with the same key until
+    ``evict`` frees it (least recently used first)."""
 
     def __init__(self):
-        self._beams = {}
+        self._beams = {}      # key -> (beam, phasors, p_inc, bytes)
+        self._refs = {}       # key -> models holding the beam
+        self._used = {}       # key -> last use (a counter)
+        self._tick = 0
 
-    def bytes_for(self, params):
+    @staticmethod
+    def beam_bytes(params):
         g = params.geom_params
-        if beam_key(params) in self._beams:
-            return 0
         return 4 * 4 * g.size_t_inc * g.size_y * g.size_x
 
+    def bytes_for(self, params):
+        """HBM a model of ``params`` adds for its beam: 0 when resident."""
+        if params.geom_params.size_t_inc == 0 or beam_key(params) in self._beams:
+            return 0
+        return self.beam_bytes(params)
+
+    def resident_bytes(self):
+        return sum(v[3] for v in self._beams.values())
+
     def get(self, params):
+        """(beam, incident phasors, incident power), generated if absent;
+        counts a reference until ``unref``."""
         k = beam_key(params)
         hit = self._beams.get(k)
         if hit is None:
             beam, ph = synthesize_on_device(params)
-            hit = (beam, ph, incident_power(ph, params.geom_params.cell_size))
+            hit = (beam, ph, incident_power(ph, params.geom_params.cell_size),
+                   self.beam_bytes(params))
             self._beams[k] = hit
-        return hit
+        self._refs[k] = self._refs.get(k, 0) + 1
+        self._tick += 1
+        self._used[k] = self._tick
+        return hit[:3]
+
+    def warm(self, params):
+        """Generate the beam of ``params`` ahead of use, holding no reference."""
+        self.get(params)
+        self.unref(params)
+
+    def unref(self, params):
+        k = beam_key(params)
+        if self._refs.get(k, 0) > 0:
+            self._refs[k] -= 1
 
     def put(self, params, beam, inc_phasors):
-        self._beams[beam_key(params)] = (beam, inc_phasors,
-                                         incident_power(inc_phasors, params.geom_params.cell_size))
+        k = beam_key(params)
+        self._beams[k] = (beam, inc_phasors,
+                          incident_power(inc_phasors, params.geom_params.cell_size),
+                          self.beam_bytes(params))
+        self._refs.setdefault(k, 0)
+        self._tick += 1
+        self._used[k] = self._tick
+
+    def evict(self, nbytes, keep=None):
+        """Free unreferenced beams, least recently used first, until at least
+        ``nbytes`` are freed; never the beam of ``keep`` (params).  Returns
+        the bytes freed."""
+        keep_k = beam_key(keep) if keep is not None else None
+        freed = 0
+        for k in sorted(self._beams, key=lambda k: self._used.get(k, 0)):
+            if freed >= nbytes:
+                break
+            if k == keep_k or self._refs.get(k, 0) > 0:
+                continue
+            b, _, _, nb = self._beams.pop(k)
+            b.release()
+            self._refs.pop(k, None)
+            self._used.pop(k, None)
+            freed += nb
+        return freed
 
     def __len__(self):
         return len(self._beams)
 
     def release(self):
-        for b, _, _ in self._beams.values():
+        for b, _, _, _ in self._beams.values():
             b.release()
         self._beams.clear()
+        self._refs.clear()
+        self._used.clear()
 
 
 def beam_key(params):
@@ -76,6 +130,7 @@ class DeviceModel:
         self.geom = common.main_geom(params)
         self.sim = common.new_main_sim(params)
         self.beam, self.inc_phasors, self.p_inc = (None, None, None)
+        self._beams = beams
         if g.size_t_inc > 0:
             self.beam, self.inc_phasors, self.p_inc = beams.get(params)
             self.sim.attach_beam(self.beam)
@@ -126,6 +181,9 @@ class DeviceModel:
 
     def release(self):
         self.sim.destroy()
+        if self.beam is not None and self._beams is not None:
+            self._beams.unref(self.params)
+            self.beam = None
 
 
 class DeviceBatch:

@@ -93,6 +93,8 @@ class MainSimHip:
         if Ca.shape != shape or Cb.shape != shape:
             raise ValueError(f'material arrays must have shape {shape}, '
                              f'got {Ca.shape} and {Cb.shape}')
+        if not (np.isfinite(Ca).all() and np.isfinite(Cb).all()):
+            raise ValueError('material arrays hold non-finite values')
         inc = self._attach_beam()
         self.sim.reset()
         self.sim.set_materials(Ca, Cb)

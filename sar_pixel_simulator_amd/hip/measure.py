@@ -41,7 +41,7 @@ def batched_vs_plain(count=8, chunk=64, device=0, log=print):
     cases = batched_workload(count)
     beams = B.BeamCache()
     for pc, _, _, _ in cases:
-        beams.get(pc)
+        beams.warm(pc)
     t_beams = time.perf_counter() - t0
     cells = [pc.geom_params.size_x * pc.geom_params.size_y * pc.geom_params.size_z
              for pc, _, _, _ in cases]
@@ -169,7 +169,7 @@ def serving(n=24, rate_hz=None, budget_gb=None, max_slots=16, chunk=96, device=0
     reqs = serving_requests(n, rate_hz=1.0, device_materials=device_materials)
     be = HipBackend(device, use_graph=use_graph)
     for r in reqs:                         # beams first: reported separately
-        be.beams.get(r.params)
+        be.beams.warm(r.params)
     fps = [be.footprint(r) for r in reqs]
     # the same requests run one at a time (admit, run, sigma0, release) give
     # the plain service rate, which sets the arrival rate

@@ -17,8 +17,16 @@ at every chunk boundary the loop
    that their admission at the next boundary does not stall the device.
 
 Requests that arrive while others run join at the next boundary, so their
-steps overlap on the device.  Each member's arithmetic does not depend on the
-other members, so a result is bit-identical alone and in the mix.
+steps overlap on the device.  The HBM in use is the members' own state plus
+every incident beam resident on the device: a beam is counted once, from
+its creation until it is freed, whichever requests share it; beams no
+member references are kept for later requests and evicted (least recently
+used first) when an admission needs their memory.  ``submit`` refuses a
+request whose inputs are malformed (shape, non-finite or non-positive
+values) or that could never fit; ``cancel`` withdraws a queued request at
+once and a running one at the next boundary.  Each member's arithmetic
+does not depend on the other members, so a result is bit-identical alone
+and in the mix.
 
 The device side is behind a small backend interface (``HipBackend``) so the
 scheduling logic can be exercised without a GPU.
@@ -64,6 +72,42 @@ class AdmissionError(RuntimeError):
     """The request can never fit the HBM budget."""
 
 
+def _finite_positive(name, a, positive):
+    a = np.asarray(a)
+    if not np.isfinite(a).all():
+        raise ValueError(f'{name} holds non-finite values')
+    if positive and not (a > 0).all():
+        raise ValueError(f'{name} must be positive')
+
+
+def check_request(req):
+    """ValueError unless the request's material inputs have the shapes its
+    geometry gives and finite values: Ca, Cb (nz, ny, nx); a MaterialModel's
+    permittivity (> 0) and conductivity (nz_mat, ny, nx), as
+    compute_material_arrays expects (materials/materials.py:23-68).  A
+    MaterialSpec is checked by the device builder."""
+    g = req.params.geom_params
+    if req.spec is not None:
+        return
+    if req.Ca is not None or req.Cb is not None:
+        want = (g.size_z, g.size_y, g.size_x)
+        for name, a in (('Ca', req.Ca), ('Cb', req.Cb)):
+            if a is None or np.shape(a) != want:
+                raise ValueError(f'{name} must have shape {want}, got '
+                                 f'{None if a is None else np.shape(a)}')
+            _finite_positive(name, a, False)
+        return
+    m = req.material
+    if m is None:
+        raise ValueError(f'request {req.rid} carries no material')
+    want = (g.size_z_material, g.size_y, g.size_x)
+    for name, a, pos in (('relative_permittivity', m.relative_permittivity, True),
+                         ('conductivity', m.conductivity, False)):
+        if np.shape(a) != want:
+            raise ValueError(f'{name} must have shape {want}, got {np.shape(a)}')
+        _finite_positive(name, a, pos)
+
+
 class HipBackend:
     """Device operations the loop needs, on libfdtd_hip."""
 
@@ -79,6 +123,19 @@ class HipBackend:
     def footprint(self, req):
         from .batch import DeviceModel
         return DeviceModel.footprint(req.params) + self.beams.bytes_for(req.params)
+
+    def beam_bytes_for(self, req):
+        """Part of footprint() that is the request's beam, 0 when resident."""
+        return self.beams.bytes_for(req.params)
+
+    def resident_beam_bytes(self):
+        return self.beams.resident_bytes()
+
+    def evict_beams(self, nbytes, keep):
+        return self.beams.evict(nbytes, keep=keep.params)
+
+    def check(self, req):
+        check_request(req)
 
     def steps(self, req):
         return req.params.geom_params.size_t
@@ -143,15 +200,51 @@ class Server:
         self.chunks = 0
         self.peak_used = 0
         self.peak_slots = 0
+        self.cancelled = set()           # running requests to withdraw at the next boundary
         # host seconds spent at chunk boundaries, by phase
         self.phase_s = dict.fromkeys(('wait', 'retire', 'admit', 'batch', 'prepare'), 0.0)
 
+    # -- memory ---------------------------------------------------------------
+    def _split_fp(self, req):
+        """(bytes of the member's own state, bytes of its beam if not resident)."""
+        fp = self.backend.footprint(req)
+        beam = getattr(self.backend, 'beam_bytes_for', None)
+        b = beam(req) if beam else 0
+        return fp - b, b
+
+    def beam_bytes(self):
+        f = getattr(self.backend, 'resident_beam_bytes', None)
+        return f() if f else 0
+
+    def in_use(self):
+        """HBM held now: members' state plus every resident beam."""
+        return self.used + self.beam_bytes()
+
     # -- queue ----------------------------------------------------------------
     def submit(self, req: Request):
+        check = getattr(self.backend, 'check', None)
+        if check:
+            check(req)                      # ValueError on malformed inputs
         fp = self.backend.footprint(req)
         if fp > self.budget:
             raise AdmissionError(f'request {req.rid} needs {fp} bytes, budget {self.budget}')
         self.queue.append(req)
+
+    def cancel(self, rid):
+        """Withdraw request ``rid``: a queued one at once, a running one at
+        the next chunk boundary (its device work in flight completes first).
+        Its result carries error 'cancelled'.  False if it is not here."""
+        for req in self.queue:
+            if req.rid == rid:
+                self.queue.remove(req)
+                now = self.clock()
+                self.results.append(Result(rid, req.name, req.arrival, now, now, 0, None,
+                                           'cancelled'))
+                return True
+        if any(e[0].rid == rid for e in self.active):
+            self.cancelled.add(rid)
+            return True
+        return False
 
     def idle(self):
         return not self.queue and not self.active
@@ -162,7 +255,14 @@ class Server:
         changed = False
         for entry in self.active:
             req, member, fp, t_admit, done, nt = entry
-            if done >= nt:
+            if req.rid in self.cancelled:
+                self.cancelled.discard(req.rid)
+                self.backend.release(member)
+                self.used -= fp
+                self.results.append(Result(req.rid, req.name, req.arrival, t_admit,
+                                           self.clock(), done, None, 'cancelled'))
+                changed = True
+            elif done >= nt:
                 out, err = None, None
                 try:
                     out = self.backend.collect(member)
@@ -182,8 +282,15 @@ class Server:
         changed = False
         while self.queue and len(self.active) < self.max_slots:
             req = self.queue[0]
-            fp = self.backend.footprint(req)
-            if self.used + fp > self.budget:
+            fp, beam = self._split_fp(req)
+            over = self.in_use() + fp + beam - self.budget
+            if over > 0:
+                evict = getattr(self.backend, 'evict_beams', None)
+                if evict:
+                    evict(over, req)            # unreferenced cached beams, LRU first
+                    fp, beam = self._split_fp(req)
+                    over = self.in_use() + fp + beam - self.budget
+            if over > 0:
                 break                           # FIFO: wait for memory to free up
             self.queue.popleft()
             try:
@@ -196,7 +303,7 @@ class Server:
             self.used += fp
             self.active.append([req, member, fp, self.clock(), 0, self.backend.steps(req)])
             changed = True
-        self.peak_used = max(self.peak_used, self.used)
+        self.peak_used = max(self.peak_used, self.in_use())
         self.peak_slots = max(self.peak_slots, len(self.active))
         return changed
 

@@ -64,14 +64,22 @@ class IncidentBeam:
 
     def load_existing(self):
         """Read and validate the cached beam, then upload it to the device."""
-        with np.load(inc_filename(self.params), allow_pickle=False) as f:
-            fields = [f[c] for c in ('ex', 'ey', 'hx', 'hy')]
-            self.check_array_sizes(*fields)
-            self.check_metadata(json.loads(str(f['attrs'])))
-            self.phasors = {n: f['phasors/' + n] for n in native.INC_PHASOR_NAMES}
+        fields, self.phasors = self.read_file(inc_filename(self.params))
         g = self.geom_params
         self.beam = native.Beam(g.size_t_inc, g.size_y, g.size_x, *fields)
         _RESIDENT[inc_filename(self.params)] = (self.beam, self.phasors)
+
+    def read_file(self, fname):
+        """(ex, ey, hx, hy) and the incident phasors of a cached beam file,
+        after the reference's checks: ValueError for a wrong shape or dtype,
+        a metadata attribute that differs from the current parameters, or an
+        incomplete incident run (incidentbeam.py:82-165)."""
+        with np.load(fname, allow_pickle=False) as f:
+            fields = [f[c] for c in ('ex', 'ey', 'hx', 'hy')]
+            self.check_array_sizes(*fields)
+            self.check_metadata(json.loads(str(f['attrs'])))
+            phasors = {n: f['phasors/' + n] for n in native.INC_PHASOR_NAMES}
+        return fields, phasors
 
     def check_array_sizes(self, inc_ex, inc_ey, inc_hx, inc_hy):
         """ValueError on a wrong shape or dtype (incidentbeam.py:82-103)."""
@@ -91,7 +99,9 @@ class IncidentBeam:
         want = beam_metadata(self.params)
         for key in META_KEYS:
             if meta.get(key) != want[key]:
-                raise ValueError(f'Incident beam metadata inconsistency ({key})')
+                # the reference names a size_t_inc mismatch "(duration)"
+                label = 'duration' if key == 'size_t_inc' else key
+                raise ValueError(f'Incident beam metadata inconsistency ({label})')
         if meta.get('incident_complete') is not True:
             raise ValueError('Incident beam simulation was not completed')
 
@@ -116,17 +126,25 @@ class IncidentBeam:
         """Write the beam, its phasors and metadata (the incident HDF5 layout,
         cpu/incident_sim.py:95-98, :167-171, :195-224) as .npz."""
         g = self.geom_params
-        arrays = {c: self.beam.component(i) for i, c in enumerate(('ex', 'ey', 'hx', 'hy'))}
-        for n in native.INC_PHASOR_NAMES:
-            arrays['phasors/' + n] = self.phasors[n]
-        meta = beam_metadata(self.params)
-        meta['incident_complete'] = True
-        arrays['attrs'] = np.array(json.dumps(meta))
         fname = inc_filename(self.params)
-        tmp = fname + '.tmp.npz'
-        np.savez(tmp, **arrays)
-        os.replace(tmp, fname)
+        write_beam_file(fname, [self.beam.component(i) for i in range(4)], self.phasors,
+                        self.params)
         logger.info(f'Incident beam written to {fname} ({g.size_t_inc} steps)')
+
+
+def write_beam_file(fname, fields, phasors, params: ParameterCollection, complete=True):
+    """Cached-beam file: ex, ey, hx, hy (nt_inc, ny, nx), the four incident
+    phasors and the metadata the reader checks; written to a temporary name
+    and renamed, so a reader never sees a partial file."""
+    arrays = {c: a for c, a in zip(('ex', 'ey', 'hx', 'hy'), fields)}
+    for n in native.INC_PHASOR_NAMES:
+        arrays['phasors/' + n] = phasors[n]
+    meta = beam_metadata(params)
+    meta['incident_complete'] = bool(complete)
+    arrays['attrs'] = np.array(json.dumps(meta))
+    tmp = fname + '.tmp.npz'
+    np.savez(tmp, **arrays)
+    os.replace(tmp, fname)
 
 
 def beam_metadata(params: ParameterCollection):

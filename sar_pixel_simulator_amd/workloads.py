@@ -103,14 +103,17 @@ def large_material(pc, seed=0):
 
 # ---------------------------------------------------------------------------
 # Largest single domain: 32 x 32 lambda lateral, 4 lambda material depth,
-# 0.021 lambda cells, material fraction 0.7 (1523 x 1523 x 271), 21 periods,
-# VV, theta 30; 1.5 lambda of snow over rough soil.
+# 0.021 lambda cells, material fraction 0.61 (1523 x 1523 x 311 = 7.2e8
+# cells), 21 periods (2000 steps), VV, theta 30; 1.5 lambda of snow over
+# rough soil.  ``lateral`` shrinks the lateral size only (the same depth,
+# cells, statistics and illumination as a fraction of the width): 8 lambda
+# gives 381 x 381 x 311, small enough for the FP64 oracle.
 # ---------------------------------------------------------------------------
-def largest_params(**kw):
+def largest_params(lateral=32.0, **kw):
     pc = base_params(polarization='VV', theta=30.0, incident_duration=10.0,
-                     illumination_width=24.0, simulation_duration=21.0,
-                     material_fraction=0.7, **kw)
-    set_geom_params(pc, 32.0, 32.0, 4.0, 21.0, 0.021, 'wavelength')
+                     illumination_width=0.75 * lateral, simulation_duration=21.0,
+                     material_fraction=0.61, **kw)
+    set_geom_params(pc, lateral, lateral, 4.0, 21.0, 0.021, 'wavelength')
     return pc
 
 

@@ -1,0 +1,336 @@
+"""Failure handling on the host (no device calls): the incident-beam cache
+file round trip and its rejections (incidentbeam.py:82-165), skip-if-exists
+of results (cpu/main_sim.py:64-79, gpu/main_sim.py:98-110), refusal to
+overwrite results (gpu/main_sim.py:398-401), malformed material inputs
+rejected before any device work, and the serving loop's admission checks,
+cancellation and beam accounting."""
+import copy
+import os
+
+import numpy as np
+import pytest
+
+from sar_pixel_simulator_amd import _native as native
+from sar_pixel_simulator_amd import workloads
+from sar_pixel_simulator_amd.hip import main_sim as MS
+from sar_pixel_simulator_amd.hip import common
+from sar_pixel_simulator_amd.hip.serve import Request, Server, check_request
+from sar_pixel_simulator_amd.incidentbeam import IncidentBeam, write_beam_file
+from sar_pixel_simulator_amd.materials import compute_material_arrays, synthetic
+from sar_pixel_simulator_amd.parameters import inc_filename, results_filename
+
+
+def _params(tmp_path, **kw):
+    pc = workloads.small_params(nx=25, ny=23, nz=27, nz_mat=9, **kw)
+    pc.processing_params.project_dir = str(tmp_path)
+    return pc
+
+
+def _beam_arrays(pc, seed=0):
+    g = pc.geom_params
+    rng = np.random.default_rng(seed)
+    fields = [rng.standard_normal((g.size_t_inc, g.size_y, g.size_x)).astype(np.float32)
+              for _ in range(4)]
+    n = g.size_pml
+    ph = {k: (rng.standard_normal((g.size_y - 2 * n, g.size_x - 2 * n))
+              + 1j * rng.standard_normal((g.size_y - 2 * n, g.size_x - 2 * n))).astype(np.complex64)
+          for k in native.INC_PHASOR_NAMES}
+    return fields, ph
+
+
+# --- incident cache ---------------------------------------------------------
+def test_incident_cache_round_trip(tmp_path):
+    pc = _params(tmp_path)
+    fields, ph = _beam_arrays(pc)
+    fname = inc_filename(pc)
+    write_beam_file(fname, fields, ph, pc)
+    assert os.path.isfile(fname) and not os.path.exists(fname + '.tmp.npz')
+    got_f, got_ph = IncidentBeam(pc).read_file(fname)
+    for a, b in zip(got_f, fields):
+        assert a.dtype == np.float32 and np.array_equal(a, b)
+    for k in native.INC_PHASOR_NAMES:
+        assert np.array_equal(got_ph[k], ph[k])
+    assert IncidentBeam(pc).inc_beam_exists()
+
+
+@pytest.mark.parametrize('field,value', [('theta_angle', 30.0), ('frequency', 5.0e9),
+                                         ('polarization', 'VV'), ('duration', 3.0),
+                                         ('illumination_width', 1.5), ('phi_angle', 10.0)])
+def test_incident_cache_metadata_mismatch_rejected(tmp_path, field, value):
+    pc = _params(tmp_path)
+    fields, ph = _beam_arrays(pc)
+    fname = os.path.join(str(tmp_path), 'beam.npz')
+    write_beam_file(fname, fields, ph, pc)
+    other = copy.deepcopy(pc)
+    setattr(other.inc_beam_params, field, value)
+    with pytest.raises(ValueError, match='metadata inconsistency'):
+        IncidentBeam(other).read_file(fname)
+
+
+def test_incident_cache_geometry_mismatch_rejected(tmp_path):
+    pc = _params(tmp_path)
+    fields, ph = _beam_arrays(pc)
+    fname = os.path.join(str(tmp_path), 'beam.npz')
+    write_beam_file(fname, fields, ph, pc)
+    other = copy.deepcopy(pc)
+    other.geom_params.size_pml = 8
+    with pytest.raises(ValueError, match=r'metadata inconsistency \(size_pml\)'):
+        IncidentBeam(other).read_file(fname)
+
+
+def test_incomplete_incident_run_rejected(tmp_path):
+    pc = _params(tmp_path)
+    fields, ph = _beam_arrays(pc)
+    fname = os.path.join(str(tmp_path), 'beam.npz')
+    write_beam_file(fname, fields, ph, pc, complete=False)
+    with pytest.raises(ValueError, match='not completed'):
+        IncidentBeam(pc).read_file(fname)
+
+
+def test_incident_cache_wrong_shape_or_dtype_rejected(tmp_path):
+    pc = _params(tmp_path)
+    fields, ph = _beam_arrays(pc)
+    fname = os.path.join(str(tmp_path), 'beam.npz')
+    write_beam_file(fname, [f[:-1] for f in fields], ph, pc)
+    with pytest.raises(ValueError, match='wrong shape'):
+        IncidentBeam(pc).read_file(fname)
+    write_beam_file(fname, [f.astype(np.float64) for f in fields], ph, pc)
+    with pytest.raises(ValueError, match='incorrect datatype'):
+        IncidentBeam(pc).read_file(fname)
+
+
+# --- results files ----------------------------------------------------------
+def _host_only_sim(pc):
+    """A MainSimHip whose device calls fail: the checks under test must
+    return or raise before any of them."""
+    sim = MS.MainSimHip.__new__(MS.MainSimHip)
+    sim.params = pc
+    sim.gpu_num = 0
+    sim.geom = common.main_geom(pc)
+
+    def no_device(*a, **k):
+        raise AssertionError('device work was started')
+    sim.simulate = no_device
+    sim.simulate_spec = no_device
+    sim._attach_beam = no_device
+    return sim
+
+
+def test_existing_results_are_skipped(tmp_path):
+    pc = _params(tmp_path)
+    fname = results_filename(pc, 'model_a')
+    with open(fname, 'wb') as f:
+        f.write(b'previous results')
+    sim = _host_only_sim(pc)
+    g = pc.geom_params
+    z = np.zeros((g.size_z, g.size_y, g.size_x), np.float32)
+    sim.run(z, z, 'model_a', {})                     # returns without device work
+    spec = type('Spec', (), {'name': 'model_a', 'metadata': {}})()
+    sim.run_spec(spec)
+    with open(fname, 'rb') as f:
+        assert f.read() == b'previous results'
+
+
+def test_results_are_never_overwritten(tmp_path):
+    pc = _params(tmp_path)
+    fname = results_filename(pc, 'model_b')
+    with open(fname, 'wb') as f:
+        f.write(b'keep me')
+    with pytest.raises(RuntimeError, match='Overwriting'):
+        MS.save_phasor_file(fname, {}, {}, pc, (1, 1, 1), {})
+    with open(fname, 'rb') as f:
+        assert f.read() == b'keep me'
+
+
+def test_simulate_rejects_bad_shape_and_nan_before_device_work(tmp_path):
+    pc = _params(tmp_path)
+    sim = _host_only_sim(pc)
+    del sim.simulate                                 # the real method, checks first
+    g = pc.geom_params
+    good = np.ones((g.size_z, g.size_y, g.size_x), np.float32)
+    with pytest.raises(ValueError, match='shape'):
+        sim.simulate(good[:-1], good[:-1])
+    bad = good.copy()
+    bad[3, 4, 5] = np.nan
+    with pytest.raises(ValueError, match='non-finite'):
+        sim.simulate(bad, good)
+    bad[3, 4, 5] = np.inf
+    with pytest.raises(ValueError, match='non-finite'):
+        sim.simulate(good, bad)
+
+
+# --- serving admission, cancellation, beams ---------------------------------
+def _canopy_request(pc, rid, **kw):
+    g = pc.geom_params
+    m = synthetic.rough_soil(g.size_x, g.size_y, g.size_z_material, g.cell_size,
+                             workloads.lam_of(pc), 5, rid)
+    return Request(rid=rid, params=pc, material=m, name=f'r{rid}', **kw), m
+
+
+def test_check_request_rejects_malformed_inputs(tmp_path):
+    pc = _params(tmp_path)
+    req, m = _canopy_request(pc, 0)
+    check_request(req)                               # well formed
+    Ca, Cb = compute_material_arrays(pc, m)
+    check_request(Request(rid=1, params=pc, Ca=Ca, Cb=Cb))
+    with pytest.raises(ValueError, match='Ca must have shape'):
+        check_request(Request(rid=2, params=pc, Ca=Ca[1:], Cb=Cb))
+    with pytest.raises(ValueError, match='Cb must have shape'):
+        check_request(Request(rid=3, params=pc, Ca=Ca))
+    Cb2 = Cb.copy()
+    Cb2[0, 0, 0] = np.nan
+    with pytest.raises(ValueError, match='non-finite'):
+        check_request(Request(rid=4, params=pc, Ca=Ca, Cb=Cb2))
+    m2 = copy.deepcopy(m)
+    m2.relative_permittivity = np.array(m2.relative_permittivity, np.float64)
+    m2.relative_permittivity[1, 2, 3] = 0.0
+    with pytest.raises(ValueError, match='positive'):
+        check_request(Request(rid=5, params=pc, material=m2))
+    m3 = copy.deepcopy(m)
+    m3.conductivity = np.asarray(m3.conductivity)[:, :-1]
+    with pytest.raises(ValueError, match='conductivity must have shape'):
+        check_request(Request(rid=6, params=pc, material=m3))
+    with pytest.raises(ValueError, match='no material'):
+        check_request(Request(rid=7, params=pc))
+
+
+class _Member:
+    def __init__(self, req, nt):
+        self.req, self.nt = req, nt
+
+
+class BeamBackend:
+    """Fake device with a shared-beam cache: each request names a beam key
+    in ``req.name``; a beam costs ``beam_size`` bytes once while resident."""
+
+    def __init__(self, model_size, beam_size, steps):
+        self.model_size, self.beam_size, self.steps_n = model_size, beam_size, steps
+        self.resident = {}                           # key -> references
+        self.evicted = []
+        self.checked = []
+
+    def check(self, req):
+        self.checked.append(req.rid)
+        if req.name == 'bad':
+            raise ValueError('malformed')
+
+    def footprint(self, req):
+        return self.model_size + self.beam_bytes_for(req)
+
+    def beam_bytes_for(self, req):
+        return 0 if req.name in self.resident else self.beam_size
+
+    def resident_beam_bytes(self):
+        return self.beam_size * len(self.resident)
+
+    def evict_beams(self, nbytes, keep):
+        freed = 0
+        for k in [k for k, n in self.resident.items() if n == 0 and k != keep.name]:
+            if freed >= nbytes:
+                break
+            del self.resident[k]
+            self.evicted.append(k)
+            freed += self.beam_size
+        return freed
+
+    def steps(self, req):
+        return self.steps_n
+
+    def admit(self, req):
+        self.resident[req.name] = self.resident.get(req.name, 0) + 1
+        return _Member(req, self.steps_n)
+
+    def make_batch(self, members):
+        return list(members)
+
+    def run(self, batch, n):
+        pass
+
+    def sync(self, batch):
+        pass
+
+    def drop_batch(self, batch):
+        pass
+
+    def collect(self, m):
+        return m.req.rid
+
+    def release(self, m):
+        self.resident[m.req.name] -= 1
+
+
+class Clock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        self.t += 1.0
+        return self.t
+
+
+def test_shared_beam_is_counted_once_and_until_freed():
+    # two requests share beam 'a': 10 + 10 model bytes and one 50-byte beam
+    be = BeamBackend(10, 50, 8)
+    s = Server(be, hbm_budget=200, chunk_steps=8, clock=Clock())
+    s.submit(Request(rid=0, params=None, name='a'))
+    s.submit(Request(rid=1, params=None, name='a'))
+    s.step()
+    assert len(s.active) == 2 and s.in_use() == 10 + 10 + 50
+    s.step()                                         # both retire
+    assert s.used == 0
+    # the beam stays resident (cached for later requests) and stays counted
+    assert s.in_use() == 50 and s.peak_used == 70
+
+
+def test_admission_evicts_unreferenced_beams_when_memory_binds():
+    be = BeamBackend(10, 50, 8)
+    s = Server(be, hbm_budget=80, chunk_steps=8, clock=Clock())
+    s.submit(Request(rid=0, params=None, name='a'))
+    s.drain()
+    assert s.in_use() == 50                          # beam 'a' cached, unreferenced
+    s.submit(Request(rid=1, params=None, name='b'))  # needs 60 more: 'a' must go
+    s.drain()
+    assert be.evicted == ['a'] and s.peak_used <= 80
+    assert sorted(r.rid for r in s.results if r.error is None) == [0, 1]
+    s.submit(Request(rid=2, params=None, name='b'))  # 'b' resident: no eviction, no new beam
+    s.drain()
+    assert be.evicted == ['a'] and s.in_use() == 50
+
+
+def test_referenced_beams_are_not_evicted():
+    be = BeamBackend(10, 50, 64)
+    s = Server(be, hbm_budget=80, chunk_steps=8, clock=Clock())
+    s.submit(Request(rid=0, params=None, name='a'))
+    s.step()                                         # 0 runs, holding 'a'
+    s.submit(Request(rid=1, params=None, name='b'))
+    s.step()
+    assert be.evicted == [] and [e[0].rid for e in s.active] == [0]   # 1 waits
+    s.drain()
+    assert be.evicted == ['a'] and len(s.results) == 2
+
+
+def test_submit_runs_the_backend_check():
+    be = BeamBackend(10, 50, 8)
+    s = Server(be, hbm_budget=200, chunk_steps=8, clock=Clock())
+    with pytest.raises(ValueError, match='malformed'):
+        s.submit(Request(rid=0, params=None, name='bad'))
+    assert not s.queue and be.checked == [0]
+
+
+def test_cancel_queued_and_running_requests():
+    be = BeamBackend(10, 0, 64)
+    s = Server(be, hbm_budget=1000, max_slots=1, chunk_steps=8, clock=Clock())
+    for k in range(3):
+        s.submit(Request(rid=k, params=None, name='a'))
+    s.step()                                         # 0 runs, 1 and 2 queued
+    assert s.cancel(2) and [r.rid for r in s.queue] == [1]
+    assert s.cancel(0)                               # running: withdrawn at the next boundary
+    assert not s.cancel(7)
+    s.step()
+    assert [e[0].rid for e in s.active] == [1]
+    s.drain()
+    res = {r.rid: r for r in s.results}
+    assert res[2].error == 'cancelled' and res[2].steps == 0
+    assert res[0].error == 'cancelled' and 0 < res[0].steps < 64
+    assert res[1].error is None and res[1].output == 1
+    assert s.used == 0

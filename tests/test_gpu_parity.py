@@ -82,14 +82,14 @@ def test_fields_psi_phasors_bitwise(soil60, fused):
 # (33 cells: one partly idle tile); fused stepper with 2, 4 and 8 rows per
 # wavefront, z ranges that start mid-column (recomputed plane below) and
 # strips whose first row is recomputed
-@pytest.mark.parametrize('fused,launch', [(False, (1, 0, 1)), (False, (2, 0, 2)),
-                                          (False, (4, 0, 4)), (False, (4, 7, 4)),
-                                          (False, (1, 5, 2)), (False, (2, 16, 4)),
-                                          (False, (2, 3, 1)), (False, (1, 1, 4)),
-                                          (True, (2, 0, 1)), (True, (2, 0, 4)),
-                                          (True, (8, 7, 1)), (True, (4, 5, 1)),
-                                          (True, (2, 1, 4)), (True, (8, 16, 0)),
-                                          (True, (4, 3, 2)), (True, (4, 2, 0)),
+@pytest.mark.parametrize('fused,launch', [(False, (1, 0, 1, 1, 1)), (False, (2, 0, 2, 1, 1)),
+                                          (False, (4, 0, 4, 1, 1)), (False, (4, 7, 4, 1, 1)),
+                                          (False, (1, 5, 2, 1, 1)), (False, (2, 16, 4, 1, 1)),
+                                          (False, (2, 3, 1, 1, 1)), (False, (1, 1, 4, 1, 1)),
+                                          (True, (2, 0, 1, 1, 1)), (True, (2, 0, 4, 1, 1)),
+                                          (True, (8, 7, 1, 1, 1)), (True, (4, 5, 1, 1, 1)),
+                                          (True, (2, 1, 4, 1, 1)), (True, (8, 16, 0, 1, 1)),
+                                          (True, (4, 3, 2, 1, 1)), (True, (4, 2, 0, 1, 1)),
                                           (False, (1, 1, 1, 4, 0)), (False, (1, 2, 2, 4, 0)),
                                           (False, (2, 4, 1, 2, 0)), (False, (1, 16, 1, 2, 1)),
                                           (False, (1, 0, 4, 4, 1)), (False, (2, 3, 2, 2, 1)),
@@ -215,7 +215,7 @@ def test_uneven_grid_bitwise():
     say('case: uneven 37x29x41, 40 steps')
     beam = H.synthetic_beam(pc)
     ref = H.oracle_main(pc, Ca, Cb, beam)
-    sim = make_sim(pc, Ca, Cb, beam, launch=(8, 9, 1), fused=True)
+    sim = make_sim(pc, Ca, Cb, beam, launch=(8, 9, 1, 1, 1), fused=True)
     sim.run(40)
     sim.sync()
     for n in native.FIELD_NAMES:
@@ -314,22 +314,22 @@ def test_minimum_interior_bitwise(nx):
 
 def test_invalid_launch_is_rejected_and_previous_kept(soil60):
     pc, Ca, Cb, beam, ref = soil60
-    sim = make_sim(pc, Ca, Cb, beam, launch=(2, 0, 4), fused=False)
+    sim = make_sim(pc, Ca, Cb, beam, launch=(2, 0, 4, 1, 1), fused=False)
     with pytest.raises(native.FdtdError):
-        sim.set_launch(8, 0, 4)          # 8 rows per tile is a fused-stepper shape
+        sim.set_launch(8, 0, 4, 1, 1)          # 8 rows per tile is a fused-stepper shape
     with pytest.raises(native.FdtdError):
-        sim.set_launch(2, 0, 3)
+        sim.set_launch(2, 0, 3, 1, 1)
     with pytest.raises(native.FdtdError):
-        sim.set_launch(2, 0, 1, 4)       # 8 wavefronts per block
+        sim.set_launch(2, 0, 1, 4, 1)       # 8 wavefronts per block
     with pytest.raises(native.FdtdError):
-        sim.set_launch(1, 0, 1, 3)
+        sim.set_launch(1, 0, 1, 3, 1)
     with pytest.raises(native.FdtdError):
         sim.set_launch(1, 4, 1, 4, 3)    # block orders are 0, 1, 2
     fsim = make_sim(pc, Ca, Cb, beam, fused=True)
     with pytest.raises(native.FdtdError):
-        fsim.set_launch(8, 0, 2)         # 8 rows of 2-cell lanes exceed the fused LDS budget
+        fsim.set_launch(8, 0, 2, 1, 1)         # 8 rows of 2-cell lanes exceed the fused LDS budget
     with pytest.raises(native.FdtdError):
-        fsim.set_launch(2, 0, 1, 2)      # one wavefront per strip
+        fsim.set_launch(2, 0, 1, 2, 1)      # one wavefront per strip
     with pytest.raises(native.FdtdError):
         fsim.set_launch(2, 0, 1, 1, 2)   # banded order is a split-stepper order
     fsim.destroy()
@@ -456,15 +456,21 @@ def test_split_and_rows_agree_at_largest_size():
     b.release()
 
 
-@pytest.mark.parametrize('fused,launch', [(True, None), (False, None), (False, (2, 8, 4)),
+@pytest.mark.parametrize('fused,launch', [(True, None), (False, None), (False, (2, 8, 4, 1, 1)),
                                           (False, (1, 1, 1, 4, 0)), (False, (1, 4, 1, 4, 2)),
-                                          (2, None)],
+                                          (2, None), (False, 'mixed-widths')],
                          ids=['fused', 'split-auto', 'split-4-per-lane', 'split-4-waves-flat',
-                              'split-banded', 'rows'])
+                              'split-banded', 'rows', 'split-auto-mixed-widths'])
 def test_batch_equals_solo(fused, launch):
     cases = [H.soil_case(nx=33, ny=31, nz=35, nz_mat=14, steps=30, seed=s) for s in (1, 2)]
     cases.append(H.soil_case(nx=45, ny=39, nz=37, nz_mat=15, steps=30, seed=5))
-    say('case: batch of 3 uneven sims, 30 steps, fused', fused, 'launch', launch)
+    if launch == 'mixed-widths':
+        # automatic shape: the 201-cell member gets 4-wavefront-wide blocks
+        # next to the 33- and 45-cell members' 1-wavefront blocks
+        cases.append(H.soil_case(nx=201, ny=23, nz=25, nz_mat=9, steps=30, seed=6))
+        cases.append(H.soil_case(nx=101, ny=23, nz=27, nz_mat=9, steps=30, seed=8))
+        launch = None
+    say('case: batch of', len(cases), 'uneven sims, 30 steps, fused', fused, 'launch', launch)
     solo, batch_sims = [], []
     for pc, Ca, Cb in cases:
         beam = H.synthetic_beam(pc)
