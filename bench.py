@@ -155,19 +155,19 @@ def measure_traffic(kernel, child_args, keep_dir=None):
 
 def largest_leg(fused, steps=10, warmup=3, log=print):
     """Throughput at the largest single domain (32 x 32 x 4 lambda at 0.021
-    lambda: 1523 x 1523 x 271 = 6.3e8 cells, snow over rough soil, VV theta
-    30), everything resident in HBM (the 50.9 GB beam buffer included; its
-    first planes hold an analytic beam of the real shape)."""
+    lambda: 1523 x 1523 x 311 = 7.2e8 cells, 1.5 lambda of snow over rough
+    soil, VV theta 30), everything resident in HBM (the 50.9 GB beam buffer
+    included; its first planes hold an analytic beam of the real shape).
+    The materials are drawn on the device (same statistics as the host
+    generator; the step rate does not depend on the values)."""
     from sar_pixel_simulator_amd import _native as native, workloads
     from sar_pixel_simulator_amd.hip import common, tuning
-    from sar_pixel_simulator_amd.materials import compute_material_arrays
+    from sar_pixel_simulator_amd.materials import device
     pc = workloads.largest_params()
     g = pc.geom_params
     t0 = time.perf_counter()
-    mat = workloads.largest_material(pc, seed=0)
-    Ca, Cb = compute_material_arrays(pc, mat)
-    del mat
-    log(f'bench: largest {g.size_x}x{g.size_y}x{g.size_z} material in {time.perf_counter() - t0:.1f} s')
+    snow = int(round(1.5 / 0.021))
+    spec = device.snow_over_soil_spec(pc, g.size_z_material - snow, snow, seed=0)
     geom = common.main_geom(pc)
     sim = native.Sim(geom, common.coeffs_for(pc))
     sim.set_stepper(fused)
@@ -176,9 +176,10 @@ def largest_leg(fused, steps=10, warmup=3, log=print):
     beam = native.Beam(g.size_t_inc, g.size_y, g.size_x)
     beam.upload(0, *workloads.synthetic_beam_planes(pc, 0, warmup + 2 * steps + 2))
     sim.attach_beam(beam)
-    sim.set_materials(Ca, Cb)
-    uni_z = uniform_top(Ca, Cb)
-    del Ca, Cb
+    sim.build_materials(*device.to_c(spec, pc))
+    sim.sync()
+    log(f'bench: largest {g.size_x}x{g.size_y}x{g.size_z} state resident in {time.perf_counter() - t0:.1f} s')
+    uni_z = g.size_z_material          # vacuum (uniform Ca, Cb) above the material
     sim.run(warmup)
     sim.sync()
     t0 = time.perf_counter()
@@ -191,7 +192,7 @@ def largest_leg(fused, steps=10, warmup=3, log=print):
     step_bytes = (ab['h'] + ab['e'] - 24 * cells if fused else ab['h'] + ab['e']) + ab['dft']
     ms = wall * 1e3 / steps
     out = {'workload': f'largest single domain: {g.size_x}x{g.size_y}x{g.size_z} cells (32x32x4 '
-                       'lambda at 0.021 lambda), snow over rough soil, VV theta 30, '
+                       'lambda at 0.021 lambda), 1.5 lambda of snow over rough soil, VV theta 30, '
                        f'{g.size_t}-step run',
            'cells': cells, 'steps': steps, 'ms_per_step': round(ms, 4),
            'Mcell_updates_per_s': round(cells / (ms * 1e-3) / 1e6, 1),
@@ -229,6 +230,11 @@ def main():
     ap.add_argument('--pmc-dir', default=None,
                     help='keep the PMC databases in this directory')
     ap.add_argument('--pmc-child', action='store_true', help=argparse.SUPPRESS)
+    ap.add_argument('--batch-count', type=int, default=32,
+                    help='batched leg: soil realizations (x HH/VV)')
+    ap.add_argument('--serve-requests', type=int, default=200, help='serving legs: requests')
+    ap.add_argument('--reference-loop', type=int, default=24,
+                    help='serving leg: requests timed through the reference-style loop')
     ap.add_argument('--largest-only', action='store_true',
                     help='run only the largest-domain leg and print its JSON (for profiling it)')
     args = ap.parse_args()
@@ -387,12 +393,13 @@ def main():
 
         def log(msg):
             print(msg, file=sys.stderr, flush=True)
-        print('bench: batched vs plain (uneven soil models)', file=sys.stderr, flush=True)
-        extras['batched_vs_plain'] = measure.batched_vs_plain(count=8, log=log)
-        print('bench: serving (Poisson canopy requests)', file=sys.stderr, flush=True)
-        extras['serving'] = measure.serving(n=48, log=log)
+        print('bench: batched vs plain (64 uneven soil models)', file=sys.stderr, flush=True)
+        extras['batched_vs_plain'] = measure.batched_vs_plain(count=args.batch_count, log=log)
         print('bench: serving, materials built on the device', file=sys.stderr, flush=True)
-        extras['serving_device_materials'] = measure.serving(n=48, log=log, device_materials=True)
+        extras['serving_device_materials'] = measure.serving(
+            n=args.serve_requests, log=log, device_materials=True, reference_loop=args.reference_loop)
+        print('bench: serving, host material volumes', file=sys.stderr, flush=True)
+        extras['serving'] = measure.serving(n=args.serve_requests, log=log)
         print('bench: material setup, host vs device', file=sys.stderr, flush=True)
         extras['material_setup'] = measure.material_setup(log=log)
         print('bench: largest single domain', file=sys.stderr, flush=True)

@@ -292,6 +292,21 @@ int fdtd_batch_set_launch(FdtdBatch *b, int ty, int zchunk, int xvec, int xwaves
 int fdtd_batch_run(FdtdBatch *b, int nsteps, int use_graph);
 int fdtd_batch_sync(FdtdBatch *b);
 int fdtd_batch_time_steps(FdtdBatch *b, int nsteps, int use_graph, float *ms_total);
+/* No counterpart in the reference.  How a batch steps its members:
+ * FDTD_BATCH_PACKED (the default) puts every member in one tile list, one
+ * launch per kernel per step for all of them; FDTD_BATCH_SERIAL runs each
+ * member's own launches (its own tile shape), a chunk of steps per member,
+ * one member after another on each of `lanes` concurrent streams (members
+ * dealt largest first to the lane with the fewest cells); FDTD_BATCH_HYBRID
+ * packs the members with fewer than small_cells cells and runs the others
+ * as in FDTD_BATCH_SERIAL.  Launches fork from and join back into the
+ * batch's stream (one graph per chunk when captured); fdtd_batch_sync waits
+ * for all of them.  Results are identical in every mode. */
+#define FDTD_BATCH_PACKED 0
+#define FDTD_BATCH_SERIAL 1
+#define FDTD_BATCH_HYBRID 2
+#define FDTD_BATCH_MAX_LANES 8
+int fdtd_batch_set_mode(FdtdBatch *b, int mode, int64_t small_cells, int lanes);
 
 /* ---- measurement support (no counterpart in the reference) ------------ */
 /* Copy nbytes between two fresh device buffers reps times (after one

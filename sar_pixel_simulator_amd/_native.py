@@ -52,7 +52,7 @@ EXPORTS = [
     'fdtd_beam_create', 'fdtd_beam_upload', 'fdtd_beam_get', 'fdtd_beam_release',
     'fdtd_beam_synthesize',
     'fdtd_batch_create', 'fdtd_batch_destroy', 'fdtd_batch_set_launch',
-    'fdtd_batch_run', 'fdtd_batch_sync', 'fdtd_batch_time_steps',
+    'fdtd_batch_run', 'fdtd_batch_sync', 'fdtd_batch_time_steps', 'fdtd_batch_set_mode',
     'fdtd_probe_copy', 'fdtd_probe_streams', 'fdtd_probe_stencil',
 ]
 
@@ -178,6 +178,7 @@ _SIGS = {
     'fdtd_beam_release': ([_P], None),
     'fdtd_beam_synthesize': ([ctypes.POINTER(FdtdGeom), ctypes.POINTER(FdtdCoeffs),
                               ctypes.POINTER(FdtdBeamSpec), ctypes.POINTER(_P), _FP], _I),
+    'fdtd_batch_set_mode': ([_P, _I, ctypes.c_int64, _I], _I),
     'fdtd_batch_create': ([ctypes.POINTER(_P), _I, ctypes.POINTER(_P)], _I),
     'fdtd_batch_destroy': ([_P], None),
     'fdtd_batch_set_launch': ([_P, _I, _I, _I, _I, _I], _I),
@@ -515,12 +516,17 @@ class Sim:
                                          None, None, None))
         return {'total': tot.value}
 
-    def near2far(self, thetas_deg, phi_deg, r, freq):
+    def near2far(self, thetas_deg, phi_deg, r, freq, noalloc=False):
+        """Far field of the top-face phasors.  ``noalloc`` uses the entry
+        point that allocates no device memory (at most N2F_MAX_THETA angles
+        and N2F_MAX_SIDE phasor samples per side; bit-identical results), so
+        the call does not wait for other streams' work as a device free
+        does."""
         th = np.ascontiguousarray(thetas_deg, dtype=np.float64)
         out = np.empty((4, th.size), np.complex128)
-        _check(lib().fdtd_sim_near2far(self._h, _dptr(th), th.size, float(phi_deg),
-                                       float(r), float(freq),
-                                       out.view(np.float64).ctypes.data_as(_DP)))
+        fn = lib().fdtd_sim_near2far_noalloc if noalloc else lib().fdtd_sim_near2far
+        _check(fn(self._h, _dptr(th), th.size, float(phi_deg), float(r), float(freq),
+                  out.view(np.float64).ctypes.data_as(_DP)))
         return {'E_theta': out[0], 'E_phi': out[1], 'H_theta': out[2],
                 'H_phi': out[3]}
 
@@ -554,6 +560,10 @@ def near2far_host(ex_u, ey_u, hx_u, hy_u, nx_ph, ny_ph, nz_ph, dx, thetas_deg,
     return {'E_theta': out[0], 'E_phi': out[1], 'H_theta': out[2], 'H_phi': out[3]}
 
 
+BATCH_PACKED, BATCH_SERIAL, BATCH_HYBRID = 0, 1, 2
+N2F_MAX_THETA, N2F_MAX_SIDE = 181, 2047
+
+
 class Batch:
     """Several simulations stepped by one launch per kernel per step."""
 
@@ -570,6 +580,13 @@ class Batch:
         block width."""
         _check(lib().fdtd_batch_set_launch(self._h, int(ty), int(zchunk), int(xvec), int(xwaves),
                                            int(xcd_order)))
+
+    def set_mode(self, mode, small_cells=0, lanes=1):
+        """BATCH_PACKED: every member in one tile list; BATCH_SERIAL: each
+        member's own launches, one member after another on each of ``lanes``
+        concurrent streams; BATCH_HYBRID: members below ``small_cells`` cells
+        packed, the others serial."""
+        _check(lib().fdtd_batch_set_mode(self._h, int(mode), int(small_cells), int(lanes)))
 
     def run(self, nsteps, use_graph=True):
         _check(lib().fdtd_batch_run(self._h, int(nsteps), 1 if use_graph else 0))

@@ -137,6 +137,7 @@ struct Group {
     hipStream_t stream = nullptr;
     Desc *d_descs = nullptr;
     int4 *d_tiles = nullptr;
+    size_t descs_cap = 0, tiles_cap = 0;   // allocated lengths, reused while they suffice
     int ntiles = 0, nmodels = 0, mode = FDTD_MODE_MAIN, fused = 0;
     std::vector<std::pair<int, int>> slabs;   // split stepper: tile ranges stepped H then E
     int xvec = 1;                  // x cells per lane of the kernels in use
@@ -168,13 +169,33 @@ struct FdtdSim {
     int *step = nullptr;
     FdtdBeam *beam = nullptr;
     Group self;                  // batch of one
+    void *mat_stage = nullptr;   // device staging of material specs, kept (a free waits for the device)
+    size_t mat_stage_len = 0;
+    float *vol_stage = nullptr;  // device staging of host volume uploads, kept likewise
+    size_t vol_stage_len = 0;
     float2 *scratch_c64 = nullptr;
     long long scratch_len = 0;
     bool mat_set = false;        // materials uploaded or uniform value given
 };
 
+// A batch steps its members in one of three ways (fdtd_batch_set_mode):
+// packed, every member in one tile list (one launch per kernel per step for
+// all of them); serial, each member's own direct launches one member after
+// another; hybrid, members below a cell count packed and the rest serial.
+// Everything runs on the batch's stream (grp.stream), captured per chunk of
+// steps into one graph.
 struct FdtdBatch {
-    Group grp;
+    Group grp;                        // the packed members (may be none)
+    std::vector<FdtdSim *> all;       // every member, in creation order
+    std::vector<FdtdSim *> serial;    // members stepped with their own launches
+    std::vector<int> lane;            // serial member k runs on streams[lane[k]]
+    std::vector<hipStream_t> streams; // [0] = grp.stream; more for concurrent serial lanes
+    std::vector<hipEvent_t> fork;     // capture: per lane, joined back into grp.stream
+    int mode = FDTD_BATCH_PACKED;
+    int nlanes = 1;
+    long long small_cells = 0;
+    std::map<int, hipGraphExec_t> graphs;
+    std::map<int, hipGraph_t> graph_src;
 };
 
 // ---------------------------------------------------------------------------
@@ -273,6 +294,7 @@ static long long footprint_bytes(const FdtdGeom *g) {
     long long b = 6 * field_stride(s.ncell) * 4 * ns + (s.psi_total + (ns - 1) * psi_h_len(s)) * 4 + s.acc_total * 16;
     if (g->mode == FDTD_MODE_MAIN) b += 2 * s.ncell * 4 + tap_len(g) * 16;
     b += (long long)(g->nt + 1) * (32 + 32) + 4096;
+    b += (long long)std::min<size_t>(32u << 20, (size_t)g->nx * g->ny * g->nz * sizeof(float));  // upload staging
     return b;
 }
 
@@ -453,12 +475,24 @@ static int group_build(Group &G) {
         G.max_dft_items = std::max(G.max_dft_items, s->h.dft_items);
     }
     group_clear_graphs(G);
-    (void)hipFree(G.d_descs);
-    (void)hipFree(G.d_tiles);
-    G.d_descs = nullptr;
-    G.d_tiles = nullptr;
-    HIP_TRY(hipMalloc(&G.d_descs, descs.size() * sizeof(Desc)));
-    HIP_TRY(hipMalloc(&G.d_tiles, tiles.size() * sizeof(int4)));
+    // a device free waits for all work on the device, so the tile list and
+    // descriptors keep their allocations while they are large enough: a
+    // rebuild while other streams run (a serving loop's admissions) then
+    // does not wait for them
+    if (descs.size() > G.descs_cap) {
+        (void)hipFree(G.d_descs);
+        G.d_descs = nullptr;
+        G.descs_cap = 0;
+        HIP_TRY(hipMalloc(&G.d_descs, descs.size() * sizeof(Desc)));
+        G.descs_cap = descs.size();
+    }
+    if (tiles.size() > G.tiles_cap) {
+        (void)hipFree(G.d_tiles);
+        G.d_tiles = nullptr;
+        G.tiles_cap = 0;
+        HIP_TRY(hipMalloc(&G.d_tiles, tiles.size() * sizeof(int4)));
+        G.tiles_cap = tiles.size();
+    }
     HIP_TRY(hipMemcpyAsync(G.d_descs, descs.data(), descs.size() * sizeof(Desc), hipMemcpyHostToDevice,
                            G.stream));
     HIP_TRY(hipMemcpyAsync(G.d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice,
@@ -509,12 +543,15 @@ static void group_free(Group &G) {
 enum KernelId { KH = 0, KE = 1, KD = 2, KF = 3 };
 
 template <typename K>
-static void launch_split(K direct, K listed, bool is_direct, dim3 grid, dim3 blk, Group &G, const int4 *tl,
-                         int nt, int shape, const TileGrid &tg) {
-    hipLaunchKernelGGL(is_direct ? direct : listed, grid, blk, 0, G.stream, G.d_descs, tl, nt, shape, tg, G.step1);
+static void launch_split(K direct, K listed, bool is_direct, dim3 grid, dim3 blk, Group &G, hipStream_t st,
+                         const int4 *tl, int nt, int shape, const TileGrid &tg) {
+    hipLaunchKernelGGL(is_direct ? direct : listed, grid, blk, 0, st, G.d_descs, tl, nt, shape, tg, G.step1);
 }
 
-static void launch_kernel(Group &G, KernelId k, int t0 = 0, int t1 = -1) {
+// One kernel of a step of group G on stream st (G's own stream unless a
+// batch runs the group's launches on its stream).
+static void launch_kernel(Group &G, KernelId k, int t0 = 0, int t1 = -1, hipStream_t st = nullptr) {
+    if (!st) st = G.stream;
     const dim3 blk(kWave, G.ty * G.xw);
     const int lxw = G.xw == 4 ? 2 : G.xw == 2 ? 1 : 0;
     const int shape = lxw | (G.launch.xcd == 1 ? kShapeXcd : 0);
@@ -534,7 +571,7 @@ static void launch_kernel(Group &G, KernelId k, int t0 = 0, int t1 = -1) {
         const int r = G.ty, v = G.xvec;
         auto *fn = v == 2 ? (r == 7 ? &k_step_rows<2, 7> : &k_step_rows<2, 3>)
                           : (r == 7 ? &k_step_rows<1, 7> : &k_step_rows<1, 3>);
-        hipLaunchKernelGGL(fn, dim3(G.ntiles), dim3(kWave, r + 1), 0, G.stream, G.d_descs, G.d_tiles, G.ntiles,
+        hipLaunchKernelGGL(fn, dim3(G.ntiles), dim3(kWave, r + 1), 0, st, G.d_descs, G.d_tiles, G.ntiles,
                            shape);
     } else if (k == KF) {
         const dim3 grid((G.ntiles + 3) / 4), b4(kWave, 4);
@@ -543,40 +580,40 @@ static void launch_kernel(Group &G, KernelId k, int t0 = 0, int t1 = -1) {
         if (v == 4) fn = &k_step_strip<4, 2>;
         else if (v == 2) fn = r == 4 ? &k_step_strip<2, 4> : &k_step_strip<2, 2>;
         else fn = r == 8 ? &k_step_strip<1, 8> : r == 4 ? &k_step_strip<1, 4> : &k_step_strip<1, 2>;
-        hipLaunchKernelGGL(fn, grid, b4, 0, G.stream, G.d_descs, G.d_tiles, G.ntiles, shape);
+        hipLaunchKernelGGL(fn, grid, b4, 0, st, G.d_descs, G.d_tiles, G.ntiles, shape);
     } else if (k == KH) {
         if (G.xvec == 4)
-            launch_split(k_h_half<4, true>, k_h_half<4, false>, tg.ntx > 0, sgrid, blk, G, tl, nt, shape, tg);
+            launch_split(k_h_half<4, true>, k_h_half<4, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt, shape, tg);
         else if (G.xvec == 2)
-            launch_split(k_h_half<2, true>, k_h_half<2, false>, tg.ntx > 0, sgrid, blk, G, tl, nt, shape, tg);
+            launch_split(k_h_half<2, true>, k_h_half<2, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt, shape, tg);
         else
-            launch_split(k_h_half<1, true>, k_h_half<1, false>, tg.ntx > 0, sgrid, blk, G, tl, nt, shape, tg);
+            launch_split(k_h_half<1, true>, k_h_half<1, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt, shape, tg);
     } else if (k == KE) {
         if (G.xvec == 4)
-            launch_split(k_e_half<4, true>, k_e_half<4, false>, tg.ntx > 0, sgrid, blk, G, tl, nt, shape, tg);
+            launch_split(k_e_half<4, true>, k_e_half<4, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt, shape, tg);
         else if (G.xvec == 2)
-            launch_split(k_e_half<2, true>, k_e_half<2, false>, tg.ntx > 0, sgrid, blk, G, tl, nt, shape, tg);
+            launch_split(k_e_half<2, true>, k_e_half<2, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt, shape, tg);
         else
-            launch_split(k_e_half<1, true>, k_e_half<1, false>, tg.ntx > 0, sgrid, blk, G, tl, nt, shape, tg);
+            launch_split(k_e_half<1, true>, k_e_half<1, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt, shape, tg);
     } else {
         const unsigned gx = (unsigned)std::max(1LL, (G.max_dft_items + 255) / 256);
-        hipLaunchKernelGGL(k_dft, dim3(gx, G.nmodels), dim3(256), 0, G.stream, G.d_descs);
+        hipLaunchKernelGGL(k_dft, dim3(gx, G.nmodels), dim3(256), 0, st, G.d_descs);
     }
 }
 
-static void launch_step(Group &G) {
+static void launch_step(Group &G, hipStream_t st = nullptr) {
     if (G.mode == FDTD_MODE_MAIN && G.fused) {   // H then E in one kernel, phasors
-        launch_kernel(G, KF);
+        launch_kernel(G, KF, 0, -1, st);
     } else if (G.mode == FDTD_MODE_MAIN) {       // cpu/main_sim.py:140-195: H, E, phasors
         for (const auto &sl : G.slabs) {
-            launch_kernel(G, KH, sl.first, sl.second);
-            launch_kernel(G, KE, sl.first, sl.second);
+            launch_kernel(G, KH, sl.first, sl.second, st);
+            launch_kernel(G, KE, sl.first, sl.second, st);
         }
     } else {                          // cpu/incident_sim.py:134-161: E, H, record + phasors
-        launch_kernel(G, KE);
-        launch_kernel(G, KH);
+        launch_kernel(G, KE, 0, -1, st);
+        launch_kernel(G, KH, 0, -1, st);
     }
-    launch_kernel(G, KD);
+    launch_kernel(G, KD, 0, -1, st);
 }
 
 static int group_run(Group &G, int nsteps, bool use_graph) {
@@ -821,22 +858,26 @@ static int sim_alloc(FdtdSim *s) {
     if (g->mode == FDTD_MODE_MAIN) HIP_TRY(hipMalloc(&s->mat, 2 * z.ncell * sizeof(float)));
     HIP_TRY(hipMalloc(&s->psi, psi_alloc * sizeof(float)));
     HIP_TRY(hipMalloc(&s->acc, z.acc_total * sizeof(float4)));
+    // initialisation is ordered on the simulation's own stream, so creating
+    // a simulation does not wait for work other streams have queued
+    hipStream_t st = s->self.stream;
     if (tap_len(g)) {
         HIP_TRY(hipMalloc(&s->tapx, tap_len(g) * sizeof(float4)));
-        HIP_TRY(hipMemset(s->tapx, 0, tap_len(g) * sizeof(float4)));
+        HIP_TRY(hipMemsetAsync(s->tapx, 0, tap_len(g) * sizeof(float4), st));
     }
     std::vector<double4> w64;
     std::vector<float4> w32;
     make_twiddles(g, &s->c, w64, w32);
     HIP_TRY(hipMalloc(&s->tw64, w64.size() * sizeof(double4)));
     HIP_TRY(hipMalloc(&s->tw32, w32.size() * sizeof(float4)));
-    HIP_TRY(hipMemcpy(s->tw64, w64.data(), w64.size() * sizeof(double4), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(s->tw32, w32.data(), w32.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(s->tw64, w64.data(), w64.size() * sizeof(double4), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(s->tw32, w32.data(), w32.size() * sizeof(float4), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMalloc(&s->step, 2 * sizeof(int)));   // counter, step of the DFT
     // the field arrays' padding columns are read as x+1 neighbours never,
     // but keep all device memory defined
-    HIP_TRY(hipMemset(s->fields, 0, 6 * field_stride(z.ncell) * ns * sizeof(float)));
-    if (s->mat) HIP_TRY(hipMemset(s->mat, 0, 2 * z.ncell * sizeof(float)));
+    HIP_TRY(hipMemsetAsync(s->fields, 0, 6 * field_stride(z.ncell) * ns * sizeof(float), st));
+    if (s->mat) HIP_TRY(hipMemsetAsync(s->mat, 0, 2 * z.ncell * sizeof(float), st));
+    HIP_TRY(hipStreamSynchronize(st));                // w64, w32 are host temporaries
 
     Desc &d = s->h;
     memset(&d, 0, sizeof d);
@@ -907,6 +948,8 @@ static void sim_free(FdtdSim *s) {
     (void)hipFree(s->src);
     (void)hipFree(s->step);
     (void)hipFree(s->scratch_c64);
+    (void)hipFree(s->mat_stage);
+    (void)hipFree(s->vol_stage);
     beam_unref(s->beam);
 }
 
@@ -929,12 +972,10 @@ extern "C" int fdtd_sim_create(const FdtdGeom *g, const FdtdCoeffs *c, FdtdSim *
         delete s;
         return fail(FDTD_EHIP, "hipSetDevice: %s", hipGetErrorString(e));
     }
-    rc = sim_alloc(s);
-    if (rc == FDTD_OK) {
-        s->self.device = s->device;
-        e = hipStreamCreateWithFlags(&s->self.stream, hipStreamNonBlocking);
-        if (e != hipSuccess) rc = fail(FDTD_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
-    }
+    s->self.device = s->device;
+    e = hipStreamCreateWithFlags(&s->self.stream, hipStreamNonBlocking);
+    if (e != hipSuccess) rc = fail(FDTD_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    if (rc == FDTD_OK) rc = sim_alloc(s);
     if (rc == FDTD_OK) {
         s->self.members = {s};
         rc = group_build(s->self);
@@ -960,25 +1001,44 @@ extern "C" void fdtd_sim_destroy(FdtdSim *s) {
 }
 
 // Host (nz, ny, nx) <-> device (nz, ny, pitch).  Pageable host memory goes
-// through one contiguous device staging buffer with a single bulk copy; the
-// pitch change is a device-to-device 2-D copy.  (A pitched copy straight from
-// pageable memory is staged row by row by the runtime: ~1e5 small copies for
-// a large grid.)
+// through a contiguous device staging buffer in bulk copies of whole rows;
+// the pitch change is a device-to-device 2-D copy.  (A pitched copy straight
+// from pageable memory is staged row by row by the runtime: ~1e5 small
+// copies for a large grid.)  The staging buffer (at most kStageBytes) is the
+// simulation's own and kept until it is destroyed: a device free waits for
+// all work on the device, which would stall an upload made while other
+// simulations run.
+static constexpr size_t kStageBytes = 32u << 20;
+
+static size_t stage_bytes(const FdtdGeom *g) {
+    return std::min(kStageBytes, (size_t)g->nx * g->ny * g->nz * sizeof(float));
+}
+
 static int copy_volume_in(FdtdSim *s, float *dst, const float *src) {
     const size_t nx = (size_t)s->g.nx, rows = (size_t)s->g.ny * s->g.nz;
     if ((long long)nx == s->pitch) {
         HIP_TRY(hipMemcpyAsync(dst, src, nx * rows * sizeof(float), hipMemcpyHostToDevice, s->self.stream));
         return FDTD_OK;
     }
-    float *tmp = nullptr;
-    HIP_TRY(hipMalloc(&tmp, nx * rows * sizeof(float)));
-    hipError_t e = hipMemcpyAsync(tmp, src, nx * rows * sizeof(float), hipMemcpyHostToDevice, s->self.stream);
-    if (e == hipSuccess)
-        e = hipMemcpy2DAsync(dst, s->pitch * sizeof(float), tmp, nx * sizeof(float), nx * sizeof(float), rows,
-                             hipMemcpyDeviceToDevice, s->self.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->self.stream);
-    (void)hipFree(tmp);
-    if (e != hipSuccess) return fail(FDTD_EHIP, "volume upload failed: %s", hipGetErrorString(e));
+    const size_t want = stage_bytes(&s->g);
+    if (s->vol_stage_len < want) {
+        (void)hipFree(s->vol_stage);
+        s->vol_stage = nullptr;
+        s->vol_stage_len = 0;
+        HIP_TRY(hipMalloc(&s->vol_stage, want));
+        s->vol_stage_len = want;
+    }
+    const size_t per = std::max<size_t>(1, s->vol_stage_len / (nx * sizeof(float)));
+    for (size_t r0 = 0; r0 < rows; r0 += per) {
+        const size_t n = std::min(per, rows - r0);
+        HIP_TRY(hipMemcpyAsync(s->vol_stage, src + r0 * nx, n * nx * sizeof(float), hipMemcpyHostToDevice,
+                               s->self.stream));
+        HIP_TRY(hipMemcpy2DAsync(dst + r0 * s->pitch, s->pitch * sizeof(float), s->vol_stage,
+                                 nx * sizeof(float), nx * sizeof(float), n, hipMemcpyDeviceToDevice,
+                                 s->self.stream));
+        // the staging buffer is reused by the next rows
+        HIP_TRY(hipStreamSynchronize(s->self.stream));
+    }
     return FDTD_OK;
 }
 
@@ -1062,19 +1122,29 @@ extern "C" int fdtd_sim_build_materials(FdtdSim *s, const FdtdMatSpec *spec, con
     HIP_TRY(hipSetDevice(s->device));
     HIP_TRY(hipStreamSynchronize(s->self.stream));
     const size_t plane = (size_t)nx * ny;
-    int32_t *d_surf = nullptr;
-    FdtdMatCylinder *d_cyl = nullptr;
+    // surface maps and cylinder list staged in one device buffer of the
+    // simulation, grown when too small and kept until it is destroyed
+    const size_t surf_bytes = (m.nlayers * plane * sizeof(int32_t) + 255) & ~(size_t)255;
+    const size_t need = surf_bytes + m.ncyl * sizeof(FdtdMatCylinder);
+    if (need > s->mat_stage_len) {
+        (void)hipFree(s->mat_stage);
+        s->mat_stage = nullptr;
+        s->mat_stage_len = 0;
+        HIP_TRY(hipMalloc(&s->mat_stage, need));
+        s->mat_stage_len = need;
+    }
+    int32_t *d_surf = m.nlayers > 0 ? (int32_t *)s->mat_stage : nullptr;
+    FdtdMatCylinder *d_cyl = m.ncyl > 0 ? (FdtdMatCylinder *)((char *)s->mat_stage + surf_bytes) : nullptr;
     float *Ca = s->mat, *Cb = s->mat + s->ncell;
     int rc = FDTD_OK;
     auto hip_ok = [&](hipError_t e, const char *what) {
         if (e != hipSuccess && rc == FDTD_OK) rc = fail(FDTD_EHIP, "%s: %s", what, hipGetErrorString(e));
         return rc == FDTD_OK;
     };
-    if (m.nlayers > 0 && hip_ok(hipMalloc(&d_surf, m.nlayers * plane * sizeof(int32_t)), "hipMalloc"))
+    if (m.nlayers > 0)
         hip_ok(hipMemcpyAsync(d_surf, surfaces, m.nlayers * plane * sizeof(int32_t), hipMemcpyHostToDevice,
                               s->self.stream), "hipMemcpyAsync");
     if (rc == FDTD_OK && m.ncyl > 0 &&
-        hip_ok(hipMalloc(&d_cyl, m.ncyl * sizeof(FdtdMatCylinder)), "hipMalloc") &&
         hip_ok(hipMemcpyAsync(d_cyl, cyl, m.ncyl * sizeof(FdtdMatCylinder), hipMemcpyHostToDevice,
                               s->self.stream), "hipMemcpyAsync") &&
         hip_ok(hipMemsetAsync(Cb, 0, s->ncell * sizeof(float), s->self.stream), "hipMemsetAsync")) {
@@ -1091,8 +1161,7 @@ extern "C" int fdtd_sim_build_materials(FdtdSim *s, const FdtdMatSpec *spec, con
         hip_ok(hipGetLastError(), "k_mat_build");
     }
     hip_ok(hipStreamSynchronize(s->self.stream), "hipStreamSynchronize");
-    if (d_surf) (void)hipFree(d_surf);
-    if (d_cyl) (void)hipFree(d_cyl);
+
     if (rc) return rc;
     // planes from nz_mat up are the vacuum compute_material_arrays appends;
     // its Ca, Cb by the same float32 operations
@@ -1680,6 +1749,122 @@ extern "C" int fdtd_beam_synthesize(const FdtdGeom *g, const FdtdCoeffs *c, cons
 // ---------------------------------------------------------------------------
 // batches
 // ---------------------------------------------------------------------------
+static void batch_clear_graphs(FdtdBatch *b) {
+    for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto &kv : b->graph_src) (void)hipGraphDestroy(kv.second);
+    b->graphs.clear();
+    b->graph_src.clear();
+}
+
+static long long sim_cells(const FdtdSim *s) { return (long long)s->g.nx * s->g.ny * s->g.nz; }
+
+// Split the members between the packed group and the serial list for the
+// batch's mode, deal the serial members to the batch's lanes (largest
+// first, each to the lane with the fewest cells so far), and rebuild the
+// packed group's tile list.
+static int batch_build(FdtdBatch *b) {
+    batch_clear_graphs(b);
+    std::vector<FdtdSim *> packed;
+    b->serial.clear();
+    for (FdtdSim *s : b->all) {
+        const bool pack = b->mode == FDTD_BATCH_PACKED ||
+                          (b->mode == FDTD_BATCH_HYBRID && sim_cells(s) < b->small_cells);
+        (pack ? packed : b->serial).push_back(s);
+    }
+    std::stable_sort(b->serial.begin(), b->serial.end(),
+                     [](const FdtdSim *a, const FdtdSim *c) { return sim_cells(a) > sim_cells(c); });
+    std::vector<long long> load(b->nlanes, 0);
+    b->lane.assign(b->serial.size(), 0);
+    for (size_t k = 0; k < b->serial.size(); ++k) {
+        const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        b->lane[k] = l;
+        load[l] += sim_cells(b->serial[k]);
+    }
+    while ((int)b->streams.size() < b->nlanes) {
+        hipStream_t st;
+        HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        b->streams.push_back(st);
+        hipEvent_t ev;
+        HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        b->fork.push_back(ev);
+    }
+    b->grp.members = packed;
+    if (packed.empty()) {
+        group_clear_graphs(b->grp);
+        b->grp.ntiles = b->grp.nmodels = 0;
+        return FDTD_OK;
+    }
+    return group_build(b->grp);
+}
+
+// The launches of len steps: the packed group step by step, then each
+// serial member's len steps with its own tile shape on its lane's stream.
+// Lanes other than the batch stream start after an event recorded on it
+// and end with an event it waits for, so a chunk is one fork-join (and one
+// graph when captured).  Members are independent, so neither the order nor
+// the concurrency changes results; a member's steps back to back keep its
+// working set in the caches.
+static int batch_launch(FdtdBatch *b, int len) {
+    hipStream_t st0 = b->grp.stream;
+    const bool lanes = b->nlanes > 1 && !b->serial.empty();
+    if (lanes) {
+        HIP_TRY(hipEventRecord(b->fork[0], st0));
+        for (int l = 1; l < b->nlanes; ++l) HIP_TRY(hipStreamWaitEvent(b->streams[l], b->fork[0], 0));
+    }
+    if (!b->grp.members.empty())
+        for (int s = 0; s < len; ++s) launch_step(b->grp);
+    for (size_t k = 0; k < b->serial.size(); ++k) {
+        hipStream_t st = b->lane[k] == 0 ? st0 : b->streams[b->lane[k]];
+        for (int s = 0; s < len; ++s) launch_step(b->serial[k]->self, st);
+    }
+    if (lanes)
+        for (int l = 1; l < b->nlanes; ++l) {
+            HIP_TRY(hipEventRecord(b->fork[l], b->streams[l]));
+            HIP_TRY(hipStreamWaitEvent(st0, b->fork[l], 0));
+        }
+    return FDTD_OK;
+}
+
+static int batch_run(FdtdBatch *b, int nsteps, bool use_graph) {
+    CHECK(nsteps >= 0, "negative step count");
+    if (b->mode == FDTD_BATCH_PACKED) return group_run(b->grp, nsteps, use_graph);
+    if (nsteps == 0) return FDTD_OK;
+    HIP_TRY(hipSetDevice(b->grp.device));
+    hipStream_t st = b->grp.stream;
+    if (!use_graph) {
+        int rc = batch_launch(b, nsteps);
+        if (rc) return rc;
+        HIP_TRY(hipGetLastError());
+        return FDTD_OK;
+    }
+    const int chunk = std::min(nsteps, 64);
+    auto replay = [&](int len) -> int {
+        auto it = b->graphs.find(len);
+        if (it == b->graphs.end()) {
+            hipGraph_t graph;
+            HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            int rc = batch_launch(b, len);
+            hipError_t ec = hipStreamEndCapture(st, &graph);
+            if (rc) return rc;
+            HIP_TRY(ec);
+            hipGraphExec_t exec;
+            HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            b->graph_src[len] = graph;
+            it = b->graphs.emplace(len, exec).first;
+        }
+        HIP_TRY(hipGraphLaunch(it->second, st));
+        return FDTD_OK;
+    };
+    int done = 0;
+    while (done + chunk <= nsteps) {
+        int rc = replay(chunk);
+        if (rc) return rc;
+        done += chunk;
+    }
+    if (done < nsteps) return replay(nsteps - done);
+    return FDTD_OK;
+}
+
 extern "C" int fdtd_batch_create(FdtdSim *const *sims, int n, FdtdBatch **out) {
     CHECK(sims && n > 0 && out, "bad arguments");
     *out = nullptr;
@@ -1687,10 +1872,11 @@ extern "C" int fdtd_batch_create(FdtdSim *const *sims, int n, FdtdBatch **out) {
         CHECK(sims[i], "sim %d is NULL", i);
         int rc = check_ready(sims[i]);
         if (rc) return rc;
+        CHECK(sims[i]->device == sims[0]->device, "batch members must share a device");
     }
     FdtdBatch *b = new FdtdBatch();
     b->grp.device = sims[0]->device;
-    b->grp.members.assign(sims, sims + n);
+    b->all.assign(sims, sims + n);
     b->grp.launch = default_launch(sims[0]->h.fused);
     (void)hipSetDevice(b->grp.device);
     hipError_t e = hipStreamCreateWithFlags(&b->grp.stream, hipStreamNonBlocking);
@@ -1698,13 +1884,21 @@ extern "C" int fdtd_batch_create(FdtdSim *const *sims, int n, FdtdBatch **out) {
         delete b;
         return fail(FDTD_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
     }
-    // members' own streams may still hold work (uploads, resets)
-    for (int i = 0; i < n; ++i) (void)hipStreamSynchronize(sims[i]->self.stream);
-    int rc = group_build(b->grp);
-    if (rc) {
-        std::string keep = g_err;
+    b->streams.push_back(b->grp.stream);
+    hipEvent_t ev0;
+    e = hipEventCreateWithFlags(&ev0, hipEventDisableTiming);
+    if (e != hipSuccess) {
         group_free(b->grp);
         delete b;
+        return fail(FDTD_EHIP, "hipEventCreate: %s", hipGetErrorString(e));
+    }
+    b->fork.push_back(ev0);
+    // members' own streams may still hold work (uploads, resets)
+    for (int i = 0; i < n; ++i) (void)hipStreamSynchronize(sims[i]->self.stream);
+    int rc = batch_build(b);
+    if (rc) {
+        std::string keep = g_err;
+        fdtd_batch_destroy(b);
         g_err = keep;
         return rc;
     }
@@ -1715,7 +1909,10 @@ extern "C" int fdtd_batch_create(FdtdSim *const *sims, int n, FdtdBatch **out) {
 extern "C" void fdtd_batch_destroy(FdtdBatch *b) {
     if (!b) return;
     (void)hipSetDevice(b->grp.device);
-    (void)hipStreamSynchronize(b->grp.stream);
+    for (hipStream_t st : b->streams) (void)hipStreamSynchronize(st);
+    batch_clear_graphs(b);
+    for (size_t l = 1; l < b->streams.size(); ++l) (void)hipStreamDestroy(b->streams[l]);
+    for (hipEvent_t ev : b->fork) (void)hipEventDestroy(ev);
     group_free(b->grp);
     delete b;
 }
@@ -1729,11 +1926,36 @@ extern "C" int fdtd_batch_set_launch(FdtdBatch *b, int ty, int zchunk, int xvec,
     b->grp.launch.xvec = xvec;
     b->grp.launch.xw = xwaves;
     b->grp.launch.xcd = xcd_order;
-    int rc = group_build(b->grp);
+    int rc = batch_build(b);
     if (rc) {
         std::string err = g_err;
         b->grp.launch = keep;
-        (void)group_build(b->grp);
+        (void)batch_build(b);
+        g_err = err;
+    }
+    return rc;
+}
+
+extern "C" int fdtd_batch_set_mode(FdtdBatch *b, int mode, int64_t small_cells, int lanes) {
+    CHECK(b, "NULL batch");
+    CHECK(mode == FDTD_BATCH_PACKED || mode == FDTD_BATCH_SERIAL || mode == FDTD_BATCH_HYBRID,
+          "unknown batch mode %d", mode);
+    CHECK(small_cells >= 0, "negative cell count");
+    CHECK(lanes >= 1 && lanes <= FDTD_BATCH_MAX_LANES, "lanes %d outside [1, %d]", lanes, FDTD_BATCH_MAX_LANES);
+    HIP_TRY(hipSetDevice(b->grp.device));
+    for (hipStream_t st : b->streams) HIP_TRY(hipStreamSynchronize(st));
+    const int keep_mode = b->mode, keep_lanes = b->nlanes;
+    const long long keep_cells = b->small_cells;
+    b->mode = mode;
+    b->small_cells = small_cells;
+    b->nlanes = lanes;
+    int rc = batch_build(b);
+    if (rc) {
+        std::string err = g_err;
+        b->mode = keep_mode;
+        b->small_cells = keep_cells;
+        b->nlanes = keep_lanes;
+        (void)batch_build(b);
         g_err = err;
     }
     return rc;
@@ -1741,7 +1963,7 @@ extern "C" int fdtd_batch_set_launch(FdtdBatch *b, int ty, int zchunk, int xvec,
 
 extern "C" int fdtd_batch_run(FdtdBatch *b, int nsteps, int use_graph) {
     CHECK(b, "NULL batch");
-    return group_run(b->grp, nsteps, use_graph != 0);
+    return batch_run(b, nsteps, use_graph != 0);
 }
 
 extern "C" int fdtd_batch_sync(FdtdBatch *b) {
@@ -1753,5 +1975,23 @@ extern "C" int fdtd_batch_sync(FdtdBatch *b) {
 
 extern "C" int fdtd_batch_time_steps(FdtdBatch *b, int nsteps, int use_graph, float *ms_total) {
     CHECK(b, "NULL batch");
-    return time_group(b->grp, nsteps, use_graph != 0, ms_total, nullptr, nullptr, nullptr);
+    CHECK(nsteps > 0, "nsteps must be positive");
+    if (b->mode == FDTD_BATCH_PACKED)
+        return time_group(b->grp, nsteps, use_graph != 0, ms_total, nullptr, nullptr, nullptr);
+    HIP_TRY(hipSetDevice(b->grp.device));
+    hipEvent_t ev[2];
+    HIP_TRY(hipEventCreate(&ev[0]));
+    HIP_TRY(hipEventCreate(&ev[1]));
+    HIP_TRY(hipEventRecord(ev[0], b->grp.stream));
+    int rc = batch_run(b, nsteps, use_graph != 0);
+    if (rc == FDTD_OK) {
+        HIP_TRY(hipEventRecord(ev[1], b->grp.stream));
+        HIP_TRY(hipEventSynchronize(ev[1]));
+        float tot = 0;
+        HIP_TRY(hipEventElapsedTime(&tot, ev[0], ev[1]));
+        if (ms_total) *ms_total = tot;
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    return rc;
 }

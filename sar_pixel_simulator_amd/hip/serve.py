@@ -5,23 +5,29 @@ Replaces the reference's one-process-per-GPU MPI farm
 runs one model and reports SUCCESS).  Here requests queue on the host;
 at every chunk boundary the loop
 
-1. retires members that have run all their steps: far field and sigma0
-   from the device-resident phasor sums, then frees their memory;
-2. admits queued requests in arrival order while a slot is free and the
-   request's HBM footprint (fields, coefficients, CPML state, phasor sums,
-   plus its incident beam unless already resident) fits the budget;
+1. waits for the chunk in flight and frees the members retired at the
+   previous boundary;
+2. takes finished (and cancelled) members out of the batch;
 3. rebuilds the batch when membership changed and starts the next chunk of
-   time steps for every member, one launch per kernel per step;
-4. while that chunk runs on the device, prepares queued requests on the
-   host (the Yee coefficients of requests that carry a MaterialModel), so
-   that their admission at the next boundary does not stall the device.
+   time steps for every member (each member's chunk with its own launches,
+   one member after another or on a few concurrent lanes, or all members in
+   one tile list: hip.batch.DeviceBatch);
+4. while that chunk runs on the device: admits queued requests in arrival
+   order while a slot is free and the request's HBM footprint (fields,
+   coefficients, CPML state, phasor sums, plus its incident beam unless
+   already resident) fits the budget -- they join at the next boundary --,
+   prepares queued requests on the host (the Yee coefficients of requests
+   that carry a MaterialModel) and collects the retired members' far field
+   and sigma0 from their device-resident phasor sums.
 
-Requests that arrive while others run join at the next boundary, so their
-steps overlap on the device.  The HBM in use is the members' own state plus
+Requests that arrive while others run join at a boundary, so their steps
+overlap on the device.  The HBM in use is the members' own state plus
 every incident beam resident on the device: a beam is counted once, from
 its creation until it is freed, whichever requests share it; beams no
 member references are kept for later requests and evicted (least recently
-used first) when an admission needs their memory.  ``submit`` refuses a
+used first) when an admission needs their memory; so are the device
+simulations of finished requests, kept for the next request on the same
+grid (hip.batch.SimPool).  ``submit`` refuses a
 request whose inputs are malformed (shape, non-finite or non-positive
 values) or that could never fit; ``cancel`` withdraws a queued request at
 once and a running one at the next boundary.  Each member's arithmetic
@@ -111,28 +117,39 @@ def check_request(req):
 class HipBackend:
     """Device operations the loop needs, on libfdtd_hip."""
 
-    def __init__(self, device=0, launch=None, use_graph=True):
+    def __init__(self, device=0, launch=None, use_graph=None, mode='auto', recycle=True):
         from .. import _native as native
-        from .batch import BeamCache
+        from .batch import BeamCache, SimPool
         native.init(device)
         self.native = native
         self.beams = BeamCache()
+        self.pool = SimPool() if recycle else None
         self.launch = launch
-        self.use_graph = use_graph
+        self.use_graph = use_graph          # None: the batch mode's own choice
+        self.mode = mode
+
+    def model_bytes(self, req):
+        """Device state of the request's simulation (beam excluded)."""
+        from .batch import DeviceModel
+        return DeviceModel.footprint(req.params)
 
     def footprint(self, req):
-        from .batch import DeviceModel
-        return DeviceModel.footprint(req.params) + self.beams.bytes_for(req.params)
+        """Device memory an admission adds: the simulation unless a kept one
+        on the same grid is recycled, and the beam unless resident."""
+        recycled = self.pool is not None and self.pool.has(req.params)
+        return (0 if recycled else self.model_bytes(req)) + self.beams.bytes_for(req.params)
 
-    def beam_bytes_for(self, req):
-        """Part of footprint() that is the request's beam, 0 when resident."""
-        return self.beams.bytes_for(req.params)
+    def cached_bytes(self):
+        """Resident beams and kept simulations, whether or not in use."""
+        return self.beams.resident_bytes() + (self.pool.bytes() if self.pool is not None else 0)
 
-    def resident_beam_bytes(self):
-        return self.beams.resident_bytes()
-
-    def evict_beams(self, nbytes, keep):
-        return self.beams.evict(nbytes, keep=keep.params)
+    def evict_cached(self, nbytes, keep):
+        """Free kept simulations, then unreferenced beams, until ``nbytes``
+        are freed; never what ``keep`` (a request) would reuse."""
+        freed = self.pool.evict(nbytes, keep=keep.params) if self.pool is not None else 0
+        if freed < nbytes:
+            freed += self.beams.evict(nbytes - freed, keep=keep.params)
+        return freed
 
     def check(self, req):
         check_request(req)
@@ -152,10 +169,10 @@ class HipBackend:
         from .batch import DeviceModel
         from ..materials import compute_material_arrays
         if req.spec is not None:
-            return DeviceModel.from_spec(req.params, req.spec, self.beams, req.name)
+            return DeviceModel.from_spec(req.params, req.spec, self.beams, req.name, pool=self.pool)
         Ca, Cb = (req.Ca, req.Cb) if req.Ca is not None else \
             compute_material_arrays(req.params, req.material)
-        model = DeviceModel(req.params, Ca, Cb, self.beams, req.name)
+        model = DeviceModel(req.params, Ca, Cb, self.beams, req.name, pool=self.pool)
         if req.prepared:                    # uploaded: the host copy is no longer needed
             req.Ca = req.Cb = None
             req.prepared = False
@@ -163,7 +180,7 @@ class HipBackend:
 
     def make_batch(self, members):
         from .batch import DeviceBatch
-        return DeviceBatch(members, self.launch)
+        return DeviceBatch(members, self.launch, self.mode)
 
     def run(self, batch, nsteps):
         batch.run(nsteps, use_graph=self.use_graph)
@@ -201,24 +218,30 @@ class Server:
         self.peak_used = 0
         self.peak_slots = 0
         self.cancelled = set()           # running requests to withdraw at the next boundary
+        self.retiring = []               # left the active set; results not yet collected
+        self.to_free = []                # results collected; memory freed at the next boundary
+        self.joined = False              # admitted since the batch was built
+        self.running = []                # entries stepped by the chunk in flight
         # host seconds spent at chunk boundaries, by phase
         self.phase_s = dict.fromkeys(('wait', 'retire', 'admit', 'batch', 'prepare'), 0.0)
 
     # -- memory ---------------------------------------------------------------
-    def _split_fp(self, req):
-        """(bytes of the member's own state, bytes of its beam if not resident)."""
-        fp = self.backend.footprint(req)
-        beam = getattr(self.backend, 'beam_bytes_for', None)
-        b = beam(req) if beam else 0
-        return fp - b, b
+    def _need(self, req):
+        """(bytes the member will hold while active, bytes its admission adds
+        to the device: less when a kept simulation or a resident beam is
+        reused)."""
+        new = self.backend.footprint(req)
+        mb = getattr(self.backend, 'model_bytes', None)
+        return (mb(req) if mb else new), new
 
-    def beam_bytes(self):
-        f = getattr(self.backend, 'resident_beam_bytes', None)
+    def cached_bytes(self):
+        f = getattr(self.backend, 'cached_bytes', None)
         return f() if f else 0
 
     def in_use(self):
-        """HBM held now: members' state plus every resident beam."""
-        return self.used + self.beam_bytes()
+        """HBM held now: active members, members awaiting their free, and
+        every resident beam and kept simulation."""
+        return self.used + self.cached_bytes()
 
     # -- queue ----------------------------------------------------------------
     def submit(self, req: Request):
@@ -246,50 +269,61 @@ class Server:
             return True
         return False
 
-    def idle(self):
-        return not self.queue and not self.active
-
     # -- one boundary + chunk --------------------------------------------------
     def _retire(self):
+        """Take finished and cancelled members out of the active set; their
+        results are collected after the next chunk is launched and their
+        memory is freed at the boundary after that (_collect)."""
         keep = []
-        changed = False
         for entry in self.active:
+            req = entry[0]
+            if req.rid in self.cancelled or entry[4] >= entry[5]:
+                self.retiring.append(entry)
+            else:
+                keep.append(entry)
+        changed = len(keep) != len(self.active)
+        self.active = keep
+        return changed
+
+    def _collect(self):
+        """Far field and sigma0 of the retiring members (device work on their
+        own streams, overlapping the chunk in flight), their results, and
+        their release deferred to the next boundary."""
+        for entry in self.retiring:
             req, member, fp, t_admit, done, nt = entry
             if req.rid in self.cancelled:
                 self.cancelled.discard(req.rid)
-                self.backend.release(member)
-                self.used -= fp
                 self.results.append(Result(req.rid, req.name, req.arrival, t_admit,
                                            self.clock(), done, None, 'cancelled'))
-                changed = True
-            elif done >= nt:
+            else:
                 out, err = None, None
                 try:
                     out = self.backend.collect(member)
                 except Exception as e:          # a failing request does not stop the loop
                     err = f'{type(e).__name__}: {e}'
-                self.backend.release(member)
-                self.used -= fp
                 self.results.append(Result(req.rid, req.name, req.arrival, t_admit,
                                            self.clock(), nt, out, err))
-                changed = True
-            else:
-                keep.append(entry)
-        self.active = keep
-        return changed
+            self.to_free.append(entry)
+        self.retiring = []
+
+    def _free(self):
+        for req, member, fp, _, _, _ in self.to_free:
+            self.backend.release(member)
+            self.used -= fp
+        self.to_free = []
 
     def _admit(self):
         changed = False
         while self.queue and len(self.active) < self.max_slots:
             req = self.queue[0]
-            fp, beam = self._split_fp(req)
-            over = self.in_use() + fp + beam - self.budget
+            fp, new = self._need(req)
+            over = self.in_use() + new - self.budget
             if over > 0:
-                evict = getattr(self.backend, 'evict_beams', None)
+                evict = getattr(self.backend, 'evict_cached', None)
                 if evict:
-                    evict(over, req)            # unreferenced cached beams, LRU first
-                    fp, beam = self._split_fp(req)
-                    over = self.in_use() + fp + beam - self.budget
+                    evict(over, req)            # kept simulations, unreferenced beams
+                    fp, new = self._need(req)
+                    over = self.in_use() + new - self.budget
             if over > 0:
                 break                           # FIFO: wait for memory to free up
             self.queue.popleft()
@@ -310,9 +344,10 @@ class Server:
     def _finish_chunk(self):
         if self.inflight:
             self.backend.sync(self.batch)
-            for e in self.active:
+            for e in self.running:              # members admitted since did not step
                 e[4] = min(e[5], e[4] + self.inflight)
             self.inflight = 0
+            self.running = []
 
     def _prepare_queued(self):
         prepare = getattr(self.backend, 'prepare', None)
@@ -326,37 +361,53 @@ class Server:
                 pass
 
     def step(self):
-        """Finish the chunk in flight, retire, admit, start the next chunk and
-        prepare queued requests while it runs.  Returns the number of active
-        members."""
+        """One boundary: wait for the chunk in flight, free the members
+        retired at the previous boundary, retire finished and cancelled
+        members, start the next chunk of the remaining ones, and while it
+        runs admit queued requests (they join at the next boundary), prepare
+        the ones still queued and collect the retired members' results.
+        Returns the number of active members."""
         ph = self.phase_s
         t0 = time.perf_counter()
         self._finish_chunk()
         t1 = time.perf_counter()
-        changed = self._retire()
+        self._free()
+        changed = self._retire() or self.joined
+        self.joined = False
         t2 = time.perf_counter()
-        changed = self._admit() or changed
+        if not self.active:
+            # nothing to run: admit first, then start them
+            changed = self._admit() or changed
         t3 = time.perf_counter()
-        ph['wait'] += t1 - t0
-        ph['retire'] += t2 - t1
-        ph['admit'] += t3 - t2
         if changed and self.batch is not None:
             self.backend.drop_batch(self.batch)
             self.batch = None
-        if not self.active:
-            ph['batch'] += time.perf_counter() - t3
-            return 0
-        if self.batch is None:
-            self.batch = self.backend.make_batch([e[1] for e in self.active])
-        n = min(self.chunk, max(e[5] - e[4] for e in self.active))
-        self.backend.run(self.batch, n)        # asynchronous
-        self.inflight = n
-        self.chunks += 1
+        if self.active:
+            if self.batch is None:
+                self.batch = self.backend.make_batch([e[1] for e in self.active])
+            n = min(self.chunk, max(e[5] - e[4] for e in self.active))
+            self.backend.run(self.batch, n)        # asynchronous
+            self.inflight = n
+            self.running = list(self.active)
+            self.chunks += 1
         t4 = time.perf_counter()
-        ph['batch'] += t4 - t3
+        self.joined = self._admit()
+        t5 = time.perf_counter()
         self._prepare_queued()
-        ph['prepare'] += time.perf_counter() - t4
+        t6 = time.perf_counter()
+        # last: reading a far field back may wait for the chunk's kernels
+        self._collect()
+        t7 = time.perf_counter()
+        ph['wait'] += t1 - t0
+        ph['retire'] += (t2 - t1) + (t7 - t6)
+        ph['admit'] += (t3 - t2) + (t5 - t4)
+        ph['batch'] += t4 - t3
+        ph['prepare'] += t6 - t5
         return len(self.active)
+
+    def idle(self):
+        return (not self.queue and not self.active and not self.retiring
+                and not self.to_free and not self.inflight)
 
     def drain(self):
         while not self.idle():

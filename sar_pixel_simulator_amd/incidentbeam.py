@@ -219,6 +219,18 @@ def params_device(params: ParameterCollection):
     return d[0] if isinstance(d, (list, tuple)) else int(d)
 
 
+def make_resident(params: ParameterCollection, beam, phasors):
+    """Register a device-resident beam (and its incident phasors) as this
+    process's beam for ``params``' key, as setup() does after generating it."""
+    _RESIDENT[inc_filename(params)] = (beam, phasors)
+
+
+def forget_resident(params: ParameterCollection):
+    """Unregister the beam of ``params``' key without releasing it (its owner
+    releases it)."""
+    _RESIDENT.pop(inc_filename(params), None)
+
+
 def clear_resident():
     """Drop this process's device-resident beams."""
     for beam, _ in _RESIDENT.values():

@@ -201,7 +201,8 @@ class _Member:
 
 class BeamBackend:
     """Fake device with a shared-beam cache: each request names a beam key
-    in ``req.name``; a beam costs ``beam_size`` bytes once while resident."""
+    in ``req.name``; a beam costs ``beam_size`` bytes once while resident
+    (HipBackend counts kept simulations the same way)."""
 
     def __init__(self, model_size, beam_size, steps):
         self.model_size, self.beam_size, self.steps_n = model_size, beam_size, steps
@@ -214,16 +215,16 @@ class BeamBackend:
         if req.name == 'bad':
             raise ValueError('malformed')
 
+    def model_bytes(self, req):
+        return self.model_size
+
     def footprint(self, req):
-        return self.model_size + self.beam_bytes_for(req)
+        return self.model_size + (0 if req.name in self.resident else self.beam_size)
 
-    def beam_bytes_for(self, req):
-        return 0 if req.name in self.resident else self.beam_size
-
-    def resident_beam_bytes(self):
+    def cached_bytes(self):
         return self.beam_size * len(self.resident)
 
-    def evict_beams(self, nbytes, keep):
+    def evict_cached(self, nbytes, keep):
         freed = 0
         for k in [k for k, n in self.resident.items() if n == 0 and k != keep.name]:
             if freed >= nbytes:
@@ -276,7 +277,9 @@ def test_shared_beam_is_counted_once_and_until_freed():
     s.submit(Request(rid=1, params=None, name='a'))
     s.step()
     assert len(s.active) == 2 and s.in_use() == 10 + 10 + 50
-    s.step()                                         # both retire
+    s.step()                                         # both retire; freed a boundary later
+    assert not s.active and s.used == 20
+    s.drain()
     assert s.used == 0
     # the beam stays resident (cached for later requests) and stays counted
     assert s.in_use() == 50 and s.peak_used == 70
@@ -334,3 +337,56 @@ def test_cancel_queued_and_running_requests():
     assert res[0].error == 'cancelled' and 0 < res[0].steps < 64
     assert res[1].error is None and res[1].output == 1
     assert s.used == 0
+
+
+class PoolBackend(BeamBackend):
+    """BeamBackend whose finished simulations are kept for the next request
+    on the same grid (``req.params`` names the grid here), as HipBackend's
+    SimPool does: a recycled admission adds no bytes."""
+
+    def __init__(self, *a, **kw):
+        super().__init__(*a, **kw)
+        self.kept = []                               # grids of kept simulations
+        self.recycled = 0
+
+    def footprint(self, req):
+        base = 0 if req.params in self.kept else self.model_size
+        return base + (0 if req.name in self.resident else self.beam_size)
+
+    def cached_bytes(self):
+        return super().cached_bytes() + self.model_size * len(self.kept)
+
+    def admit(self, req):
+        if req.params in self.kept:
+            self.kept.remove(req.params)
+            self.recycled += 1
+        return super().admit(req)
+
+    def release(self, m):
+        super().release(m)
+        self.kept.append(m.req.params)
+
+    def evict_cached(self, nbytes, keep):
+        freed = 0
+        for g in [g for g in self.kept if g != keep.params]:
+            if freed >= nbytes:
+                break
+            self.kept.remove(g)
+            freed += self.model_size
+        if freed < nbytes:
+            freed += super().evict_cached(nbytes - freed, keep)
+        return freed
+
+
+def test_kept_simulations_are_recycled_and_counted():
+    be = PoolBackend(30, 0, 8)
+    s = Server(be, hbm_budget=70, max_slots=8, chunk_steps=8, clock=Clock())
+    for k in range(4):                               # two grids, two requests each
+        s.submit(Request(rid=k, params=('g', k % 2), name='a'))
+    s.drain()
+    assert all(r.error is None for r in s.results) and len(s.results) == 4
+    assert be.recycled == 2 and s.peak_used <= 70
+    assert s.used == 0 and s.in_use() == 60          # both grids' simulations kept
+    s.submit(Request(rid=9, params=('g', 2), name='a'))  # a new grid: one kept one must go
+    s.drain()
+    assert s.in_use() == 60 and len(s.results) == 5

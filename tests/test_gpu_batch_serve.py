@@ -237,3 +237,48 @@ def test_sigma0_served_request_vs_fp64_oracle():
     say(pc.inc_beam_params.polarization, 'theta', pc.inc_beam_params.theta_angle,
         'sigma0 dB: device', _db(dev), 'fp64 oracle', _db(want))
     assert abs(_db(dev) - _db(want)) <= 0.05
+
+
+def test_recycled_simulation_equals_fresh():
+    """A simulation kept in the pool after one model and reused for the next
+    model on the same grid (another material, another polarisation and
+    angle, so another beam) gives the fresh simulation's phasors and sigma0
+    bit for bit; a host-volume model after a device-built one too."""
+    from sar_pixel_simulator_amd.materials import device
+    say('case: recycled simulations vs fresh')
+    native.init(0)
+    beams = B.BeamCache()
+    pool = B.SimPool()
+    pc0, m0 = _request_params(0)
+    pc1 = copy.deepcopy(pc0)
+    pc1.inc_beam_params.polarization = 'VV'
+    pc1.inc_beam_params.theta_angle = 40.0
+    g = pc1.geom_params
+    m1 = synthetic.rough_soil(g.size_x, g.size_y, g.size_z_material, g.cell_size,
+                              workloads.lam_of(pc1), g.size_z_material - 3, 777)
+    spec = device.canopy_over_soil_spec(pc0, g.size_z_material // 2, seed=31, density=20.0)
+    fresh = []
+    for pc, kind in ((pc0, 'spec'), (pc1, 'host')):
+        dm = (B.DeviceModel.from_spec(pc, spec, beams) if kind == 'spec'
+              else B.DeviceModel.from_material(pc, m1, beams))
+        B.run_plain([dm])
+        fresh.append((dm.sim.phasors(), dm.sigma0()))
+        dm.release()
+    first = B.DeviceModel.from_material(pc0, m0, beams, pool=pool)
+    B.run_plain([first])
+    first.release()                                  # kept
+    assert pool.has(pc0) and pool.bytes() > 0
+    for (want_ph, want_s), (pc, kind) in zip(fresh, ((pc0, 'spec'), (pc1, 'host'))):
+        dm = (B.DeviceModel.from_spec(pc, spec, beams, pool=pool) if kind == 'spec'
+              else B.DeviceModel.from_material(pc, m1, beams, pool=pool))
+        assert not pool.has(pc)                      # the kept one was taken
+        B.run_plain([dm])
+        ph = dm.sim.phasors()
+        for n in native.PHASOR_NAMES:
+            assert np.array_equal(ph[n], want_ph[n]), (kind, n)
+        got_s = dm.sigma0()
+        for key in ('sigma_h', 'sigma_v'):
+            assert np.array_equal(got_s[key], want_s[key]), (kind, key)
+        dm.release()
+    pool.release()
+    beams.release()

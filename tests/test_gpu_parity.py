@@ -458,9 +458,11 @@ def test_split_and_rows_agree_at_largest_size():
 
 @pytest.mark.parametrize('fused,launch', [(True, None), (False, None), (False, (2, 8, 4, 1, 1)),
                                           (False, (1, 1, 1, 4, 0)), (False, (1, 4, 1, 4, 2)),
-                                          (2, None), (False, 'mixed-widths')],
+                                          (2, None), (False, 'mixed-widths'), (False, 'serial'),
+                                          (False, 'hybrid'), (2, 'hybrid')],
                          ids=['fused', 'split-auto', 'split-4-per-lane', 'split-4-waves-flat',
-                              'split-banded', 'rows', 'split-auto-mixed-widths'])
+                              'split-banded', 'rows', 'split-auto-mixed-widths', 'split-serial',
+                              'split-hybrid', 'rows-hybrid'])
 def test_batch_equals_solo(fused, launch):
     cases = [H.soil_case(nx=33, ny=31, nz=35, nz_mat=14, steps=30, seed=s) for s in (1, 2)]
     cases.append(H.soil_case(nx=45, ny=39, nz=37, nz_mat=15, steps=30, seed=5))
@@ -469,6 +471,13 @@ def test_batch_equals_solo(fused, launch):
         # next to the 33- and 45-cell members' 1-wavefront blocks
         cases.append(H.soil_case(nx=201, ny=23, nz=25, nz_mat=9, steps=30, seed=6))
         cases.append(H.soil_case(nx=101, ny=23, nz=27, nz_mat=9, steps=30, seed=8))
+        launch = None
+    mode = None
+    if launch in ('serial', 'hybrid'):
+        # serial: every member's own launches in turn, on two concurrent
+        # lanes; hybrid: the two 33-cell members packed, the 45-cell one
+        # serial on the batch stream
+        mode = (native.BATCH_SERIAL, 0, 2) if launch == 'serial' else (native.BATCH_HYBRID, 40000, 1)
         launch = None
     say('case: batch of', len(cases), 'uneven sims, 30 steps, fused', fused, 'launch', launch)
     solo, batch_sims = [], []
@@ -487,6 +496,8 @@ def test_batch_equals_solo(fused, launch):
         b = native.Batch(batch_sims)
         if launch:
             b.set_launch(*launch)
+        if mode:
+            b.set_mode(*mode)
         b.run(17, use_graph=use_graph)
         b.run(13, use_graph=use_graph)
         b.sync()

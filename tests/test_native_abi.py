@@ -46,3 +46,14 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(native, '_lib', None)
     with pytest.raises(native.NativeLibraryError):
         native.lib()
+
+
+def test_python_constants_match_header():
+    src = open(os.path.join(ROOT, 'include', 'fdtd_hip.h')).read()
+
+    def define(name):
+        return int(re.search(rf'#define {name} (\d+)', src).group(1))
+    assert native.N2F_MAX_THETA == define('FDTD_N2F_MAX_THETA')
+    assert native.N2F_MAX_SIDE == define('FDTD_N2F_MAX_SIDE')
+    assert (native.BATCH_PACKED, native.BATCH_SERIAL, native.BATCH_HYBRID) == \
+        (define('FDTD_BATCH_PACKED'), define('FDTD_BATCH_SERIAL'), define('FDTD_BATCH_HYBRID'))

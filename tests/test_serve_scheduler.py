@@ -122,7 +122,10 @@ def test_late_arrival_joins_running_batch():
     s.step()
     s.step()
     s.submit(Request(rid=1, params=None))   # arrives while 0 is running
-    s.step()
+    s.step()                                # admitted while 0's next chunk runs
+    assert [m.req.rid for m in be.batches[-1].members] == [0]
+    assert [e[0].rid for e in s.active] == [0, 1]
+    s.step()                                # joins at the following boundary
     assert [m.req.rid for m in be.batches[-1].members] == [0, 1]
     s.drain()
     res = {r.rid: r for r in s.results}
