@@ -49,12 +49,65 @@ def uniform_top(Ca, Cb):
     return z
 
 
+#: Host cores a one-GPU job may use on the MI355X box (its CPU share for
+#: one GPU); the CPU baseline uses all of them, or fewer if fewer are visible.
+HOST_CORE_SHARE = 16
+
+
 def cpu_threads():
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+    return max(1, min(n, HOST_CORE_SHARE))
+
+
+def cfg0_end_to_end(log=print):
+    """The plain CPU-sized workload (81 x 81 x 81 cells, 480 steps, rough
+    soil, HH theta 20) end to end, one model: on the host through the oracle
+    (tapered-wave excitation, incident FDTD, main FDTD, top-face far field
+    and sigma0, float32 fields, OpenMP on the host cores) and on the device
+    (beam synthesis, main run, far field and sigma0)."""
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    sys.path.insert(0, os.path.join(ROOT, 'tests'))
+    import beam_oracle
+    import helpers as H
+    import oracle
+    from sar_pixel_simulator_amd import _native as native, workloads
+    from sar_pixel_simulator_amd.hip import batch as B
+    from sar_pixel_simulator_amd.materials import compute_material_arrays
+    oracle.build()
+    pc = workloads.plain_params()
+    m = workloads.plain_material(pc, seed=0)
+    g = pc.geom_params
+    nthreads = cpu_threads()
+    t0 = time.perf_counter()
+    pex, pey = beam_oracle.create_inc_excitation(pc)
+    rec, inc = H.oracle_incident(pc, pex, pey, nthreads=nthreads)
+    Ca, Cb = compute_material_arrays(pc, m)
+    sim = H.oracle_main(pc, Ca, Cb, rec, nthreads=nthreads)
+    n = g.size_pml
+    nxp, nyp, nzp = g.size_x - 2 * n, g.size_y - 2 * n, g.size_z - 2 * n
+    th = np.linspace(-80.0, 80.0, 161)
+    ff = oracle.far_field(sim.phasors(), nxp, nyp, nzp, g.cell_size, 2000.0, th, [0.0],
+                          pc.inc_beam_params.frequency)
+    oracle.sigma0(ff, oracle.incident_power(inc, g.cell_size), nxp, nyp, g.cell_size)
+    t_cpu = time.perf_counter() - t0
+    native.init(0)
+    t0 = time.perf_counter()
+    beams = B.BeamCache()
+    dm = B.DeviceModel.from_material(pc, m, beams)
+    B.run_plain([dm])
+    dm.sigma0()
+    dm.release()
+    beams.release()
+    t_gpu = time.perf_counter() - t0
+    out = {'workload': f'plain CPU-sized run {g.size_x}x{g.size_y}x{g.size_z}, {g.size_t} steps, '
+                       'one model end to end (beam, incident run, main run, far field, sigma0)',
+           'cpu_s': round(t_cpu, 3), 'cpu_cores': nthreads, 'gpu_s': round(t_gpu, 3),
+           'gpu_vs_cpu': round(t_cpu / t_gpu, 2)}
+    log(f'bench: cfg0 end to end {out}')
+    return out
 
 
 def cpu_baseline(pc, Ca, Cb, target_s=15.0):
@@ -83,7 +136,7 @@ def cpu_baseline(pc, Ca, Cb, target_s=15.0):
     el = time.perf_counter() - t0
     cells = g.size_x * g.size_y * g.size_z
     return {'value': cells * n / el / 1e6, 'unit': 'Mcell-updates/s', 'cores': nthreads,
-            'kind': 'port',
+            'host_cpus_visible': os.cpu_count(), 'kind': 'port',
             'sample': f'{n} time steps (after 1 warm-up) of the full {g.size_x}x{g.size_y}x'
                       f'{g.size_z} domain with injection active, {el:.1f} s'}
 
@@ -386,6 +439,8 @@ def main():
     if not args.no_cpu:
         print('bench: CPU baseline sample', file=sys.stderr, flush=True)
         cpu = cpu_baseline(pc, Ca, Cb)
+        cpu['plain_end_to_end'] = cfg0_end_to_end(
+            log=lambda m: print(m, file=sys.stderr, flush=True))
 
     extras = {}
     if not args.no_extras:

@@ -232,7 +232,7 @@ int fdtd_sim_near2far(FdtdSim *s, const double *thetas_deg, int ntheta,
  * one GPU (several host threads) race and must be serialised by the caller. */
 #define FDTD_N2F_MAX_THETA 181
 #define FDTD_N2F_MAX_SIDE 2047
-#define FDTD_N2F_SCRATCH_BYTES 12620032
+#define FDTD_N2F_SCRATCH_BYTES 12621568
 int fdtd_sim_near2far_noalloc(FdtdSim *s, const double *thetas_deg, int ntheta,
                               double phi_deg, double r, double freq, double *out);
 /* Same transform of caller-supplied complex64 top-face phasors, each
@@ -241,6 +241,19 @@ int fdtd_near2far_host(const float *ex_u, const float *ey_u, const float *hx_u,
                        const float *hy_u, int nx_ph, int ny_ph, int nz_ph,
                        double dx, const double *thetas_deg, int ntheta,
                        double phi_deg, double r, double freq, double *out);
+/* The transform over a theta x phi grid in one launch, for a phi array as
+ * FarFieldTransform.__call__ takes it (near2far.py:28-112, its phi loop):
+ * out = E_theta, E_phi, H_theta, H_phi, each (nphi, ntheta) complex128,
+ * i.e. 8*nphi*ntheta doubles. */
+int fdtd_sim_near2far_grid(FdtdSim *s, const double *thetas_deg, int ntheta,
+                           const double *phis_deg, int nphi, double r,
+                           double freq, double *out);
+int fdtd_near2far_host_grid(const float *ex_u, const float *ey_u,
+                            const float *hx_u, const float *hy_u, int nx_ph,
+                            int ny_ph, int nz_ph, double dx,
+                            const double *thetas_deg, int ntheta,
+                            const double *phis_deg, int nphi, double r,
+                            double freq, double *out);
 
 /* ---- incident beam (the (nt_inc, ny, nx) ex/ey/hx/hy datasets of the
  * incident HDF5 file, cpu/incident_sim.py:95-98), resident in HBM --------- */
@@ -291,15 +304,22 @@ int fdtd_batch_set_launch(FdtdBatch *b, int ty, int zchunk, int xvec, int xwaves
  * sequence into a HIP graph on first use and replays it. */
 int fdtd_batch_run(FdtdBatch *b, int nsteps, int use_graph);
 int fdtd_batch_sync(FdtdBatch *b);
+/* No counterpart in the reference.  1 while steps queued on the batch are
+ * still running, 0 once they are done (the batch's stream is idle); does not
+ * wait.  A serving loop uses it to do host work only while the device is
+ * busy. */
+int fdtd_batch_busy(FdtdBatch *b);
 int fdtd_batch_time_steps(FdtdBatch *b, int nsteps, int use_graph, float *ms_total);
 /* No counterpart in the reference.  How a batch steps its members:
  * FDTD_BATCH_PACKED (the default) puts every member in one tile list, one
  * launch per kernel per step for all of them; FDTD_BATCH_SERIAL runs each
  * member's own launches (its own tile shape), a chunk of steps per member,
  * one member after another on each of `lanes` concurrent streams (members
- * dealt largest first to the lane with the fewest cells); FDTD_BATCH_HYBRID
- * packs the members with fewer than small_cells cells and runs the others
- * as in FDTD_BATCH_SERIAL.  Launches fork from and join back into the
+ * dealt largest first to the lane with the fewest cells) -- with
+ * small_cells > 0 only the members below small_cells are dealt to lanes,
+ * and the others follow one after another once those are done;
+ * FDTD_BATCH_HYBRID packs the members with fewer than small_cells cells and
+ * runs the others as in FDTD_BATCH_SERIAL with small_cells 0.  Launches fork from and join back into the
  * batch's stream (one graph per chunk when captured); fdtd_batch_sync waits
  * for all of them.  Results are identical in every mode. */
 #define FDTD_BATCH_PACKED 0

@@ -49,10 +49,11 @@ EXPORTS = [
     'fdtd_sim_get_psi', 'fdtd_sim_set_stepper', 'fdtd_sim_set_launch',
     'fdtd_sim_time_steps',
     'fdtd_sim_near2far', 'fdtd_sim_near2far_noalloc', 'fdtd_near2far_host',
+    'fdtd_sim_near2far_grid', 'fdtd_near2far_host_grid',
     'fdtd_beam_create', 'fdtd_beam_upload', 'fdtd_beam_get', 'fdtd_beam_release',
     'fdtd_beam_synthesize',
     'fdtd_batch_create', 'fdtd_batch_destroy', 'fdtd_batch_set_launch',
-    'fdtd_batch_run', 'fdtd_batch_sync', 'fdtd_batch_time_steps', 'fdtd_batch_set_mode',
+    'fdtd_batch_run', 'fdtd_batch_sync', 'fdtd_batch_time_steps', 'fdtd_batch_set_mode', 'fdtd_batch_busy',
     'fdtd_probe_copy', 'fdtd_probe_streams', 'fdtd_probe_stencil',
 ]
 
@@ -172,12 +173,16 @@ _SIGS = {
     'fdtd_sim_near2far_noalloc': ([_P, _DP, _I, _D, _D, _D, _DP], _I),
     'fdtd_near2far_host': ([_FP, _FP, _FP, _FP, _I, _I, _I, _D, _DP, _I, _D,
                             _D, _D, _DP], _I),
+    'fdtd_sim_near2far_grid': ([_P, _DP, _I, _DP, _I, _D, _D, _DP], _I),
+    'fdtd_near2far_host_grid': ([_FP, _FP, _FP, _FP, _I, _I, _I, _D, _DP, _I, _DP, _I,
+                                 _D, _D, _DP], _I),
     'fdtd_beam_create': ([_I, _I, _I, _FP, _FP, _FP, _FP, ctypes.POINTER(_P)], _I),
     'fdtd_beam_upload': ([_P, _I, _I, _FP, _FP, _FP, _FP], _I),
     'fdtd_beam_get': ([_P, _I, _FP], _I),
     'fdtd_beam_release': ([_P], None),
     'fdtd_beam_synthesize': ([ctypes.POINTER(FdtdGeom), ctypes.POINTER(FdtdCoeffs),
                               ctypes.POINTER(FdtdBeamSpec), ctypes.POINTER(_P), _FP], _I),
+    'fdtd_batch_busy': ([_P], _I),
     'fdtd_batch_set_mode': ([_P, _I, ctypes.c_int64, _I], _I),
     'fdtd_batch_create': ([ctypes.POINTER(_P), _I, ctypes.POINTER(_P)], _I),
     'fdtd_batch_destroy': ([_P], None),
@@ -530,6 +535,17 @@ class Sim:
         return {'E_theta': out[0], 'E_phi': out[1], 'H_theta': out[2],
                 'H_phi': out[3]}
 
+    def near2far_grid(self, thetas_deg, phis_deg, r, freq):
+        """Far field over the theta x phi grid in one launch: each component
+        (nphi, ntheta) complex128."""
+        th = np.ascontiguousarray(thetas_deg, dtype=np.float64)
+        ph = np.ascontiguousarray(phis_deg, dtype=np.float64)
+        out = np.empty((4, ph.size, th.size), np.complex128)
+        _check(lib().fdtd_sim_near2far_grid(self._h, _dptr(th), th.size, _dptr(ph), ph.size,
+                                            float(r), float(freq),
+                                            out.view(np.float64).ctypes.data_as(_DP)))
+        return {'E_theta': out[0], 'E_phi': out[1], 'H_theta': out[2], 'H_phi': out[3]}
+
     def destroy(self):
         if self._h:
             lib().fdtd_sim_destroy(self._h)
@@ -564,6 +580,24 @@ BATCH_PACKED, BATCH_SERIAL, BATCH_HYBRID = 0, 1, 2
 N2F_MAX_THETA, N2F_MAX_SIDE = 181, 2047
 
 
+def near2far_host_grid(ex_u, ey_u, hx_u, hy_u, nx_ph, ny_ph, nz_ph, dx, thetas_deg,
+                       phis_deg, r, freq):
+    """Far field of caller-supplied top-face phasors over the theta x phi
+    grid in one launch: each component (nphi, ntheta) complex128."""
+    arrs = [np.ascontiguousarray(a, dtype=np.complex64) for a in (ex_u, ey_u, hx_u, hy_u)]
+    for a in arrs:
+        if a.shape != (ny_ph, nx_ph):
+            raise ValueError(f'phasor plane shape {a.shape} != {(ny_ph, nx_ph)}')
+    th = np.ascontiguousarray(thetas_deg, dtype=np.float64)
+    ph = np.ascontiguousarray(phis_deg, dtype=np.float64)
+    out = np.empty((4, ph.size, th.size), np.complex128)
+    _check(lib().fdtd_near2far_host_grid(
+        *[a.view(np.float32).ctypes.data_as(_FP) for a in arrs],
+        int(nx_ph), int(ny_ph), int(nz_ph), float(dx), _dptr(th), th.size, _dptr(ph), ph.size,
+        float(r), float(freq), out.view(np.float64).ctypes.data_as(_DP)))
+    return {'E_theta': out[0], 'E_phi': out[1], 'H_theta': out[2], 'H_phi': out[3]}
+
+
 class Batch:
     """Several simulations stepped by one launch per kernel per step."""
 
@@ -593,6 +627,13 @@ class Batch:
 
     def sync(self):
         _check(lib().fdtd_batch_sync(self._h))
+
+    def busy(self):
+        """True while queued steps are still running (does not wait)."""
+        rc = lib().fdtd_batch_busy(self._h)
+        if rc < 0:
+            _check(rc)
+        return rc == 1
 
     def time_steps(self, nsteps, use_graph=True):
         tot = ctypes.c_float()

@@ -27,7 +27,7 @@ __global__ void k_step_strip(const Desc *, const int4 *, int, int);
 template <int V, int R>
 __global__ void k_step_rows(const Desc *, const int4 *, int, int);
 __global__ void k_phasor_out(const float4 *, long long, float2 *);
-__global__ void k_n2f_tables(const double *, int, int, double, double, double, double, double, double2 *,
+__global__ void k_n2f_tables(const double *, const double *, int, int, double, double, double, double2 *,
                              double2 *, double2 *, double4 *);
 template <typename P>
 __global__ void k_n2f_partial(const P *, const P *, const P *, const P *, int, int, int, const double2 *,
@@ -1457,10 +1457,11 @@ extern "C" int fdtd_sim_time_steps(FdtdSim *s, int nsteps, float *ms_total, floa
 // near-to-far
 // ---------------------------------------------------------------------------
 // Device work area of one near-to-far call, carved from one buffer in
-// 256-byte aligned pieces: the theta list, the column and row phase tables,
-// the top-face factors, the per-(theta, split) partial sums and the output.
+// 256-byte aligned pieces: the direction lists (theta, phi), the column and
+// row phase tables, the top-face factors, the per-(direction, split)
+// partial sums and the output.
 struct N2fWork {
-    double *thetas;
+    double *thetas, *phis;
     double2 *phx, *phy, *zph;
     double4 *trig;
     double *part;
@@ -1471,25 +1472,26 @@ static constexpr size_t n2f_round(size_t b) { return (b + 255) & ~(size_t)255; }
 
 static int n2f_splits(int nyp) { return std::max(1, std::min(64, nyp / 8)); }
 
-static size_t n2f_carve(char *base, int ntheta, int nxp, int nyp, N2fWork *w) {
-    const size_t T = (size_t)ntheta;
-    const size_t sizes[7] = {T * sizeof(double), T * nxp * sizeof(double2), T * nyp * sizeof(double2),
-                             T * sizeof(double2), T * sizeof(double4),
+static size_t n2f_carve(char *base, int ndir, int nxp, int nyp, N2fWork *w) {
+    const size_t T = (size_t)ndir;
+    const size_t sizes[8] = {T * sizeof(double), T * sizeof(double), T * nxp * sizeof(double2),
+                             T * nyp * sizeof(double2), T * sizeof(double2), T * sizeof(double4),
                              T * n2f_splits(nyp) * 8 * sizeof(double), 4 * T * sizeof(double2)};
-    void *ptrs[7];
+    void *ptrs[8];
     size_t off = 0;
-    for (int i = 0; i < 7; ++i) {
+    for (int i = 0; i < 8; ++i) {
         ptrs[i] = base ? base + off : nullptr;
         off += n2f_round(sizes[i]);
     }
     if (w) {
         w->thetas = (double *)ptrs[0];
-        w->phx = (double2 *)ptrs[1];
-        w->phy = (double2 *)ptrs[2];
-        w->zph = (double2 *)ptrs[3];
-        w->trig = (double4 *)ptrs[4];
-        w->part = (double *)ptrs[5];
-        w->out = (double2 *)ptrs[6];
+        w->phis = (double *)ptrs[1];
+        w->phx = (double2 *)ptrs[2];
+        w->phy = (double2 *)ptrs[3];
+        w->zph = (double2 *)ptrs[4];
+        w->trig = (double4 *)ptrs[5];
+        w->part = (double *)ptrs[6];
+        w->out = (double2 *)ptrs[7];
     }
     return off;
 }
@@ -1500,7 +1502,7 @@ static size_t n2f_carve(char *base, int ntheta, int nxp, int nyp, N2fWork *w) {
 // Host code reaches it only through the address hipGetSymbolAddress returns
 // for the current device (n2f_scratch_ptr), never through its name.
 static constexpr size_t kN2fScratchBytes =
-    n2f_round(FDTD_N2F_MAX_THETA * sizeof(double)) +
+    2 * n2f_round(FDTD_N2F_MAX_THETA * sizeof(double)) +
     2 * n2f_round((size_t)FDTD_N2F_MAX_THETA * FDTD_N2F_MAX_SIDE * sizeof(double2)) +
     n2f_round(FDTD_N2F_MAX_THETA * sizeof(double2)) + n2f_round(FDTD_N2F_MAX_THETA * sizeof(double4)) +
     n2f_round((size_t)FDTD_N2F_MAX_THETA * 64 * 8 * sizeof(double)) +
@@ -1533,21 +1535,21 @@ static int n2f_scratch_ptr(int device, void **out) {
     return FDTD_OK;
 }
 
-// One near-to-far transform on stream st.  work: a device area of at least
-// n2f_carve(nullptr, ...) bytes, or NULL to allocate one for the call.
-// Every piece of the area is written before it is read, so nothing needs
-// clearing between calls.
+// Near-to-far transform on stream st for ndir directions (thetas_deg[i],
+// phis_deg[i]).  work: a device area of at least n2f_carve(nullptr, ...)
+// bytes, or NULL to allocate one for the call.  Every piece of the area is
+// written before it is read, so nothing needs clearing between calls.
+// out: 4 x ndir complex128 (E_theta, E_phi, H_theta, H_phi).
 template <typename P>
 static int near2far_run(int device, hipStream_t st, const P *ex, const P *ey, const P *hx, const P *hy,
-                        int nxp, int nyp, int nzp, double dx, const double *thetas_deg, int ntheta,
-                        double phi_deg, double r, double freq, double *out, void *work) {
+                        int nxp, int nyp, int nzp, double dx, const double *thetas_deg,
+                        const double *phis_deg, int ndir, double r, double freq, double *out, void *work) {
     CHECK(nxp >= 3 && nyp >= 3, "phasor plane %d x %d too small", nyp, nxp);
     CHECK(nxp % 2 == 1 && nyp % 2 == 1, "composite Simpson needs odd sample counts (got %d x %d)", nyp, nxp);
-    CHECK(ntheta > 0 && thetas_deg && out, "bad theta arguments");
+    CHECK(ndir > 0 && thetas_deg && phis_deg && out, "bad angle arguments");
     const double pi = 3.141592653589793;
     const double eta0 = 376.730313668, c0 = 299792458.0;
     const double lam = c0 / freq, k = 2.0 * pi / lam;
-    const double phi = phi_deg * (pi / 180.0);
     const double z_top = (double)((nzp - 1) / 2) * dx;   // z_loc[-1] (near2far.py:73, :97)
     // ff_weight = 1j * k * exp(-1j * k * r) / (4 pi r)   (near2far.py:76-77)
     const double kr = k * r;
@@ -1556,26 +1558,27 @@ static int near2far_run(int device, hipStream_t st, const P *ex, const P *ey, co
     HIP_TRY(hipSetDevice(device));
     char *own = nullptr;
     if (!work) {
-        HIP_TRY(hipMalloc(&own, n2f_carve(nullptr, ntheta, nxp, nyp, nullptr)));
+        HIP_TRY(hipMalloc(&own, n2f_carve(nullptr, ndir, nxp, nyp, nullptr)));
         work = own;
     }
     N2fWork w;
-    n2f_carve((char *)work, ntheta, nxp, nyp, &w);
+    n2f_carve((char *)work, ndir, nxp, nyp, &w);
     int rc = FDTD_OK;
-    hipError_t e = hipMemcpyAsync(w.thetas, thetas_deg, (size_t)ntheta * sizeof(double),
+    hipError_t e = hipMemcpyAsync(w.thetas, thetas_deg, (size_t)ndir * sizeof(double),
                                   hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(w.phis, phis_deg, (size_t)ndir * sizeof(double), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_n2f_tables, dim3(ntheta, (std::max(nxp, nyp) + 255) / 256), dim3(256), 0, st,
-                           w.thetas, nxp, nyp, dx, k, std::cos(phi), std::sin(phi), z_top, w.phx, w.phy,
-                           w.zph, w.trig);
-        hipLaunchKernelGGL(k_n2f_partial<P>, dim3(ntheta, nsplit), dim3(256), 0, st, ex, ey, hx, hy, nxp, nyp,
+        hipLaunchKernelGGL(k_n2f_tables, dim3(ndir, (std::max(nxp, nyp) + 255) / 256), dim3(256), 0, st,
+                           w.thetas, w.phis, nxp, nyp, dx, k, z_top, w.phx, w.phy, w.zph, w.trig);
+        hipLaunchKernelGGL(k_n2f_partial<P>, dim3(ndir, nsplit), dim3(256), 0, st, ex, ey, hx, hy, nxp, nyp,
                            nsplit, w.phx, w.phy, w.part);
-        hipLaunchKernelGGL(k_n2f_final, dim3((ntheta + 63) / 64), dim3(64), 0, st, w.part, ntheta, nsplit,
+        hipLaunchKernelGGL(k_n2f_final, dim3((ndir + 63) / 64), dim3(64), 0, st, w.part, ndir, nsplit,
                            w.trig, w.zph, wgt, eta0, w.out);
         e = hipGetLastError();
     }
     if (e == hipSuccess)
-        e = hipMemcpyAsync(out, w.out, (size_t)4 * ntheta * sizeof(double2), hipMemcpyDeviceToHost, st);
+        e = hipMemcpyAsync(out, w.out, (size_t)4 * ndir * sizeof(double2), hipMemcpyDeviceToHost, st);
     const hipError_t es = hipStreamSynchronize(st);
     if (e == hipSuccess) e = es;
     if (e != hipSuccess) rc = fail(FDTD_EHIP, "near-to-far on device %d failed: %s", device, hipGetErrorString(e));
@@ -1583,14 +1586,27 @@ static int near2far_run(int device, hipStream_t st, const P *ex, const P *ey, co
     return rc;
 }
 
-static int near2far_of_sim(FdtdSim *s, const double *thetas_deg, int ntheta, double phi_deg, double r,
+// Direction lists of a theta x phi grid, phi-major (the reference's
+// (nphi, ntheta) output order).
+static void n2f_grid(const double *thetas, int ntheta, const double *phis, int nphi, std::vector<double> &th,
+                     std::vector<double> &ph) {
+    th.resize((size_t)ntheta * nphi);
+    ph.resize(th.size());
+    for (int p = 0; p < nphi; ++p)
+        for (int t = 0; t < ntheta; ++t) {
+            th[(size_t)p * ntheta + t] = thetas[t];
+            ph[(size_t)p * ntheta + t] = phis[p];
+        }
+}
+
+static int near2far_of_sim(FdtdSim *s, const double *thetas_deg, const double *phis_deg, int ndir, double r,
                            double freq, double *out, void *work) {
     HIP_TRY(hipSetDevice(s->device));
     const float4 *u = s->acc;   // ex_ph_u ey_ph_u hx_ph_u hy_ph_u = phasors 4..7
     const int np = s->g.npml;
     return near2far_run<float4>(s->device, s->self.stream, u + s->h.acc_off[4], u + s->h.acc_off[5],
                                 u + s->h.acc_off[6], u + s->h.acc_off[7], s->g.nx - 2 * np, s->g.ny - 2 * np,
-                                s->g.nz - 2 * np, s->c.cell_size, thetas_deg, ntheta, phi_deg, r, freq, out,
+                                s->g.nz - 2 * np, s->c.cell_size, thetas_deg, phis_deg, ndir, r, freq, out,
                                 work);
 }
 
@@ -1598,7 +1614,21 @@ extern "C" int fdtd_sim_near2far(FdtdSim *s, const double *thetas_deg, int nthet
                                  double freq, double *out) {
     CHECK(s, "NULL sim");
     CHECK(s->g.mode == FDTD_MODE_MAIN, "near-to-far needs a MAIN simulation");
-    return near2far_of_sim(s, thetas_deg, ntheta, phi_deg, r, freq, out, nullptr);
+    CHECK(ntheta > 0 && thetas_deg, "bad theta arguments");
+    std::vector<double> th, ph;
+    n2f_grid(thetas_deg, ntheta, &phi_deg, 1, th, ph);
+    return near2far_of_sim(s, th.data(), ph.data(), ntheta, r, freq, out, nullptr);
+}
+
+extern "C" int fdtd_sim_near2far_grid(FdtdSim *s, const double *thetas_deg, int ntheta, const double *phis_deg,
+                                      int nphi, double r, double freq, double *out) {
+    CHECK(s, "NULL sim");
+    CHECK(s->g.mode == FDTD_MODE_MAIN, "near-to-far needs a MAIN simulation");
+    CHECK(ntheta > 0 && nphi > 0 && thetas_deg && phis_deg, "bad angle arguments");
+    CHECK((long long)ntheta * nphi < (1LL << 31), "too many directions");
+    std::vector<double> th, ph;
+    n2f_grid(thetas_deg, ntheta, phis_deg, nphi, th, ph);
+    return near2far_of_sim(s, th.data(), ph.data(), (int)th.size(), r, freq, out, nullptr);
 }
 
 extern "C" int fdtd_sim_near2far_noalloc(FdtdSim *s, const double *thetas_deg, int ntheta, double phi_deg,
@@ -1607,6 +1637,7 @@ extern "C" int fdtd_sim_near2far_noalloc(FdtdSim *s, const double *thetas_deg, i
     CHECK(s->g.mode == FDTD_MODE_MAIN, "near-to-far needs a MAIN simulation");
     CHECK(ntheta >= 1 && ntheta <= FDTD_N2F_MAX_THETA, "ntheta %d outside [1, %d] (preallocated work area)",
           ntheta, FDTD_N2F_MAX_THETA);
+    CHECK(thetas_deg, "bad theta arguments");
     const int nxp = s->g.nx - 2 * s->g.npml, nyp = s->g.ny - 2 * s->g.npml;
     CHECK(nxp <= FDTD_N2F_MAX_SIDE && nyp <= FDTD_N2F_MAX_SIDE,
           "phasor plane %d x %d exceeds %d samples per side (preallocated work area)", nyp, nxp,
@@ -1614,12 +1645,18 @@ extern "C" int fdtd_sim_near2far_noalloc(FdtdSim *s, const double *thetas_deg, i
     void *work = nullptr;
     int rc = n2f_scratch_ptr(s->device, &work);
     if (rc) return rc;
-    return near2far_of_sim(s, thetas_deg, ntheta, phi_deg, r, freq, out, work);
+    // the direction lists live on the stack: no allocation
+    double th[FDTD_N2F_MAX_THETA], ph[FDTD_N2F_MAX_THETA];
+    for (int t = 0; t < ntheta; ++t) {
+        th[t] = thetas_deg[t];
+        ph[t] = phi_deg;
+    }
+    return near2far_of_sim(s, th, ph, ntheta, r, freq, out, work);
 }
 
-extern "C" int fdtd_near2far_host(const float *ex_u, const float *ey_u, const float *hx_u, const float *hy_u,
-                                  int nx_ph, int ny_ph, int nz_ph, double dx, const double *thetas_deg,
-                                  int ntheta, double phi_deg, double r, double freq, double *out) {
+static int near2far_host_planes(const float *ex_u, const float *ey_u, const float *hx_u, const float *hy_u,
+                                int nx_ph, int ny_ph, int nz_ph, double dx, const double *th, const double *ph,
+                                int ndir, double r, double freq, double *out) {
     CHECK(ex_u && ey_u && hx_u && hy_u, "NULL phasor");
     CHECK(nx_ph > 0 && ny_ph > 0, "bad plane size");
     HIP_TRY(hipSetDevice(g_device));
@@ -1631,12 +1668,34 @@ extern "C" int fdtd_near2far_host(const float *ex_u, const float *ey_u, const fl
     const float *src[4] = {ex_u, ey_u, hx_u, hy_u};
     for (int c = 0; c < 4; ++c)
         HIP_TRY(hipMemcpyAsync(d + c * n, src[c], n * sizeof(float2), hipMemcpyHostToDevice, st));
-    int rc = near2far_run<float2>(g_device, st, d, d + n, d + 2 * n, d + 3 * n, nx_ph, ny_ph, nz_ph, dx,
-                                  thetas_deg, ntheta, phi_deg, r, freq, out, nullptr);
+    int rc = near2far_run<float2>(g_device, st, d, d + n, d + 2 * n, d + 3 * n, nx_ph, ny_ph, nz_ph, dx, th, ph,
+                                  ndir, r, freq, out, nullptr);
     (void)hipStreamSynchronize(st);
     (void)hipStreamDestroy(st);
     (void)hipFree(d);
     return rc;
+}
+
+extern "C" int fdtd_near2far_host(const float *ex_u, const float *ey_u, const float *hx_u, const float *hy_u,
+                                  int nx_ph, int ny_ph, int nz_ph, double dx, const double *thetas_deg,
+                                  int ntheta, double phi_deg, double r, double freq, double *out) {
+    CHECK(ntheta > 0 && thetas_deg, "bad theta arguments");
+    std::vector<double> th, ph;
+    n2f_grid(thetas_deg, ntheta, &phi_deg, 1, th, ph);
+    return near2far_host_planes(ex_u, ey_u, hx_u, hy_u, nx_ph, ny_ph, nz_ph, dx, th.data(), ph.data(), ntheta,
+                                r, freq, out);
+}
+
+extern "C" int fdtd_near2far_host_grid(const float *ex_u, const float *ey_u, const float *hx_u,
+                                       const float *hy_u, int nx_ph, int ny_ph, int nz_ph, double dx,
+                                       const double *thetas_deg, int ntheta, const double *phis_deg, int nphi,
+                                       double r, double freq, double *out) {
+    CHECK(ntheta > 0 && nphi > 0 && thetas_deg && phis_deg, "bad angle arguments");
+    CHECK((long long)ntheta * nphi < (1LL << 31), "too many directions");
+    std::vector<double> th, ph;
+    n2f_grid(thetas_deg, ntheta, phis_deg, nphi, th, ph);
+    return near2far_host_planes(ex_u, ey_u, hx_u, hy_u, nx_ph, ny_ph, nz_ph, dx, th.data(), ph.data(),
+                                (int)th.size(), r, freq, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -1759,9 +1818,12 @@ static void batch_clear_graphs(FdtdBatch *b) {
 static long long sim_cells(const FdtdSim *s) { return (long long)s->g.nx * s->g.ny * s->g.nz; }
 
 // Split the members between the packed group and the serial list for the
-// batch's mode, deal the serial members to the batch's lanes (largest
-// first, each to the lane with the fewest cells so far), and rebuild the
-// packed group's tile list.
+// batch's mode and deal the serial members to lanes, then rebuild the
+// packed group's tile list.  Lanes: in SERIAL mode with small_cells > 0 the
+// members below small_cells are dealt over the lanes (largest first, each
+// to the lane with the fewest cells so far) and step first, concurrently;
+// the others then step one after another on the batch stream (lane -1).
+// Otherwise every serial member is dealt over the lanes.
 static int batch_build(FdtdBatch *b) {
     batch_clear_graphs(b);
     std::vector<FdtdSim *> packed;
@@ -1773,9 +1835,14 @@ static int batch_build(FdtdBatch *b) {
     }
     std::stable_sort(b->serial.begin(), b->serial.end(),
                      [](const FdtdSim *a, const FdtdSim *c) { return sim_cells(a) > sim_cells(c); });
+    const bool split = b->mode == FDTD_BATCH_SERIAL && b->small_cells > 0;
     std::vector<long long> load(b->nlanes, 0);
     b->lane.assign(b->serial.size(), 0);
     for (size_t k = 0; k < b->serial.size(); ++k) {
+        if (split && sim_cells(b->serial[k]) >= b->small_cells) {
+            b->lane[k] = -1;
+            continue;
+        }
         const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
         b->lane[k] = l;
         load[l] += sim_cells(b->serial[k]);
@@ -1797,31 +1864,37 @@ static int batch_build(FdtdBatch *b) {
     return group_build(b->grp);
 }
 
-// The launches of len steps: the packed group step by step, then each
-// serial member's len steps with its own tile shape on its lane's stream.
-// Lanes other than the batch stream start after an event recorded on it
-// and end with an event it waits for, so a chunk is one fork-join (and one
-// graph when captured).  Members are independent, so neither the order nor
-// the concurrency changes results; a member's steps back to back keep its
-// working set in the caches.
+// The launches of len steps: the packed group step by step, then the
+// members dealt to lanes, each member's len steps with its own tile shape
+// on its lane's stream, then the members of lane -1 one after another on
+// the batch stream.  Lanes other than the batch stream start after an event
+// recorded on it and end with an event it waits for, so a chunk is one
+// fork-join (and one graph when captured).  Members are independent, so
+// neither the order nor the concurrency changes results; a member's steps
+// back to back keep its working set in the caches.
 static int batch_launch(FdtdBatch *b, int len) {
     hipStream_t st0 = b->grp.stream;
-    const bool lanes = b->nlanes > 1 && !b->serial.empty();
-    if (lanes) {
+    bool dealt = false;
+    for (int l : b->lane) dealt = dealt || l > 0;
+    if (dealt) {
         HIP_TRY(hipEventRecord(b->fork[0], st0));
         for (int l = 1; l < b->nlanes; ++l) HIP_TRY(hipStreamWaitEvent(b->streams[l], b->fork[0], 0));
     }
     if (!b->grp.members.empty())
         for (int s = 0; s < len; ++s) launch_step(b->grp);
     for (size_t k = 0; k < b->serial.size(); ++k) {
+        if (b->lane[k] < 0) continue;
         hipStream_t st = b->lane[k] == 0 ? st0 : b->streams[b->lane[k]];
         for (int s = 0; s < len; ++s) launch_step(b->serial[k]->self, st);
     }
-    if (lanes)
+    if (dealt)
         for (int l = 1; l < b->nlanes; ++l) {
             HIP_TRY(hipEventRecord(b->fork[l], b->streams[l]));
             HIP_TRY(hipStreamWaitEvent(st0, b->fork[l], 0));
         }
+    for (size_t k = 0; k < b->serial.size(); ++k)
+        if (b->lane[k] < 0)
+            for (int s = 0; s < len; ++s) launch_step(b->serial[k]->self, st0);
     return FDTD_OK;
 }
 
@@ -1964,6 +2037,15 @@ extern "C" int fdtd_batch_set_mode(FdtdBatch *b, int mode, int64_t small_cells, 
 extern "C" int fdtd_batch_run(FdtdBatch *b, int nsteps, int use_graph) {
     CHECK(b, "NULL batch");
     return batch_run(b, nsteps, use_graph != 0);
+}
+
+extern "C" int fdtd_batch_busy(FdtdBatch *b) {
+    CHECK(b, "NULL batch");
+    HIP_TRY(hipSetDevice(b->grp.device));
+    const hipError_t e = hipStreamQuery(b->grp.stream);
+    if (e == hipSuccess) return 0;
+    if (e == hipErrorNotReady) return 1;
+    return fail(FDTD_EHIP, "hipStreamQuery: %s", hipGetErrorString(e));
 }
 
 extern "C" int fdtd_batch_sync(FdtdBatch *b) {

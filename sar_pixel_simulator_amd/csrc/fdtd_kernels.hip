@@ -1627,24 +1627,27 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Phase tables of one call: per theta, the Simpson-weighted column factors
-// w_i exp(jk x_i sin th cos ph) and row factors w_j exp(jk y_j sin th sin ph)
-// with x_i = (i - (nxp-1)/2) dx (near2far.py:71-73, :126-131), the top-face
-// factor exp(jk z_top cos th) and (cos th, sin th, cos ph, sin ph).
-// Grid (ntheta, ceil(max(nxp, nyp) / 256)).
+// Phase tables of one call: per direction (theta, phi), the Simpson-weighted
+// column factors w_i exp(jk x_i sin th cos ph) and row factors
+// w_j exp(jk y_j sin th sin ph) with x_i = (i - (nxp-1)/2) dx
+// (near2far.py:71-73, :126-131), the top-face factor exp(jk z_top cos th)
+// and (cos th, sin th, cos ph, sin ph).  Grid (ndir, ceil(max(nxp, nyp) /
+// 256)).
 __device__ __forceinline__ double simpson_w(int n, int i) {
     return (i == 0 || i == n - 1) ? 1.0 : ((i & 1) ? 4.0 : 2.0);
 }
 
 __global__ void __launch_bounds__(256)
-k_n2f_tables(const double *__restrict__ thetas_deg, int nxp, int nyp, double dx, double k,
-             double cp, double sp, double z_top, double2 *__restrict__ phx,
-             double2 *__restrict__ phy, double2 *__restrict__ zph, double4 *__restrict__ trig) {
+k_n2f_tables(const double *__restrict__ thetas_deg, const double *__restrict__ phis_deg, int nxp, int nyp,
+             double dx, double k, double z_top, double2 *__restrict__ phx, double2 *__restrict__ phy,
+             double2 *__restrict__ zph, double4 *__restrict__ trig) {
     const int t = blockIdx.x;
     const int i = blockIdx.y * blockDim.x + threadIdx.x;
     const double th = thetas_deg[t] * (3.141592653589793 / 180.0);
-    double st, ct;
+    const double ph = phis_deg[t] * (3.141592653589793 / 180.0);
+    double st, ct, sp, cp;
     sincos(th, &st, &ct);
+    sincos(ph, &sp, &cp);
     if (i < nxp) {
         const double xv = (double)(i - (nxp - 1) / 2) * dx, w = simpson_w(nxp, i) * dx / 3.0;
         double s, c;

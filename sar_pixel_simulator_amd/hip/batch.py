@@ -285,11 +285,15 @@ SMALL_LANES = 4
 
 def auto_mode(models):
     """(mode, small_cells, lanes) for a batch of ``models``: every member
-    with its own launches, a chunk of steps per member; four concurrent
-    lanes when most members are small, else one member after another."""
-    cells = sorted(m.cells for m in models)
-    lanes = SMALL_LANES if cells[len(cells) // 2] < SMALL_CELLS else 1
-    return native.BATCH_SERIAL, 0, min(lanes, len(models))
+    with its own launches, a chunk of steps per member; the small members
+    on four concurrent lanes, then the large ones one after another."""
+    small = sum(m.cells < SMALL_CELLS for m in models)
+    if small == 0:
+        return native.BATCH_SERIAL, 0, 1
+    lanes = min(SMALL_LANES, small)
+    if small == len(models):
+        return native.BATCH_SERIAL, 0, lanes
+    return native.BATCH_SERIAL, SMALL_CELLS, lanes
 
 
 class DeviceBatch:
@@ -326,6 +330,9 @@ class DeviceBatch:
 
     def sync(self):
         self.batch.sync()
+
+    def busy(self):
+        return self.batch.busy()
 
     def destroy(self):
         self.batch.destroy()

@@ -338,35 +338,34 @@ def serving(n=200, rate_factor=8.0, budget_gb=None, max_slots=256, chunk=96, dev
     return st
 
 
-def batch_step_rate(count=8, steps=30, launch=None, device=0):
+def batch_step_rate(count=32, steps=30, mode='auto', device=0):
     """Steady-state step rate of the batched workload as one batch: all
-    members resident, ``steps`` graph-replayed steps after a short warm-up.
-    Returns ms per step, Gcell-updates/s and the algorithmic rate (the
-    split kernels' and the DFT's compulsory bytes per step over the step)."""
-    from .roofline import algorithmic_bytes
+    members resident, ``steps`` steps after a short warm-up, in batch mode
+    ``mode`` ('auto', 'packed' or a fdtd_batch_set_mode tuple).  Returns ms
+    per step, Gcell-updates/s and the algorithmic rate (the split kernels'
+    and the DFT's compulsory bytes per step over the step)."""
     native.init(device)
     cases = batched_workload(count)
     beams = B.BeamCache()
     models = _models_for(cases, beams)
-    alg, cells = 0, 0
-    for pc, _, _, _ in cases:
-        g = pc.geom_params
-        a = algorithmic_bytes(g.size_x, g.size_y, g.size_z, g.size_pml, None)
-        alg += a['h'] + a['e'] + a['dft']
-        cells += g.size_x * g.size_y * g.size_z
-    b = native.Batch([m.sim for m in models])
-    if launch:
-        b.set_launch(*launch)
-    b.run(5, False)
-    b.sync()
-    ms = b.time_steps(steps, True) / steps
-    b.destroy()
+    alg = _batch_alg_bytes(cases)
+    cells = sum(pc.geom_params.size_x * pc.geom_params.size_y * pc.geom_params.size_z
+                for pc, _, _, _ in cases)
+    db = B.DeviceBatch(models, mode=mode)
+    db.run(5)
+    db.sync()
+    t0 = time.perf_counter()
+    db.run(steps)
+    db.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    used = db.mode
+    db.destroy()
     for m in models:
         m.release()
     beams.release()
-    return {'models': len(cases), 'cells': cells, 'steps': steps, 'ms_per_step': round(ms, 4),
-            'Gcell_updates_per_s': round(cells / ms / 1e6, 1),
-            'alg_TBs': round(alg / ms / 1e9, 2)}
+    return {'models': len(cases), 'cells': cells, 'steps': steps, 'mode': list(used),
+            'ms_per_step': round(ms, 4), 'Gcell_updates_per_s': round(cells / ms / 1e6, 1),
+            'alg_TBs': round(alg / ms / 1e9, 3)}
 
 
 if __name__ == '__main__':
@@ -379,5 +378,9 @@ if __name__ == '__main__':
     if 'serving' in what:
         out['serving'] = serving()
     if 'batch_step' in what:
-        out['batch_step'] = batch_step_rate()
+        # batch_step [count] [packed]
+        rest = what[what.index('batch_step') + 1:]
+        count = int(rest[0]) if rest and rest[0].isdigit() else 32
+        mode = 'packed' if 'packed' in rest else 'auto'
+        out['batch_step'] = batch_step_rate(count=count, mode=mode)
     print(json.dumps(out))

@@ -188,6 +188,9 @@ class HipBackend:
     def sync(self, batch):
         batch.sync()
 
+    def busy(self, batch):
+        return batch.busy()
+
     def drop_batch(self, batch):
         batch.destroy()
 
@@ -312,9 +315,21 @@ class Server:
             self.used -= fp
         self.to_free = []
 
-    def _admit(self):
+    def _device_idle(self):
+        """True when the chunk in flight has finished: host work should stop
+        so that the next chunk starts (backends without a query: never)."""
+        busy = getattr(self.backend, 'busy', None)
+        return bool(self.inflight and busy is not None and not busy(self.batch))
+
+    def _admit(self, first_only=False):
+        """Admit queued requests in arrival order while slots and memory
+        allow; stop early once the device has run out of work (the next
+        chunk should start first), or after one admission when nothing runs
+        yet (``first_only``)."""
         changed = False
         while self.queue and len(self.active) < self.max_slots:
+            if changed and (first_only or self._device_idle()):
+                break
             req = self.queue[0]
             fp, new = self._need(req)
             over = self.in_use() + new - self.budget
@@ -355,6 +370,8 @@ class Server:
             return
         free = max(1, self.max_slots - len(self.active))
         for req in list(self.queue)[:free]:
+            if self._device_idle():
+                break
             try:
                 prepare(req)
             except Exception:               # admit() repeats the work and reports it
@@ -376,8 +393,8 @@ class Server:
         self.joined = False
         t2 = time.perf_counter()
         if not self.active:
-            # nothing to run: admit first, then start them
-            changed = self._admit() or changed
+            # nothing to run: admit one, start it, admit more while it runs
+            changed = self._admit(first_only=True) or changed
         t3 = time.perf_counter()
         if changed and self.batch is not None:
             self.backend.drop_batch(self.batch)

@@ -1,9 +1,10 @@
 """Near-to-far transform (Taflove ch. 8) over the top face of the phasor box.
 
 Same call signature and outputs as fdtdsarsim/near2far.py:11-112; the
-surface integrals run in FP64 on the device (libfdtd_hip
-``fdtd_near2far_host``): J = (-Hy_u, Hx_u), M = (Ey_u, -Ex_u), composite
-Simpson weights in x and y, r, theta/phi in degrees.
+surface integrals run in FP64 on the device over the whole theta x phi grid
+in one launch (libfdtd_hip ``fdtd_near2far_host_grid``): J = (-Hy_u, Hx_u),
+M = (Ey_u, -Ex_u), composite Simpson weights in x and y, r, theta/phi in
+degrees.
 """
 from typing import Mapping
 
@@ -23,10 +24,6 @@ class FarFieldTransform:
         if thetas.ndim != 1 or phis.ndim != 1:
             raise RuntimeError('Theta and phi arguments must be 1D arrays')
         planes = [np.asarray(phasors[n]) for n in ('ex_ph_u', 'ey_ph_u', 'hx_ph_u', 'hy_ph_u')]
-        out = {k: np.zeros((phis.size, thetas.size), np.complex128)
-               for k in ('E_theta', 'E_phi', 'H_theta', 'H_phi')}
-        for i, phi in enumerate(phis):
-            ff = native.near2far_host(*planes, nx, ny, nz, dx, thetas, phi, r, freq)
-            for k in out:
-                out[k][i] = ff[k]
+        # the whole theta x phi grid in one launch
+        out = native.near2far_host_grid(*planes, nx, ny, nz, dx, thetas, phis, r, freq)
         return {k: np.squeeze(v) for k, v in out.items()}

@@ -275,10 +275,11 @@ def test_shared_beam_is_counted_once_and_until_freed():
     s = Server(be, hbm_budget=200, chunk_steps=8, clock=Clock())
     s.submit(Request(rid=0, params=None, name='a'))
     s.submit(Request(rid=1, params=None, name='a'))
-    s.step()
+    s.step()                                         # 0 starts; 1 is admitted while it runs
     assert len(s.active) == 2 and s.in_use() == 10 + 10 + 50
-    s.step()                                         # both retire; freed a boundary later
-    assert not s.active and s.used == 20
+    s.step()                                         # 0 retires, 1 runs
+    s.step()                                         # 1 retires; 0 was freed
+    assert not s.active and s.used == 10
     s.drain()
     assert s.used == 0
     # the beam stays resident (cached for later requests) and stays counted

@@ -459,10 +459,10 @@ def test_split_and_rows_agree_at_largest_size():
 @pytest.mark.parametrize('fused,launch', [(True, None), (False, None), (False, (2, 8, 4, 1, 1)),
                                           (False, (1, 1, 1, 4, 0)), (False, (1, 4, 1, 4, 2)),
                                           (2, None), (False, 'mixed-widths'), (False, 'serial'),
-                                          (False, 'hybrid'), (2, 'hybrid')],
+                                          (False, 'serial-split'), (False, 'hybrid'), (2, 'hybrid')],
                          ids=['fused', 'split-auto', 'split-4-per-lane', 'split-4-waves-flat',
                               'split-banded', 'rows', 'split-auto-mixed-widths', 'split-serial',
-                              'split-hybrid', 'rows-hybrid'])
+                              'split-serial-small-lanes', 'split-hybrid', 'rows-hybrid'])
 def test_batch_equals_solo(fused, launch):
     cases = [H.soil_case(nx=33, ny=31, nz=35, nz_mat=14, steps=30, seed=s) for s in (1, 2)]
     cases.append(H.soil_case(nx=45, ny=39, nz=37, nz_mat=15, steps=30, seed=5))
@@ -473,11 +473,13 @@ def test_batch_equals_solo(fused, launch):
         cases.append(H.soil_case(nx=101, ny=23, nz=27, nz_mat=9, steps=30, seed=8))
         launch = None
     mode = None
-    if launch in ('serial', 'hybrid'):
+    if launch in ('serial', 'serial-split', 'hybrid'):
         # serial: every member's own launches in turn, on two concurrent
-        # lanes; hybrid: the two 33-cell members packed, the 45-cell one
-        # serial on the batch stream
-        mode = (native.BATCH_SERIAL, 0, 2) if launch == 'serial' else (native.BATCH_HYBRID, 40000, 1)
+        # lanes; serial-split: the two 33-cell members on two lanes, then
+        # the 45-cell one; hybrid: the two 33-cell members packed, the
+        # 45-cell one serial on the batch stream
+        mode = {'serial': (native.BATCH_SERIAL, 0, 2), 'serial-split': (native.BATCH_SERIAL, 40000, 2),
+                'hybrid': (native.BATCH_HYBRID, 40000, 1)}[launch]
         launch = None
     say('case: batch of', len(cases), 'uneven sims, 30 steps, fused', fused, 'launch', launch)
     solo, batch_sims = [], []
@@ -676,3 +678,75 @@ def test_bandwidth_probes_run_and_reject_bad_arguments():
         native.probe_stencil(64, 8, 8, 0, 0)
     with pytest.raises(native.FdtdError):
         native.probe_streams(1 << 16, 5, 3)
+
+
+def test_near2far_grid_matches_oracle(soil60):
+    """The far field over a theta x phi grid in one launch, from host
+    phasors and from the device accumulators, against the oracle's FP64
+    scipy-Simpson transform (relative 1e-9); each phi row equals the
+    single-phi transform bit for bit."""
+    pc, Ca, Cb, beam, ref = soil60
+    g = pc.geom_params
+    oph = ref.phasors()
+    nxp, nyp, nzp = g.size_x - 20, g.size_y - 20, g.size_z - 20
+    th = np.linspace(-80.0, 80.0, 17)
+    phis = np.arange(0.0, 360.0, 30.0)
+    f = pc.inc_beam_params.frequency
+    want = oracle.far_field(oph, nxp, nyp, nzp, g.cell_size, 2000.0, th, phis, f)
+    native.init(0)
+    got = native.near2far_host_grid(oph['ex_ph_u'], oph['ey_ph_u'], oph['hx_ph_u'], oph['hy_ph_u'],
+                                    nxp, nyp, nzp, g.cell_size, th, phis, 2000.0, f)
+    sim = make_sim(pc, Ca, Cb, beam)
+    sim.run(60)
+    sim.sync()
+    got2 = sim.near2far_grid(th, phis, 2000.0, f)
+    for k in ('E_theta', 'E_phi', 'H_theta', 'H_phi'):
+        scale = np.abs(want[k]).max()
+        assert got[k].shape == (phis.size, th.size)
+        assert np.abs(got[k] - want[k]).max() <= 1e-9 * scale, k
+        assert np.abs(got2[k] - want[k]).max() <= 1e-9 * scale, k
+    for p, phi in enumerate(phis):
+        one = sim.near2far(th, phi, 2000.0, f)
+        for k in one:
+            assert np.array_equal(one[k], got2[k][p]), (phi, k)
+    sim.destroy()
+
+
+def test_pec_box_powers_and_hemisphere_on_device():
+    """Device pipeline (device beam synthesis, split stepper, compensated
+    DFT) on a flat PEC plate under a normally incident 2-lambda beam: the
+    incident power returns through the top half of the phasor box, none
+    reaches the bottom face, and the far field of the top face, computed
+    over the upper hemisphere in one launch, carries the power crossing it."""
+    from sar_pixel_simulator_amd import processresults, workloads
+    from sar_pixel_simulator_amd.hip import batch as B
+    from sar_pixel_simulator_amd.materials import compute_material_arrays, flat_surface
+    pc = workloads.base_params(theta=0.0, illumination_width=2.0, incident_duration=4.0,
+                               simulation_duration=11.0)
+    workloads.explicit_geometry(pc, 101, 101, 41, 14, 0.05)
+    g = pc.geom_params
+    say('case: PEC box 101 x 101 x 41 on the device')
+    native.init(0)
+    beams = B.BeamCache()
+    Ca, Cb = compute_material_arrays(pc, next(flat_surface(pc, 1.0, 1.0e10)))
+    dm = B.DeviceModel(pc, Ca, Cb, beams)
+    B.run_plain([dm])
+    ph = dict(dm.sim.phasors())
+    ph.update(dm.inc_phasors)
+    pw = processresults.near_field_power(ph, g.cell_size, g.size_z_material)
+    dth, dphi, r = 2.0, 10.0, 2000.0
+    th = np.arange(dth / 2, 90.0, dth)
+    phis = np.arange(0.0, 360.0, dphi)
+    ff = dm.sim.near2far_grid(th, phis, r, pc.inc_beam_params.frequency)
+    p_h, p_v = processresults.farfield_power(ff['E_theta'], ff['E_phi'], ff['H_theta'], ff['H_phi'])
+    w = np.sin(np.deg2rad(th))[None, :] * np.deg2rad(dth) * np.deg2rad(dphi) * r * r
+    hemi = float(((p_h + p_v) * w).sum())
+    p_inc = dm.p_inc
+    say('powers / P_inc: top', pw['top_total'] / p_inc, 'bottom', pw['bottom_total'] / p_inc,
+        'hemisphere', hemi / p_inc)
+    dm.release()
+    beams.release()
+    assert pw['top_total'] == pytest.approx(p_inc, rel=0.02)
+    assert abs(pw['bottom_total']) < 0.01 * p_inc
+    assert hemi == pytest.approx(p_inc, rel=0.02)
+    assert hemi == pytest.approx(pw['up'], rel=0.01)

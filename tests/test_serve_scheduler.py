@@ -230,3 +230,34 @@ def test_hip_backend_prepare_computes_host_coefficients():
     spec = Request(rid=2, params=pc, spec=object())
     HipBackend.prepare(None, spec)
     assert not spec.prepared and spec.Ca is None
+
+
+class BusyBackend(FakeBackend):
+    """FakeBackend whose device reports the chunk as finished (busy False)
+    or still running (busy True)."""
+
+    def __init__(self, *a, busy=True, **kw):
+        super().__init__(*a, **kw)
+        self.is_busy = busy
+        self.queries = 0
+
+    def busy(self, batch):
+        self.queries += 1
+        return self.is_busy
+
+
+@pytest.mark.parametrize('busy', [True, False])
+def test_host_work_only_while_the_device_is_busy(busy):
+    # nothing runs yet: one request is admitted and started first; the rest
+    # are admitted while its chunk runs -- all of them while the device
+    # reports work, one per boundary once it reports none
+    be = BusyBackend({i: 1 for i in range(5)}, {i: 64 for i in range(5)}, busy=busy)
+    s = Server(be, hbm_budget=100, chunk_steps=8, clock=Clock())
+    for r in _reqs(5):
+        s.submit(r)
+    s.step()
+    assert [m.req.rid for m in be.batches[0].members] == [0]
+    assert len(s.active) == (5 if busy else 2)
+    s.drain()
+    assert sorted(r.rid for r in s.results) == list(range(5))
+    assert all(r.error is None and r.output['steps_run'] == 64 for r in s.results)
