@@ -354,7 +354,16 @@ def run_plain(models, sync=True):
 
 
 def run_batched(models, chunk=64, use_graph=None, launch=None, mode='auto'):
-    """All models together; steps until the longest is done."""
+    """All models together; steps until the longest is done.  ``mode``
+    'tuned' takes the execution the tuner cached for this batch's grids,
+    measuring the candidates first when none is cached ('tune': always);
+    otherwise as DeviceBatch."""
+    if mode in ('tune', 'tuned'):
+        from . import tuning
+        hit = tuning.batch_mode_for(models) if mode == 'tuned' else None
+        if hit is None:
+            hit, _ = tuning.autotune_batch(models)
+        mode, launch = hit
     b = DeviceBatch(models, launch, mode)
     nmax = max(m.nt for m in models)
     done = 0

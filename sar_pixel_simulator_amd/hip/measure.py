@@ -79,13 +79,21 @@ def batched_vs_plain(count=32, chunk=96, device=0, log=print, step_probe=True):
     t_plain = time.perf_counter() - t0
     log(f'batched_vs_plain: plain {t_plain:.2f} s')
 
-    # batched: all resident, stepped together
+    # batched: all resident, stepped together in the execution the tuner
+    # picks for this batch (its measurement, a one-time cost per batch
+    # composition kept in the tuning cache, is reported apart)
+    from . import tuning
     t0 = time.perf_counter()
     models = _models_for(cases, beams)
-    mode = B.auto_mode(models)
-    B.run_batched(models, chunk=chunk)
+    t1 = time.perf_counter()
+    (mode, launch), tuned = tuning.autotune_batch(models, save=False)
+    t_tune = time.perf_counter() - t1
+    B.run_batched(models, chunk=chunk, launch=launch, mode=mode)
     batch_back = [m.sigma0()['sigma_h'] for m in models]
-    t_batch = time.perf_counter() - t0
+    t_batch = time.perf_counter() - t0 - t_tune
+    label = min(tuned, key=tuned.get)
+    log(f'batched_vs_plain: tuner picked {label} in {t_tune:.2f} s: '
+        f'{ {k: round(v, 3) for k, v in tuned.items()} }')
     same = all(np.array_equal(a, b) for a, b in zip(plain_back, batch_back))
     log(f'batched_vs_plain: batched {t_batch:.2f} s, identical results: {same}')
     alg = _batch_alg_bytes(cases)
@@ -128,7 +136,8 @@ def batched_vs_plain(count=32, chunk=96, device=0, log=print, step_probe=True):
            'plain_models_per_hour': round(n / t_plain * 3600, 1),
            'batched_models_per_hour': round(n / t_batch * 3600, 1),
            'speedup': round(t_plain / t_batch, 3),
-           'batch_mode': {'mode': ['packed', 'serial', 'hybrid'][mode[0]], 'lanes': mode[2]},
+           'batch_execution': label, 'batch_tune_s': round(t_tune, 2),
+           'batch_tune_ms_per_step': {k: round(v, 4) for k, v in tuned.items()},
            'batched_Mcell_updates_per_s': round(cu / t_batch / 1e6, 1),
            'plain_Mcell_updates_per_s': round(cu / t_plain / 1e6, 1),
            'step_alg_bytes': int(alg),

@@ -89,6 +89,69 @@ def autotune(sim, nx, ny, nz, fused=True, arch='gfx950', steps=10, candidates=No
     return best, results
 
 
+#: Execution candidates for a batch: (label, DeviceBatch mode argument).
+#: 'auto' is the rule of hip.batch.auto_mode; the others fix the mode.
+BATCH_CANDIDATES = [
+    ('packed-z2', ('packed', (0, 2, 0, 0, 0))),
+    ('packed-z4', ('packed', (0, 4, 0, 0, 0))),
+    ('serial-1', ((1, 0, 1), None)),
+    ('serial-2', ((1, 0, 2), None)),
+    ('serial-4', ((1, 0, 4), None)),
+    ('serial-4-small-first', ((1, 8_000_000, 4), None)),
+]
+
+
+def batch_key(models, arch='gfx950'):
+    """Cache key of a batch: its members' grids, sorted."""
+    sizes = sorted(tuple(m.sim.shape) for m in models)
+    return f'batch:{arch}:' + ';'.join('x'.join(map(str, sz[::-1])) for sz in sizes)
+
+
+def batch_mode_for(models, arch='gfx950'):
+    """Cached execution of this batch (a (mode, launch) pair for
+    DeviceBatch), else None."""
+    hit = _load().get(batch_key(models, arch))
+    if not hit:
+        return None
+    launch = tuple(hit['launch']) if hit.get('launch') else None
+    mode = hit['mode'] if isinstance(hit['mode'], str) else tuple(hit['mode'])
+    return mode, launch
+
+
+def autotune_batch(models, steps=8, arch='gfx950', candidates=None, save=True):
+    """Time each execution candidate on the batch of ``models`` (ready
+    DeviceModels, before their run: their state is reset afterwards, so the
+    results do not depend on the tuning) and keep the fastest.  Returns
+    ((mode, launch), {label: ms_per_step})."""
+    import time
+    from .batch import DeviceBatch
+    results, choice = {}, {}
+    for label, (mode, launch) in candidates or BATCH_CANDIDATES:
+        db = DeviceBatch(models, launch=launch, mode=mode)
+        db.run(2)
+        db.sync()
+        t0 = time.perf_counter()
+        db.run(steps)
+        db.sync()
+        results[label] = (time.perf_counter() - t0) * 1e3 / steps
+        choice[label] = (mode, launch)
+        db.destroy()
+    for m in models:
+        m.sim.reset()
+        m.steps_done = 0
+    best = min(results, key=results.get)
+    if save:
+        mode, launch = choice[best]
+        data = _load()
+        data[batch_key(models, arch)] = {'label': best, 'mode': mode, 'launch': launch,
+                                         'ms_per_step': results[best]}
+        tmp = CACHE + '.tmp'
+        with open(tmp, 'w') as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        os.replace(tmp, CACHE)
+    return choice[best], results
+
+
 def _tune_workload(name, fused):
     """Build a ready simulation of a named workload with uniform-value inputs
     (timing does not depend on the values) and autotune it."""

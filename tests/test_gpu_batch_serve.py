@@ -282,3 +282,27 @@ def test_recycled_simulation_equals_fresh():
         dm.release()
     pool.release()
     beams.release()
+
+
+def test_tuned_batch_equals_plain():
+    """The batch autotuner steps the members in each candidate execution,
+    resets them and runs the fastest: results equal each model run alone."""
+    say('case: tuned batch of 4 vs plain')
+    native.init(0)
+    beams = B.BeamCache()
+    cases = [_request_params(k) for k in range(4)]
+    plain = []
+    for pc, m in cases:
+        dm = B.DeviceModel.from_material(pc, m, beams)
+        B.run_plain([dm])
+        plain.append(dm.sim.phasors())
+        dm.release()
+    models = [B.DeviceModel.from_material(pc, m, beams) for pc, m in cases]
+    B.run_batched(models, chunk=24, mode='tune')
+    for k, dm in enumerate(models):
+        assert dm.sim.step_count() == dm.nt
+        ph = dm.sim.phasors()
+        for n in native.PHASOR_NAMES:
+            assert np.array_equal(ph[n], plain[k][n]), (k, n)
+        dm.release()
+    beams.release()
