@@ -35,6 +35,7 @@ METRIC = 'FDTD Mcell-updates/s + material-models/hr batched vs plain; p50/p99 la
 HBM_PEAK_GBS = 8000.0
 PROBE_BYTES = 1 << 30          # calibration copy: 1 GiB read + 1 GiB written
 LAUNCH_KEYS = ('ty', 'zchunk', 'xvec', 'xwaves', 'xcd_order')
+STEPPER_IDS = {'split': 0, 'fused': 1, 'rows': 2, 'halo': 3}
 PMC_COUNTERS = ('FETCH_SIZE', 'WRITE_SIZE')
 
 
@@ -241,7 +242,7 @@ def largest_leg(fused, steps=10, warmup=3, log=print):
     wall = time.perf_counter() - t0
     k = sim.time_steps(steps, per_kernel=True)
     cells = g.size_x * g.size_y * g.size_z
-    ab = algorithmic_bytes(g.size_x, g.size_y, g.size_z, g.size_pml, uni_z, taps=not fused)
+    ab = algorithmic_bytes(g.size_x, g.size_y, g.size_z, g.size_pml, uni_z, taps=fused in (0, 3))
     step_bytes = (ab['h'] + ab['e'] - 24 * cells if fused else ab['h'] + ab['e']) + ab['dft']
     ms = wall * 1e3 / steps
     out = {'workload': f'largest single domain: {g.size_x}x{g.size_y}x{g.size_z} cells (32x32x4 '
@@ -252,7 +253,11 @@ def largest_leg(fused, steps=10, warmup=3, log=print):
            'step_alg_GBs': round(step_bytes / (ms * 1e-3) / 1e9, 1),
            'kernels_ms': {kk: round(v, 4) for kk, v in k.items()},
            'kernels_alg_GBs': ({kk: round(ab[kk] / (k[kk] * 1e-3) / 1e9, 1) for kk in ('h', 'e', 'dft')}
-                               if not fused and min(k['h'], k['e'], k['dft']) > 0 else None),
+                               if not fused and min(k['h'], k['e'], k['dft']) > 0 else
+                               {'fused': round((ab['h'] + ab['e'] - 24 * cells) / (k['h'] * 1e-3) / 1e9, 1),
+                                'dft': round(ab['dft'] / (k['dft'] * 1e-3) / 1e9, 1)}
+                               if fused and min(k['h'], k['dft']) > 0 else None),
+           'stepper': tuning.STEPPERS[int(fused)],
            'launch': dict(zip(LAUNCH_KEYS, shape)),
            'device_bytes': int(native.footprint(geom) + 16 * g.size_t_inc * g.size_y * g.size_x)}
     sim.destroy()
@@ -268,7 +273,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-extras', action='store_true',
                     help='skip the batched-vs-plain and serving legs')
-    ap.add_argument('--stepper', choices=['split', 'fused', 'rows'], default=None,
+    ap.add_argument('--stepper', choices=['split', 'fused', 'rows', 'halo'], default=None,
                     help='time-step kernels (default: the library default)')
     ap.add_argument('--ty', type=int, default=0)
     ap.add_argument('--zchunk', type=int, default=-1)
@@ -306,8 +311,9 @@ def main():
 
     if args.largest_only:
         native.init(0)
-        fused = (int(common.FUSED_STEPPER) if args.stepper is None
-                 else {'split': 0, 'fused': 1, 'rows': 2}[args.stepper])
+        lg = workloads.largest_params().geom_params
+        fused = (common.stepper_for(lg.size_x, lg.size_y, lg.size_z) if args.stepper is None
+                 else STEPPER_IDS[args.stepper])
         print(json.dumps(largest_leg(fused, steps=args.steps, warmup=args.warmup,
                                      log=lambda m: print(m, file=sys.stderr, flush=True))),
               flush=True)
@@ -323,10 +329,10 @@ def main():
 
     native.init(0)
     info = native.device_info()
-    fused = (int(common.FUSED_STEPPER) if args.stepper is None
-             else {'split': 0, 'fused': 1, 'rows': 2}[args.stepper])
+    fused = (common.stepper_for(g.size_x, g.size_y, g.size_z) if args.stepper is None
+             else STEPPER_IDS[args.stepper])
     stepper_name = tuning.STEPPERS[fused]
-    fused_kernel = 'k_step_rows' if fused == 2 else 'k_step_strip'
+    fused_kernel = {1: 'k_step_strip', 2: 'k_step_rows', 3: 'k_step_halo'}.get(fused)
     sim = native.Sim(common.main_geom(pc), common.coeffs_for(pc))
     sim.set_stepper(fused)
     ty, zc, xv, xw, xo = tuning.launch_for(g.size_x, g.size_y, g.size_z, fused=fused)
@@ -374,7 +380,7 @@ def main():
     # per-kernel HIP-event times on the library's stream
     k = sim.time_steps(min(args.steps, 50), per_kernel=True)
     uni_z = uniform_top(Ca, Cb)
-    ab = algorithmic_bytes(g.size_x, g.size_y, g.size_z, g.size_pml, uni_z, taps=not fused)
+    ab = algorithmic_bytes(g.size_x, g.size_y, g.size_z, g.size_pml, uni_z, taps=fused in (0, 3))
 
     def gbs(nbytes, ms):
         return round(nbytes / (ms * 1e-3) / 1e9, 1) if ms > 0 else None
@@ -406,14 +412,15 @@ def main():
     # flat: the practical ceiling for that mix
     mix = (8, 6) if fused else ((8, 3) if kern['name'] == 'k_e_half' else (6, 3))
     mix_ms, mix_gbs = native.probe_streams(cells, *mix, reps=5)
-    # the split kernels' own access pattern (tile shape, z march, neighbour
-    # rows) without their arithmetic: the practical ceiling of that pattern
+    # the kernel's own access pattern (z march, neighbour rows; for the halo
+    # stepper the row below re-read and both buffer sets) without its
+    # arithmetic: the practical ceiling of that pattern
     stencil = None
-    if not fused:
-        kind = 1 if kern['name'] == 'k_e_half' else 0
+    if fused in (0, 3):
+        kind = 2 if fused == 3 else 1 if kern['name'] == 'k_e_half' else 0
         st_ms, st_gbs = native.probe_stencil(g.size_x, g.size_y, g.size_z, zc if zc > 0 else g.size_z,
                                              kind, reps=5)
-        stencil = {'half': 'E' if kind else 'H', 'zchunk': zc, 'GBs': round(st_gbs, 1)}
+        stencil = {'half': ('H', 'E', 'fused step')[kind], 'zchunk': zc, 'GBs': round(st_gbs, 1)}
     traffic, tdetail = None, {}
     if not args.no_traffic:
         print('bench: PMC traffic passes (FETCH_SIZE, WRITE_SIZE)', file=sys.stderr, flush=True)
@@ -458,7 +465,8 @@ def main():
         print('bench: material setup, host vs device', file=sys.stderr, flush=True)
         extras['material_setup'] = measure.material_setup(log=log)
         print('bench: largest single domain', file=sys.stderr, flush=True)
-        extras['largest'] = largest_leg(fused, log=log)
+        lg = workloads.largest_params().geom_params
+        extras['largest'] = largest_leg(common.stepper_for(lg.size_x, lg.size_y, lg.size_z), log=log)
 
     out = {'metric': METRIC, 'value': round(value, 1), 'unit': 'Mcell-updates/s',
            'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,

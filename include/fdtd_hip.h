@@ -342,13 +342,15 @@ int fdtd_probe_copy(int64_t nbytes, int vec, int reps, float *ms_per_copy);
  * reuse or arithmetic -- 6+3 (H half), 8+3 (E half), 8+6 (fused step), and
  * 1+1.  Reports the average HIP-event time of one pass. */
 int fdtd_probe_streams(int64_t n, int nread, int nwrite, int reps, float *ms_per_pass);
-/* No counterpart in the reference.  The split kernels' access pattern
+/* No counterpart in the reference.  The stencil kernels' access pattern
  * without their arithmetic over an nx x ny x nz grid laid out like a
  * simulation's fields: blocks of 256 lanes along x, tiles in plain order, a
  * z march over zchunk planes carrying the plane-to-plane values in
- * registers, and the y +- 1 row loads.  kind 0: the H half (9 algorithmic
- * streams per cell), 1: the E half (11).  Reports the average HIP-event
- * time of one pass: the practical ceiling of that pattern. */
+ * registers, and the y +- 1 row loads.  kind 0: the split H half (9
+ * algorithmic streams per cell), 1: the split E half (11), 2: the halo
+ * stepper's fused step (14; the row below's E and H re-read, both buffer
+ * sets).  Reports the average HIP-event time of one pass: the practical
+ * ceiling of that pattern. */
 int fdtd_probe_stencil(int nx, int ny, int nz, int zchunk, int kind, int reps, float *ms_per_pass);
 
 #ifdef __cplusplus

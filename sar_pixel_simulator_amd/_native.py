@@ -274,12 +274,13 @@ def probe_streams(n, nread, nwrite, reps=5):
 
 def probe_stencil(nx, ny, nz, zchunk, kind, reps=5):
     """Average time (ms) of one pass of the stencil-pattern probe (kind 0:
-    H half, 9 streams per cell; 1: E half, 11); returns (ms, GB/s over the
-    algorithmic streams)."""
+    split H half, 9 streams per cell; 1: split E half, 11; 2: the halo
+    stepper's fused step, 14); returns (ms, GB/s over the algorithmic
+    streams)."""
     ms = ctypes.c_float()
     _check(lib().fdtd_probe_stencil(int(nx), int(ny), int(nz), int(zchunk), int(kind), int(reps),
                                     ctypes.byref(ms)))
-    streams = 9 if kind == 0 else 11
+    streams = {0: 9, 1: 11, 2: 14}[int(kind)]
     return ms.value, streams * 4 * nx * ny * nz / (ms.value * 1e-3) / 1e9
 
 
@@ -505,8 +506,10 @@ class Sim:
     def set_stepper(self, fused):
         """0 / False: split H and E kernels; 1 / True: fused H+E kernel per
         step on row strips; 2: fused H+E kernel with one row per
-        wavefront."""
+        wavefront; 3: fused H+E kernel with one row per wavefront that
+        recomputes its neighbours' H (the split stepper's tiles)."""
         _check(lib().fdtd_sim_set_stepper(self._h, int(fused)))
+        self.stepper = int(fused)
 
     def time_steps(self, nsteps, per_kernel=True):
         """Run ``nsteps`` and return HIP-event times (ms) measured on the

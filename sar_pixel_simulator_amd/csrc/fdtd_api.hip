@@ -26,6 +26,8 @@ template <int V, int R>
 __global__ void k_step_strip(const Desc *, const int4 *, int, int);
 template <int V, int R>
 __global__ void k_step_rows(const Desc *, const int4 *, int, int);
+template <int V, bool DIRECT>
+__global__ void k_step_halo(const Desc *, const int4 *, int, int, TileGrid, const int *);
 __global__ void k_phasor_out(const float4 *, long long, float2 *);
 __global__ void k_n2f_tables(const double *, const double *, int, int, double, double, double, double2 *,
                              double2 *, double2 *, double4 *);
@@ -115,10 +117,17 @@ static void beam_unref(FdtdBeam *b) {
 // Launch shape a simulation or batch starts with, per stepper.  Split:
 // one x cell per lane, blocks of four wavefronts as wide along x as the
 // grid fills, 2-plane chunks, plain block order (measured fastest at
-// 761 x 761 x 203).  Fused: 4-row strips, 8-plane chunks.
+// 761 x 761 x 203).  Halo: one x cell per lane, blocks of 2 x 2
+// wavefronts (two rows of 128 cells), 8-plane chunks (fastest there).
+// Fused: 4-row strips, 8-plane chunks.
 static Launch default_launch(int fused) {
     Launch l;
-    if (fused == 2) {
+    if (fused == 3) {
+        l.ty = 2;
+        l.zchunk = 8;
+        l.xvec = 1;
+        l.xw = 2;
+    } else if (fused == 2) {
         l.ty = 7;
         l.zchunk = 8;
         l.xvec = 1;
@@ -398,7 +407,7 @@ static int group_build(Group &G) {
         CHECK(G.launch.xcd != 2, "row-parallel stepper: block order must be 0 (plain) or 1 (XCD-contiguous)");
         xw = 1;
         G.xvec = xv ? xv : 1;
-    } else if (G.fused) {
+    } else if (G.fused == 1) {
         CHECK(ty == 2 || ty == 4 || ty == 8, "fused stepper: ty (rows per wavefront) must be 2, 4 or 8 (got %d)",
               ty);
         CHECK(xw <= 1, "fused stepper: one wavefront per strip (xwaves must be 1)");
@@ -406,9 +415,16 @@ static int group_build(Group &G) {
         xw = 1;
         G.xvec = xv ? xv : auto_xvec(G.members, 1);
     } else {
-        CHECK(ty == 0 || ty == 1 || ty == 2 || ty == 4, "split stepper: ty must be 0, 1, 2 or 4 (got %d)", ty);
+        // the halo stepper's blocks may hold up to 8 wavefronts
+        const int max_waves = G.fused == 3 ? 8 : 4;
+        CHECK(ty == 0 || ty == 1 || ty == 2 || ty == 4 || (ty == 8 && max_waves == 8),
+              "split stepper: ty must be 0, 1, 2 or 4 (halo stepper: also 8) (got %d)", ty);
         // automatic: one cell per lane unless the block width is fixed
         G.xvec = xv ? xv : xw ? auto_xvec(G.members, xw) : 1;
+        if (G.fused == 3) {   // halo stepper: the split shapes, 1 or 2 cells per lane
+            CHECK(xv == 0 || xv == 1 || xv == 2, "halo stepper: xvec must be 1 or 2 (got %d)", xv);
+            G.xvec = std::min(G.xvec, 2);
+        }
         // a batch with a fully automatic shape gives each member its own
         // block width (4-wavefront blocks, 4 / xw rows); its tiles carry it
         per_member = !xw && !ty && !xv && G.members.size() > 1;
@@ -418,7 +434,8 @@ static int group_build(Group &G) {
         }
         if (!xw) xw = auto_xwaves(G.members, G.xvec);
         if (!ty) ty = std::max(1, 4 / xw);
-        CHECK(xw * ty <= 4, "split stepper: xwaves x ty = %d exceeds 4 wavefronts per block", xw * ty);
+        CHECK(xw * ty <= max_waves, "split stepper: xwaves x ty = %d exceeds %d wavefronts per block", xw * ty,
+              max_waves);
     }
     G.ty = ty;
     G.xw = xw;
@@ -440,7 +457,7 @@ static int group_build(Group &G) {
         FdtdSim *s = G.members[m];
         // split tiles carry log2(xwaves) above the member index
         const int xw_m = per_member ? auto_xwaves(std::vector<FdtdSim *>{s}, G.xvec) : xw;
-        const int lxw_m = G.fused ? 0 : xw_m == 4 ? 2 : xw_m == 2 ? 1 : 0;
+        const int lxw_m = G.fused == 1 || G.fused == 2 ? 0 : xw_m == 4 ? 2 : xw_m == 2 ? 1 : 0;
         const int tx_m = kWave * G.xvec * xw_m, ty_m = per_member ? 4 / xw_m : tys;
         const int tag = (int)m | (lxw_m << 16);
         CHECK(s->g.mode == G.mode, "a group cannot mix MAIN and INCIDENT simulations");
@@ -502,7 +519,7 @@ static int group_build(Group &G) {
     G.nmodels = (int)descs.size();
     G.tg = TileGrid{};
     G.step1 = nullptr;
-    if (G.members.size() == 1 && !G.fused && G.launch.xcd == 0) {
+    if (G.members.size() == 1 && (G.fused == 0 || G.fused == 3) && G.launch.xcd == 0) {
         const FdtdSim *m = G.members[0];
         const int zc = G.launch.zchunk > 0 ? G.launch.zchunk : m->g.nz;
         G.tg = TileGrid{(m->g.nx + tx - 1) / tx, (m->g.ny + tys - 1) / tys, tx, tys, zc, m->g.nz, 0, 0};
@@ -567,7 +584,14 @@ static void launch_kernel(Group &G, KernelId k, int t0 = 0, int t1 = -1, hipStre
         tg.t0 = t0 / per;
         sgrid = dim3((unsigned)tg.ntx, (unsigned)tg.nyb, (unsigned)(nt / per));
     }
-    if (k == KF && G.fused == 2) {
+    if (k == KF && G.fused == 3) {
+        if (G.xvec == 2)
+            launch_split(k_step_halo<2, true>, k_step_halo<2, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt, shape,
+                         tg);
+        else
+            launch_split(k_step_halo<1, true>, k_step_halo<1, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt, shape,
+                         tg);
+    } else if (k == KF && G.fused == 2) {
         const int r = G.ty, v = G.xvec;
         auto *fn = v == 2 ? (r == 7 ? &k_step_rows<2, 7> : &k_step_rows<2, 3>)
                           : (r == 7 ? &k_step_rows<1, 7> : &k_step_rows<1, 3>);
@@ -790,7 +814,7 @@ extern "C" int fdtd_probe_streams(int64_t n, int nread, int nwrite, int reps, fl
 }
 
 extern "C" int fdtd_probe_stencil(int nx, int ny, int nz, int zchunk, int kind, int reps, float *ms_per_pass) {
-    CHECK(kind == 0 || kind == 1, "kind must be 0 (H half) or 1 (E half)");
+    CHECK(kind >= 0 && kind <= 2, "kind must be 0 (H half), 1 (E half) or 2 (halo stepper)");
     CHECK(nx >= 1 && ny >= 1 && nz >= 1 && zchunk >= 1, "bad sizes");
     CHECK(reps >= 1 && ms_per_pass, "bad reps / output");
     const long long sy = pitch_of(nx), sz = sy * ny, ncell = sz * nz;
@@ -808,7 +832,7 @@ extern "C" int fdtd_probe_stencil(int nx, int ny, int nz, int zchunk, int kind, 
         if (e != hipSuccess && rc == FDTD_OK) rc = fail(FDTD_EHIP, "%s: %s", what, hipGetErrorString(e));
         return rc == FDTD_OK;
     };
-    const size_t bytes = (size_t)8 * stride * sizeof(float);
+    const size_t bytes = (size_t)(kind == 2 ? 14 : 8) * stride * sizeof(float);
     if (hip_ok(hipMalloc(&base, bytes), "hipMalloc") && hip_ok(hipStreamCreate(&st), "hipStreamCreate") &&
         hip_ok(hipEventCreate(&ev[0]), "hipEventCreate") && hip_ok(hipEventCreate(&ev[1]), "hipEventCreate") &&
         hip_ok(hipMemsetAsync(base, 0, bytes, st), "hipMemsetAsync")) {
@@ -816,6 +840,9 @@ extern "C" int fdtd_probe_stencil(int nx, int ny, int nz, int zchunk, int kind, 
             if (r == 1) hip_ok(hipEventRecord(ev[0], st), "hipEventRecord");
             if (kind == 0)
                 hipLaunchKernelGGL(k_stencil_probe<0>, dim3((unsigned)blocks), dim3(256), 0, st, base, stride, nx,
+                                   ny, nz, (uint32_t)sy, (uint32_t)sz, zchunk, ntx);
+            else if (kind == 2)
+                hipLaunchKernelGGL(k_stencil_probe<2>, dim3((unsigned)blocks), dim3(256), 0, st, base, stride, nx,
                                    ny, nz, (uint32_t)sy, (uint32_t)sz, zchunk, ntx);
             else
                 hipLaunchKernelGGL(k_stencil_probe<1>, dim3((unsigned)blocks), dim3(256), 0, st, base, stride, nx,
@@ -1366,7 +1393,8 @@ extern "C" int fdtd_sim_get_psi(FdtdSim *s, int index, float *out) {
 
 extern "C" int fdtd_sim_set_stepper(FdtdSim *s, int fused) {
     CHECK(s, "NULL sim");
-    CHECK(fused == 0 || fused == 1 || fused == 2, "stepper must be 0 (split), 1 (fused strips) or 2 (fused rows)");
+    CHECK(fused >= 0 && fused <= 3,
+          "stepper must be 0 (split), 1 (fused strips), 2 (fused rows) or 3 (fused, halo recomputed)");
     CHECK(!fused || s->g.mode == FDTD_MODE_MAIN, "the fused stepper runs MAIN simulations");
     int t = 0;
     int rc = fdtd_sim_step_count(s, &t);

@@ -46,7 +46,7 @@ struct alignas(16) Desc {
     int nx, ny, nz, npml;
     int tfsf_z, read_out, nt, nt_inc;
     int mode, dft_accum, uni_z, n_src;   // planes z >= uni_z hold (ca_u, cb_u) everywhere
-    int nt_beam, nfaces, fused, pad1;    // fused: the fused stepper advances this sim
+    int nt_beam, nfaces, fused, pad1;    // fused: stepper (0 split; 1, 2, 3 fused)
     long long sy, sz;              // field strides (elements)
     long long du_sz, lr_sz, fb_sz; // psi plane strides
     float Db, dtm0, dx, dte0dx;
@@ -71,7 +71,8 @@ struct alignas(16) Desc {
     int *step;                     // [0] step counter, [1] the step k_dft runs
     // MAIN: copies of the x-face columns, [2][nz][ny] records of two float4,
     // (ey, ez, hy(x-1), hy), (hz(x-1), hz, 0, 0) at x = npml + 1 (face 0)
-    // and nx - npml - 1 (face 1), written by the split E kernel
+    // and nx - npml - 1 (face 1), written by the split E kernel or the
+    // halo stepper
     float4 *tapx;
 };
 

@@ -4,6 +4,8 @@
 // the reference's one-rounding-per-operation arithmetic
 // (fdtdsarsim/cpu/field_updates.py, numba without fastmath) so that results
 // are bit-identical to the CPU path on the same inputs.
+#include <type_traits>
+
 #include "fdtd_kernels.h"
 
 namespace fdtd {
@@ -251,21 +253,26 @@ __device__ __forceinline__ void e_psi_load(PsiPre &q, const StepK &k, const gflo
 // them in sequence per cell gives the values of the reference's separate
 // passes exactly.  psi is read from the H block hr and, when hw is not
 // null, the updated value is written to hw (hw == hr updates in place).
+// INNER: the caller guarantees the cell lies inside the absorber's inner
+// boundary on all six faces (np <= x, y, z and x, y, z < n - np, so also
+// away from the grid's outer planes): the update is then the Yee step and
+// the injection only, the same operations in the same order.
 __device__ __forceinline__ void h_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
                                        const EAround &e, float &hx, float &hy, float &hz,
                                        const gfloat *hr, gfloat *hw, const PsiPre *pre);
 
+template <bool INNER = false>
 __device__ __forceinline__ void h_cell(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
                                        const EAround &e, float &hx, float &hy, float &hz,
                                        const gfloat *hr, gfloat *hw, const PsiPre *pre = nullptr) {
     const int nx = k.nx, ny = k.ny, nz = k.nz, np = k.np;
     const float Db = k.Db;
-    const bool xin = x < nx - 1, yin = y < ny - 1;
-    if (z < nz - 1) {
-        if (x >= 1 && xin && yin) hx -= Db * ((e.ez_yp - e.ez_c) - (e.ey_n - e.ey_c));
-        if (y >= 1 && yin && xin) hy -= Db * ((e.ex_n - e.ex_c) - (e.ez_xp - e.ez_c));
-        if (z >= 1 && yin && xin) hz -= Db * ((e.ey_xp - e.ey_c) - (e.ex_yp - e.ex_c));
-        if (z < np - 1 || z >= nz - np || x < np - 1 || x >= nx - np || y < np - 1 || y >= ny - np)
+    const bool xin = INNER || x < nx - 1, yin = INNER || y < ny - 1;
+    if (INNER || z < nz - 1) {
+        if (INNER || (x >= 1 && xin && yin)) hx -= Db * ((e.ez_yp - e.ez_c) - (e.ey_n - e.ey_c));
+        if (INNER || (y >= 1 && yin && xin)) hy -= Db * ((e.ex_n - e.ex_c) - (e.ez_xp - e.ez_c));
+        if (INNER || (z >= 1 && yin && xin)) hz -= Db * ((e.ey_xp - e.ey_c) - (e.ex_yp - e.ex_c));
+        if (!INNER && (z < np - 1 || z >= nz - np || x < np - 1 || x >= nx - np || y < np - 1 || y >= ny - np))
             h_cpml(d, k, x, y, z, e, hx, hy, hz, hr, hw, pre);
     }
     if (k.inj && z == k.tfsf_z) {
@@ -382,22 +389,23 @@ struct HAround {
 // (:66-191, psi updated in place), then the H_inc injection at tfsf_z (MAIN,
 // cpu/main_sim.py:158-161) or the hard source at the feed-in plane
 // (INCIDENT, cpu/incident_sim.py:142-144).  ca, cb are read only below the
-// top plane (z < nz - 1).
+// top plane (z < nz - 1).  INNER as in h_cell.
 __device__ __forceinline__ void e_cpml(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
                                        float cb, const HAround &h, float &ex, float &ey, float &ez,
                                        const PsiPre *pre);
 
+template <bool INNER = false>
 __device__ __forceinline__ void e_cell(const Desc *__restrict__ d, const StepK &k, int x, int y, int z,
                                        float ca, float cb, const HAround &h, float &ex, float &ey,
                                        float &ez, const PsiPre *pre = nullptr) {
     const int nx = k.nx, ny = k.ny, nz = k.nz, np = k.np;
-    const bool xlo = x >= 1, ylo = y >= 1, zlo = z >= 1;
-    const bool xin = x < nx - 1, yin = y < ny - 1;
-    if (z < nz - 1) {
+    const bool xlo = INNER || x >= 1, ylo = INNER || y >= 1, zlo = INNER || z >= 1;
+    const bool xin = INNER || x < nx - 1, yin = INNER || y < ny - 1;
+    if (INNER || z < nz - 1) {
         if (zlo && ylo && yin && xin) ex = (ex * ca) + (cb * ((h.hz_c - h.hz_ym) - (h.hy_c - h.hy_zm)));
         if (zlo && yin && xlo && xin) ey = (ey * ca) + (cb * ((h.hx_c - h.hx_zm) - (h.hz_c - h.hz_xm)));
         if (ylo && yin && xlo && xin) ez = (ez * ca) + (cb * ((h.hy_c - h.hy_xm) - (h.hx_c - h.hx_ym)));
-        if (z < np || z >= nz - np || x < np || x >= nx - np || y < np || y >= ny - np)
+        if (!INNER && (z < np || z >= nz - np || x < np || x >= nx - np || y < np || y >= ny - np))
             e_cpml(d, k, x, y, z, cb, h, ex, ey, ez, pre);
     }
     if (z == k.tfsf_z) {
@@ -1355,6 +1363,300 @@ template __global__ void k_step_strip<2, 4>(const Desc *, const int4 *, int, int
 template __global__ void k_step_strip<4, 2>(const Desc *, const int4 *, int, int);
 
 // ---------------------------------------------------------------------------
+// Fused stepper, halo form (stepper 3).  The split kernels' blocks and
+// tiles (xw wavefronts side by side along x on each of ty rows, one row per
+// wavefront over a z chunk), with H(t+1/2) and E(t+1) in one kernel.  Rows
+// are independent: each wavefront recomputes from the old level what it
+// needs of its neighbours' H(t+1/2), bit-identical to the owner's value and
+// never stored, so there is no LDS hand-over and no barrier:
+//   - hx, hz of the row below (y - 1), the E update's y - 1 operands, on
+//     every plane;
+//   - hx, hy of the row at z0 - 1, the chunk's first z - 1 operands;
+//   - hy, hz at x0 - 1, by lane 0.
+// Per plane a lane loads E(t) at z + 1 and the old H of its row and of the
+// row below, ez and ex of the row above, and Ca, Cb; E(t) at z of both rows
+// and H(t+1/2) at z - 1 of its row stay in registers from the plane below.
+// The neighbour rows' values are the other wavefronts' own loads and mostly
+// hit L2.  Fields and H-CPML state are double-buffered (step t reads set
+// t&1, writes set 1-(t&1)); the E-CPML state is updated in place, as only
+// its cell's lane touches it.  The wavefronts covering a DFT x-face column
+// copy it to Desc::tapx as the split E kernel does.
+// ---------------------------------------------------------------------------
+template <int V, bool DIRECT>
+__global__ void __launch_bounds__(512)
+k_step_halo(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles, int shape, TileGrid tg,
+            const int *__restrict__ stepp) {
+    int4 tl;
+    if (!split_tile<DIRECT>(tiles, ntiles, shape, tg, tl)) return;
+    const Desc *__restrict__ d = descs + (tl.x & 0xffff);
+    // the step selects the buffer sets: kept uniform (scalar), so the field
+    // pointers stay in SGPRs
+    const int t = DIRECT ? *stepp : __builtin_amdgcn_readfirstlane(load_step(d->step));
+    const StepK kg = step_geom(d);
+    const StepTime st = step_time(d);
+    scalars_resident(kg, st);
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    note_step(d, tl, w, t);
+    const int lxw = DIRECT ? shape & 3 : tl.x >> 16;   // listed tiles carry their block width
+    const int y = tl.z + (w >> lxw);
+    const int nx = kg.nx, ny = kg.ny, nz = kg.nz;
+    const int x0 = tl.y + V * ((w & ((1 << lxw) - 1)) << 6);   // the wavefront's first cell
+    if (y >= ny || x0 >= nx || t >= st.nt) return;    // uniform per wavefront
+    const StepK k = step_at(kg, st, t);
+    const int lane = threadIdx.x;
+    const int xb = x0 + V * lane;                      // the lane's first cell
+    const bool act = xb < nx;
+    const bool next_mem = lane == kWave - 1 && xb + V < nx;
+    const bool halo_x = lane == 0 && x0 > 0;
+    const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
+    const uint32_t sy = k.sy, sz = k.sz;
+    const int p = t & 1;
+    const gfloat *fo[6];
+    gfloat *fn[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        fo[c] = gp(d->f[p][c]);
+        fn[c] = gp(d->f[1 - p][c]);
+    }
+    const gfloat *psi_o = gp(d->psi[p][pHxz_d]);      // H-CPML state of this step
+    gfloat *psi_n = gp(d->psi[1 - p][pHxz_d]);        // and of the next
+    const gfloat *Ca = gp(d->ca), *Cb = gp(d->cb);
+    const int uni_z = d->uni_z;
+    const float ca_u = d->ca_u, cb_u = d->cb_u;
+    const bool ylo = y >= 1, yin = y < ny - 1;
+    gchar *const tap = d->tapx && wave_taps<V>(x0, nx, k.np) ? (gchar *)d->tapx : nullptr;
+
+    uint32_t o = (uint32_t)z0 * sz + (uint32_t)y * sy + (uint32_t)xb;
+    // E(t) at z of the row and of the row below; H(t+1/2) at z - 1 of the row
+    float exc[V], eyc[V], exm[V], eym[V], hxzm[V], hyzm[V];
+    vset(exc, 0.f); vset(eyc, 0.f); vset(exm, 0.f); vset(eym, 0.f); vset(hxzm, 0.f); vset(hyzm, 0.f);
+    if (act) {
+        vld(exc, at(fo[0], o));
+        vld(eyc, at(fo[1], o));
+        if (ylo) {
+            vld(exm, at(fo[0], o - sy));
+            vld(eym, at(fo[1], o - sy));
+        }
+    }
+    if (z0 > 0) {
+        // hx, hy of the row at z0 - 1 (below the top plane: z0 <= nz - 1)
+        const uint32_t oh = o - sz;
+        float ex[V], ey[V], ez[V], ezyp[V], exyp[V], hz[V];
+        vset(ex, 0.f); vset(ey, 0.f); vset(ez, 0.f); vset(ezyp, 0.f); vset(exyp, 0.f); vset(hz, 0.f);
+        float ey_x = 0.f, ez_x = 0.f;
+        if (act) {
+            vld(ex, at(fo[0], oh));
+            vld(ey, at(fo[1], oh));
+            vld(ez, at(fo[2], oh));
+            vld(hxzm, at(fo[3], oh));
+            vld(hyzm, at(fo[4], oh));
+            if (yin) {
+                vld(ezyp, at(fo[2], oh + sy));
+                vld(exyp, at(fo[0], oh + sy));
+            }
+            if (next_mem) {
+                ey_x = *at(fo[1], oh + V);
+                ez_x = *at(fo[2], oh + V);
+            }
+        }
+        float ey_next = __shfl_down(ey[0], 1, kWave);
+        float ez_next = __shfl_down(ez[0], 1, kWave);
+        if (lane == kWave - 1) {
+            ey_next = ey_x;
+            ez_next = ez_x;
+        }
+#pragma unroll
+        for (int c = 0; c < V; ++c) {
+            if (xb + c < nx) {
+                EAround e;
+                e.ex_c = ex[c]; e.ex_n = exc[c]; e.ex_yp = exyp[c];
+                e.ey_c = ey[c]; e.ey_n = eyc[c];
+                e.ey_xp = c + 1 < V ? ey[c + 1 < V ? c + 1 : 0] : ey_next;
+                e.ez_c = ez[c]; e.ez_yp = ezyp[c];
+                e.ez_xp = c + 1 < V ? ez[c + 1 < V ? c + 1 : 0] : ez_next;
+                h_cell(d, k, xb + c, y, z0 - 1, e, hxzm[c], hyzm[c], hz[c], psi_o, nullptr);
+            }
+        }
+    }
+    // One plane of the march.  IN (a std::integral_constant): every cell
+    // the wavefront updates or recomputes on this plane lies inside the
+    // absorber's inner boundary (the wavefront's cells, its row and the row
+    // below, the cell left of it), so all lanes are active, every
+    // neighbour exists, and the cell updates reduce to the Yee step and the
+    // injection (h_cell / e_cell INNER): the same arithmetic, without the
+    // per-cell boundary and absorber tests.
+    auto plane = [&](int z, auto IN) {
+        constexpr bool I = decltype(IN)::value;
+        const bool zin = I || z < nz - 1;
+        const bool lact = I || act, lyin = I || yin, lylo = I || ylo;
+        // ---- loads: the row (H old, E(t) at z + 1, ez at z), the row above
+        // (ez, ex), the row below (hx, hz old, E(t) at z + 1, ez), Ca, Cb.
+        // Values a lane does not load are never read for a stored result
+        // (inactive lanes, missing neighbours), but are zeroed as the split
+        // kernels do, except on interior planes, where every lane loads all.
+        float exn[V], eyn[V], ez[V], ezyp[V], exyp[V], hx[V], hy[V], hz[V], ca[V], cb[V];
+        float exmn[V], eymn[V], ezm[V], hxm[V], hzm[V];
+        if (!I) {
+            vset(exn, 0.f); vset(eyn, 0.f); vset(ez, 0.f); vset(ezyp, 0.f); vset(exyp, 0.f);
+            vset(hx, 0.f); vset(hy, 0.f); vset(hz, 0.f);
+            vset(exmn, 0.f); vset(eymn, 0.f); vset(ezm, 0.f); vset(hxm, 0.f); vset(hzm, 0.f);
+        }
+        vset(ca, ca_u);
+        vset(cb, cb_u);
+        float ey_x = 0.f, ez_x = 0.f, eym_x = 0.f;
+        if (lact) {
+            vld(hx, at(fo[3], o));
+            vld(hy, at(fo[4], o));
+            vld(hz, at(fo[5], o));
+            vld(ez, at(fo[2], o));
+            if (zin) {
+                vld(exn, at(fo[0], o + sz));
+                vld(eyn, at(fo[1], o + sz));
+                if (lyin) {
+                    vld(ezyp, at(fo[2], o + sy));
+                    vld(exyp, at(fo[0], o + sy));
+                }
+                if (next_mem) {
+                    ey_x = *at(fo[1], o + V);
+                    ez_x = *at(fo[2], o + V);
+                }
+                if (lylo) {
+                    const uint32_t om = o - sy;
+                    vld(hxm, at(fo[3], om));
+                    vld(hzm, at(fo[5], om));
+                    vld(exmn, at(fo[0], om + sz));
+                    vld(eymn, at(fo[1], om + sz));
+                    vld(ezm, at(fo[2], om));
+                    if (next_mem) eym_x = *at(fo[1], om + V);
+                }
+                if (z < uni_z) {
+                    vld_stream(ca, at(Ca, o));
+                    vld_stream(cb, at(Cb, o));
+                }
+            }
+        }
+        // lane 0: the inputs of hy, hz at x0 - 1
+        EAround eh = {};
+        float hy_h = 0.f, hz_h = 0.f;
+        const bool do_halo = halo_x && zin;
+        if (do_halo) {
+            const uint32_t oh = o - 1;
+            eh.ex_c = *at(fo[0], oh);
+            eh.ey_c = *at(fo[1], oh);
+            eh.ez_c = *at(fo[2], oh);
+            eh.ex_n = *at(fo[0], oh + sz);
+            eh.ex_yp = lyin ? *at(fo[0], oh + sy) : 0.f;
+            hy_h = *at(fo[4], oh);
+            hz_h = *at(fo[5], oh);
+        }
+        // ---- H(t+1/2) of the row, then hx, hz of the row below
+        float ey_next = __shfl_down(eyc[0], 1, kWave);
+        float ez_next = __shfl_down(ez[0], 1, kWave);
+        float eym_next = __shfl_down(eym[0], 1, kWave);
+        if (lane == kWave - 1) {
+            ey_next = ey_x;
+            ez_next = ez_x;
+            eym_next = eym_x;
+        }
+#pragma unroll
+        for (int c = 0; c < V; ++c) {
+            if (I || xb + c < nx) {
+                EAround e;
+                e.ex_c = exc[c]; e.ex_n = exn[c]; e.ex_yp = exyp[c];
+                e.ey_c = eyc[c]; e.ey_n = eyn[c];
+                e.ey_xp = c + 1 < V ? eyc[c + 1 < V ? c + 1 : 0] : ey_next;
+                e.ez_c = ez[c]; e.ez_yp = ezyp[c];
+                e.ez_xp = c + 1 < V ? ez[c + 1 < V ? c + 1 : 0] : ez_next;
+                h_cell<I>(d, k, xb + c, y, z, e, hx[c], hy[c], hz[c], psi_o, psi_n);
+            }
+        }
+        if (lylo && zin) {
+#pragma unroll
+            for (int c = 0; c < V; ++c) {
+                if (I || xb + c < nx) {
+                    // hy of the row below is not needed: its operands
+                    // (ex at z + 1, ez at x + 1) are left out
+                    EAround e;
+                    e.ex_c = exm[c]; e.ex_n = exm[c]; e.ex_yp = exc[c];
+                    e.ey_c = eym[c]; e.ey_n = eymn[c];
+                    e.ey_xp = c + 1 < V ? eym[c + 1 < V ? c + 1 : 0] : eym_next;
+                    e.ez_c = ezm[c]; e.ez_yp = ez[c]; e.ez_xp = ezm[c];
+                    float hy_m = 0.f;
+                    h_cell<I>(d, k, xb + c, y - 1, z, e, hxm[c], hy_m, hzm[c], psi_o, nullptr);
+                }
+            }
+        }
+        // ---- E(t+1) of the row
+        float hz_prev = __shfl_up(hz[V - 1], 1, kWave);
+        float hy_prev = __shfl_up(hy[V - 1], 1, kWave);
+        if (lane == 0) {
+            hz_prev = 0.f;
+            hy_prev = 0.f;
+            if (do_halo) {
+                // only hy, hz of this cell are used: hx's operands are left out
+                eh.ey_xp = eyc[0];
+                eh.ez_xp = ez[0];
+                eh.ey_n = eh.ey_c;
+                eh.ez_yp = eh.ez_c;
+                float hx_h = 0.f;
+                h_cell<I>(d, k, x0 - 1, y, z, eh, hx_h, hy_h, hz_h, psi_o, nullptr);
+                hz_prev = hz_h;
+                hy_prev = hy_h;
+            }
+        }
+        float exu[V], eyu[V], ezu[V];
+        vcopy(exu, exc);
+        vcopy(eyu, eyc);
+        vcopy(ezu, ez);
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+            if (I || xb + q < nx) {
+                HAround h;
+                h.hx_c = hx[q]; h.hy_c = hy[q]; h.hz_c = hz[q];
+                h.hx_zm = hxzm[q]; h.hy_zm = hyzm[q];
+                h.hz_ym = hzm[q]; h.hx_ym = hxm[q];
+                h.hz_xm = q > 0 ? hz[q > 0 ? q - 1 : 0] : hz_prev;
+                h.hy_xm = q > 0 ? hy[q > 0 ? q - 1 : 0] : hy_prev;
+                e_cell<I>(d, k, xb + q, y, z, ca[q], cb[q], h, exu[q], eyu[q], ezu[q]);
+            }
+        }
+        if (lact) {
+            vst_stream(at(fn[3], o), hx);
+            vst_stream(at(fn[4], o), hy);
+            vst_stream(at(fn[5], o), hz);
+            vst_stream(at(fn[0], o), exu);
+            vst_stream(at(fn[1], o), eyu);
+            vst_stream(at(fn[2], o), ezu);
+        }
+        if (!I && tap) e_tap<V>(tap, xb, nx, k.np, nz, ny, z, y, eyu, ezu, hy, hz, hy_prev, hz_prev);
+        // hand over to the plane above
+        vcopy(exc, exn);
+        vcopy(eyc, eyn);
+        vcopy(exm, exmn);
+        vcopy(eym, eymn);
+        vcopy(hxzm, hx);
+        vcopy(hyzm, hy);
+    };
+    // interior wavefront: its cells and the cell left of it lie in
+    // [np, nx - np) along x, its row and the row below in [np, ny - np),
+    // and it copies no DFT column; its planes np .. nz - np - 1 are interior
+    const int np = k.np;
+    const bool wave_in = !tap && x0 >= np && x0 + kWave * V <= nx - np && y >= np && y < ny - np;
+#pragma unroll 1
+    for (int z = z0; z < z1; ++z, o += sz) {
+        if (wave_in && z >= np && z < nz - np)
+            plane(z, std::true_type{});
+        else
+            plane(z, std::false_type{});
+    }
+}
+
+template __global__ void k_step_halo<1, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_step_halo<1, true>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_step_halo<2, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_step_halo<2, true>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+
+// ---------------------------------------------------------------------------
 // Running single-frequency DFT on the phasor planes
 // (cpu/field_updates.py:373-491, called at cpu/main_sim.py:166-195 and
 // cpu/incident_sim.py:159-161).  E samples are 2-point averages, H samples
@@ -1439,8 +1741,8 @@ k_dft(const Desc *__restrict__ descs) {
             break;
         }
         case kFaceX: {   // (a, b) = (z, y); ey ez hy hz
-            if (d->tapx && !d->fused) {
-                // the split E kernel's copies of the face columns:
+            if (d->tapx && (d->fused == 0 || d->fused == 3)) {
+                // the split E kernel's (or halo stepper's) copies of the face columns:
                 // (ey, ez, hy(x-1), hy), (hz(x-1), hz, -, -) per (z, y)
                 const int ny = d->ny;
                 const float4 *T = d->tapx + 2 * (pos == np + 1 ? 0 : (long long)d->nz * ny);
@@ -1556,7 +1858,10 @@ k_stream_probe(const float *__restrict__ in, float *__restrict__ out, long long 
 // the y +- 1 row loads.  KIND 0 is the H half (reads ex ey ez hx hy hz, ex
 // and ez of row y+1, ex ey of plane z+1; writes hx hy hz), KIND 1 the E half
 // (reads ex ey ez hx hy hz ca cb, hz and hx of row y-1, hx hy of plane z-1;
-// writes ex ey ez).  Algorithmic streams: 9 and 11 floats per cell.
+// writes ex ey ez).  KIND 2 is the halo stepper's one kernel per step
+// (reads ex ey ez hx hy hz ca cb of its row, ex ey ez hx hz of row y-1, ex ez
+// of row y+1; writes the six fields to the other buffer set, streamed).
+// Algorithmic streams: 9, 11 and 14 floats per cell.
 template <int KIND>
 __global__ void __launch_bounds__(256)
 k_stencil_probe(float *__restrict__ base, long long stride, int nx, int ny, int nz, uint32_t sy, uint32_t sz,
@@ -1585,6 +1890,53 @@ k_stencil_probe(float *__restrict__ base, long long stride, int nx, int ny, int 
             c0 = n0;
             c1 = n1;
         }
+    } else if (KIND == 2) {
+        const gfloat *A6 = gp(base + 6 * stride), *A7 = gp(base + 7 * stride);
+        gfloat *B[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) B[c] = gp(base + (8 + c) * stride);
+        const bool ylo = y >= 1, yin = y + 1 < ny;
+        float exc = *at(A0, o), eyc = *at(A1, o);
+        float exm = ylo ? *at(A0, o - sy) : 0.f, eym = ylo ? *at(A1, o - sy) : 0.f;
+        float hxz = 0.f, hyz = 0.f;
+        for (int z = z0; z < z1; ++z, o += sz) {
+            const bool zin = z + 1 < nz;
+            const float exn = zin ? *at(A0, o + sz) : 0.f, eyn = zin ? *at(A1, o + sz) : 0.f;
+            const float ez = *at(A2, o);
+            float hx = *at(A3, o), hy = *at(A4, o), hz = *at(A5, o);
+            const float ca = __builtin_nontemporal_load(at(A6, o)), cb = __builtin_nontemporal_load(at(A7, o));
+            const float ezp = yin ? *at(A2, o + sy) : 0.f, exp_ = yin ? *at(A0, o + sy) : 0.f;
+            float hxm = 0.f, hzm = 0.f, exmn = 0.f, eymn = 0.f, ezm = 0.f;
+            if (ylo) {
+                hxm = *at(A3, o - sy);
+                hzm = *at(A5, o - sy);
+                ezm = *at(A2, o - sy);
+                if (zin) {
+                    exmn = *at(A0, o - sy + sz);
+                    eymn = *at(A1, o - sy + sz);
+                }
+            }
+            const float eyx = __shfl_down(eyc, 1, kWave), ezx = __shfl_down(ez, 1, kWave);
+            const float eymx = __shfl_down(eym, 1, kWave);
+            hx -= 0.5f * ((ezp - ez) - (eyn - eyc));
+            hy -= 0.5f * ((exn - exc) - (ezx - ez));
+            hz -= 0.5f * ((eyx - eyc) - (exp_ - exc));
+            hxm -= 0.5f * ((ez - ezm) - (eymn - eym));
+            hzm -= 0.5f * ((eymx - eym) - (exc - exm));
+            const float hyx = __shfl_up(hy, 1, kWave), hzx = __shfl_up(hz, 1, kWave);
+            __builtin_nontemporal_store(hx, at(B[3], o));
+            __builtin_nontemporal_store(hy, at(B[4], o));
+            __builtin_nontemporal_store(hz, at(B[5], o));
+            __builtin_nontemporal_store(ca * exc + cb * ((hz - hzm) - (hy - hyz)), at(B[0], o));
+            __builtin_nontemporal_store(ca * eyc + cb * ((hx - hxz) - (hz - hzx)), at(B[1], o));
+            __builtin_nontemporal_store(ca * ez + cb * ((hy - hyx) - (hx - hxm)), at(B[2], o));
+            exc = exn;
+            eyc = eyn;
+            exm = exmn;
+            eym = eymn;
+            hxz = hx;
+            hyz = hy;
+        }
     } else {
         const gfloat *A6 = gp(base + 6 * stride), *A7 = gp(base + 7 * stride);
         const bool ylo = y >= 1;
@@ -1605,6 +1957,7 @@ k_stencil_probe(float *__restrict__ base, long long stride, int nx, int ny, int 
 
 template __global__ void k_stencil_probe<0>(float *, long long, int, int, int, uint32_t, uint32_t, int, int);
 template __global__ void k_stencil_probe<1>(float *, long long, int, int, int, uint32_t, uint32_t, int, int);
+template __global__ void k_stencil_probe<2>(float *, long long, int, int, int, uint32_t, uint32_t, int, int);
 
 template __global__ void k_stream_probe<1, 1>(const float *, float *, long long);
 template __global__ void k_stream_probe<6, 3>(const float *, float *, long long);

@@ -5,7 +5,8 @@ xvec consecutive x cells (1, 2 or 4; 0 = chosen by the library from the
 grid width) and marches z over zchunk cells (0 = whole column).  Split
 stepper: blocks of xwaves x ty wavefronts (at most 4), xwaves side by side
 along x on each of ty rows.  Fused stepper: ty in {2, 4, 8} rows walked by
-one wavefront, xwaves 1.  xcd_order 1 deals each XCD a contiguous range of
+one wavefront, xwaves 1.  Halo stepper: the split stepper's blocks, 1 or 2
+cells per lane.  xcd_order 1 deals each XCD a contiguous range of
 tiles, 0 runs them in plain order.
 ``autotune`` measures candidates on the device and records the best per
 (stepper, arch, grid) in a JSON cache; ``launch_for`` reads it, falling back
@@ -23,11 +24,14 @@ CANDIDATES = {
                 for zc in (1, 2, 3, 4, 6, 8, 16) for xo in (0, 1)],
     'rows': [(r, zc, xv, 1, xo) for r in (3, 7) for zc in (4, 8, 16, 32) for xv in (1, 2)
              for xo in (0, 1)],
+    'halo': [(ty, zc, xv, xw, 0) for xv in (1, 2) for xw, ty in ((4, 1), (2, 1), (2, 2), (1, 4))
+             for zc in (4, 8, 16)],
 }
 # library defaults: 0 = chosen from the grid (split: one cell per lane,
 # four wavefronts per block, as many along x as the grid fills)
-DEFAULT = {'fused': (4, 8, 0, 1, 0), 'split': (0, 2, 0, 0, 0), 'rows': (7, 16, 2, 1, 0)}
-STEPPERS = {0: 'split', 1: 'fused', 2: 'rows'}
+DEFAULT = {'fused': (4, 8, 0, 1, 0), 'split': (0, 2, 0, 0, 0), 'rows': (7, 16, 2, 1, 0),
+           'halo': (2, 8, 1, 2, 0)}
+STEPPERS = {0: 'split', 1: 'fused', 2: 'rows', 3: 'halo'}
 
 
 def _stepper(fused):

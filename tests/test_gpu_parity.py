@@ -58,7 +58,7 @@ def soil60():
     return pc, Ca, Cb, beam, ref
 
 
-@pytest.mark.parametrize('fused', [False, True, 2], ids=['split', 'fused', 'rows'])
+@pytest.mark.parametrize('fused', [False, True, 2, 3], ids=['split', 'fused', 'rows', 'halo'])
 def test_fields_psi_phasors_bitwise(soil60, fused):
     pc, Ca, Cb, beam, ref = soil60
     say('case: soil 33x31x35, 60 steps, reference DFT, fused', fused)
@@ -98,7 +98,11 @@ def test_fields_psi_phasors_bitwise(soil60, fused):
                                           (False, (1, 0, 2, 4, 2)), (False, (4, 1, 1, 1, 2)),
                                           (2, (3, 0, 1, 1, 0)), (2, (7, 5, 1, 1, 0)),
                                           (2, (3, 1, 2, 1, 1)), (2, (7, 8, 2, 1, 0)),
-                                          (2, (3, 16, 0, 1, 0))])
+                                          (2, (3, 16, 0, 1, 0)),
+                                          (3, (1, 0, 1, 4, 0)), (3, (1, 2, 1, 4, 0)),
+                                          (3, (2, 3, 2, 2, 1)), (3, (4, 1, 1, 1, 0)),
+                                          (3, (1, 5, 2, 4, 0)), (3, (1, 7, 1, 2, 2)),
+                                          (3, (2, 16, 0, 0, 0)), (3, (0, 8, 0, 0, 0))])
 def test_launch_shapes_bitwise(soil60, fused, launch):
     pc, Ca, Cb, beam, ref = soil60
     say('case: fused', fused, 'launch', launch)
@@ -113,7 +117,7 @@ def test_launch_shapes_bitwise(soil60, fused, launch):
     sim.destroy()
 
 
-STEPPERS = pytest.mark.parametrize('fused', [False, True, 2], ids=['split', 'fused', 'rows'])
+STEPPERS = pytest.mark.parametrize('fused', [False, True, 2, 3], ids=['split', 'fused', 'rows', 'halo'])
 
 
 @STEPPERS
@@ -149,9 +153,9 @@ def taps139():
 
 @pytest.mark.parametrize('fused,launch', [(False, None), (False, (1, 2, 1, 4, 0)),
                                           (False, (1, 3, 2, 2, 0)), (False, (2, 5, 4, 2, 1)),
-                                          (True, None), (2, None)],
+                                          (True, None), (2, None), (3, None), (3, (1, 3, 2, 2, 0))],
                          ids=['split-auto', 'split-1-cell', 'split-2-cells', 'split-4-cells-xcd',
-                              'fused', 'rows'])
+                              'fused', 'rows', 'halo', 'halo-2-cells'])
 def test_compensated_x_faces_close_and_closer_to_exact(taps139, fused, launch):
     """The shipped default (split stepper, compensated DFT) at phasor level
     where the x-face taps sit on wavefront edges: every phasor within 1e-5
@@ -230,8 +234,10 @@ def wide30():
     return pc, Ca, Cb, beam, H.oracle_main(pc, Ca, Cb, beam)
 
 
-@pytest.mark.parametrize('fused,xwaves', [(False, 1), (True, 1), (False, 4), (False, 2), (2, 1)],
-                         ids=['split', 'fused', 'split-4-waves', 'split-2-waves', 'rows'])
+@pytest.mark.parametrize('fused,xwaves', [(False, 1), (True, 1), (False, 4), (False, 2), (2, 1), (3, 4),
+                                          (3, 1)],
+                         ids=['split', 'fused', 'split-4-waves', 'split-2-waves', 'rows', 'halo-4-waves',
+                              'halo'])
 @pytest.mark.parametrize('xvec', [4, 2, 1])
 def test_wide_grid_lane_widths_bitwise(wide30, fused, xwaves, xvec):
     # 301 cells in x: two 256-cell tiles at 4 cells per lane (the second
@@ -242,9 +248,9 @@ def test_wide_grid_lane_widths_bitwise(wide30, fused, xwaves, xvec):
     launch = (2, 8, xvec) if xwaves == 1 else (4 // xwaves, 2, xvec, xwaves, 2 if xvec == 1 else 0)
     if fused == 2:
         launch = (3, 4, xvec, 1, 0)
-    if fused == 2 and xvec == 4:
-        # the row-parallel stepper takes 1 or 2 cells per lane: the shape is
-        # refused on the host and the simulation keeps its previous one
+    if fused in (2, 3) and xvec == 4:
+        # the row-parallel and halo steppers take 1 or 2 cells per lane: the
+        # shape is refused on the host and the simulation keeps its previous one
         sim = make_sim(pc, Ca, Cb, beam, fused=fused)
         with pytest.raises(native.FdtdError, match='xvec must be 1 or 2'):
             sim.set_launch(*launch)
@@ -392,15 +398,15 @@ def test_incident_launch_shapes_bitwise(incident31, launch):
 
 
 def test_split_and_fused_agree_long_run():
-    """300 steps on the plain workload's grid: both steppers bit-identical."""
+    """300 steps on the plain workload's grid: all four steppers bit-identical."""
     from sar_pixel_simulator_amd import workloads
     from sar_pixel_simulator_amd.materials import compute_material_arrays
     pc = workloads.plain_params()
     Ca, Cb = compute_material_arrays(pc, workloads.plain_material(pc, 2))
     beam = H.synthetic_beam(pc)
-    say('case: 81^3, 300 steps, split vs both fused steppers')
+    say('case: 81^3, 300 steps, split vs the three fused steppers')
     out = []
-    for fused in (False, True, 2):
+    for fused in (False, True, 2, 3):
         sim = make_sim(pc, Ca, Cb, beam, fused=fused)
         sim.run(300)
         sim.sync()
@@ -414,10 +420,11 @@ def test_split_and_fused_agree_long_run():
 
 
 def test_split_and_rows_agree_at_largest_size():
-    """The largest single domain, 1523 x 1523 x 271 (6.3e8 cells), 240 steps
+    """The largest single domain, 1523 x 1523 x 311 (7.2e8 cells), 240 steps
     on device-built snow over rough soil: the split stepper and the
     row-parallel fused stepper, two independent implementations of the
-    step, give bit-identical fields and phasors."""
+    step, and the halo-recomputing fused stepper give bit-identical fields
+    and phasors."""
     from sar_pixel_simulator_amd import workloads
     from sar_pixel_simulator_amd.materials import device
     nsteps = 240
@@ -427,7 +434,7 @@ def test_split_and_rows_agree_at_largest_size():
     snow = int(round(1.5 / 0.021))
     spec = device.snow_over_soil_spec(pc, g.size_z_material - snow, snow, seed=3)
     beam = workloads.synthetic_beam_planes(pc, 0, nsteps)
-    say('case: largest 1523x1523x271, 240 steps, split vs row-parallel')
+    say('case: largest', g.size_x, g.size_y, g.size_z, '240 steps, split vs row-parallel and halo')
 
     def run(fused):
         native.init(0)
@@ -446,23 +453,27 @@ def test_split_and_rows_agree_at_largest_size():
     sim.destroy()
     b.release()
     assert np.abs(ref['ey']).max() > 0 and np.abs(ref_ph['ey_ph_l']).max() > 0
-    sim, b = run(2)
-    for n in native.FIELD_NAMES:
-        assert_same(sim.field(n), ref.pop(n), n)
-    ph = sim.phasors()
-    for n in native.PHASOR_NAMES:
-        assert_same(ph[n], ref_ph[n], n)
-    sim.destroy()
-    b.release()
+    for fused in (2, 3):
+        say('  stepper', fused)
+        sim, b = run(fused)
+        for n in native.FIELD_NAMES:
+            assert_same(sim.field(n), ref[n], f'{n} stepper {fused}')
+        ph = sim.phasors()
+        for n in native.PHASOR_NAMES:
+            assert_same(ph[n], ref_ph[n], f'{n} stepper {fused}')
+        sim.destroy()
+        b.release()
 
 
 @pytest.mark.parametrize('fused,launch', [(True, None), (False, None), (False, (2, 8, 4, 1, 1)),
                                           (False, (1, 1, 1, 4, 0)), (False, (1, 4, 1, 4, 2)),
                                           (2, None), (False, 'mixed-widths'), (False, 'serial'),
-                                          (False, 'serial-split'), (False, 'hybrid'), (2, 'hybrid')],
+                                          (False, 'serial-split'), (False, 'hybrid'), (2, 'hybrid'),
+                                          (3, None), (3, 'mixed-widths'), (3, 'serial-split')],
                          ids=['fused', 'split-auto', 'split-4-per-lane', 'split-4-waves-flat',
                               'split-banded', 'rows', 'split-auto-mixed-widths', 'split-serial',
-                              'split-serial-small-lanes', 'split-hybrid', 'rows-hybrid'])
+                              'split-serial-small-lanes', 'split-hybrid', 'rows-hybrid', 'halo-auto',
+                              'halo-auto-mixed-widths', 'halo-serial-small-lanes'])
 def test_batch_equals_solo(fused, launch):
     cases = [H.soil_case(nx=33, ny=31, nz=35, nz_mat=14, steps=30, seed=s) for s in (1, 2)]
     cases.append(H.soil_case(nx=45, ny=39, nz=37, nz_mat=15, steps=30, seed=5))
@@ -668,12 +679,12 @@ def test_bandwidth_probes_run_and_reject_bad_arguments():
     assert ms > 0 and gbs > 0
     ms, gbs = native.probe_streams(1 << 16, 6, 3, reps=2)
     assert ms > 0 and gbs > 0
-    for kind in (0, 1):
+    for kind in (0, 1, 2):
         for zc in (1, 4, 64):
             ms, gbs = native.probe_stencil(301, 17, 9, zc, kind, reps=2)
             assert ms > 0 and gbs > 0
     with pytest.raises(native.FdtdError):
-        native.probe_stencil(64, 8, 8, 4, 2)
+        native.probe_stencil(64, 8, 8, 4, 3)
     with pytest.raises(native.FdtdError):
         native.probe_stencil(64, 8, 8, 0, 0)
     with pytest.raises(native.FdtdError):
