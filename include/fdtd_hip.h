@@ -191,17 +191,22 @@ int fdtd_sim_set_field(FdtdSim *s, int comp, const float *in);
 int fdtd_sim_get_psi(FdtdSim *s, int index, float *out);
 /* No counterpart in the reference (its step is 26 fixed launches,
  * gpu/main_sim.py:216-355).  MAIN only, before the first step: 0 = split
- * stepper (an H kernel, then an E kernel, in place; the default); 1 = fused
- * stepper on row strips (one kernel computes H then E per step, one
- * wavefront walks a strip's rows; fields double-buffered); 2 = fused
+ * stepper (an H kernel, then an E kernel, in place; the library default);
+ * 1 = fused stepper on row strips (one kernel computes H then E per step,
+ * one wavefront walks a strip's rows; fields double-buffered); 2 = fused
  * stepper with one row per wavefront (ty = 3 or 7 rows per block, xvec 1 or
- * 2).  All give identical results. */
+ * 2); 3 = halo stepper: one kernel per step on the split stepper's tiles
+ * (one row per wavefront, up to 8 wavefronts per block, xvec 1 or 2), each
+ * wavefront recomputing the H of the row below, of its chunk's first plane
+ * and of the cell left of it instead of reading it (fastest from about 64M
+ * cells).  All give identical results. */
 int fdtd_sim_set_stepper(FdtdSim *s, int fused);
 /* No counterpart in the reference.  Tile shape: each lane owns xvec
  * consecutive x cells (1, 2, 4) and
  * marches z over zchunk cells (0 = whole column).  Split stepper: blocks of
  * xwaves x ty wavefronts (at most 4), xwaves (1, 2, 4) side by side along
- * x on each of ty (1, 2, 4) rows.  Fused stepper: xwaves 1, ty 2, 4 or 8
+ * x on each of ty (1, 2, 4) rows; halo stepper likewise, at most 8
+ * wavefronts (ty also 8).  Fused stepper: xwaves 1, ty 2, 4 or 8
  * rows per wavefront with (4 ty + 2) xvec <= 64 (LDS).  0 for ty, xvec or
  * xwaves: chosen from the grid.  xcd_order: 0 runs tiles in plain order, 1
  * deals each XCD a contiguous range of tiles, 2 (split only) deals each XCD
