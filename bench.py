@@ -403,6 +403,34 @@ def main():
     per_kernel['step_alg_GBs'] = gbs(step_alg, ms_step)
     achieved = kern['bytes'] / (kern['ms'] * 1e-3) / 1e9
     sim.destroy()
+    split_ref = None
+    if fused and not args.no_extras:
+        # the same workload on the split stepper, for comparison in the same
+        # run: step time and its kernels against their algorithmic bytes
+        print('bench: split stepper on the same workload', file=sys.stderr, flush=True)
+        ssim = native.Sim(common.main_geom(pc), common.coeffs_for(pc))
+        ssim.set_stepper(0)
+        sshape = tuning.launch_for(g.size_x, g.size_y, g.size_z, fused=0)
+        ssim.set_launch(*sshape)
+        ssim.attach_beam(beam)
+        ssim.set_materials(Ca, Cb)
+        ssim.run(args.warmup)
+        ssim.sync()
+        t0 = time.perf_counter()
+        ssim.run(args.steps)
+        ssim.sync()
+        s_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        sk = ssim.time_steps(min(args.steps, 50), per_kernel=True)
+        sab = algorithmic_bytes(g.size_x, g.size_y, g.size_z, g.size_pml, uni_z, taps=True)
+        split_ref = {'ms_per_step': round(s_ms, 4), 'Mcell_updates_per_s': round(cells / (s_ms * 1e-3) / 1e6, 1),
+                     'launch': dict(zip(LAUNCH_KEYS, sshape)),
+                     'k_h_half_ms': round(sk['h'], 4), 'k_e_half_ms': round(sk['e'], 4),
+                     'k_dft_ms': round(sk['dft'], 4),
+                     'k_e_half_frac': round(sab['e'] / (sk['e'] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     'k_h_half_frac': round(sab['h'] / (sk['h'] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     'step_alg_bytes': sab['h'] + sab['e'] + sab['dft'],
+                     'split_step_over_this_step': round(s_ms / ms_step, 4)}
+        ssim.destroy()
     beam.release()
     # practical ceiling: a plain streaming copy with the same 4-byte accesses
     # (and with 16-byte accesses)
@@ -481,6 +509,8 @@ def main():
                       'device': info['name'], 'arch': info['arch'],
                       'cu_count': info['cu_count']},
            'roofline': roof, 'kernels': per_kernel, 'cpu_baseline': cpu}
+    if split_ref:
+        out['split_stepper_same_workload'] = split_ref
     out.update(extras)
     print(json.dumps(out), flush=True)
 

@@ -27,3 +27,28 @@ def test_batch_cache_round_trip(tmp_path, monkeypatch):
         json.dump(data, f)
     assert tuning.batch_mode_for(models) == ((1, 0, 4), None)
     assert tuning.batch_mode_for(models[:1]) == ('packed', (0, 4, 0, 0, 0))
+
+
+def test_default_stepper_by_grid_size(monkeypatch):
+    from sar_pixel_simulator_amd.hip import common
+    monkeypatch.setattr(common, 'FUSED_STEPPER', None)
+    # the batched soil sizes and the serving canopy sizes stay on the split
+    # stepper; the single large run and the largest domain take the halo one
+    assert common.stepper_for(267, 267, 133) == 0
+    assert common.stepper_for(381, 381, 311) == 0
+    assert common.stepper_for(761, 761, 203) == 3
+    assert common.stepper_for(1523, 1523, 311) == 3
+    n = common.HALO_MIN_CELLS
+    assert common.stepper_for(n, 1, 1) == 3 and common.stepper_for(n - 1, 1, 1) == 0
+    monkeypatch.setattr(common, 'FUSED_STEPPER', 0)
+    assert common.stepper_for(761, 761, 203) == 0
+
+
+def test_every_stepper_has_a_default_shape():
+    for fused, name in tuning.STEPPERS.items():
+        assert name in tuning.DEFAULT and name in tuning.CANDIDATES
+        assert len(tuning.launch_for(1, 1, 1, fused=fused)) == 5
+    # the halo stepper's candidates are the split stepper's block shapes
+    # with at most 8 wavefronts and 1 or 2 cells per lane
+    for ty, zc, xv, xw, xo in tuning.CANDIDATES['halo']:
+        assert xv in (1, 2) and ty * xw <= 8
