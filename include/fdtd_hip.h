@@ -209,9 +209,11 @@ int fdtd_sim_set_stepper(FdtdSim *s, int fused);
  * wavefronts (ty also 8).  Fused stepper: xwaves 1, ty 2, 4 or 8
  * rows per wavefront with (4 ty + 2) xvec <= 64 (LDS).  0 for ty, xvec or
  * xwaves: chosen from the grid.  xcd_order: 0 runs tiles in plain order, 1
- * deals each XCD a contiguous range of tiles, 2 (split only) deals each XCD
- * a band of rows within every z chunk.  An invalid shape is rejected and
- * the previous one kept. */
+ * deals each XCD a contiguous range of tiles, 2 (split and halo only) deals
+ * each XCD a band of rows within every z chunk, 3 + b (split and halo only,
+ * b = 0..3) deals the XCDs bands of 2^b row blocks in turn, so that an
+ * XCD's consecutive blocks are x and y neighbours.  An invalid shape is
+ * rejected and the previous one kept. */
 int fdtd_sim_set_launch(FdtdSim *s, int ty, int zchunk, int xvec, int xwaves, int xcd_order);
 /* No counterpart in the reference.  Run nsteps and report HIP-event times
  * on the library stream: whole run and per-kernel averages (ms), the latter

@@ -399,19 +399,22 @@ static int group_build(Group &G) {
     CHECK(G.members.size() < (1u << 16), "a group holds at most 65535 simulations");
     CHECK(xv == 0 || xv == 1 || xv == 2 || xv == 4, "xvec must be 0, 1, 2 or 4 (got %d)", xv);
     CHECK(xw == 0 || xw == 1 || xw == 2 || xw == 4, "xwaves must be 0, 1, 2 or 4 (got %d)", xw);
-    CHECK(G.launch.xcd >= 0 && G.launch.xcd <= 2, "block order must be 0, 1 or 2 (got %d)", G.launch.xcd);
+    CHECK(G.launch.xcd >= 0 && G.launch.xcd <= kXcdFine + 3,
+          "block order must be 0 (plain), 1 (XCD-contiguous), 2 (banded) or 3-6 (fine bands of 1-8 row blocks) "
+          "(got %d)", G.launch.xcd);
+    const int band = G.launch.xcd >= kXcdFine ? 1 << (G.launch.xcd - kXcdFine) : 0;
     if (G.fused == 2) {
         CHECK(ty == 3 || ty == 7, "row-parallel stepper: ty (rows per block) must be 3 or 7 (got %d)", ty);
         CHECK(xw <= 1, "row-parallel stepper: xwaves must be 1");
         CHECK(xv == 0 || xv == 1 || xv == 2, "row-parallel stepper: xvec must be 1 or 2 (got %d)", xv);
-        CHECK(G.launch.xcd != 2, "row-parallel stepper: block order must be 0 (plain) or 1 (XCD-contiguous)");
+        CHECK(G.launch.xcd < 2, "row-parallel stepper: block order must be 0 (plain) or 1 (XCD-contiguous)");
         xw = 1;
         G.xvec = xv ? xv : 1;
     } else if (G.fused == 1) {
         CHECK(ty == 2 || ty == 4 || ty == 8, "fused stepper: ty (rows per wavefront) must be 2, 4 or 8 (got %d)",
               ty);
         CHECK(xw <= 1, "fused stepper: one wavefront per strip (xwaves must be 1)");
-        CHECK(G.launch.xcd != 2, "fused stepper: block order must be 0 (plain) or 1 (XCD-contiguous)");
+        CHECK(G.launch.xcd < 2, "fused stepper: block order must be 0 (plain) or 1 (XCD-contiguous)");
         xw = 1;
         G.xvec = xv ? xv : auto_xvec(G.members, 1);
     } else {
@@ -472,6 +475,21 @@ static int group_build(Group &G) {
             for (int y0 = 0; y0 < s->g.ny; y0 += ty_m)
                 for (int x0 = 0; x0 < s->g.nx; x0 += tx_m)
                     chunk.push_back(make_int4(tag, x0, y0, z0 | (z1 << 16)));
+            if (band) {
+                // fine bands (the tile grid's order, TileGrid): panels of
+                // kXcds bands of `band` row blocks; slot i of a panel runs on
+                // XCD i % kXcds and takes the (i / kXcds)-th tile of its band,
+                // x fastest.  Slots past the last row block are empty tiles.
+                const int ntx = (s->g.nx + tx_m - 1) / tx_m, nyb = (s->g.ny + ty_m - 1) / ty_m;
+                const int per_panel = kXcds * band * ntx, panels = (nyb + kXcds * band - 1) / (kXcds * band);
+                for (int p = 0; p < panels; ++p)
+                    for (int i = 0; i < per_panel; ++i) {
+                        const int kk = i / kXcds, q = kk / ntx;
+                        const int yb = (p * kXcds + i % kXcds) * band + q;
+                        tiles.push_back(yb < nyb ? chunk[(size_t)yb * ntx + (kk - q * ntx)] : make_int4(0, 0, 0, 0));
+                    }
+                continue;
+            }
             if (G.launch.xcd != 2) {
                 tiles.insert(tiles.end(), chunk.begin(), chunk.end());
                 continue;
@@ -519,11 +537,12 @@ static int group_build(Group &G) {
     G.nmodels = (int)descs.size();
     G.tg = TileGrid{};
     G.step1 = nullptr;
-    if (G.members.size() == 1 && (G.fused == 0 || G.fused == 3) && G.launch.xcd == 0) {
+    if (G.members.size() == 1 && (G.fused == 0 || G.fused == 3) && (G.launch.xcd == 0 || band)) {
         const FdtdSim *m = G.members[0];
         const int zc = G.launch.zchunk > 0 ? G.launch.zchunk : m->g.nz;
-        G.tg = TileGrid{(m->g.nx + tx - 1) / tx, (m->g.ny + tys - 1) / tys, tx, tys, zc, m->g.nz, 0, 0};
-        CHECK((long long)G.tg.ntx * G.tg.nyb * ((m->g.nz + zc - 1) / zc) == G.ntiles, "tile grid mismatch");
+        G.tg = TileGrid{(m->g.nx + tx - 1) / tx, (m->g.ny + tys - 1) / tys, tx, tys, zc, m->g.nz, 0, band};
+        CHECK((long long)tilegrid_gx(G.tg) * tilegrid_gy(G.tg) * ((m->g.nz + zc - 1) / zc) == G.ntiles,
+              "tile grid mismatch");
         G.step1 = m->h.step;
     }
     G.slabs.assign(1, {0, G.ntiles});
@@ -539,6 +558,10 @@ static int group_build(Group &G) {
         const int zc = G.launch.zchunk > 0 ? G.launch.zchunk : m->g.nz;
         int per_chunk = (int)(((m->g.ny + tys - 1) / tys) * ((m->g.nx + tx - 1) / tx));
         if (G.launch.xcd == 2) per_chunk = (per_chunk + kXcds - 1) / kXcds * kXcds;
+        if (band) {
+            const TileGrid tg{(m->g.nx + tx - 1) / tx, (m->g.ny + tys - 1) / tys, tx, tys, zc, m->g.nz, 0, band};
+            per_chunk = tilegrid_gx(tg) * tilegrid_gy(tg);
+        }
         const long long chunk_bytes = 32LL * zc * m->g.ny * m->g.nx;
         const int chunks = (int)std::max(1LL, sb / chunk_bytes);
         G.slabs.clear();
@@ -580,9 +603,9 @@ static void launch_kernel(Group &G, KernelId k, int t0 = 0, int t1 = -1, hipStre
     TileGrid tg = G.tg;
     dim3 sgrid((unsigned)nt);
     if (tg.ntx > 0) {
-        const int per = tg.ntx * tg.nyb;
+        const int gx = tilegrid_gx(tg), gy = tilegrid_gy(tg), per = gx * gy;
         tg.t0 = t0 / per;
-        sgrid = dim3((unsigned)tg.ntx, (unsigned)tg.nyb, (unsigned)(nt / per));
+        sgrid = dim3((unsigned)gx, (unsigned)gy, (unsigned)(nt / per));
     }
     if (k == KF && G.fused == 3) {
         if (G.xvec == 2)

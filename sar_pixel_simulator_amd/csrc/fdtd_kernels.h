@@ -89,19 +89,38 @@ struct Launch {
     int zchunk = 2;  // z cells marched per tile (0 = whole column)
     int xvec = 0;    // x cells per lane
     int xw = 0;      // split stepper: wavefronts side by side along x per block
-    int xcd = 0;     // 1: XCD-contiguous block order, 0: plain block order
+    int xcd = 0;     // 0: plain block order, 1: XCD-contiguous, 2: banded,
+                     // 3 + b: fine bands of 2^b row blocks per XCD
 };
+
+// Block order xcd >= kXcdFine: fine bands of 2^(xcd - kXcdFine) row blocks.
+constexpr int kXcdFine = 3;
 
 // Shape word bit of the split and strip kernels: XCD-contiguous block order.
 constexpr int kShapeXcd = 4;
 
 // Tile grid of a single simulation's split-kernel launch (ntx > 0), a 3-D
-// grid of ntx x nyb x (z chunks) blocks: block (i, j, k) covers x tile i of
-// tx cells, rows j ty .. j ty + ty - 1 and z chunk t0 + k of zc planes.
+// grid of blocks over z chunks t0 + k of zc planes.  band = 0 (plain
+// order): ntx x nyb x (z chunks) blocks, block (i, j, k) covers x tile i of
+// tx cells and rows j ty .. j ty + ty - 1.  band > 0 (fine bands): the
+// chunk's row blocks form panels of kXcds bands of `band` row blocks, and
+// block (i, p, k) of a (kXcds band ntx) x (panels) x (z chunks) grid runs
+// on XCD i % kXcds (the dispatcher deals workgroups round-robin), taking
+// the (i / kXcds)-th tile, x fastest, of band i % kXcds of panel p.  An
+// XCD's consecutive blocks then hold x neighbours and, ntx blocks apart,
+// y neighbours, so the rows and columns a tile re-reads of its neighbours
+// pass through one L2, while all XCDs stream through the same narrow
+// window of rows.  Blocks past the last row block are empty.
 // ntx = 0: tiles come from the group's list.
 struct TileGrid {
-    int ntx, nyb, tx, ty, zc, nz, t0, pad;
+    int ntx, nyb, tx, ty, zc, nz, t0, band;
 };
+
+// Blocks of one z chunk of a tile grid (the launch's x extent times its y extent).
+__host__ __device__ inline int tilegrid_gx(const TileGrid &g) { return g.band ? kXcds * g.band * g.ntx : g.ntx; }
+__host__ __device__ inline int tilegrid_gy(const TileGrid &g) {
+    return g.band ? (g.nyb + kXcds * g.band - 1) / (kXcds * g.band) : g.nyb;
+}
 
 // Buffer set holding the fields at step t.
 __host__ __device__ inline int current_set(const Desc &d, int t) { return d.fused ? (t & 1) : 0; }

@@ -688,7 +688,16 @@ __device__ __forceinline__ bool split_tile(const int4 *__restrict__ tiles, int n
                                            const TileGrid &tg, int4 &tl) {
     if (DIRECT) {
         const int z0 = (int)(blockIdx.z + tg.t0) * tg.zc, z1 = min(tg.nz, z0 + tg.zc);
-        tl = make_int4(0, (int)blockIdx.x * tg.tx, (int)blockIdx.y * tg.ty, z0 | (z1 << 16));
+        int xb = (int)blockIdx.x, yb = (int)blockIdx.y;
+        if (tg.band) {
+            // fine bands: slot i of panel p; XCD i % 8 takes the
+            // (i / 8)-th tile of its band, x fastest
+            const int k = xb / kXcds, q = k / tg.ntx;
+            yb = (yb * kXcds + xb % kXcds) * tg.band + q;
+            xb = k - q * tg.ntx;
+            if (yb >= tg.nyb) return false;
+        }
+        tl = make_int4(0, xb * tg.tx, yb * tg.ty, z0 | (z1 << 16));
         return true;
     }
     tl = tiles[block_tile(shape, ntiles)];

@@ -7,7 +7,9 @@ stepper: blocks of xwaves x ty wavefronts (at most 4), xwaves side by side
 along x on each of ty rows.  Fused stepper: ty in {2, 4, 8} rows walked by
 one wavefront, xwaves 1.  Halo stepper: the split stepper's blocks, 1 or 2
 cells per lane.  xcd_order 1 deals each XCD a contiguous range of
-tiles, 0 runs them in plain order.
+tiles, 0 runs them in plain order, 2 gives each XCD a band of rows per z
+chunk and 3 + b (split and halo steppers) deals the XCDs bands of 2^b row
+blocks in turn.
 ``autotune`` measures candidates on the device and records the best per
 (stepper, arch, grid) in a JSON cache; ``launch_for`` reads it, falling back
 to a default.

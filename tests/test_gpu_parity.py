@@ -102,7 +102,13 @@ def test_fields_psi_phasors_bitwise(soil60, fused):
                                           (3, (1, 0, 1, 4, 0)), (3, (1, 2, 1, 4, 0)),
                                           (3, (2, 3, 2, 2, 1)), (3, (4, 1, 1, 1, 0)),
                                           (3, (1, 5, 2, 4, 0)), (3, (1, 7, 1, 2, 2)),
-                                          (3, (2, 16, 0, 0, 0)), (3, (0, 8, 0, 0, 0))])
+                                          (3, (2, 16, 0, 0, 0)), (3, (0, 8, 0, 0, 0)),
+                                          # fine bands of 1, 2, 4, 8 row blocks per XCD (31 rows:
+                                          # the last panel partly empty)
+                                          (False, (1, 2, 1, 4, 3)), (False, (2, 3, 1, 2, 4)),
+                                          (False, (4, 1, 1, 1, 6)), (3, (2, 8, 1, 2, 4)),
+                                          (3, (1, 5, 1, 4, 3)), (3, (2, 3, 2, 2, 5)),
+                                          (3, (1, 7, 1, 1, 6))])
 def test_launch_shapes_bitwise(soil60, fused, launch):
     pc, Ca, Cb, beam, ref = soil60
     say('case: fused', fused, 'launch', launch)
@@ -190,8 +196,8 @@ def test_steps_past_end_are_noops(soil60, fused):
     sim.destroy()
 
 
-@pytest.mark.parametrize('launch', [(1, 2, 1, 4, 0), (2, 3, 1, 1, 1), (1, 2, 1, 4, 2)],
-                         ids=['direct-grid', 'xcd-order', 'banded'])
+@pytest.mark.parametrize('launch', [(1, 2, 1, 4, 0), (2, 3, 1, 1, 1), (1, 2, 1, 4, 2), (1, 2, 1, 4, 4)],
+                         ids=['direct-grid', 'xcd-order', 'banded', 'fine-bands'])
 def test_slabs_bitwise(soil60, launch):
     """H and E launched slab by slab (FDTD_SLAB_BYTES), several z chunks a
     slab: the same fields as the oracle."""
@@ -469,11 +475,13 @@ def test_split_and_rows_agree_at_largest_size():
                                           (False, (1, 1, 1, 4, 0)), (False, (1, 4, 1, 4, 2)),
                                           (2, None), (False, 'mixed-widths'), (False, 'serial'),
                                           (False, 'serial-split'), (False, 'hybrid'), (2, 'hybrid'),
-                                          (3, None), (3, 'mixed-widths'), (3, 'serial-split')],
+                                          (3, None), (3, 'mixed-widths'), (3, 'serial-split'),
+                                          (False, (1, 2, 1, 4, 4)), (3, (2, 8, 1, 2, 3))],
                          ids=['fused', 'split-auto', 'split-4-per-lane', 'split-4-waves-flat',
                               'split-banded', 'rows', 'split-auto-mixed-widths', 'split-serial',
                               'split-serial-small-lanes', 'split-hybrid', 'rows-hybrid', 'halo-auto',
-                              'halo-auto-mixed-widths', 'halo-serial-small-lanes'])
+                              'halo-auto-mixed-widths', 'halo-serial-small-lanes', 'split-fine-bands',
+                              'halo-fine-bands'])
 def test_batch_equals_solo(fused, launch):
     cases = [H.soil_case(nx=33, ny=31, nz=35, nz_mat=14, steps=30, seed=s) for s in (1, 2)]
     cases.append(H.soil_case(nx=45, ny=39, nz=37, nz_mat=15, steps=30, seed=5))
