@@ -17,9 +17,9 @@
 #include "fdtd_kernels.h"
 
 namespace fdtd {
-template <int V, bool DIRECT>
+template <int V, bool DIRECT, bool FLAT>
 __global__ void k_h_half(const Desc *, const int4 *, int, int, TileGrid, const int *);
-template <int V, bool DIRECT>
+template <int V, bool DIRECT, bool FLAT>
 __global__ void k_e_half(const Desc *, const int4 *, int, int, TileGrid, const int *);
 __global__ void k_dft(const Desc *);
 template <int V, int R>
@@ -150,6 +150,7 @@ struct Group {
     int ntiles = 0, nmodels = 0, mode = FDTD_MODE_MAIN, fused = 0;
     std::vector<std::pair<int, int>> slabs;   // split stepper: tile ranges stepped H then E
     int xvec = 1;                  // x cells per lane of the kernels in use
+    bool flat = false;             // split stepper on flat tiles (kXwFlat)
     int ty = 1, xw = 1;            // rows and x wavefronts per block (split), rows per strip (fused)
     TileGrid tg = {};              // split kernels, single simulation: tiles from the block index
     const int *step1 = nullptr;    // single simulation: its step counter
@@ -368,6 +369,23 @@ static int auto_xvec(const std::vector<FdtdSim *> &members, int xw) {
     return 1;
 }
 
+// Whether flat tiles (kXwFlat) hold the members' cells in at most 95 % of
+// the lane slots of row tiles xw_of(member) wavefronts wide (one cell per
+// lane): narrow rows padded to whole wavefronts leave lanes idle that flat
+// tiles, whose wavefronts run on into the next row, fill.  Every member's
+// row pitch must hold a wavefront (sy >= 64).
+template <typename XwOf>
+static bool flat_pays(const std::vector<FdtdSim *> &members, XwOf xw_of) {
+    double rows_slots = 0, flat_slots = 0;
+    for (const FdtdSim *s : members) {
+        if (s->h.sy < kWave) return false;
+        const long long tw = (long long)kWave * xw_of(s);
+        rows_slots += (double)s->g.nz * s->g.ny * ((s->g.nx + tw - 1) / tw * tw);
+        flat_slots += (double)s->g.nz * (((long long)s->g.ny * s->h.sy + 4 * kWave - 1) / (4 * kWave) * 4 * kWave);
+    }
+    return flat_slots <= 0.95 * rows_slots;
+}
+
 // Widest block (4 or 2 wavefronts along x, else 1) whose tiles of
 // 64 xvec xw cells leave at most 30 % of the lanes idle.
 static int auto_xwaves(const std::vector<FdtdSim *> &members, int v) {
@@ -396,9 +414,19 @@ static int group_build(Group &G) {
     const int xv = G.launch.xvec;
     int ty = G.launch.ty, xw = G.launch.xw;
     bool per_member = false;
+    G.flat = false;
     CHECK(G.members.size() < (1u << 16), "a group holds at most 65535 simulations");
     CHECK(xv == 0 || xv == 1 || xv == 2 || xv == 4, "xvec must be 0, 1, 2 or 4 (got %d)", xv);
-    CHECK(xw == 0 || xw == 1 || xw == 2 || xw == 4, "xwaves must be 0, 1, 2 or 4 (got %d)", xw);
+    CHECK(xw == 0 || xw == 1 || xw == 2 || xw == 4 || xw == kXwFlat,
+          "xwaves must be 0, 1, 2, 4 or -1 (flat tiles) (got %d)", xw);
+    if (xw == kXwFlat) {
+        CHECK(G.members[0]->h.fused == 0, "flat tiles are a split-stepper shape");
+        CHECK(xv == 0 || xv == 1, "flat tiles hold one x cell per lane (xvec 0 or 1, got %d)", xv);
+        CHECK(ty == 0, "flat tiles take no ty (got %d)", ty);
+        CHECK(G.launch.xcd == 0, "flat tiles run in plain block order (got %d)", G.launch.xcd);
+        for (FdtdSim *s : G.members)
+            CHECK(s->h.sy >= kWave, "flat tiles need a row pitch of at least %d cells (nx = %d)", kWave, s->g.nx);
+    }
     CHECK(G.launch.xcd >= 0 && G.launch.xcd <= kXcdFine + 3,
           "block order must be 0 (plain), 1 (XCD-contiguous), 2 (banded) or 3-6 (fine bands of 1-8 row blocks) "
           "(got %d)", G.launch.xcd);
@@ -431,6 +459,21 @@ static int group_build(Group &G) {
         // a batch with a fully automatic shape gives each member its own
         // block width (4-wavefront blocks, 4 / xw rows); its tiles carry it
         per_member = !xw && !ty && !xv && G.members.size() > 1;
+        // a fully automatic split-stepper shape runs on flat tiles where
+        // they fill the lanes better (flat_pays), for one simulation or a
+        // batch (then every member flat)
+        const bool automatic = !xw && !ty && !xv && G.launch.xcd == 0;
+        if (xw == kXwFlat ||
+            (automatic && G.fused == 0 &&
+             flat_pays(G.members, [&](const FdtdSim *s) {
+                 return auto_xwaves(std::vector<FdtdSim *>{const_cast<FdtdSim *>(s)}, 1);
+             }))) {
+            G.flat = true;
+            per_member = false;
+            G.xvec = 1;
+            xw = 4;
+            ty = 1;
+        }
         if (per_member) {
             xw = 4;
             ty = 1;
@@ -469,6 +512,17 @@ static int group_build(Group &G) {
         descs.push_back(s->h);
         const int nz = s->g.nz, zc = G.launch.zchunk > 0 ? G.launch.zchunk : nz;
         std::vector<int4> chunk;
+        if (G.flat) {
+            // flat tiles: blocks of 4 wavefronts over 256 consecutive cells
+            // of the plane's row-major (y, x) index
+            const long long span = (long long)s->g.ny * s->h.sy;
+            CHECK(span < (1LL << 31), "plane of %lld cells too large for flat tiles", span);
+            for (int z0 = 0; z0 < nz; z0 += zc)
+                for (long long f0 = 0; f0 < span; f0 += 4 * kWave)
+                    tiles.push_back(make_int4(tag, (int)f0, 0, z0 | (std::min(nz, z0 + zc) << 16)));
+            G.max_dft_items = std::max(G.max_dft_items, s->h.dft_items);
+            continue;
+        }
         for (int z0 = 0; z0 < nz; z0 += zc) {
             const int z1 = std::min(nz, z0 + zc);
             chunk.clear();
@@ -537,10 +591,17 @@ static int group_build(Group &G) {
     G.nmodels = (int)descs.size();
     G.tg = TileGrid{};
     G.step1 = nullptr;
-    if (G.members.size() == 1 && (G.fused == 0 || G.fused == 3) && (G.launch.xcd == 0 || band)) {
+    if (G.members.size() == 1 && G.flat) {
         const FdtdSim *m = G.members[0];
         const int zc = G.launch.zchunk > 0 ? G.launch.zchunk : m->g.nz;
-        G.tg = TileGrid{(m->g.nx + tx - 1) / tx, (m->g.ny + tys - 1) / tys, tx, tys, zc, m->g.nz, 0, band};
+        const long long span = (long long)m->g.ny * m->h.sy;
+        G.tg = TileGrid{(int)((span + 4 * kWave - 1) / (4 * kWave)), 1, 4 * kWave, 0, zc, m->g.nz, 0, 0, 1};
+        CHECK((long long)G.tg.ntx * ((m->g.nz + zc - 1) / zc) == G.ntiles, "tile grid mismatch");
+        G.step1 = m->h.step;
+    } else if (G.members.size() == 1 && (G.fused == 0 || G.fused == 3) && (G.launch.xcd == 0 || band)) {
+        const FdtdSim *m = G.members[0];
+        const int zc = G.launch.zchunk > 0 ? G.launch.zchunk : m->g.nz;
+        G.tg = TileGrid{(m->g.nx + tx - 1) / tx, (m->g.ny + tys - 1) / tys, tx, tys, zc, m->g.nz, 0, band, 0};
         CHECK((long long)tilegrid_gx(G.tg) * tilegrid_gy(G.tg) * ((m->g.nz + zc - 1) / zc) == G.ntiles,
               "tile grid mismatch");
         G.step1 = m->h.step;
@@ -559,9 +620,10 @@ static int group_build(Group &G) {
         int per_chunk = (int)(((m->g.ny + tys - 1) / tys) * ((m->g.nx + tx - 1) / tx));
         if (G.launch.xcd == 2) per_chunk = (per_chunk + kXcds - 1) / kXcds * kXcds;
         if (band) {
-            const TileGrid tg{(m->g.nx + tx - 1) / tx, (m->g.ny + tys - 1) / tys, tx, tys, zc, m->g.nz, 0, band};
+            const TileGrid tg{(m->g.nx + tx - 1) / tx, (m->g.ny + tys - 1) / tys, tx, tys, zc, m->g.nz, 0, band, 0};
             per_chunk = tilegrid_gx(tg) * tilegrid_gy(tg);
         }
+        if (G.flat) per_chunk = (int)(((long long)m->g.ny * m->h.sy + 4 * kWave - 1) / (4 * kWave));
         const long long chunk_bytes = 32LL * zc * m->g.ny * m->g.nx;
         const int chunks = (int)std::max(1LL, sb / chunk_bytes);
         G.slabs.clear();
@@ -629,19 +691,31 @@ static void launch_kernel(Group &G, KernelId k, int t0 = 0, int t1 = -1, hipStre
         else fn = r == 8 ? &k_step_strip<1, 8> : r == 4 ? &k_step_strip<1, 4> : &k_step_strip<1, 2>;
         hipLaunchKernelGGL(fn, grid, b4, 0, st, G.d_descs, G.d_tiles, G.ntiles, shape);
     } else if (k == KH) {
-        if (G.xvec == 4)
-            launch_split(k_h_half<4, true>, k_h_half<4, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt, shape, tg);
+        if (G.flat)
+            launch_split(k_h_half<1, true, true>, k_h_half<1, false, true>, tg.ntx > 0, sgrid, blk, G, st, tl, nt,
+                         shape, tg);
+        else if (G.xvec == 4)
+            launch_split(k_h_half<4, true, false>, k_h_half<4, false, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt,
+                         shape, tg);
         else if (G.xvec == 2)
-            launch_split(k_h_half<2, true>, k_h_half<2, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt, shape, tg);
+            launch_split(k_h_half<2, true, false>, k_h_half<2, false, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt,
+                         shape, tg);
         else
-            launch_split(k_h_half<1, true>, k_h_half<1, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt, shape, tg);
+            launch_split(k_h_half<1, true, false>, k_h_half<1, false, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt,
+                         shape, tg);
     } else if (k == KE) {
-        if (G.xvec == 4)
-            launch_split(k_e_half<4, true>, k_e_half<4, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt, shape, tg);
+        if (G.flat)
+            launch_split(k_e_half<1, true, true>, k_e_half<1, false, true>, tg.ntx > 0, sgrid, blk, G, st, tl, nt,
+                         shape, tg);
+        else if (G.xvec == 4)
+            launch_split(k_e_half<4, true, false>, k_e_half<4, false, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt,
+                         shape, tg);
         else if (G.xvec == 2)
-            launch_split(k_e_half<2, true>, k_e_half<2, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt, shape, tg);
+            launch_split(k_e_half<2, true, false>, k_e_half<2, false, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt,
+                         shape, tg);
         else
-            launch_split(k_e_half<1, true>, k_e_half<1, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt, shape, tg);
+            launch_split(k_e_half<1, true, false>, k_e_half<1, false, false>, tg.ntx > 0, sgrid, blk, G, st, tl, nt,
+                         shape, tg);
     } else {
         const unsigned gx = (unsigned)std::max(1LL, (G.max_dft_items + 255) / 256);
         hipLaunchKernelGGL(k_dft, dim3(gx, G.nmodels), dim3(256), 0, st, G.d_descs);

@@ -111,9 +111,11 @@ constexpr int kShapeXcd = 4;
 // y neighbours, so the rows and columns a tile re-reads of its neighbours
 // pass through one L2, while all XCDs stream through the same narrow
 // window of rows.  Blocks past the last row block are empty.
-// ntx = 0: tiles come from the group's list.
+// flat = 1 (flat tiles, band 0): ntx x 1 x (z chunks) blocks, block i
+// covering cells i tx .. i tx + tx - 1 of the plane's row-major (y, x)
+// index.  ntx = 0: tiles come from the group's list.
 struct TileGrid {
-    int ntx, nyb, tx, ty, zc, nz, t0, band;
+    int ntx, nyb, tx, ty, zc, nz, t0, band, flat;
 };
 
 // Blocks of one z chunk of a tile grid (the launch's x extent times its y extent).
@@ -121,6 +123,12 @@ __host__ __device__ inline int tilegrid_gx(const TileGrid &g) { return g.band ? 
 __host__ __device__ inline int tilegrid_gy(const TileGrid &g) {
     return g.band ? (g.nyb + kXcds * g.band - 1) / (kXcds * g.band) : g.nyb;
 }
+
+// Launch::xw value selecting flat tiles for the split stepper: blocks of 4
+// wavefronts, each over 64 consecutive cells of a plane's row-major (y, x)
+// index (rows padded to the pitch share wavefronts).  Needs one x cell per
+// lane and a row pitch of at least 64 floats.
+constexpr int kXwFlat = -1;
 
 // Buffer set holding the fields at step t.
 __host__ __device__ inline int current_set(const Desc &d, int t) { return d.fused ? (t & 1) : 0; }

@@ -679,6 +679,37 @@ __device__ __forceinline__ void e_tap(gchar *tap, int xb, int nx, int np, int nz
     }
 }
 
+// Cell of a split-kernel lane (one x cell per lane in flat tiles).  Row
+// tiles: the wavefront's row is y = tl.z + w / xw (uniform), its x range
+// starts at the tile's x0.  Flat tiles (FLAT): wavefront w of the block
+// covers the 64 consecutive cells fw .. fw + 63 of the plane's row-major
+// (y, x) index, fw = tl.y + 64 w; rows are padded to the pitch sy >= 64, so
+// a wavefront spans at most two rows, and the lanes past the row's last
+// cell (x >= nx) or past the last row (y >= ny) hold no cell.  Narrow rows
+// then share wavefronts instead of each being padded to whole ones.
+// Returns false when the whole wavefront lies past the grid.
+template <int V, bool FLAT>
+__device__ __forceinline__ bool lane_cell(const int4 &tl, int w, int lxw, int ny, uint32_t sy, int &y, int &xb) {
+    static_assert(V == 1 || !FLAT, "flat tiles hold one x cell per lane");
+    const int lane = threadIdx.x;
+    if (FLAT) {
+        const int fw = tl.y + (w << 6);
+        const int yw = __builtin_amdgcn_readfirstlane(fw / (int)sy);
+        if (yw >= ny) return false;                    // uniform per wavefront
+        y = yw;
+        xb = fw - yw * (int)sy + lane;
+        if (xb >= (int)sy) {
+            xb -= (int)sy;
+            ++y;
+        }
+        return true;
+    }
+    y = tl.z + (w >> lxw);
+    if (y >= ny) return false;                         // uniform per wavefront (one row)
+    xb = tl.y + V * (((w & ((1 << lxw) - 1)) << 6) + lane);
+    return true;
+}
+
 // Tile of a split-kernel block: for a single simulation (DIRECT), from
 // the 3-D grid (x tile, row block, z chunk: x fastest, as the plain order;
 // no dependent load); else from the group's tile list.  Returns false for a
@@ -688,6 +719,11 @@ __device__ __forceinline__ bool split_tile(const int4 *__restrict__ tiles, int n
                                            const TileGrid &tg, int4 &tl) {
     if (DIRECT) {
         const int z0 = (int)(blockIdx.z + tg.t0) * tg.zc, z1 = min(tg.nz, z0 + tg.zc);
+        if (tg.flat) {
+            // flat tiles: block i covers the plane's cells from i tx on
+            tl = make_int4(0, (int)blockIdx.x * tg.tx, 0, z0 | (z1 << 16));
+            return true;
+        }
         int xb = (int)blockIdx.x, yb = (int)blockIdx.y;
         if (tg.band) {
             // fine bands: slot i of panel p; XCD i % 8 takes the
@@ -710,7 +746,7 @@ __device__ __forceinline__ bool split_tile(const int4 *__restrict__ tiles, int n
 // after that; no field store precedes the test (the H kernel's one-lane copy
 // of the counter for k_dft does, note_step).  The z loop then computes a
 // plane, stores it and loads the next one.
-template <int V, bool DIRECT>
+template <int V, bool DIRECT, bool FLAT>
 __global__ void __launch_bounds__(256)
 k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles, int shape, TileGrid tg,
          const int *__restrict__ stepp) {
@@ -726,18 +762,17 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     scalars_resident(kg, st);
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     note_step(d, tl, w, t);
-    const int lxw = DIRECT ? shape & 3 : tl.x >> 16;   // listed tiles carry their block width
-    const int y = tl.z + (w >> lxw);
+    const int lxw = DIRECT ? shape & 3 : (tl.x >> 16) & 3;   // listed tiles carry their block width
     const int nx = d->nx, ny = d->ny, nz = d->nz;
-    if (y >= ny) return;                               // uniform per wavefront (one row)
+    const uint32_t sy = (uint32_t)d->sy, sz = (uint32_t)d->sz;
+    int y, xb;                                         // the lane's row and first cell
+    if (!lane_cell<V, FLAT>(tl, w, lxw, ny, sy, y, xb)) return;
     const int lane = threadIdx.x;
-    const int xb = tl.y + V * (((w & ((1 << lxw) - 1)) << 6) + lane);   // the lane's first cell
-    const bool act = xb < nx;
+    const bool act = xb < nx && (!FLAT || y < ny);
     // the x + 1 neighbour of the lane's last cell is needed when that cell
     // is below nx - 1; for lane 63 it lies in the next tile
-    const bool next_mem = lane == kWave - 1 && xb + V < nx;
+    const bool next_mem = lane == kWave - 1 && xb + V < nx && act;
     const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
-    const uint32_t sy = (uint32_t)d->sy, sz = (uint32_t)d->sz;
     const gfloat *Ex = gp(d->f[0][0]), *Ey = gp(d->f[0][1]), *Ez = gp(d->f[0][2]);
     gfloat *Hx = gp(d->f[0][3]), *Hy = gp(d->f[0][4]), *Hz = gp(d->f[0][5]);
     gfloat *psiH = gp(d->psi[0][pHxz_d]);
@@ -757,7 +792,7 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
 #pragma unroll
     for (int c = 0; c < V; ++c) {
         h_xprof_load(pq[c], d, kg, xb + c);
-        if (xb + c < nx) h_psi_load(pq[c], kg, psiH, xb + c, y, z0);
+        if (xb + c < nx && act) h_psi_load(pq[c], kg, psiH, xb + c, y, z0);
     }
     if (t >= st.nt) return;                            // uniform per block
     const StepK k = step_at(kg, st, t);
@@ -770,7 +805,7 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
         }
 #pragma unroll
         for (int c = 0; c < V; ++c) {
-            if (xb + c < k.nx) {
+            if (xb + c < k.nx && act) {
                 EAround e;
                 e.ex_c = exc[c]; e.ex_n = P.exn[c]; e.ex_yp = P.exyp[c];
                 e.ey_c = eyc[c]; e.ey_n = P.eyn[c];
@@ -792,7 +827,7 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
         h_plane_load(P, Ex, Ey, Ez, Hx, Hy, Hz, o, sy, sz, act, z < nz - 1, yin, next_mem);
 #pragma unroll
         for (int c = 0; c < V; ++c)
-            if (xb + c < nx) h_psi_load(pq[c], k, psiH, xb + c, y, z);
+            if (xb + c < nx && act) h_psi_load(pq[c], k, psiH, xb + c, y, z);
     }
 }
 
@@ -839,7 +874,7 @@ __device__ __forceinline__ void e_plane_load(EPlane<V> &Q, const gfloat *Ex, con
     }
 }
 
-template <int V, bool DIRECT>
+template <int V, bool DIRECT, bool FLAT>
 __global__ void __launch_bounds__(256)
 k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int ntiles, int shape, TileGrid tg,
          const int *__restrict__ stepp) {
@@ -854,23 +889,30 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     const StepTime st = step_time(d);
     scalars_resident(kg, st);
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
-    const int lxw = DIRECT ? shape & 3 : tl.x >> 16;   // listed tiles carry their block width
-    const int y = tl.z + (w >> lxw);
+    const int lxw = DIRECT ? shape & 3 : (tl.x >> 16) & 3;   // listed tiles carry their block width
     const int nx = d->nx, ny = d->ny, nz = d->nz;
-    if (y >= ny) return;                               // uniform per wavefront (one row)
+    const uint32_t sy = (uint32_t)d->sy, sz = (uint32_t)d->sz;
+    int y, xb;                                         // the lane's row and first cell
+    if (!lane_cell<V, FLAT>(tl, w, lxw, ny, sy, y, xb)) return;
     const int lane = threadIdx.x;
-    const int xb = tl.y + V * (((w & ((1 << lxw) - 1)) << 6) + lane);
-    const bool act = xb < nx;
+    const bool act = xb < nx && (!FLAT || y < ny);
     const bool prev_mem = lane == 0 && xb >= 1 && act;    // x - 1 lies in the previous tile
     const int z0 = tl.w & 0xffff, z1 = tl.w >> 16;
-    const uint32_t sy = (uint32_t)d->sy, sz = (uint32_t)d->sz;
     gfloat *Ex = gp(d->f[0][0]), *Ey = gp(d->f[0][1]), *Ez = gp(d->f[0][2]);
     const gfloat *Hx = gp(d->f[0][3]), *Hy = gp(d->f[0][4]), *Hz = gp(d->f[0][5]);
     const gfloat *Ca = gp(d->ca), *Cb = gp(d->cb);
     const bool ylo = y >= 1;
     const int uni_z = d->uni_z;
-    const int x0w = tl.y + V * ((w & ((1 << lxw) - 1)) << 6);   // the wavefront's first cell
-    gchar *const tap = d->tapx && wave_taps<V>(x0w, nx, d->npml) ? (gchar *)d->tapx : nullptr;
+    // the wavefronts whose cells include a DFT x-face column copy it
+    bool taps;
+    if (FLAT) {
+        const int np = d->npml;
+        taps = __ballot(act && (xb == np + 1 || xb == nx - np - 1)) != 0;
+    } else {
+        const int x0w = tl.y + V * ((w & ((1 << lxw) - 1)) << 6);   // the wavefront's first cell
+        taps = wave_taps<V>(x0w, nx, d->npml);
+    }
+    gchar *const tap = d->tapx && taps ? (gchar *)d->tapx : nullptr;
     const float ca_u = d->ca_u, cb_u = d->cb_u;
 
     uint32_t o = (uint32_t)z0 * sz + (uint32_t)y * sy + (uint32_t)xb;
@@ -888,7 +930,7 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
 #pragma unroll
     for (int c = 0; c < V; ++c) {
         e_xprof_load(pq[c], d, kg, xb + c);
-        if (xb + c < nx) e_psi_load(pq[c], kg, kg.psiE, xb + c, y, z0);
+        if (xb + c < nx && act) e_psi_load(pq[c], kg, kg.psiE, xb + c, y, z0);
     }
     if (t >= st.nt) return;                            // uniform per block
     const StepK k = step_at(kg, st, t);
@@ -901,7 +943,7 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
         }
 #pragma unroll
         for (int c = 0; c < V; ++c) {
-            if (xb + c < k.nx) {
+            if (xb + c < k.nx && act) {
                 HAround h;
                 h.hx_c = P.hx[c]; h.hy_c = P.hy[c]; h.hz_c = P.hz[c];
                 h.hx_zm = hxzm[c]; h.hy_zm = hyzm[c];
@@ -916,7 +958,7 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
             vst_stream(at(Ey, o), P.ey);
             vst_stream(at(Ez, o), P.ez);
         }
-        if (tap) e_tap<V>(tap, xb, nx, k.np, nz, ny, z, y, P.ey, P.ez, P.hy, P.hz, hy_prev, hz_prev);
+        if (tap && act) e_tap<V>(tap, xb, nx, k.np, nz, ny, z, y, P.ey, P.ez, P.hy, P.hz, hy_prev, hz_prev);
         if (++z >= z1) break;
         o += sz;
         vcopy(hxzm, P.hx);
@@ -925,22 +967,26 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
                      z < nz - 1 && z < uni_z, ylo, prev_mem);
 #pragma unroll
         for (int c = 0; c < V; ++c)
-            if (xb + c < nx) e_psi_load(pq[c], k, k.psiE, xb + c, y, z);
+            if (xb + c < nx && act) e_psi_load(pq[c], k, k.psiE, xb + c, y, z);
     }
 }
 
-template __global__ void k_h_half<1, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
-template __global__ void k_h_half<1, true>(const Desc *, const int4 *, int, int, TileGrid, const int *);
-template __global__ void k_h_half<2, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
-template __global__ void k_h_half<2, true>(const Desc *, const int4 *, int, int, TileGrid, const int *);
-template __global__ void k_h_half<4, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
-template __global__ void k_h_half<4, true>(const Desc *, const int4 *, int, int, TileGrid, const int *);
-template __global__ void k_e_half<1, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
-template __global__ void k_e_half<1, true>(const Desc *, const int4 *, int, int, TileGrid, const int *);
-template __global__ void k_e_half<2, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
-template __global__ void k_e_half<2, true>(const Desc *, const int4 *, int, int, TileGrid, const int *);
-template __global__ void k_e_half<4, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
-template __global__ void k_e_half<4, true>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_h_half<1, false, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_h_half<1, true, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_h_half<2, false, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_h_half<2, true, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_h_half<4, false, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_h_half<4, true, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_h_half<1, false, true>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_h_half<1, true, true>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_e_half<1, false, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_e_half<1, true, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_e_half<2, false, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_e_half<2, true, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_e_half<4, false, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_e_half<4, true, false>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_e_half<1, false, true>(const Desc *, const int4 *, int, int, TileGrid, const int *);
+template __global__ void k_e_half<1, true, true>(const Desc *, const int4 *, int, int, TileGrid, const int *);
 
 // ---------------------------------------------------------------------------
 // Fused stepper (MAIN): one kernel per time step computes H(t+1/2) and then

@@ -6,7 +6,9 @@ grid width) and marches z over zchunk cells (0 = whole column).  Split
 stepper: blocks of xwaves x ty wavefronts (at most 4), xwaves side by side
 along x on each of ty rows.  Fused stepper: ty in {2, 4, 8} rows walked by
 one wavefront, xwaves 1.  Halo stepper: the split stepper's blocks, 1 or 2
-cells per lane.  xcd_order 1 deals each XCD a contiguous range of
+cells per lane.  xwaves -1 (split stepper, ty 0, one cell per lane): flat
+tiles, each wavefront over 64 consecutive cells of a plane's row-major
+(y, x) index, so that narrow rows share wavefronts.  xcd_order 1 deals each XCD a contiguous range of
 tiles, 0 runs them in plain order, 2 gives each XCD a band of rows per z
 chunk and 3 + b (split and halo steppers) deals the XCDs bands of 2^b row
 blocks in turn.
@@ -23,7 +25,8 @@ CANDIDATES = {
               for r in rows for zc in (4, 8, 16, 32) for xo in (1, 0)],
     'split': [(ty, zc, xv, 1, 1) for xv in (4, 2, 1) for ty in (1, 2, 4) for zc in (8, 16, 32)]
              + [(ty, zc, xv, xw, xo) for xv in (1, 2) for xw, ty in ((4, 1), (2, 1), (2, 2))
-                for zc in (1, 2, 3, 4, 6, 8, 16) for xo in (0, 1)],
+                for zc in (1, 2, 3, 4, 6, 8, 16) for xo in (0, 1)]
+             + [(0, zc, 1, -1, 0) for zc in (1, 2, 3, 4, 6)],
     'rows': [(r, zc, xv, 1, xo) for r in (3, 7) for zc in (4, 8, 16, 32) for xv in (1, 2)
              for xo in (0, 1)],
     'halo': [(ty, zc, xv, xw, 0) for xv in (1, 2) for xw, ty in ((4, 1), (2, 1), (2, 2), (1, 4))

@@ -159,9 +159,10 @@ def taps139():
 
 @pytest.mark.parametrize('fused,launch', [(False, None), (False, (1, 2, 1, 4, 0)),
                                           (False, (1, 3, 2, 2, 0)), (False, (2, 5, 4, 2, 1)),
+                                          (False, (0, 3, 1, -1, 0)),
                                           (True, None), (2, None), (3, None), (3, (1, 3, 2, 2, 0))],
                          ids=['split-auto', 'split-1-cell', 'split-2-cells', 'split-4-cells-xcd',
-                              'fused', 'rows', 'halo', 'halo-2-cells'])
+                              'split-flat', 'fused', 'rows', 'halo', 'halo-2-cells'])
 def test_compensated_x_faces_close_and_closer_to_exact(taps139, fused, launch):
     """The shipped default (split stepper, compensated DFT) at phasor level
     where the x-face taps sit on wavefront edges: every phasor within 1e-5
@@ -287,7 +288,7 @@ def test_x_face_taps_bitwise(nx):
     for n in ('ey_ph_l', 'hy_ph_l', 'ez_ph_r', 'hz_ph_r'):
         assert np.abs(oph[n]).max() > 1e-4, n
     for launch in ((1, 2, 1, 4, 0), (1, 3, 1, 1, 0), (2, 2, 2, 2, 0), (1, 4, 2, 4, 0),
-                   (4, 0, 4, 1, 0), (2, 5, 4, 2, 1), (1, 1, 1, 4, 2)):
+                   (4, 0, 4, 1, 0), (2, 5, 4, 2, 1), (1, 1, 1, 4, 2), (0, 2, 1, -1, 0), (0, 3, 0, -1, 0)):
         say('  launch', launch)
         sim = make_sim(pc, Ca, Cb, beam, launch=launch, fused=False)
         sim.run(150)
@@ -324,6 +325,97 @@ def test_minimum_interior_bitwise(nx):
         sim.destroy()
 
 
+# flat tiles: wavefronts over 64 consecutive cells of a plane's row-major
+# (y, x) index, running on into the next row (pitches 112, 80 and 144; at
+# 139 cells the right DFT x face, x = 128, is a wavefront's first lane in
+# the first row and mid-wavefront in others); None: the automatic shape,
+# which takes flat tiles for these widths
+@pytest.fixture(scope='module', params=[(101, 23, 27, 9), (67, 29, 31, 12), (139, 27, 29, 11)],
+                ids=['101', '67', '139'])
+def flat_case(request):
+    nx, ny, nz, nz_mat = request.param
+    pc, Ca, Cb = H.soil_case(nx=nx, ny=ny, nz=nz, nz_mat=nz_mat, steps=50, seed=nx,
+                             dft_accumulation='reference')
+    beam = H.synthetic_beam(pc)
+    return pc, Ca, Cb, beam, H.oracle_main(pc, Ca, Cb, beam)
+
+
+@pytest.mark.parametrize('launch', [None, (0, 2, 1, -1, 0), (0, 5, 0, -1, 0), (0, 0, 1, -1, 0),
+                                    (0, 1, 1, -1, 0)],
+                         ids=['auto', 'flat-2', 'flat-5', 'flat-column', 'flat-1'])
+def test_flat_tiles_bitwise(flat_case, launch):
+    pc, Ca, Cb, beam, ref = flat_case
+    g = pc.geom_params
+    say('case: flat tiles', g.size_x, 'x', g.size_y, 'x', g.size_z, 'launch', launch)
+    sim = make_sim(pc, Ca, Cb, beam, launch=launch, fused=False)
+    sim.run(50)
+    sim.sync()
+    for n in native.FIELD_NAMES:
+        assert_same(sim.field(n), ref.fields[n], f'{n} {launch}')
+    for i, n in enumerate(native.PSI_NAMES):
+        assert_same(sim.psi(i), ref.psi[n], f'{n} {launch}')
+    ph, oph = sim.phasors(), ref.phasors()
+    for n in native.PHASOR_NAMES:
+        assert_same(ph[n], oph[n], f'{n} {launch}')
+    sim.destroy()
+
+
+def test_flat_batch_equals_solo():
+    """Members of widths 101, 67 and 139 (pitches 112, 80, 144) stepped as
+    one batch on flat tiles (automatic shape, explicit flat tiles, and each
+    member's own launches on lanes): bit-identical to each run alone on row
+    tiles."""
+    cases = [H.soil_case(nx=nx, ny=ny, nz=nz, nz_mat=m, steps=30, seed=nx)
+             for nx, ny, nz, m in ((101, 23, 27, 9), (67, 29, 31, 12), (139, 27, 29, 11))]
+    solo, sims = [], []
+    for pc, Ca, Cb in cases:
+        beam = H.synthetic_beam(pc)
+        s = make_sim(pc, Ca, Cb, beam, launch=(1, 2, 1, 1, 0))
+        s.run(30)
+        s.sync()
+        solo.append({n: s.field(n) for n in native.FIELD_NAMES})
+        solo[-1].update(s.phasors())
+        s.destroy()
+        sims.append(make_sim(pc, Ca, Cb, beam))
+    for launch, mode in ((None, None), ((0, 4, 0, -1, 0), None), (None, (native.BATCH_SERIAL, 0, 2))):
+        say('case: flat batch, launch', launch, 'mode', mode)
+        for s in sims:
+            s.reset()
+        b = native.Batch(sims)
+        if launch:
+            b.set_launch(*launch)
+        if mode:
+            b.set_mode(*mode)
+        b.run(11, use_graph=False)
+        b.run(19, use_graph=True)
+        b.sync()
+        for k, s in enumerate(sims):
+            assert s.step_count() == 30
+            for n in native.FIELD_NAMES:
+                assert_same(s.field(n), solo[k][n], f'member {k} {n} {launch} {mode}')
+            ph = s.phasors()
+            for n in native.PHASOR_NAMES:
+                assert_same(ph[n], solo[k][n], f'member {k} {n} {launch} {mode}')
+        b.destroy()
+    for s in sims:
+        s.destroy()
+
+
+def test_incident_run_flat_bitwise():
+    # 67 cells wide (pitch 80): the incident run's automatic shape takes flat tiles
+    pc = H.soil_case(nx=67, ny=33, nz=35, nz_mat=14, dft_accumulation='reference')[0]
+    say('case: incident run 67x33 (flat tiles), nt', pc.geom_params.size_t_inc)
+    pex, pey = H.excitation(pc)
+    rec, oph = H.oracle_incident(pc, pex, pey)
+    from sar_pixel_simulator_amd.incidentbeam import run_incident
+    beam, ph = run_incident(pc, pex, pey, device=0)
+    for c in range(4):
+        assert_same(beam.component(c), rec[c], f'beam component {c}')
+    for n in native.INC_PHASOR_NAMES:
+        assert_same(ph[n], oph[n], n)
+    beam.release()
+
+
 def test_invalid_launch_is_rejected_and_previous_kept(soil60):
     pc, Ca, Cb, beam, ref = soil60
     sim = make_sim(pc, Ca, Cb, beam, launch=(2, 0, 4, 1, 1), fused=False)
@@ -336,8 +428,12 @@ def test_invalid_launch_is_rejected_and_previous_kept(soil60):
     with pytest.raises(native.FdtdError):
         sim.set_launch(1, 0, 1, 3, 1)
     with pytest.raises(native.FdtdError):
-        sim.set_launch(1, 4, 1, 4, 3)    # block orders are 0, 1, 2
+        sim.set_launch(1, 4, 1, 4, 7)    # block orders are 0 - 6
+    with pytest.raises(native.FdtdError):
+        sim.set_launch(0, 2, 1, -1, 0)   # flat tiles need a row pitch of 64 (33 cells: 48)
     fsim = make_sim(pc, Ca, Cb, beam, fused=True)
+    with pytest.raises(native.FdtdError):
+        fsim.set_launch(0, 2, 1, -1, 0)  # flat tiles are a split-stepper shape
     with pytest.raises(native.FdtdError):
         fsim.set_launch(8, 0, 2, 1, 1)         # 8 rows of 2-cell lanes exceed the fused LDS budget
     with pytest.raises(native.FdtdError):
