@@ -38,6 +38,46 @@ def batched_workload(count, seed=1000):
     return cases
 
 
+def _batched_reference_loop(n, beams, seed=1000, log=print):
+    """Models/hour of the reference-style loop (experiment.py:69-97 with the
+    HIP backend) over the first ``n`` models of the batched workload: per
+    model its host soil volume (workloads.batched_material, as the
+    reference's soil generators yield models), compute_material_arrays,
+    MainSimHip.run (upload, all steps, phasors to the host, results file)
+    and process_results (far field, powers, sigma0 into the file).  The
+    incident beams are the resident ones of ``beams`` (generated beforehand,
+    as in every leg)."""
+    import tempfile
+    from ..incidentbeam import forget_resident, make_resident
+    from ..processresults import process_results
+    from ..parameters import results_filename
+    from .main_sim import MainSimHip
+    t_total = 0.0
+    done = 0
+    with tempfile.TemporaryDirectory(prefix='fdtd_refloop_') as proj:
+        for i, pc in workloads.batched_params_list(count=(n + 1) // 2, seed=seed):
+            if done == n:
+                break
+            pc.processing_params.project_dir = proj
+            beam, ph = beams.lookup(pc)
+            make_resident(pc, beam, ph)
+            name = f'soil_{i}_{pc.inc_beam_params.polarization}'
+            t0 = time.perf_counter()
+            m = workloads.batched_material(pc, i)
+            Ca, Cb = compute_material_arrays(pc, m)
+            sim = MainSimHip(pc)
+            sim.run(Ca, Cb, name, {})
+            sim.close()
+            process_results(results_filename(pc, name))
+            t_total += time.perf_counter() - t0
+            forget_resident(pc)
+            done += 1
+    out = {'models': done, 's_per_model': round(t_total / done, 4),
+           'models_per_hour': round(done / t_total * 3600.0, 1)}
+    log(f'batched_vs_plain: reference-style loop {out}')
+    return out
+
+
 def _batch_alg_bytes(cases):
     from .roofline import algorithmic_bytes
     alg = 0
@@ -48,11 +88,15 @@ def _batch_alg_bytes(cases):
     return alg
 
 
-def batched_vs_plain(count=32, chunk=96, device=0, log=print, step_probe=True):
+def batched_vs_plain(count=32, chunk=96, device=0, log=print, step_probe=True, reference_loop=16):
     """The batched workload (``count`` soil realizations x HH/VV) one model
     at a time and as one device batch: models/hour of each, identical
     results, and the batch's step rate in each execution mode (the bound
-    on batched / plain that the kernels allow)."""
+    on batched / plain that the kernels allow).  With ``reference_loop`` > 0
+    that many of the models also run through the reference-style loop
+    (host volume, Ca/Cb, MainSimHip.run writing the results file,
+    process_results), against which the batch is compared with its host
+    volumes counted."""
     native.init(device)
     t0 = time.perf_counter()
     cases = batched_workload(count)
@@ -125,6 +169,7 @@ def batched_vs_plain(count=32, chunk=96, device=0, log=print, step_probe=True):
         log(f'batched_vs_plain: step rates {modes}')
     for m in models:
         m.release()
+    ref = _batched_reference_loop(min(reference_loop, len(cases)), beams, log=log) if reference_loop else None
     beams.release()
     n = len(cases)
     cu = float(np.dot(cells, steps))
@@ -144,6 +189,14 @@ def batched_vs_plain(count=32, chunk=96, device=0, log=print, step_probe=True):
            'identical_results': bool(same),
            'grid_cells_min_max': [int(min(cells)), int(max(cells))],
            'steps': int(max(steps))}
+    if ref:
+        # the reference-style loop generates each model's host volume inside
+        # its loop; the batch's volumes were generated before it, so they are
+        # counted here
+        with_host = n / (t_batch + t_host) * 3600
+        out['reference_loop'] = ref
+        out['batched_models_per_hour_with_host_volumes'] = round(with_host, 1)
+        out['batched_vs_reference_loop'] = round(with_host / ref['models_per_hour'], 3)
     if modes:
         out['step_rate_by_mode'] = modes
         best = min(v['ms_per_step'] for k, v in modes.items() if k != 'one_at_a_time')
