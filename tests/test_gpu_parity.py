@@ -360,6 +360,28 @@ def test_flat_tiles_bitwise(flat_case, launch):
     sim.destroy()
 
 
+def test_flat_slabs_bitwise(flat_case):
+    """Flat tiles launched slab by slab (FDTD_SLAB_BYTES), three 2-plane
+    chunks a slab: the same fields and phasors as the oracle."""
+    import os
+    pc, Ca, Cb, beam, ref = flat_case
+    g = pc.geom_params
+    say('case: flat tiles in slabs,', g.size_x, 'x', g.size_y, 'x', g.size_z)
+    os.environ['FDTD_SLAB_BYTES'] = str(32 * 3 * 2 * g.size_y * g.size_x)
+    try:
+        sim = make_sim(pc, Ca, Cb, beam, launch=(0, 2, 1, -1, 0), fused=False)
+    finally:
+        del os.environ['FDTD_SLAB_BYTES']
+    sim.run(50)
+    sim.sync()
+    for n in native.FIELD_NAMES:
+        assert_same(sim.field(n), ref.fields[n], n)
+    ph, oph = sim.phasors(), ref.phasors()
+    for n in native.PHASOR_NAMES:
+        assert_same(ph[n], oph[n], n)
+    sim.destroy()
+
+
 def test_flat_batch_equals_solo():
     """Members of widths 101, 67 and 139 (pitches 112, 80, 144) stepped as
     one batch on flat tiles (automatic shape, explicit flat tiles, and each
