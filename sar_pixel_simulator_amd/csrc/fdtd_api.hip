@@ -377,7 +377,7 @@ static int auto_xvec(const std::vector<FdtdSim *> &members, int xw) {
 template <typename XwOf>
 static bool flat_pays(const std::vector<FdtdSim *> &members, XwOf xw_of) {
     double rows_slots = 0, flat_slots = 0;
-    for (const FdtdSim *s : members) {
+    for (FdtdSim *s : members) {
         if (s->h.sy < kWave) return false;
         const long long tw = (long long)kWave * xw_of(s);
         rows_slots += (double)s->g.nz * s->g.ny * ((s->g.nx + tw - 1) / tw * tw);
@@ -465,9 +465,7 @@ static int group_build(Group &G) {
         const bool automatic = !xw && !ty && !xv && G.launch.xcd == 0;
         if (xw == kXwFlat ||
             (automatic && G.fused == 0 &&
-             flat_pays(G.members, [&](const FdtdSim *s) {
-                 return auto_xwaves(std::vector<FdtdSim *>{const_cast<FdtdSim *>(s)}, 1);
-             }))) {
+             flat_pays(G.members, [](FdtdSim *s) { return auto_xwaves(std::vector<FdtdSim *>{s}, 1); }))) {
             G.flat = true;
             per_member = false;
             G.xvec = 1;
