@@ -34,6 +34,18 @@ __device__ __forceinline__ int block_tile(int shape, int n) {
     return (shape & kShapeXcd) ? xcd_tile(blockIdx.x, n) : (int)blockIdx.x;
 }
 
+// Neighbour lanes of a wavefront through DPP (wave_shl:1 / wave_shr:1, one
+// VALU move, no LDS round trip as a shuffle takes): lane_next(v) is lane
+// i + 1's v, lane_prev(v) lane i - 1's; lane 63 and lane 0 get 0.  Called
+// where the whole wavefront is active.
+__device__ __forceinline__ float lane_next(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float lane_prev(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+
 __device__ __forceinline__ int load_step(const int *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -603,7 +615,7 @@ __device__ __forceinline__ void vcopy(float (&r)[V], const float (&s)[V]) {
 // registers.  (Issuing plane z + 1's loads before computing plane z was
 // measured slower: the extra registers cost more occupancy than it hides.)
 // The x +- 1 neighbours come from the lane's own cells or, at its ends, from
-// the adjacent lane by a wavefront shuffle (from memory at the tile edge).
+// the adjacent lane by a DPP lane move (from memory at the tile edge).
 // ---------------------------------------------------------------------------
 
 // What the H half-step loads per plane: H(z), E(z+1) (ex, ey), ez(z),
@@ -797,8 +809,8 @@ k_h_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     if (t >= st.nt) return;                            // uniform per block
     const StepK k = step_at(kg, st, t);
     for (int z = z0;;) {
-        float ey_next = __shfl_down(eyc[0], 1, kWave);
-        float ez_next = __shfl_down(P.ez[0], 1, kWave);
+        float ey_next = lane_next(eyc[0]);
+        float ez_next = lane_next(P.ez[0]);
         if (lane == kWave - 1) {
             ey_next = P.ey_x;
             ez_next = P.ez_x;
@@ -935,8 +947,8 @@ k_e_half(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int nti
     if (t >= st.nt) return;                            // uniform per block
     const StepK k = step_at(kg, st, t);
     for (int z = z0;;) {
-        float hz_prev = __shfl_up(P.hz[V - 1], 1, kWave);
-        float hy_prev = __shfl_up(P.hy[V - 1], 1, kWave);
+        float hz_prev = lane_prev(P.hz[V - 1]);
+        float hy_prev = lane_prev(P.hy[V - 1]);
         if (lane == 0) {
             hz_prev = P.hz_p;
             hy_prev = P.hy_p;
@@ -1514,8 +1526,8 @@ k_step_halo(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int 
                 ez_x = *at(fo[2], oh + V);
             }
         }
-        float ey_next = __shfl_down(ey[0], 1, kWave);
-        float ez_next = __shfl_down(ez[0], 1, kWave);
+        float ey_next = lane_next(ey[0]);
+        float ez_next = lane_next(ez[0]);
         if (lane == kWave - 1) {
             ey_next = ey_x;
             ez_next = ez_x;
@@ -1605,9 +1617,9 @@ k_step_halo(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int 
             hz_h = *at(fo[5], oh);
         }
         // ---- H(t+1/2) of the row, then hx, hz of the row below
-        float ey_next = __shfl_down(eyc[0], 1, kWave);
-        float ez_next = __shfl_down(ez[0], 1, kWave);
-        float eym_next = __shfl_down(eym[0], 1, kWave);
+        float ey_next = lane_next(eyc[0]);
+        float ez_next = lane_next(ez[0]);
+        float eym_next = lane_next(eym[0]);
         if (lane == kWave - 1) {
             ey_next = ey_x;
             ez_next = ez_x;
@@ -1642,8 +1654,8 @@ k_step_halo(const Desc *__restrict__ descs, const int4 *__restrict__ tiles, int 
             }
         }
         // ---- E(t+1) of the row
-        float hz_prev = __shfl_up(hz[V - 1], 1, kWave);
-        float hy_prev = __shfl_up(hy[V - 1], 1, kWave);
+        float hz_prev = lane_prev(hz[V - 1]);
+        float hy_prev = lane_prev(hy[V - 1]);
         if (lane == 0) {
             hz_prev = 0.f;
             hy_prev = 0.f;
