@@ -30,12 +30,19 @@ CANDIDATES = {
     'rows': [(r, zc, xv, 1, xo) for r in (3, 7) for zc in (4, 8, 16, 32) for xv in (1, 2)
              for xo in (0, 1)],
     'halo': [(ty, zc, xv, xw, 0) for xv in (1, 2) for xw, ty in ((4, 1), (2, 1), (2, 2), (1, 4))
-             for zc in (4, 8, 16)],
+             for zc in (4, 8, 16)]
+            # fine bands of 1, 2 and 4 row blocks per XCD (the wide grids'
+            # neighbour rows then pass through one L2)
+            + [(ty, zc, 1, xw, xo) for xw, ty in ((2, 1), (4, 1), (2, 2)) for zc in (8, 16)
+               for xo in (3, 4, 5)],
 }
 # library defaults: 0 = chosen from the grid (split: one cell per lane,
-# four wavefronts per block, as many along x as the grid fills)
+# four wavefronts per block, as many along x as the grid fills); halo: one
+# row of two wavefronts per block, XCDs dealt bands of 4 row blocks (at
+# 761 x 761 x 203 and 1523 x 1523 x 311 no slower than 2 x 2 blocks in plain
+# order, and 5 % faster at the larger)
 DEFAULT = {'fused': (4, 8, 0, 1, 0), 'split': (0, 2, 0, 0, 0), 'rows': (7, 16, 2, 1, 0),
-           'halo': (2, 8, 1, 2, 0)}
+           'halo': (1, 8, 1, 2, 5)}
 STEPPERS = {0: 'split', 1: 'fused', 2: 'rows', 3: 'halo'}
 
 
@@ -191,10 +198,16 @@ def _tune_workload(name, fused):
 
 
 if __name__ == '__main__':
+    # python -m sar_pixel_simulator_amd.hip.tuning [workload ...] [--steppers 3,0]
     import sys
-    args = sys.argv[1:] or ['large']
-    for name in args:
-        for fused in (1, 0, 2):
+    args = sys.argv[1:]
+    steppers = (1, 0, 2, 3)
+    if '--steppers' in args:
+        i = args.index('--steppers')
+        steppers = tuple(int(v) for v in args[i + 1].split(','))
+        del args[i:i + 2]
+    for name in args or ['large']:
+        for fused in steppers:
             best, res = _tune_workload(name, fused)
             tag = _stepper(fused)
             for shape, ms in sorted(res.items(), key=lambda kv: kv[1]):

@@ -108,7 +108,9 @@ def test_fields_psi_phasors_bitwise(soil60, fused):
                                           (False, (1, 2, 1, 4, 3)), (False, (2, 3, 1, 2, 4)),
                                           (False, (4, 1, 1, 1, 6)), (3, (2, 8, 1, 2, 4)),
                                           (3, (1, 5, 1, 4, 3)), (3, (2, 3, 2, 2, 5)),
-                                          (3, (1, 7, 1, 1, 6))])
+                                          (3, (1, 7, 1, 1, 6)),
+                                          # the tuned halo shapes of the two largest workloads
+                                          (3, (1, 8, 1, 2, 5)), (3, (1, 16, 1, 4, 4))])
 def test_launch_shapes_bitwise(soil60, fused, launch):
     pc, Ca, Cb, beam, ref = soil60
     say('case: fused', fused, 'launch', launch)

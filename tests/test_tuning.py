@@ -52,3 +52,6 @@ def test_every_stepper_has_a_default_shape():
     # with at most 8 wavefronts and 1 or 2 cells per lane
     for ty, zc, xv, xw, xo in tuning.CANDIDATES['halo']:
         assert xv in (1, 2) and ty * xw <= 8
+        assert xo in (0, 3, 4, 5)           # plain order or fine bands of 1, 2, 4 row blocks
+    # flat tiles are a split-stepper shape of one cell per lane
+    assert all(xv == 1 and ty == 0 for ty, zc, xv, xw, xo in tuning.CANDIDATES['split'] if xw == -1)
