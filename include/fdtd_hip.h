@@ -209,10 +209,10 @@ int fdtd_sim_set_stepper(FdtdSim *s, int fused);
  * wavefronts (ty also 8).  Fused stepper: xwaves 1, ty 2, 4 or 8
  * rows per wavefront with (4 ty + 2) xvec <= 64 (LDS).  0 for ty, xvec or
  * xwaves: chosen from the grid (flat tiles, below, where they fill the lanes
- * better).  xwaves -1 (split stepper, xvec 0 or 1, ty 0, plain order, row
- * pitch >= 64): flat tiles, blocks of 4 wavefronts each over 64
- * consecutive cells of a plane's row-major (y, x) index, so that narrow rows
- * share wavefronts instead of each being padded to whole ones.  xcd_order:
+ * better).  xwaves -1 (split stepper, xvec 0 or 1, ty 0, plain order or
+ * fine bands, row pitch >= 64): flat tiles, blocks of 4 wavefronts each over
+ * 64 consecutive cells of a plane's row-major (y, x) index, so that narrow
+ * rows share wavefronts instead of each being padded to whole ones.  xcd_order:
  * 0 runs tiles in plain order, 1
  * deals each XCD a contiguous range of tiles, 2 (split and halo only) deals
  * each XCD a band of rows within every z chunk, 3 + b (split and halo only,

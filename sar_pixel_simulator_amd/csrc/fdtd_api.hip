@@ -423,7 +423,8 @@ static int group_build(Group &G) {
         CHECK(G.members[0]->h.fused == 0, "flat tiles are a split-stepper shape");
         CHECK(xv == 0 || xv == 1, "flat tiles hold one x cell per lane (xvec 0 or 1, got %d)", xv);
         CHECK(ty == 0, "flat tiles take no ty (got %d)", ty);
-        CHECK(G.launch.xcd == 0, "flat tiles run in plain block order (got %d)", G.launch.xcd);
+        CHECK(G.launch.xcd == 0 || G.launch.xcd >= kXcdFine, "flat tiles run in plain order or fine bands (got %d)",
+              G.launch.xcd);
         for (FdtdSim *s : G.members)
             CHECK(s->h.sy >= kWave, "flat tiles need a row pitch of at least %d cells (nx = %d)", kWave, s->g.nx);
     }
@@ -515,9 +516,24 @@ static int group_build(Group &G) {
             // of the plane's row-major (y, x) index
             const long long span = (long long)s->g.ny * s->h.sy;
             CHECK(span < (1LL << 31), "plane of %lld cells too large for flat tiles", span);
-            for (int z0 = 0; z0 < nz; z0 += zc)
-                for (long long f0 = 0; f0 < span; f0 += 4 * kWave)
-                    tiles.push_back(make_int4(tag, (int)f0, 0, z0 | (std::min(nz, z0 + zc) << 16)));
+            const int nfb = (int)((span + 4 * kWave - 1) / (4 * kWave));
+            for (int z0 = 0; z0 < nz; z0 += zc) {
+                const int zw = z0 | (std::min(nz, z0 + zc) << 16);
+                if (!band) {
+                    for (int b = 0; b < nfb; ++b) tiles.push_back(make_int4(tag, b * 4 * kWave, 0, zw));
+                    continue;
+                }
+                // fine bands over the flat blocks (the row blocks of one
+                // column): slot i of a panel runs on XCD i % kXcds and takes
+                // block (p kXcds + i % kXcds) band + i / kXcds; slots past
+                // the last block are empty tiles
+                const int per_panel = kXcds * band, panels = (nfb + per_panel - 1) / per_panel;
+                for (int p = 0; p < panels; ++p)
+                    for (int i = 0; i < per_panel; ++i) {
+                        const int b = (p * kXcds + i % kXcds) * band + i / kXcds;
+                        tiles.push_back(b < nfb ? make_int4(tag, b * 4 * kWave, 0, zw) : make_int4(0, 0, 0, 0));
+                    }
+            }
             G.max_dft_items = std::max(G.max_dft_items, s->h.dft_items);
             continue;
         }
@@ -593,8 +609,11 @@ static int group_build(Group &G) {
         const FdtdSim *m = G.members[0];
         const int zc = G.launch.zchunk > 0 ? G.launch.zchunk : m->g.nz;
         const long long span = (long long)m->g.ny * m->h.sy;
-        G.tg = TileGrid{(int)((span + 4 * kWave - 1) / (4 * kWave)), 1, 4 * kWave, 0, zc, m->g.nz, 0, 0, 1};
-        CHECK((long long)G.tg.ntx * ((m->g.nz + zc - 1) / zc) == G.ntiles, "tile grid mismatch");
+        const int nfb = (int)((span + 4 * kWave - 1) / (4 * kWave));
+        G.tg = band ? TileGrid{1, nfb, 4 * kWave, 0, zc, m->g.nz, 0, band, 1}
+                    : TileGrid{nfb, 1, 4 * kWave, 0, zc, m->g.nz, 0, 0, 1};
+        CHECK((long long)tilegrid_gx(G.tg) * tilegrid_gy(G.tg) * ((m->g.nz + zc - 1) / zc) == G.ntiles,
+              "tile grid mismatch");
         G.step1 = m->h.step;
     } else if (G.members.size() == 1 && (G.fused == 0 || G.fused == 3) && (G.launch.xcd == 0 || band)) {
         const FdtdSim *m = G.members[0];
@@ -621,7 +640,10 @@ static int group_build(Group &G) {
             const TileGrid tg{(m->g.nx + tx - 1) / tx, (m->g.ny + tys - 1) / tys, tx, tys, zc, m->g.nz, 0, band, 0};
             per_chunk = tilegrid_gx(tg) * tilegrid_gy(tg);
         }
-        if (G.flat) per_chunk = (int)(((long long)m->g.ny * m->h.sy + 4 * kWave - 1) / (4 * kWave));
+        if (G.flat) {
+            const int nfb = (int)(((long long)m->g.ny * m->h.sy + 4 * kWave - 1) / (4 * kWave));
+            per_chunk = band ? (nfb + kXcds * band - 1) / (kXcds * band) * (kXcds * band) : nfb;
+        }
         const long long chunk_bytes = 32LL * zc * m->g.ny * m->g.nx;
         const int chunks = (int)std::max(1LL, sb / chunk_bytes);
         G.slabs.clear();

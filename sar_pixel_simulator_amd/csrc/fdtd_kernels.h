@@ -111,9 +111,11 @@ constexpr int kShapeXcd = 4;
 // y neighbours, so the rows and columns a tile re-reads of its neighbours
 // pass through one L2, while all XCDs stream through the same narrow
 // window of rows.  Blocks past the last row block are empty.
-// flat = 1 (flat tiles, band 0): ntx x 1 x (z chunks) blocks, block i
-// covering cells i tx .. i tx + tx - 1 of the plane's row-major (y, x)
-// index.  ntx = 0: tiles come from the group's list.
+// flat = 1 (flat tiles): block b covers cells b tx .. b tx + tx - 1 of the
+// plane's row-major (y, x) index; band 0: ntx x 1 x (z chunks) blocks, block
+// (b, 0, k) for flat block b; band > 0: the flat blocks are the nyb row
+// blocks of one column (ntx = 1), dealt to the XCDs in fine bands as above.
+// ntx = 0: tiles come from the group's list.
 struct TileGrid {
     int ntx, nyb, tx, ty, zc, nz, t0, band, flat;
 };

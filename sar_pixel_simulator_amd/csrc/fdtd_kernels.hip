@@ -731,11 +731,6 @@ __device__ __forceinline__ bool split_tile(const int4 *__restrict__ tiles, int n
                                            const TileGrid &tg, int4 &tl) {
     if (DIRECT) {
         const int z0 = (int)(blockIdx.z + tg.t0) * tg.zc, z1 = min(tg.nz, z0 + tg.zc);
-        if (tg.flat) {
-            // flat tiles: block i covers the plane's cells from i tx on
-            tl = make_int4(0, (int)blockIdx.x * tg.tx, 0, z0 | (z1 << 16));
-            return true;
-        }
         int xb = (int)blockIdx.x, yb = (int)blockIdx.y;
         if (tg.band) {
             // fine bands: slot i of panel p; XCD i % 8 takes the
@@ -744,6 +739,12 @@ __device__ __forceinline__ bool split_tile(const int4 *__restrict__ tiles, int n
             yb = (yb * kXcds + xb % kXcds) * tg.band + q;
             xb = k - q * tg.ntx;
             if (yb >= tg.nyb) return false;
+        }
+        if (tg.flat) {
+            // flat tiles: block b covers the plane's cells from b tx on; in
+            // fine bands the flat blocks are the row blocks of one column
+            tl = make_int4(0, (tg.band ? yb : xb) * tg.tx, 0, z0 | (z1 << 16));
+            return true;
         }
         tl = make_int4(0, xb * tg.tx, yb * tg.ty, z0 | (z1 << 16));
         return true;

@@ -26,7 +26,7 @@ CANDIDATES = {
     'split': [(ty, zc, xv, 1, 1) for xv in (4, 2, 1) for ty in (1, 2, 4) for zc in (8, 16, 32)]
              + [(ty, zc, xv, xw, xo) for xv in (1, 2) for xw, ty in ((4, 1), (2, 1), (2, 2))
                 for zc in (1, 2, 3, 4, 6, 8, 16) for xo in (0, 1)]
-             + [(0, zc, 1, -1, 0) for zc in (1, 2, 3, 4, 6)],
+             + [(0, zc, 1, -1, xo) for zc in (1, 2, 3, 4, 6) for xo in (0, 4, 5)],
     'rows': [(r, zc, xv, 1, xo) for r in (3, 7) for zc in (4, 8, 16, 32) for xv in (1, 2)
              for xo in (0, 1)],
     'halo': [(ty, zc, xv, xw, 0) for xv in (1, 2) for xw, ty in ((4, 1), (2, 1), (2, 2), (1, 4))

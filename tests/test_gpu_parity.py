@@ -343,8 +343,10 @@ def flat_case(request):
 
 
 @pytest.mark.parametrize('launch', [None, (0, 2, 1, -1, 0), (0, 5, 0, -1, 0), (0, 0, 1, -1, 0),
-                                    (0, 1, 1, -1, 0)],
-                         ids=['auto', 'flat-2', 'flat-5', 'flat-column', 'flat-1'])
+                                    (0, 1, 1, -1, 0), (0, 2, 1, -1, 3), (0, 3, 1, -1, 5),
+                                    (0, 4, 0, -1, 6)],
+                         ids=['auto', 'flat-2', 'flat-5', 'flat-column', 'flat-1', 'flat-bands-1',
+                              'flat-bands-4', 'flat-bands-8'])
 def test_flat_tiles_bitwise(flat_case, launch):
     pc, Ca, Cb, beam, ref = flat_case
     g = pc.geom_params
@@ -371,7 +373,7 @@ def test_flat_slabs_bitwise(flat_case):
     say('case: flat tiles in slabs,', g.size_x, 'x', g.size_y, 'x', g.size_z)
     os.environ['FDTD_SLAB_BYTES'] = str(32 * 3 * 2 * g.size_y * g.size_x)
     try:
-        sim = make_sim(pc, Ca, Cb, beam, launch=(0, 2, 1, -1, 0), fused=False)
+        sim = make_sim(pc, Ca, Cb, beam, launch=(0, 2, 1, -1, 4), fused=False)
     finally:
         del os.environ['FDTD_SLAB_BYTES']
     sim.run(50)
@@ -401,7 +403,8 @@ def test_flat_batch_equals_solo():
         solo[-1].update(s.phasors())
         s.destroy()
         sims.append(make_sim(pc, Ca, Cb, beam))
-    for launch, mode in ((None, None), ((0, 4, 0, -1, 0), None), (None, (native.BATCH_SERIAL, 0, 2))):
+    for launch, mode in ((None, None), ((0, 4, 0, -1, 0), None), ((0, 3, 0, -1, 4), None),
+                         (None, (native.BATCH_SERIAL, 0, 2))):
         say('case: flat batch, launch', launch, 'mode', mode)
         for s in sims:
             s.reset()
