@@ -39,7 +39,13 @@ def new_main_sim(params, absorbers=None, fused=None):
     sim = native.Sim(main_geom(params), coeffs_for(params, absorbers))
     fused = stepper_for(g.size_x, g.size_y, g.size_z) if fused is None else fused
     sim.set_stepper(fused)
-    sim.set_launch(*tuning.launch_for(g.size_x, g.size_y, g.size_z, fused=fused))
+    shape = tuning.launch_for(g.size_x, g.size_y, g.size_z, fused=fused)
+    try:
+        sim.set_launch(*shape)
+    except native.FdtdError:
+        # a cached shape the library does not accept (a cache from another
+        # build): the stepper's default shape, which every grid takes
+        sim.set_launch(*tuning.DEFAULT[tuning.STEPPERS[int(fused)]])
     return sim
 
 

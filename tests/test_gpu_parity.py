@@ -892,3 +892,29 @@ def test_pec_box_powers_and_hemisphere_on_device():
     assert abs(pw['bottom_total']) < 0.01 * p_inc
     assert hemi == pytest.approx(p_inc, rel=0.02)
     assert hemi == pytest.approx(pw['up'], rel=0.01)
+
+
+def test_rejected_cached_shape_falls_back_to_default(soil60, tmp_path, monkeypatch):
+    """A cached tile shape the library refuses for its grid (flat tiles on a
+    33-cell row, pitch 48) gives way to the stepper's default shape: the
+    simulation still steps to the oracle's fields."""
+    import json
+    from sar_pixel_simulator_amd.hip import tuning
+    pc, Ca, Cb, beam, ref = soil60
+    g = pc.geom_params
+    cache = tmp_path / 'cache.json'
+    cache.write_text(json.dumps({f'split:gfx950:{g.size_x}x{g.size_y}x{g.size_z}':
+                                 {'ty': 0, 'zchunk': 2, 'xvec': 1, 'xwaves': -1, 'xcd_order': 0}}))
+    monkeypatch.setattr(tuning, 'CACHE', str(cache))
+    assert tuning.launch_for(g.size_x, g.size_y, g.size_z, fused=0) == (0, 2, 1, -1, 0)
+    native.init(0)
+    sim = common.new_main_sim(pc, fused=0)
+    b = native.Beam(g.size_t_inc, g.size_y, g.size_x, *beam)
+    sim.attach_beam(b)
+    sim.set_materials(Ca, Cb)
+    sim.run(60)
+    sim.sync()
+    for n in native.FIELD_NAMES:
+        assert_same(sim.field(n), ref.fields[n], n)
+    sim.destroy()
+    b.release()
