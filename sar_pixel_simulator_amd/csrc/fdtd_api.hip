@@ -623,6 +623,13 @@ static int group_build(Group &G) {
               "tile grid mismatch");
         G.step1 = m->h.step;
     }
+    if (G.tg.ntx > 0 && tilegrid_gy(G.tg) > 65535) {
+        // more row blocks than a launch grid's y extent holds (very many
+        // narrow rows): the kernels take their tiles from the list instead
+        // (a 1-D grid), built above in the same order
+        G.tg = TileGrid{};
+        G.step1 = nullptr;
+    }
     G.slabs.assign(1, {0, G.ntiles});
     // A single MAIN simulation on the split stepper runs the H and E kernels
     // slab by slab (whole z chunks, about slab_bytes of field and material

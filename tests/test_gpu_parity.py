@@ -918,3 +918,21 @@ def test_rejected_cached_shape_falls_back_to_default(soil60, tmp_path, monkeypat
         assert_same(sim.field(n), ref.fields[n], n)
     sim.destroy()
     b.release()
+
+
+def test_more_row_blocks_than_a_launch_grid_holds():
+    """22 x 66001 x 22 with one row per block: 66001 row blocks exceed a
+    launch grid's y extent (65535), so the kernels take their tiles from
+    the list; fields equal the oracle's on the split and halo steppers."""
+    pc, Ca, Cb = H.soil_case(nx=22, ny=66001, nz=22, nz_mat=8, steps=3, seed=2,
+                             dft_accumulation='reference')
+    say('case: 22 x 66001 x 22, 3 steps, one row per block')
+    beam = H.synthetic_beam(pc)
+    ref = H.oracle_main(pc, Ca, Cb, beam)
+    for fused, launch in ((False, (1, 2, 1, 1, 0)), (3, (1, 8, 1, 1, 0))):
+        sim = make_sim(pc, Ca, Cb, beam, launch=launch, fused=fused)
+        sim.run(3)
+        sim.sync()
+        for n in native.FIELD_NAMES:
+            assert_same(sim.field(n), ref.fields[n], f'{n} stepper {fused}')
+        sim.destroy()
